@@ -1,0 +1,199 @@
+"""Benchmark: converged GTEPS and time-to-eps of bin/excent's engine on one MI355X.
+
+Workload (default): seeded R-MAT scale 24, edge factor 16 (a,b,c = .57,.19,.19),
+self-loops and duplicates removed, --measure all (PageRank d = 0.85, eps 0.01;
+Katz alpha = 0.5/lambda1, beta = 1; eigenvector with lambda1 from a seeded FP64
+power iteration; eps-katz = eps-ev = 1e-6 per vertex), float32 state, the whole
+graph resident in HBM before timing (--cache chunks --preload).  The graph is
+directed, so the eigenvector's initial residual uses in-degrees (--ev-init indeg).
+
+A step is one converged run: the state is restored to its initial values (a device
+copy) and the pass loop runs until nothing is pending.  value = pushed adjacency
+entries / timed seconds / 1e9 (converged GTEPS).  roofline: the dominant push
+kernel's algorithmic bytes per launch / its average HIP-event duration.
+cpu_baseline: the oracle's OpenMP restatement of the same pass loop on the same
+graph and constants, on this host's cores, for a bounded number of seconds.
+
+Prints one JSON line on stdout; progress goes to stderr.
+"""
+import argparse
+import json
+import os
+import subprocess
+import sys
+import tempfile
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "converged GTEPS + time-to-eps, in-HBM & out-of-core; batched vs plain at FP64 err"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def read_env(path):
+    out = {}
+    for line in open(path):
+        line = line.strip()
+        if line and not line.startswith("#") and "=" in line:
+            k, v = line.split("=", 1)
+            out[k] = float(v)
+    return out
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--scale", type=int, default=24)
+    ap.add_argument("--edge-factor", type=int, default=16)
+    ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--measure", default="all")
+    ap.add_argument("--kernel", default="lb", choices=["lb", "group"])
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--workdir", default=None)
+    ap.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "push_traffic_rmat24_all.json"),
+                    help="JSON with HBM bytes per push launch from rocprofv3 --pmc runs")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
+              "the multi-GPU design is described, not built", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+
+    import excent_amd.native as nat
+
+    work = args.workdir or tempfile.mkdtemp(prefix="excent_bench_", dir=os.environ.get("TMPDIR", "/tmp"))
+    os.makedirs(work, exist_ok=True)
+    graph = os.path.join(work, f"rmat{args.scale}_ef{args.edge_factor}_s{args.seed}.egr")
+    t0 = time.time()
+    if not os.path.exists(graph):
+        rc = nat.run_entry("excent_rmat_main", ["--scale", args.scale, "--edge-factor", args.edge_factor,
+                                                "--seed", args.seed, graph])
+        assert rc == 0, "graph generation failed"
+    log(f"graph {graph} ready in {time.time() - t0:.1f}s")
+    envp = graph + ".env"
+    t0 = time.time()
+    if not os.path.exists(envp):
+        rc = nat.run_entry("excent_params_main", [graph, envp, "--directed", "--alpha-factor", "0.5", "--seed", "7",
+                                                  "--eps-abs", "1e-6"])
+        assert rc == 0, "constants failed"
+    env = read_env(envp)
+    log(f"constants {env} in {time.time() - t0:.1f}s")
+
+    excent_log = os.path.join(work, "excent.log")
+    sargs = [graph, "--measure", args.measure, "--cache", "chunks", "--preload", "--stream-ratio", "1",
+             "--timing", "--push-kernel", args.kernel, "--log", excent_log]
+    if args.measure in ("katz", "all"):
+        sargs += ["--alpha", env["ALPHA"], "--beta", 1, "--eps-katz", "1e-6"]
+    if args.measure in ("ev", "all"):
+        sargs += ["--lambda", env["LAMBDA"], "--eps-ev", "1e-6", "--ev-init", "indeg"]
+    t0 = time.time()
+    sess = nat.Session(sargs)
+    log(f"session setup {time.time() - t0:.1f}s")
+    first = True
+    for w in range(args.warmup):
+        r = sess.run(reinit=not first)
+        first = False
+        log(f"warmup {w}: {r['passes']} passes, {r['time_passes_s']:.3f}s, {r['pushed_edges'] / 1e9:.2f} G edges")
+    nat.synchronize()
+    runs = []
+    t_start = time.perf_counter()
+    for s in range(args.steps):
+        runs.append(sess.run(reinit=not first))
+        first = False
+    nat.synchronize()
+    elapsed = time.perf_counter() - t_start
+    sess.results()
+    sess.close()
+    for i, r in enumerate(runs):
+        log(f"step {i}: {r['passes']} passes, {r['time_passes_s']:.3f}s, {r['pushed_edges'] / 1e9:.3f} G edges")
+    pushed = sum(r["pushed_edges"] for r in runs)
+    gteps = pushed / elapsed / 1e9
+    # dominant push kernel (by total time) among the vertex sources
+    names = ["range", "bitmap", "list", "hot"]
+    tot_ms = [sum(r["kernel"]["src_ms"][i] for r in runs) for i in range(4)]
+    tot_launch = [sum(r["kernel"]["src_launches"][i] for r in runs) for i in range(4)]
+    tot_bytes = [sum(r["kernel"]["src_bytes"][i] for r in runs) for i in range(4)]
+    dom = int(np.argmax(tot_ms))
+    achieved = tot_bytes[dom] / (tot_ms[dom] / 1e3) / 1e9 if tot_ms[dom] > 0 else 0.0
+    traffic = None
+    if os.path.exists(args.traffic):
+        try:
+            t = json.load(open(args.traffic))
+            traffic = t.get(names[dom])
+        except Exception:
+            traffic = None
+    kern = {k: sum(r["kernel"][k] for r in runs) for k in ("snapshot_ms", "push_ms", "piece_ms", "other_ms",
+                                                          "snapshot_bytes", "push_bytes", "piece_bytes")}
+    roof = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+            "kernel": f"k_push<3,4,{dom},0> ({names[dom]} source)" if args.measure == "all" else f"k_push ({names[dom]})",
+            "launches": tot_launch[dom], "avg_launch_ms": round(tot_ms[dom] / max(1, tot_launch[dom]), 5),
+            "algorithmic_bytes_per_launch": round(tot_bytes[dom] / max(1, tot_launch[dom]), 1)}
+
+    cpu = None
+    if not args.no_cpu:
+        from oracle import oracle as orc
+        log("cpu baseline: loading graph")
+        idx, adj = orc.egr_load(graph)
+        n = len(idx) - 1
+        kinds = {"pr": [0], "katz": [1], "ev": [2], "all": [0, 1, 2]}[args.measure]
+        xs, rs, eps, scale, divdeg = [], [], [], [], []
+        indeg = np.bincount(adj, minlength=n).astype(np.float64)
+        for k in kinds:
+            if k == 0:
+                x, r = orc.init(idx, 0)
+                eps.append(0.01); scale.append(np.float32(0.85)); divdeg.append(1)
+            elif k == 1:
+                x, r = orc.init(idx, 1, beta=1.0)
+                eps.append(1e-6); scale.append(np.float32(env["ALPHA"])); divdeg.append(0)
+            else:
+                lam = float(np.float32(env["LAMBDA"]))
+                x = np.ones(n, np.float32)
+                r = (indeg / lam - 1.0).astype(np.float32)
+                eps.append(1e-6); scale.append(np.float32(1.0 / env["LAMBDA"])); divdeg.append(0)
+            xs.append(x); rs.append(r)
+        cap = orc.chunk_cap(len(adj))
+        cv0 = orc.plan_chunks(idx, cap)
+        threads = min(16, os.cpu_count() or 1)
+        passes, cpushed, csecs = orc.run_push(idx, adj, cv0, xs, rs, eps, scale, divdeg, threads=threads,
+                                              time_limit=args.cpu_seconds)
+        cpu = {"value": round(cpushed / csecs / 1e9, 4), "unit": "GTEPS", "cores": threads, "kind": "port",
+               "sample": f"first {passes} passes of the same run (same graph, constants, eps), "
+                         f"{csecs:.1f}s of a {args.cpu_seconds:.0f}s budget"}
+        log(f"cpu baseline: {passes} passes, {cpushed / 1e9:.3f} G edges in {csecs:.1f}s")
+
+    n_v, m_e = None, None
+    for line in open(excent_log):
+        if line.startswith("graph: n="):
+            n_v = int(line.split("n=")[1].split()[0])
+            m_e = int(line.split("m=")[1].split()[0])
+            break
+    out = {
+        "metric": METRIC, "value": round(gteps, 4), "unit": "GTEPS", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(elapsed / args.steps * 1e3, 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"R-MAT scale {args.scale} ef {args.edge_factor} --measure {args.measure}, "
+                               f"whole graph in HBM, converged to eps", "n": n_v, "m": m_e,
+                   "lambda1": env["LAMBDA"], "alpha": env["ALPHA"], "eps_pr": 0.01, "eps_katz": 1e-6,
+                   "eps_ev": 1e-6, "push_kernel": args.kernel, "parallelism": "single GPU"},
+        "time_to_eps_s": round(sum(r["time_passes_s"] for r in runs) / len(runs), 4),
+        "passes": [r["passes"] for r in runs],
+        "pushed_edges_per_run": runs[-1]["pushed_edges"],
+        "kernel_ms": {k: round(v / len(runs), 3) for k, v in kern.items() if k.endswith("_ms")},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,810 @@
+// engine.cpp -- bin/excent: flags, setup, the pass loop, results and report.
+//
+// Control flow, flags, defaults, messages and report lines follow the reference's
+// main() (src/excent.cu:242-805, paths relative to the reference root) so scripts that
+// drive or parse bin/excent keep working.  Device work goes through the C ABI of
+// include/excent_hip.h; this file includes no HIP header.  Differences by design:
+//   * resident chunks next to each other with the same list/range decision are pushed
+//     in one launch (the arena holds them contiguously);
+//   * the hot lists of every chunk are filled at setup (the reference fills them in
+//     pass 0 only from chunks that stream, leaving lists of skipped chunks unfilled);
+//   * the hot push reads its count on the device (no blocking count readback);
+//   * tau calibration includes the PCIe copy of what it times;
+//   * extension flags: --preload, --cache-mb MB, --push-kernel lb|group,
+//     --ev-init outdeg|indeg, --stats-json FILE, --timing, --log FILE, --device N;
+//   * a session API (excent_session_*) runs setup once and the pass loop many times
+//     on the resident graph, for benchmarks.
+#include <omp.h>
+#include <sys/resource.h>
+#include <sys/stat.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "egr_graph.hpp"
+#include "excent.h"
+#include "excent_hip.h"
+#include "planner.hpp"
+
+namespace excent {
+namespace {
+
+typedef std::chrono::steady_clock Clock;
+double secs(Clock::time_point a, Clock::time_point b) { return std::chrono::duration<double>(b - a).count(); }
+
+const char* MNAME[EXC_MAXM] = {"pr", "katz", "ev"};
+
+struct exit_error {
+    int code;
+    std::string msg;
+};
+
+[[noreturn]] void die(const std::string& msg, int code = 1) { throw exit_error{code, msg}; }
+
+void ck(int rc, const char* what) {
+    if (rc != EXC_OK) die(std::string("HIP layer error in ") + what + ": " + exc_last_error());
+}
+
+struct Options {
+    std::string graph;
+    double chunk_medges = 0, stream_ratio = 2.0, reserve_mb = 256;
+    double target_chunks = 16, min_chunk_medges = 4, list_frac = 0.5;
+    bool use_cache = true;
+    std::string cache_mode = "auto", hot_when = "end", hot_order = "sweep", trace_file;
+    bool stream_auto = true, recalibrate = false, dedup = true;
+    std::string hot_rank;
+    int group_opt = 0;
+    bool count_pushes = false;
+    std::string zero_copy = "auto";
+    int max_passes = 100000;
+    std::string dump, measure;
+    double alpha = -1, beta = 1.0, lambda = -1, eps_pr = 0.01, eps_katz = -1, eps_ev = -1;
+    // extensions
+    bool preload = false, timing = false;
+    double cache_mb = -1;
+    std::string push_kernel = "lb", ev_init = "outdeg", stats_json, log;
+    int device = 0;
+};
+
+const char* USAGE =
+    "usage: %s graph.egr --measure pr|katz|ev|all [--alpha A] [--beta 1] [--lambda L] "
+    "[--eps-pr 0.01] [--eps-katz E] [--eps-ev E] [--chunk-medges auto] [--target-chunks 16] "
+    "[--min-chunk-medges 4] [--list-frac 0.5] [--stream-ratio auto|R] "
+    "[--reserve-mb 256] [--cache chunks|hot|none] [--max-passes N] [--dump PREFIX]\n";
+
+// src/excent.cu:269-299
+Options parse(int argc, const char* const* argv) {
+    Options o;
+    if (argc < 2) {
+        fprintf(stderr, USAGE, argc ? argv[0] : "excent");
+        die("", 1);
+    }
+    o.graph = argv[1];
+    for (int i = 2; i < argc; i++) {
+        std::string a = argv[i];
+        auto next = [&]() -> const char* {
+            if (i + 1 >= argc) die("missing value for " + a);
+            return argv[++i];
+        };
+        if (a == "--chunk-medges") o.chunk_medges = atof(next());
+        else if (a == "--target-chunks") o.target_chunks = atof(next());
+        else if (a == "--min-chunk-medges") o.min_chunk_medges = atof(next());
+        else if (a == "--list-frac") o.list_frac = atof(next());
+        else if (a == "--stream-ratio") { std::string v = next(); o.stream_auto = v == "auto"; if (!o.stream_auto) o.stream_ratio = atof(v.c_str()); }
+        else if (a == "--dedup") { std::string v = next(); if (v != "on" && v != "off") die("--dedup must be on or off"); o.dedup = v == "on"; }
+        else if (a == "--hot-rank") o.hot_rank = next();
+        else if (a == "--recalibrate") o.recalibrate = true;
+        else if (a == "--reserve-mb") o.reserve_mb = atof(next());
+        else if (a == "--no-cache") o.use_cache = false;
+        else if (a == "--cache") o.cache_mode = next();
+        else if (a == "--hot-when") o.hot_when = next();
+        else if (a == "--hot-order") o.hot_order = next();
+        else if (a == "--trace-pending") o.trace_file = next();
+        else if (a == "--group") o.group_opt = atoi(next());
+        else if (a == "--count-pushes") o.count_pushes = true;
+        else if (a == "--zero-copy") o.zero_copy = next();
+        else if (a == "--max-passes") o.max_passes = atoi(next());
+        else if (a == "--dump") o.dump = next();
+        else if (a == "--measure") o.measure = next();
+        else if (a == "--alpha") o.alpha = atof(next());
+        else if (a == "--beta") o.beta = atof(next());
+        else if (a == "--lambda") o.lambda = atof(next());
+        else if (a == "--eps-pr") o.eps_pr = atof(next());
+        else if (a == "--eps-katz") o.eps_katz = atof(next());
+        else if (a == "--eps-ev") o.eps_ev = atof(next());
+        // extensions (no reference counterpart)
+        else if (a == "--preload") o.preload = true;
+        else if (a == "--timing") o.timing = true;
+        else if (a == "--cache-mb") o.cache_mb = atof(next());
+        else if (a == "--push-kernel") o.push_kernel = next();
+        else if (a == "--ev-init") o.ev_init = next();
+        else if (a == "--stats-json") o.stats_json = next();
+        else if (a == "--log") o.log = next();
+        else if (a == "--device") o.device = atoi(next());
+        else die("unknown option " + a);
+    }
+    return o;
+}
+
+// Every column id must name a vertex: the push kernels add to r[id] with no bounds check.
+void check_columns(const int32_t* a, size_t cnt, int n, const char* where) {
+    long long bad = -1;
+#pragma omp parallel for schedule(static) num_threads(16) reduction(max : bad)
+    for (long long i = 0; i < (long long)cnt; i++)
+        if ((uint32_t)a[i] >= (uint32_t)n) bad = std::max(bad, i);
+    if (bad >= 0) {
+        char b[256];
+        snprintf(b, sizeof b, "graph: column id %d out of range [0, %d) in %s", a[bad], n, where);
+        die(b);
+    }
+}
+
+struct Faults { long maj, min; };
+Faults faults() {
+    struct rusage ru;
+    getrusage(RUSAGE_SELF, &ru);
+    return {ru.ru_majflt, ru.ru_minflt};
+}
+
+}  // namespace
+
+class Engine {
+public:
+    explicit Engine(const Options& o, bool session = false) : o_(o), session_(session) {}
+    ~Engine() {
+        if (ctx_) exc_destroy(ctx_);
+        if (out_ && own_out_) fclose(out_);
+    }
+    void setup();
+    void run_passes();
+    void reinit();
+    void results();
+    void report();
+    void write_stats_json();
+    void fill_stats(excent_run_stats* s);
+
+private:
+    Options o_;
+    bool session_ = false, checkpoint_ = false;
+    FILE* out_ = stdout;
+    bool own_out_ = false;
+    Clock::time_point T0_, T_open_, T_setup_, T_iter_, T_end_;
+    std::unique_ptr<EgrGraph> g_;
+    int n_ = 0, nc_ = 0, push_kernel_ = EXC_KERNEL_LB, cus_ = 0;
+    uint64_t m_ = 0, cap_ = 0;
+    char arch_[64] = {0};
+    bool zc_ = false, use_cache_ = false, hot_mode_ = false;
+    std::vector<EdgeChunk> chunks_;
+    std::vector<int> kinds_;
+    exc_measures M_{};
+    void* ctx_ = nullptr;
+    std::vector<char> resident_, loaded_;
+    int n_resident_ = 0, nh_ = 0;
+    uint64_t resident_edges_ = 0, hot_edges_ = 0, hot_deg_min_ = 0;
+    std::vector<int> hlo_, hhi_;
+    HotSet hs_;
+    int32_t* h_edges_[2] = {nullptr, nullptr};
+    int32_t* h_vids_[2] = {nullptr, nullptr};
+    uint64_t* h_off_[2] = {nullptr, nullptr};
+    double stream_ratio_ = 2.0, t_calib_ = 0;
+    // per run
+    double t_io_ = 0, t_wait_ = 0, t_snap_ = 0;
+    uint64_t bytes_stream_ = 0, bytes_gather_ = 0;
+    int pass_ = 0, cur_ = 0;
+    uint64_t n_list_ = 0, n_range_ = 0, n_launch_ = 0;
+    long tot_maj_ = 0, tot_min_ = 0;
+    uint64_t tot_pages_ = 0, tot_runs_ = 0;
+    std::vector<double> sum_;
+    std::vector<uint32_t> indeg_;
+
+    void init_state();
+    size_t budget_cap(size_t b) const {
+        if (o_.cache_mb < 0) return b;
+        return std::min(b, (size_t)(o_.cache_mb * (1 << 20)));
+    }
+};
+
+void Engine::setup() {
+    const Options& o = o_;
+    T0_ = Clock::now();
+    if (!o.log.empty()) {
+        out_ = fopen(o.log.c_str(), "w");
+        if (!out_) { out_ = stdout; die("cannot open --log file " + o.log); }
+        own_out_ = true;
+    }
+    FILE* out = out_;
+
+    // src/excent.cu:301-316
+    std::string cache_mode = o.use_cache ? o.cache_mode : "none";
+    if (cache_mode != "chunks" && cache_mode != "hot" && cache_mode != "none" && cache_mode != "auto")
+        die("--cache must be auto, chunks, hot or none");
+    use_cache_ = cache_mode == "chunks";
+    hot_mode_ = cache_mode == "hot";
+    if (o.measure == "pr") kinds_ = {EXC_PR};
+    else if (o.measure == "katz") kinds_ = {EXC_KATZ};
+    else if (o.measure == "ev") kinds_ = {EXC_EV};
+    else if (o.measure == "all") kinds_ = {EXC_PR, EXC_KATZ, EXC_EV};
+    else die("--measure must be pr, katz, ev or all");
+    for (int k : kinds_) {
+        if (k == EXC_KATZ && (o.alpha <= 0 || o.eps_katz <= 0)) die("katz needs --alpha and --eps-katz");
+        if (k == EXC_EV && (o.lambda <= 0 || o.eps_ev <= 0)) die("ev needs --lambda and --eps-ev");
+    }
+    if (o.push_kernel == "lb") push_kernel_ = EXC_KERNEL_LB;
+    else if (o.push_kernel == "group") push_kernel_ = EXC_KERNEL_GROUP;
+    else die("--push-kernel must be lb or group");
+    if (o.ev_init != "outdeg" && o.ev_init != "indeg") die("--ev-init must be outdeg or indeg");
+    if (o.zero_copy != "auto" && o.zero_copy != "on" && o.zero_copy != "off") die("--zero-copy must be auto, on or off");
+    if (o.group_opt < 0 || o.group_opt > 64) die("--group must be 0 (automatic) or 1..64");
+    if (o.hot_when != "end" && o.hot_when != "start" && o.hot_when != "both") die("--hot-when must be end, start or both");
+    if (o.hot_order != "sweep" && o.hot_order != "ranges") die("--hot-order must be sweep or ranges");
+
+    try {
+        g_.reset(new EgrGraph(o.graph, 16, /*offsets_in_ram=*/false));
+    } catch (const egr_error& e) {
+        die(e.what());
+    }
+    EgrGraph& g = *g_;
+    n_ = g.n;
+    m_ = g.m;
+    const int n = n_;
+    const uint64_t m = m_;
+    cap_ = chunk_cap(m, o.chunk_medges, o.target_chunks, o.min_chunk_medges);
+    chunks_ = g.plan_chunks(cap_);
+    nc_ = (int)chunks_.size();
+    const int nc = nc_;
+    uint64_t max_chunk_edges = 0;
+    int max_chunk_verts = 0;
+    for (auto& c : chunks_) { max_chunk_edges = std::max(max_chunk_edges, c.edges()); max_chunk_verts = std::max(max_chunk_verts, c.v1 - c.v0); }
+    T_open_ = Clock::now();
+    fprintf(out, "graph: n=%d m=%llu chunks=%d (cap %llu, max %llu edges, %d vertices)  open %.2fs\n",
+            n, (unsigned long long)m, nc, (unsigned long long)cap_, (unsigned long long)max_chunk_edges, max_chunk_verts,
+            secs(T0_, T_open_));
+
+    int integrated = 0;
+    ck(exc_device_info(o.device, &cus_, &integrated, arch_, sizeof arch_), "device info");
+    zc_ = o.zero_copy == "on" || (o.zero_copy == "auto" && integrated);
+
+    M_.nm = (int)kinds_.size();
+    for (int i = 0; i < M_.nm; i++) {
+        const int k = kinds_[i];
+        M_.kind[i] = k;
+        M_.eps[i] = (float)(k == EXC_PR ? o.eps_pr : k == EXC_KATZ ? o.eps_katz : o.eps_ev);
+        M_.scale[i] = (float)(k == EXC_PR ? (double)0.85f : k == EXC_KATZ ? o.alpha : 1.0 / o.lambda);
+        M_.divdeg[i] = k == EXC_PR;
+    }
+    std::vector<int> cv0(nc);
+    for (int c = 0; c < nc; c++) cv0[c] = chunks_[c].v0;
+    exc_config cfg{};
+    cfg.device = o.device; cfg.n = n; cfg.m = m; cfg.nc = nc; cfg.cv0 = cv0.data();
+    cfg.max_chunk_edges = max_chunk_edges; cfg.max_chunk_verts = max_chunk_verts;
+    cfg.trace = !o.trace_file.empty(); cfg.zero_copy = zc_; cfg.push_kernel = push_kernel_; cfg.timing = o.timing;
+    ck(exc_create(&cfg, &M_, &ctx_), "create");
+    void* ctx = ctx_;
+    {   // offsets in 16M-entry slices (src/excent.cu:360-369)
+        const size_t B = (size_t)1 << 24;
+        std::vector<uint64_t> stage(std::min(B, (size_t)n + 1));
+        for (size_t v0 = 0; v0 < (size_t)n + 1; v0 += B) {
+            const size_t cnt = std::min(B, (size_t)n + 1 - v0);
+            g.read_offsets(v0, cnt, stage.data());
+            ck(exc_upload_offsets(ctx, stage.data(), v0, cnt), "offsets upload");
+        }
+    }
+    if (o.ev_init == "indeg" && std::find(kinds_.begin(), kinds_.end(), (int)EXC_EV) != kinds_.end()) {
+        indeg_.assign(n, 0);
+        const int32_t* nl = g.nlist();
+#pragma omp parallel for schedule(static) num_threads(16)
+        for (long long e = 0; e < (long long)m; e++) __atomic_fetch_add(&indeg_[nl[e]], 1u, __ATOMIC_RELAXED);
+    }
+    init_state();
+
+    resident_.assign(nc, 0);
+    loaded_.assign(nc, 0);
+    const size_t reserve = (size_t)(o.reserve_mb * (1 << 20));
+    if (cache_mode == "auto") {   // src/excent.cu:402-409
+        size_t freeB, totalB;
+        ck(exc_mem_info(o.device, &freeB, &totalB), "mem info");
+        const size_t avail = budget_cap(freeB > reserve ? freeB - reserve : 0);
+        const bool fits = freeB > reserve && m * sizeof(int32_t) <= avail;
+        use_cache_ = fits;
+        hot_mode_ = !fits;
+        fprintf(out, "cache: auto -> %s\n", fits ? "chunks (the whole graph fits)" : "high-degree lists");
+    }
+    if (use_cache_) {   // src/excent.cu:410-421: resident chunks are a prefix in file order
+        size_t freeB, totalB;
+        ck(exc_mem_info(o.device, &freeB, &totalB), "mem info");
+        size_t avail = budget_cap(freeB > reserve ? freeB - reserve : 0);
+        for (int c = 0; c < nc; c++) {
+            const size_t need = chunks_[c].edges() * sizeof(int32_t);
+            if (need > avail) break;
+            avail -= need;
+            n_resident_++;
+        }
+        while (n_resident_ > 0) {
+            const int rc = exc_cache_chunks(ctx, chunks_[n_resident_ - 1].e1);
+            if (rc == EXC_OK) break;
+            if (rc != EXC_ENOMEM) ck(rc, "edge cache");
+            n_resident_--;   // allocation failure: keep a shorter prefix
+        }
+        for (int c = 0; c < n_resident_; c++) { resident_[c] = 1; resident_edges_ += chunks_[c].edges(); }
+    }
+
+    hlo_.assign(nc, 0);
+    hhi_.assign(nc, 0);
+    if (hot_mode_) {   // src/excent.cu:428-513
+        size_t freeB, totalB;
+        ck(exc_mem_info(o.device, &freeB, &totalB), "mem info");
+        const size_t bits = (((size_t)n + 31) / 32) * sizeof(uint32_t);
+        const size_t budget = budget_cap(freeB > reserve + bits ? freeB - reserve - bits : 0);
+        std::vector<uint16_t> score;
+        if (!o.hot_rank.empty()) {
+            score.resize(n);
+            FILE* f = fopen(o.hot_rank.c_str(), "rb");
+            if (!f || fread(score.data(), sizeof(uint16_t), n, f) != (size_t)n) {
+                if (f) fclose(f);
+                char b[512];
+                snprintf(b, sizeof b, "--hot-rank: cannot read %d entries from %s", n, o.hot_rank.c_str());
+                die(b);
+            }
+            fclose(f);
+        }
+        hs_ = hot_select(n, g.offsets(), budget, score.empty() ? nullptr : score.data());
+        nh_ = (int)hs_.ids.size();
+        hot_edges_ = hs_.off.back();
+        hot_deg_min_ = hs_.deg_min;
+        const int rc = nh_ > 0 ? exc_cache_hot(ctx, nh_, hs_.ids.data(), hs_.off.data(), hs_.bitmask.data()) : EXC_ENOMEM;
+        if (rc != EXC_OK) {
+            if (rc != EXC_ENOMEM) ck(rc, "hot cache");
+            fprintf(stderr, "degree cache: could not allocate %d lists (%.2f GB); running without a cache\n", nh_, hot_edges_ * 4.0 / 1e9);
+            nh_ = 0;
+            hot_edges_ = 0;
+        } else {
+            for (int c = 0; c < nc; c++) {
+                hlo_[c] = (int)(std::lower_bound(hs_.ids.begin(), hs_.ids.end(), chunks_[c].v0) - hs_.ids.begin());
+                hhi_[c] = (int)(std::lower_bound(hs_.ids.begin(), hs_.ids.end(), chunks_[c].v1) - hs_.ids.begin());
+            }
+        }
+    }
+
+    for (int s = 0; s < 2; s++) ck(exc_slot_buffers(ctx, s, &h_edges_[s], &h_vids_[s], &h_off_[s]), "slot buffers");
+
+    if (nh_ > 0) {   // every chunk's hot lists are filled here, before the passes
+        for (int c = 0; c < nc; c++) {
+            if (hhi_[c] <= hlo_[c]) continue;
+            const EdgeChunk& ch = chunks_[c];
+            g.read_edges(ch.e0, ch.e1, h_edges_[0]);
+            check_columns(h_edges_[0], ch.edges(), n, "a hot chunk");
+            const uint64_t o0 = hs_.off[hlo_[c]], o1 = hs_.off[hhi_[c]];
+            int32_t* dst = h_edges_[1];
+            const int32_t* src = h_edges_[0];
+#pragma omp parallel for schedule(dynamic, 64) num_threads(16)
+            for (int j = hlo_[c]; j < hhi_[c]; j++)
+                memcpy(dst + (hs_.off[j] - o0), src + (g.offset(hs_.ids[j]) - ch.e0), (hs_.off[j + 1] - hs_.off[j]) * sizeof(int32_t));
+            ck(exc_cache_hot_upload(ctx, dst, o0, o1 - o0), "hot fill");
+        }
+    }
+    if (o.preload && n_resident_ > 0) {   // extension: fill the chunk cache before the passes
+        for (int c = 0; c < n_resident_; c++) {
+            const EdgeChunk& ch = chunks_[c];
+            g.read_edges(ch.e0, ch.e1, h_edges_[0]);
+            check_columns(h_edges_[0], ch.edges(), n, "a preloaded chunk");
+            ck(exc_cache_upload(ctx, h_edges_[0], ch.e0, ch.edges()), "preload");
+            loaded_[c] = 1;
+        }
+    }
+
+    // tau (src/excent.cu:515-579)
+    stream_ratio_ = o.stream_ratio;
+    bool stream_auto = o.stream_auto;
+    std::string tau_file;
+    if (stream_auto) {
+        const char* home = getenv("HOME");
+        std::string base = o.graph.substr(o.graph.find_last_of('/') + 1);
+        std::string dir = std::string(home ? home : ".") + "/.cache";
+        mkdir(dir.c_str(), 0755);
+        dir += "/excent";
+        mkdir(dir.c_str(), 0755);
+        tau_file = dir + "/" + base + "." + std::to_string(n) + "." + std::to_string(m) + ".tau";
+        FILE* f = o.recalibrate ? nullptr : fopen(tau_file.c_str(), "r");
+        if (f) {
+            if (fscanf(f, "%lf %lf", &stream_ratio_, &t_calib_) == 2) {
+                fprintf(out, "fetch rule: tau = %.3f (measured earlier on this machine in %.2fs, %s)\n", stream_ratio_, t_calib_, tau_file.c_str());
+                stream_auto = false;
+            }
+            fclose(f);
+        }
+    }
+    if (stream_auto) {
+        auto c0 = Clock::now();
+        const EdgeChunk& ch = chunks_[nc / 2];
+        g.drop_cache(ch.e0, ch.e1, ch.v0, ch.v1);
+        auto a = Clock::now();
+        g.read_edges(ch.e0, ch.e1, h_edges_[0]);
+        double t_copy = 0;
+        ck(exc_slot_copy_probe(ctx, 0, ch.edges() * sizeof(int32_t), &t_copy), "calibration copy");
+        const double t_stream = secs(a, Clock::now());
+        const double fr[3] = {0.05, 0.15, 0.4};
+        double f_got[3], t_g[3];
+        std::vector<std::pair<uint64_t, int>> byd;
+        for (int v = ch.v0; v < ch.v1; v++) byd.push_back({g.degree(v), v});
+        std::sort(byd.begin(), byd.end(), [](const auto& x, const auto& y) { return x.first > y.first; });
+        for (int j = 0; j < 3; j++) {
+            std::vector<int> pick;
+            for (uint64_t acc = 0; acc < fr[j] * ch.edges() && pick.size() < byd.size();) {
+                acc += byd[pick.size()].first;
+                pick.push_back(byd[pick.size()].second);
+            }
+            std::sort(pick.begin(), pick.end());
+            g.drop_cache(ch.e0, ch.e1, ch.v0, ch.v1);
+            a = Clock::now();
+            int k = 0;
+            for (int v : pick) h_vids_[0][k++] = v;
+            h_off_[0][0] = 0;
+            for (int i = 0; i < k; i++) h_off_[0][i + 1] = h_off_[0][i] + g.degree(h_vids_[0][i]);
+            uint64_t pg, rn;
+            if (o.dedup) g.prefetch_lists(h_vids_[0], k, &pg, &rn);
+            int32_t* he = h_edges_[0]; const int32_t* hv = h_vids_[0]; const uint64_t* ho = h_off_[0];
+#pragma omp parallel for schedule(dynamic, 256) num_threads(16)
+            for (int i = 0; i < k; i++) memcpy(he + ho[i], g.edges(hv[i]), g.degree(hv[i]) * sizeof(int32_t));
+            const uint64_t bytes = ho[k] * 4 + (uint64_t)k * 12;
+            ck(exc_slot_copy_probe(ctx, 0, std::min<uint64_t>(bytes, ch.edges() * sizeof(int32_t)), &t_copy), "calibration copy");
+            t_g[j] = secs(a, Clock::now());
+            f_got[j] = ch.edges() ? (double)ho[k] / ch.edges() : 0.0;
+        }
+        stream_ratio_ = tau_from_timings(t_stream, t_g, f_got);
+        t_calib_ = secs(c0, Clock::now());
+        fprintf(out, "fetch rule: tau = %.3f (stream %.3fs; gather %.3fs / %.3fs / %.3fs at demand %.3f / %.3f / %.3f)  calibrate %.2fs\n",
+                stream_ratio_, t_stream, t_g[0], t_g[1], t_g[2], f_got[0], f_got[1], f_got[2], t_calib_);
+        if (FILE* f = fopen(tau_file.c_str(), "w")) { fprintf(f, "%.6f %.4f\n", stream_ratio_, t_calib_); fclose(f); }
+    } else if (tau_file.empty()) {
+        fprintf(out, "fetch rule: tau = %.3f (given)\n", stream_ratio_);
+    }
+    ck(exc_pass_sync(ctx, nullptr), "setup sync");
+    ck(exc_reset_stats(ctx), "reset stats");
+    checkpoint_ = session_ && exc_state_checkpoint(ctx, 0) == EXC_OK;
+    T_setup_ = Clock::now();
+    fprintf(out, "staging: %s\n", zc_ ? "zero copy (pinned, GPU-mapped buffers read in place)" : "pinned host buffers copied to the GPU");
+    if (hot_mode_ && !o.hot_rank.empty())
+        fprintf(out, "GPU cache ranked by passes pending (%s): %d vertices (%.1f%% of edges, %.2f GB)  setup %.2fs\n",
+                o.hot_rank.c_str(), nh_, m ? 100.0 * hot_edges_ / m : 0.0, hot_edges_ * 4.0 / 1e9, secs(T_open_, T_setup_));
+    else if (hot_mode_)
+        fprintf(out, "GPU degree cache: %d vertices with degree >= %llu (%.1f%% of edges, %.2f GB)  setup %.2fs\n",
+                nh_, (unsigned long long)hot_deg_min_, m ? 100.0 * hot_edges_ / m : 0.0, hot_edges_ * 4.0 / 1e9, secs(T_open_, T_setup_));
+    else
+        fprintf(out, "GPU edge cache: %d/%d chunks resident (%.1f%% of edges)  setup %.2fs\n",
+                n_resident_, nc, m ? 100.0 * resident_edges_ / m : 0.0, secs(T_open_, T_setup_));
+}
+
+// k_init for every measure; the eigenvector's residual uses in-degrees with --ev-init indeg.
+void Engine::init_state() {
+    for (int i = 0; i < M_.nm; i++) ck(exc_init_measure(ctx_, i, (float)o_.beta, (float)o_.lambda), "init");
+    if (indeg_.empty()) return;
+    // r = indeg/lambda - 1 with x = 1: the invariant x + r = W x on a directed graph (the
+    // reference's out-degree form assumes a symmetric graph).
+    for (int i = 0; i < M_.nm; i++) {
+        if (kinds_[i] != EXC_EV) continue;
+        std::vector<float> r(n_), x(n_, 1.f);
+        const double lam = (double)(float)o_.lambda;
+#pragma omp parallel for schedule(static) num_threads(16)
+        for (int v = 0; v < n_; v++) r[v] = (float)((double)indeg_[v] / lam - 1.0);
+        ck(exc_set_state(ctx_, i, x.data(), r.data()), "ev init");
+    }
+}
+
+void Engine::reinit() {
+    // restore the initial state saved at setup (a device copy); re-initialise if it
+    // could not be saved
+    if (!checkpoint_ || exc_state_checkpoint(ctx_, 1) != EXC_OK) init_state();
+    ck(exc_pass_sync(ctx_, nullptr), "reinit sync");
+    ck(exc_reset_stats(ctx_), "reset stats");
+}
+
+// ---- passes (src/excent.cu:593-760)
+void Engine::run_passes() {
+    const Options& o = o_;
+    EgrGraph& g = *g_;
+    void* ctx = ctx_;
+    FILE* out = out_;
+    const int n = n_, nc = nc_;
+    const uint64_t m = m_;
+    t_io_ = t_wait_ = t_snap_ = 0;
+    bytes_stream_ = bytes_gather_ = 0;
+    pass_ = 0;
+    n_list_ = n_range_ = n_launch_ = 0;
+    tot_maj_ = tot_min_ = 0;
+    tot_pages_ = tot_runs_ = 0;
+    const Clock::time_point T_begin = Clock::now();
+    const size_t words = ((size_t)n + 31) / 32;
+    std::vector<uint64_t> h_cnt(2 * (size_t)nc);
+    std::vector<uint32_t> h_bitmap(words);
+    std::vector<uint64_t> chunk_edges(nc);
+    for (int c = 0; c < nc; c++) chunk_edges[c] = chunks_[c].edges();
+    uint64_t h_hotpend = 0;
+    const int gcap = 32;
+    const int G_hot = o.group_opt ? o.group_opt : pick_group(nh_ ? (double)hot_edges_ / nh_ : 32.0, gcap);
+    std::vector<std::pair<int, int>> hot_ranges;
+    if (o.hot_order == "ranges") { for (int c = 0; c < nc; c++) if (hhi_[c] > hlo_[c]) hot_ranges.push_back({hlo_[c], hhi_[c]}); }
+    else if (nh_) hot_ranges.push_back({0, nh_});
+    auto launch_hot = [&](int after_slots) {
+        for (auto [lo, hi] : hot_ranges) ck(exc_push_hot(ctx, lo, hi, G_hot, after_slots), "hot push");
+    };
+    auto group_of = [&](uint64_t k, uint64_t e) {
+        if (o.group_opt) return o.group_opt;
+        return pick_group(k ? (double)e / k : 1.0, gcap);
+    };
+    std::vector<int8_t> mode(nc);
+    for (; pass_ < o.max_passes; pass_++) {
+        auto a = Clock::now();
+        const Faults f0 = faults();
+        uint64_t pass_pages = 0, pass_runs = 0;
+        const int hot_skip = (hot_mode_ && nh_ > 0 && pass_ > 0) ? 1 : 0;
+        ck(exc_snapshot(ctx, hot_skip, h_cnt.data(), &h_hotpend), "snapshot");
+        const PlanOut po = fetch_plan(nc, h_cnt.data(), resident_, loaded_, stream_ratio_, chunk_edges, hot_mode_, pass_, hlo_, hhi_, mode.data());
+        if (po.need_bitmap) ck(exc_bitmap_to_host(ctx, h_bitmap.data()), "bitmap readback");
+        t_snap_ += secs(a, Clock::now());
+        if (po.pend_v == 0 && h_hotpend == 0) break;
+
+        int nres = 0, nstream = 0, ngather = 0;
+        if (hot_skip && h_hotpend && o.hot_when != "end") launch_hot(0);
+        for (int c = 0; c < nc;) {   // resident chunks, runs merged into one launch
+            if (mode[c] != F_RESIDENT) { c++; continue; }
+            const bool ul = use_list(h_cnt[2 * c], chunks_[c].v1 - chunks_[c].v0, o.list_frac);
+            uint64_t k = h_cnt[2 * c], e = h_cnt[2 * c + 1];
+            int c1 = c + 1;
+            if (push_kernel_ == EXC_KERNEL_LB)
+                while (c1 < nc && mode[c1] == F_RESIDENT && use_list(h_cnt[2 * c1], chunks_[c1].v1 - chunks_[c1].v0, o.list_frac) == ul) {
+                    k += h_cnt[2 * c1]; e += h_cnt[2 * c1 + 1]; c1++;
+                }
+            for (int j = c; j < c1; j++) { nres++; if (ul) n_list_++; else n_range_++; }
+            exc_chunk run{chunks_[c].v0, chunks_[c1 - 1].v1, chunks_[c].e0, chunks_[c1 - 1].e1};
+            ck(exc_push_resident(ctx, &run, ul, group_of(k, e), hot_skip), "resident push");
+            n_launch_++;
+            c = c1;
+        }
+        for (int c = 0; c < nc; c++) {
+            if (mode[c] < F_STREAM) continue;
+            const EdgeChunk& ch = chunks_[c];
+            const int s = cur_;
+            cur_ ^= 1;
+            double w = 0;
+            ck(exc_slot_acquire(ctx, s, &w), "slot acquire");
+            t_wait_ += w;
+            auto io0 = Clock::now();
+            if (mode[c] == F_STREAM) {
+                nstream++;
+                g.read_edges(ch.e0, ch.e1, h_edges_[s]);
+                if (!loaded_[c]) check_columns(h_edges_[s], ch.edges(), n, "a streamed chunk");
+                bytes_stream_ += ch.edges() * sizeof(int32_t);
+                t_io_ += secs(io0, Clock::now());
+                const bool ul = use_list(h_cnt[2 * c], ch.v1 - ch.v0, o.list_frac);
+                if (ul) n_list_++; else n_range_++;
+                exc_chunk cc{ch.v0, ch.v1, ch.e0, ch.e1};
+                ck(exc_stream_chunk(ctx, s, &cc, resident_[c] ? 1 : 0, 0, 0, ul, group_of(h_cnt[2 * c], h_cnt[2 * c + 1]), hot_skip), "stream push");
+                n_launch_++;
+                if (resident_[c]) loaded_[c] = 1;
+            } else {
+                ngather++;
+                const int k = gather_list(h_bitmap.data(), ch.v0, ch.v1, h_vids_[s]);
+                h_off_[s][0] = 0;
+                for (int i = 0; i < k; i++) h_off_[s][i + 1] = h_off_[s][i] + g.degree(h_vids_[s][i]);
+                const uint64_t ne = h_off_[s][k];
+                if (o.dedup) { uint64_t pg, rn; g.prefetch_lists(h_vids_[s], k, &pg, &rn); pass_pages += pg; pass_runs += rn; }
+                int32_t* he = h_edges_[s]; const int32_t* hv = h_vids_[s]; const uint64_t* ho = h_off_[s];
+#pragma omp parallel for schedule(dynamic, 256) num_threads(16)
+                for (int i = 0; i < k; i++) memcpy(he + ho[i], g.edges(hv[i]), g.degree(hv[i]) * sizeof(int32_t));
+                check_columns(he, ne, n, "a gathered chunk");
+                bytes_gather_ += ne * sizeof(int32_t) + (size_t)k * (sizeof(int32_t) + sizeof(uint64_t));
+                t_io_ += secs(io0, Clock::now());
+                const int G = o.group_opt ? o.group_opt : pick_group(k ? (double)ne / k : 1.0, gcap);
+                ck(exc_gather_chunk(ctx, s, k, G), "gather push");
+                if (k) n_launch_++;
+            }
+        }
+        if (hot_skip && o.hot_when != "start") launch_hot(1);
+        double w = 0;
+        ck(exc_pass_sync(ctx, &w), "pass sync");
+        t_wait_ += w;
+        const Faults f1 = faults();
+        tot_maj_ += f1.maj - f0.maj;
+        tot_min_ += f1.min - f0.min;
+        tot_pages_ += pass_pages;
+        tot_runs_ += pass_runs;
+        fprintf(out, "pass %3d: pending %llu v, demand ratio %.4f | resident %d stream %d gather %d | cached high-degree pending %llu"
+                     " | faults major %ld minor %ld | gather pages %llu runs %llu\n",
+                pass_, (unsigned long long)po.pend_v, m ? (double)po.demand / m : 0.0, nres, nstream, ngather,
+                (unsigned long long)h_hotpend, f1.maj - f0.maj, f1.min - f0.min, (unsigned long long)pass_pages,
+                (unsigned long long)pass_runs);
+    }
+    T_iter_ = Clock::now();
+    T_setup_ = T_begin;
+}
+
+void Engine::results() {
+    const Options& o = o_;
+    const int n = n_;
+    sum_.assign(M_.nm, 0.0);
+    {   // src/excent.cu:763-778
+        const size_t B = (size_t)1 << 24;
+        std::vector<float> slice(std::min(B, (size_t)n));
+        for (int i = 0; i < M_.nm; i++) {
+            FILE* df = nullptr;
+            if (!o.dump.empty()) {
+                std::string p = o.dump + "." + MNAME[kinds_[i]] + ".bin";
+                df = fopen(p.c_str(), "wb");
+                if (!df) die("cannot write " + p);
+            }
+            for (size_t v0 = 0; v0 < (size_t)n; v0 += B) {
+                const size_t cnt = std::min(B, (size_t)n - v0);
+                ck(exc_read_state(ctx_, i, 0, slice.data(), v0, cnt), "result readback");
+                for (size_t j = 0; j < cnt; j++) sum_[i] += slice[j];
+                if (df) fwrite(slice.data(), sizeof(float), cnt, df);
+            }
+            if (df) fclose(df);
+        }
+    }
+    if (!o.trace_file.empty()) {   // src/excent.cu:779-789
+        FILE* tf = fopen(o.trace_file.c_str(), "wb");
+        if (!tf) die("cannot write " + o.trace_file);
+        const size_t B = (size_t)1 << 24;
+        std::vector<uint16_t> sl(std::min(B, (size_t)n));
+        for (size_t v0 = 0; v0 < (size_t)n; v0 += B) {
+            const size_t cnt = std::min(B, (size_t)n - v0);
+            ck(exc_read_trace(ctx_, sl.data(), v0, cnt), "trace readback");
+            fwrite(sl.data(), sizeof(uint16_t), cnt, tf);
+        }
+        fclose(tf);
+    }
+    T_end_ = Clock::now();
+}
+
+void Engine::report() {
+    const Options& o = o_;
+    FILE* out = out_;
+    uint64_t pushed = 0;
+    ck(exc_pushed(ctx_, &pushed), "pushed count");
+    fprintf(out, "passes %d  measure %s (n=%d):", pass_, o.measure.c_str(), n_);
+    for (int i = 0; i < M_.nm; i++) fprintf(out, "  %s sum %.6e", MNAME[kinds_[i]], sum_[i]);
+    fprintf(out, "\n");
+    if (o.count_pushes) fprintf(out, "edges pushed: %.2f x |E|\n", m_ ? (double)pushed / m_ : 0.0);
+    fprintf(out, "launches from GPU memory: %llu pending-only, %llu whole-range\n", (unsigned long long)n_list_, (unsigned long long)n_range_);
+    fprintf(out, "bytes shipped: stream %.2f GB, gather %.2f GB (graph edges %.2f GB)\n", bytes_stream_ / 1e9, bytes_gather_ / 1e9, m_ * 4.0 / 1e9);
+    fprintf(out, "page faults in passes: major %ld, minor %ld | gather dedup %s: %llu pages in %llu runs\n",
+            tot_maj_, tot_min_, o.dedup ? "on" : "off", (unsigned long long)tot_pages_, (unsigned long long)tot_runs_);
+    fprintf(out, "time: open %.2fs setup %.2fs passes %.2fs [cpu io %.2fs, gpu wait %.2fs, snapshot %.2fs] result %.2fs | internal total %.2fs\n",
+            secs(T0_, T_open_), secs(T_open_, T_setup_), secs(T_setup_, T_iter_), t_io_, t_wait_, t_snap_,
+            secs(T_iter_, T_end_), secs(T0_, T_end_));
+    fflush(out);
+}
+
+void Engine::fill_stats(excent_run_stats* s) {
+    memset(s, 0, sizeof *s);
+    exc_stats st{};
+    ck(exc_get_stats(ctx_, &st), "stats");
+    s->time_passes_s = secs(T_setup_, T_iter_);
+    s->time_snapshot_s = t_snap_;
+    s->time_io_s = t_io_;
+    s->time_wait_s = t_wait_;
+    s->passes = pass_;
+    s->pushed_edges = st.pushed_edges;
+    s->bytes_stream = bytes_stream_;
+    s->bytes_gather = bytes_gather_;
+    s->launches = n_launch_;
+    s->kernel = st;
+}
+
+void Engine::write_stats_json() {
+    if (o_.stats_json.empty()) return;
+    FILE* j = fopen(o_.stats_json.c_str(), "w");
+    if (!j) die("cannot write " + o_.stats_json);
+    excent_run_stats rs;
+    fill_stats(&rs);
+    const exc_stats& st = rs.kernel;
+    const double tp = rs.time_passes_s;
+    fprintf(j, "{\"n\": %d, \"m\": %llu, \"chunks\": %d, \"passes\": %d, \"measure\": \"%s\", \"arch\": \"%s\", \"cus\": %d,\n",
+            n_, (unsigned long long)m_, nc_, pass_, o_.measure.c_str(), arch_, cus_);
+    fprintf(j, " \"time_open_s\": %.6f, \"time_setup_s\": %.6f, \"time_passes_s\": %.6f, \"time_io_s\": %.6f, \"time_wait_s\": %.6f, \"time_snapshot_s\": %.6f,\n",
+            secs(T0_, T_open_), secs(T_open_, T_setup_), tp, t_io_, t_wait_, t_snap_);
+    fprintf(j, " \"pushed_edges\": %llu, \"pushed_vertices\": %llu, \"pushed_edge_measures\": %llu, \"gteps\": %.6f,\n",
+            (unsigned long long)st.pushed_edges, (unsigned long long)st.pushed_vertices, (unsigned long long)st.pushed_edge_measures,
+            tp > 0 ? st.pushed_edges / tp / 1e9 : 0.0);
+    fprintf(j, " \"bytes_stream\": %llu, \"bytes_gather\": %llu, \"resident_chunks\": %d, \"hot_vertices\": %d, \"tau\": %.6f,\n",
+            (unsigned long long)bytes_stream_, (unsigned long long)bytes_gather_, n_resident_, nh_, stream_ratio_);
+    fprintf(j, " \"launches\": %llu, \"list_launches\": %llu, \"range_launches\": %llu,\n",
+            (unsigned long long)n_launch_, (unsigned long long)n_list_, (unsigned long long)n_range_);
+    fprintf(j, " \"kernel\": {\"snapshot_ms\": %.6f, \"snapshot_launches\": %llu, \"snapshot_bytes\": %.1f, \"push_ms\": %.6f, \"push_launches\": %llu, \"push_bytes\": %.1f, \"piece_ms\": %.6f, \"piece_launches\": %llu, \"piece_bytes\": %.1f, \"other_ms\": %.6f},\n",
+            st.snapshot_ms, (unsigned long long)st.snapshot_launches, st.snapshot_bytes, st.push_ms, (unsigned long long)st.push_launches,
+            st.push_bytes, st.piece_ms, (unsigned long long)st.piece_launches, st.piece_bytes, st.other_ms);
+    fprintf(j, " \"sums\": {");
+    for (int i = 0; i < M_.nm; i++) fprintf(j, "%s\"%s\": %.9e", i ? ", " : "", MNAME[kinds_[i]], sum_.size() > (size_t)i ? sum_[i] : 0.0);
+    fprintf(j, "}}\n");
+    fclose(j);
+}
+
+}  // namespace excent
+
+namespace {
+thread_local std::string g_session_err;
+
+template <class F>
+int guarded(F&& f) {
+    try {
+        return f();
+    } catch (const excent::exit_error& e) {
+        if (!e.msg.empty()) fprintf(stderr, "%s\n", e.msg.c_str());
+        g_session_err = e.msg;
+        fflush(stdout);
+        return e.code ? e.code : 1;
+    } catch (const std::exception& e) {
+        fprintf(stderr, "%s\n", e.what());
+        g_session_err = e.what();
+        fflush(stdout);
+        return 1;
+    }
+}
+}  // namespace
+
+extern "C" int excent_main(int argc, const char* const* argv) {
+    int code = 0;
+    int rc = guarded([&]() {
+        excent::Options o;
+        try {
+            o = excent::parse(argc, argv);
+        } catch (const excent::exit_error& e) {
+            code = e.code;
+            throw;
+        }
+        excent::Engine eng(o);
+        eng.setup();
+        eng.run_passes();
+        eng.results();
+        eng.report();
+        eng.write_stats_json();
+        return 0;
+    });
+    return rc ? rc : code;
+}
+
+extern "C" int excent_session_open(int argc, const char* const* argv, void** session) {
+    *session = nullptr;
+    return guarded([&]() {
+        excent::Options o = excent::parse(argc, argv);
+        std::unique_ptr<excent::Engine> eng(new excent::Engine(o, /*session=*/true));
+        eng->setup();
+        *session = eng.release();
+        return 0;
+    });
+}
+
+extern "C" int excent_session_run(void* session, int reinit, excent_run_stats* stats) {
+    if (!session) return 1;
+    return guarded([&]() {
+        auto* eng = static_cast<excent::Engine*>(session);
+        if (reinit) eng->reinit();
+        eng->run_passes();
+        if (stats) eng->fill_stats(stats);
+        return 0;
+    });
+}
+
+extern "C" int excent_session_results(void* session) {
+    if (!session) return 1;
+    return guarded([&]() {
+        auto* eng = static_cast<excent::Engine*>(session);
+        eng->results();
+        eng->report();
+        eng->write_stats_json();
+        return 0;
+    });
+}
+
+extern "C" int excent_session_close(void* session) {
+    delete static_cast<excent::Engine*>(session);
+    return 0;
+}
+
+extern "C" const char* excent_session_error(void) { return g_session_err.c_str(); }

