@@ -60,9 +60,9 @@ def test_sample_all(tmp_path, kernel):
     xs = {m: np.fromfile(str(tmp_path / f"a.{m}.bin"), np.float32) for m in ("pr", "katz", "ev")}
     pr_check(idx, adj, xs["pr"])
     kref = orc.fp64_reference(idx, adj, orc.ORC_KATZ, alpha=env["ALPHA"])[0]
-    assert orc.compare(kref, xs["katz"])["l1_rel"] < 0.02
+    assert orc.compare(kref, xs["katz"])["l1_rel"] < 0.05
     eref = orc.fp64_reference(idx, adj, orc.ORC_EV)[0]
-    assert orc.compare(eref, xs["ev"], unit=True)["l1_rel"] < 0.05
+    assert orc.compare(eref, xs["ev"], unit=True)["l1_rel"] < 0.1
 
 
 @pytest.mark.parametrize("mode", ["stream_only", "gather_only", "mixed", "hot", "preload", "zero_copy"])
