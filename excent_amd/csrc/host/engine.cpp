@@ -70,6 +70,10 @@ struct Options {
     double cache_mb = -1;
     std::string push_kernel = "lb", ev_init = "outdeg", stats_json, log;
     int device = 0;
+    // batched personalised PageRank (--measure ppr)
+    int ppr_k = 0;
+    std::string ppr_seeds, ppr_gen;
+    double eps_ppr = -1;
 };
 
 const char* USAGE =
@@ -127,6 +131,10 @@ Options parse(int argc, const char* const* argv) {
         else if (a == "--stats-json") o.stats_json = next();
         else if (a == "--log") o.log = next();
         else if (a == "--device") o.device = atoi(next());
+        else if (a == "--ppr-k") o.ppr_k = atoi(next());
+        else if (a == "--ppr-seeds") o.ppr_seeds = next();
+        else if (a == "--ppr-gen") o.ppr_gen = next();
+        else if (a == "--eps-ppr") o.eps_ppr = atof(next());
         else die("unknown option " + a);
     }
     return o;
@@ -204,6 +212,12 @@ private:
     std::vector<uint32_t> indeg_;
 
     void init_state();
+    void load_ppr_seeds(EgrGraph& g);
+    bool batched() const { return o_.measure == "ppr"; }
+    int K_ = 0;
+    std::vector<std::vector<int32_t>> seeds_;
+    std::vector<uint32_t> colpend_;
+    int active_cols_ = 0;
     size_t budget_cap(size_t b) const {
         if (o_.cache_mb < 0) return b;
         return std::min(b, (size_t)(o_.cache_mb * (1 << 20)));
@@ -230,7 +244,13 @@ void Engine::setup() {
     else if (o.measure == "katz") kinds_ = {EXC_KATZ};
     else if (o.measure == "ev") kinds_ = {EXC_EV};
     else if (o.measure == "all") kinds_ = {EXC_PR, EXC_KATZ, EXC_EV};
+    else if (o.measure == "ppr") kinds_ = {EXC_PR};
     else die("--measure must be pr, katz, ev or all");
+    if (o.measure == "ppr") {
+        if (o.eps_ppr <= 0) die("ppr needs --eps-ppr");
+        if (o.ppr_seeds.empty() == o.ppr_gen.empty()) die("ppr needs one of --ppr-seeds FILE or --ppr-gen K:S:TOP:SEED");
+        if (!o.ppr_seeds.empty() && (o.ppr_k < 1 || o.ppr_k > 64)) die("--ppr-k must be 1..64");
+    }
     for (int k : kinds_) {
         if (k == EXC_KATZ && (o.alpha <= 0 || o.eps_katz <= 0)) die("katz needs --alpha and --eps-katz");
         if (k == EXC_EV && (o.lambda <= 0 || o.eps_ev <= 0)) die("ev needs --lambda and --eps-ev");
@@ -274,7 +294,7 @@ void Engine::setup() {
     for (int i = 0; i < M_.nm; i++) {
         const int k = kinds_[i];
         M_.kind[i] = k;
-        M_.eps[i] = (float)(k == EXC_PR ? o.eps_pr : k == EXC_KATZ ? o.eps_katz : o.eps_ev);
+        M_.eps[i] = (float)(batched() ? o.eps_ppr : k == EXC_PR ? o.eps_pr : k == EXC_KATZ ? o.eps_katz : o.eps_ev);
         M_.scale[i] = (float)(k == EXC_PR ? (double)0.85f : k == EXC_KATZ ? o.alpha : 1.0 / o.lambda);
         M_.divdeg[i] = k == EXC_PR;
     }
@@ -284,6 +304,10 @@ void Engine::setup() {
     cfg.device = o.device; cfg.n = n; cfg.m = m; cfg.nc = nc; cfg.cv0 = cv0.data();
     cfg.max_chunk_edges = max_chunk_edges; cfg.max_chunk_verts = max_chunk_verts;
     cfg.trace = !o.trace_file.empty(); cfg.zero_copy = zc_; cfg.push_kernel = push_kernel_; cfg.timing = o.timing;
+    if (batched()) {
+        load_ppr_seeds(g);
+        cfg.batch_k = K_;
+    }
     ck(exc_create(&cfg, &M_, &ctx_), "create");
     void* ctx = ctx_;
     {   // offsets in 16M-entry slices (src/excent.cu:360-369)
@@ -480,8 +504,70 @@ void Engine::setup() {
                 n_resident_, nc, m ? 100.0 * resident_edges_ / m : 0.0, secs(T_open_, T_setup_));
 }
 
+// Teleport sets of the batched columns: from a file of K*S int32 vertex ids, or drawn
+// with --ppr-gen K:S:TOP:SEED -- S distinct seeds per column, TOP of them uniform over the
+// top 1% of vertices by out-degree and the rest uniform over all vertices (SplitMix64).
+void Engine::load_ppr_seeds(EgrGraph& g) {
+    const Options& o = o_;
+    if (!o.ppr_seeds.empty()) {
+        K_ = o.ppr_k;
+        FILE* f = fopen(o.ppr_seeds.c_str(), "rb");
+        if (!f) die("cannot read --ppr-seeds " + o.ppr_seeds);
+        std::vector<int32_t> all;
+        int32_t b[4096];
+        size_t r;
+        while ((r = fread(b, 4, 4096, f)) > 0) all.insert(all.end(), b, b + r);
+        fclose(f);
+        if (all.empty() || all.size() % K_) die("--ppr-seeds: the file must hold K * S int32 ids");
+        const size_t S = all.size() / K_;
+        seeds_.assign(K_, {});
+        for (int j = 0; j < K_; j++) seeds_[j].assign(all.begin() + j * S, all.begin() + (j + 1) * S);
+    } else {
+        int Kg = 0, S = 0, top = 0;
+        unsigned long long sd = 0;
+        if (sscanf(o.ppr_gen.c_str(), "%d:%d:%d:%llu", &Kg, &S, &top, &sd) != 4 || Kg < 1 || Kg > 64 || S < 1 ||
+            top < 0 || top > S || S > n_)
+            die("--ppr-gen must be K:S:TOP:SEED with 1 <= K <= 64, TOP <= S <= n");
+        K_ = Kg;
+        std::vector<int> order(n_);
+        for (int v = 0; v < n_; v++) order[v] = v;
+        const uint64_t* off = g.offsets();
+        std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return off[a + 1] - off[a] > off[b + 1] - off[b]; });
+        const int ntop = std::max(top > 0 ? 1 : 0, n_ / 100);
+        uint64_t state = sd * 0x9E3779B97F4A7C15ull + 12345;
+        auto next = [&]() {
+            uint64_t z = (state += 0x9E3779B97F4A7C15ull);
+            z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+            z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+            return z ^ (z >> 31);
+        };
+        seeds_.assign(K_, {});
+        for (int j = 0; j < K_; j++) {
+            std::vector<int32_t>& sj = seeds_[j];
+            while ((int)sj.size() < S) {
+                const bool from_top = (int)sj.size() < top;
+                const int v = from_top ? order[next() % (uint64_t)ntop] : (int)(next() % (uint64_t)n_);
+                if (std::find(sj.begin(), sj.end(), v) == sj.end()) sj.push_back(v);
+            }
+        }
+    }
+    for (auto& sj : seeds_)
+        for (int v : sj)
+            if (v < 0 || v >= n_) die("ppr seed out of range");
+}
+
 // k_init for every measure; the eigenvector's residual uses in-degrees with --ev-init indeg.
+// Batched columns start at x = r = 0 with r = (1 - d) * n / S at each of a column's S seeds
+// (teleport mass n per column, so values keep the unnormalised scale of mean about 1).
 void Engine::init_state() {
+    if (batched()) {
+        ck(exc_init_measure(ctx_, 0, 0.f, 0.f), "init");
+        for (int j = 0; j < K_; j++) {
+            const float mass = (float)((1.0 - (double)0.85f) * (double)n_ / (double)seeds_[j].size());
+            ck(exc_batch_seed(ctx_, j, seeds_[j].data(), (int)seeds_[j].size(), mass), "ppr seed");
+        }
+        return;
+    }
     for (int i = 0; i < M_.nm; i++) ck(exc_init_measure(ctx_, i, (float)o_.beta, (float)o_.lambda), "init");
     if (indeg_.empty()) return;
     // r = indeg/lambda - 1 with x = 1: the invariant x + r = W x on a directed graph (the
@@ -544,6 +630,14 @@ void Engine::run_passes() {
         uint64_t pass_pages = 0, pass_runs = 0;
         const int hot_skip = (hot_mode_ && nh_ > 0 && pass_ > 0) ? 1 : 0;
         ck(exc_snapshot(ctx, hot_skip, h_cnt.data(), &h_hotpend), "snapshot");
+        if (batched()) {   // converged columns drop out of the pushes
+            colpend_.resize(K_);
+            ck(exc_batch_colpend(ctx, colpend_.data()), "column counts");
+            uint64_t mask = 0;
+            active_cols_ = 0;
+            for (int j = 0; j < K_; j++) if (colpend_[j]) { mask |= 1ull << j; active_cols_++; }
+            ck(exc_batch_set_active(ctx, mask), "active columns");
+        }
         const PlanOut po = fetch_plan(nc, h_cnt.data(), resident_, loaded_, stream_ratio_, chunk_edges, hot_mode_, pass_, hlo_, hhi_, mode.data());
         if (po.need_bitmap) ck(exc_bitmap_to_host(ctx, h_bitmap.data()), "bitmap readback");
         t_snap_ += secs(a, Clock::now());
@@ -615,10 +709,12 @@ void Engine::run_passes() {
         tot_pages_ += pass_pages;
         tot_runs_ += pass_runs;
         fprintf(out, "pass %3d: pending %llu v, demand ratio %.4f | resident %d stream %d gather %d | cached high-degree pending %llu"
-                     " | faults major %ld minor %ld | gather pages %llu runs %llu\n",
+                     " | faults major %ld minor %ld | gather pages %llu runs %llu",
                 pass_, (unsigned long long)po.pend_v, m ? (double)po.demand / m : 0.0, nres, nstream, ngather,
                 (unsigned long long)h_hotpend, f1.maj - f0.maj, f1.min - f0.min, (unsigned long long)pass_pages,
                 (unsigned long long)pass_runs);
+        if (batched()) fprintf(out, " | active columns %d", active_cols_);
+        fprintf(out, "\n");
     }
     T_iter_ = Clock::now();
     T_setup_ = T_begin;
@@ -627,13 +723,20 @@ void Engine::run_passes() {
 void Engine::results() {
     const Options& o = o_;
     const int n = n_;
-    sum_.assign(M_.nm, 0.0);
-    {   // src/excent.cu:763-778
+    const int ncols = batched() ? K_ : M_.nm;
+    sum_.assign(ncols, 0.0);
+    {   // src/excent.cu:763-778 (batched: PREFIX.ppr.bin holds the K columns one after another)
         const size_t B = (size_t)1 << 24;
         std::vector<float> slice(std::min(B, (size_t)n));
-        for (int i = 0; i < M_.nm; i++) {
-            FILE* df = nullptr;
-            if (!o.dump.empty()) {
+        FILE* pf = nullptr;
+        if (batched() && !o.dump.empty()) {
+            std::string p = o.dump + ".ppr.bin";
+            pf = fopen(p.c_str(), "wb");
+            if (!pf) die("cannot write " + p);
+        }
+        for (int i = 0; i < ncols; i++) {
+            FILE* df = pf;
+            if (!batched() && !o.dump.empty()) {
                 std::string p = o.dump + "." + MNAME[kinds_[i]] + ".bin";
                 df = fopen(p.c_str(), "wb");
                 if (!df) die("cannot write " + p);
@@ -644,8 +747,9 @@ void Engine::results() {
                 for (size_t j = 0; j < cnt; j++) sum_[i] += slice[j];
                 if (df) fwrite(slice.data(), sizeof(float), cnt, df);
             }
-            if (df) fclose(df);
+            if (df && !batched()) fclose(df);
         }
+        if (pf) fclose(pf);
     }
     if (!o.trace_file.empty()) {   // src/excent.cu:779-789
         FILE* tf = fopen(o.trace_file.c_str(), "wb");
@@ -668,7 +772,13 @@ void Engine::report() {
     uint64_t pushed = 0;
     ck(exc_pushed(ctx_, &pushed), "pushed count");
     fprintf(out, "passes %d  measure %s (n=%d):", pass_, o.measure.c_str(), n_);
-    for (int i = 0; i < M_.nm; i++) fprintf(out, "  %s sum %.6e", MNAME[kinds_[i]], sum_[i]);
+    if (batched()) {
+        double tot = 0, lo = 1e300, hi = -1e300;
+        for (double v : sum_) { tot += v; lo = std::min(lo, v); hi = std::max(hi, v); }
+        fprintf(out, "  %d columns, ppr sum %.6e (column sums %.6e .. %.6e)", K_, tot, lo, hi);
+    } else {
+        for (int i = 0; i < M_.nm; i++) fprintf(out, "  %s sum %.6e", MNAME[kinds_[i]], sum_[i]);
+    }
     fprintf(out, "\n");
     if (o.count_pushes) fprintf(out, "edges pushed: %.2f x |E|\n", m_ ? (double)pushed / m_ : 0.0);
     fprintf(out, "launches from GPU memory: %llu pending-only, %llu whole-range\n", (unsigned long long)n_list_, (unsigned long long)n_range_);
@@ -720,7 +830,13 @@ void Engine::write_stats_json() {
             st.snapshot_ms, (unsigned long long)st.snapshot_launches, st.snapshot_bytes, st.push_ms, (unsigned long long)st.push_launches,
             st.push_bytes, st.piece_ms, (unsigned long long)st.piece_launches, st.piece_bytes, st.other_ms);
     fprintf(j, " \"sums\": {");
-    for (int i = 0; i < M_.nm; i++) fprintf(j, "%s\"%s\": %.9e", i ? ", " : "", MNAME[kinds_[i]], sum_.size() > (size_t)i ? sum_[i] : 0.0);
+    if (batched()) {
+        fprintf(j, "\"ppr\": [");
+        for (size_t i = 0; i < sum_.size(); i++) fprintf(j, "%s%.9e", i ? ", " : "", sum_[i]);
+        fprintf(j, "]");
+    } else {
+        for (int i = 0; i < M_.nm; i++) fprintf(j, "%s\"%s\": %.9e", i ? ", " : "", MNAME[kinds_[i]], sum_.size() > (size_t)i ? sum_[i] : 0.0);
+    }
     fprintf(j, "}}\n");
     fclose(j);
 }

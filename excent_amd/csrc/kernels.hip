@@ -320,8 +320,9 @@ struct PushArgs {
     const int32_t* edges;           // edge array of this launch
     uint64_t ebase;                 // global edge index of edges[0] (RANGE / BITMAP)
     Piece* pieces;
-    uint32_t* piece_count;
-    uint32_t piece_cap;
+    uint32_t* piece_count;          // this launch's piece counter
+    uint32_t* piece_zero;           // the other counter (and batched p-row counter): cleared
+    uint32_t piece_cap;             //   here for the next launch on this stream
     unsigned long long* counters;   // this launch's source row of C_FIELDS counters
     unsigned long long* overflow;
 };
@@ -454,6 +455,7 @@ __global__ __launch_bounds__(BLOCK) void k_push(PushArgs a, DevMeasures M) {
     __shared__ unsigned long long s_cnt[6];
     const int tid = threadIdx.x;
     if (tid < 6) s_cnt[tid] = 0;
+    if (blockIdx.x == 0 && tid == 0) { a.piece_zero[0] = 0u; a.piece_zero[1] = 0u; }
     __syncthreads();
 
     if constexpr (SRC == SRC_RANGE) {
@@ -511,15 +513,15 @@ __global__ __launch_bounds__(BLOCK) void k_push(PushArgs a, DevMeasures M) {
     if (tid < 6 && s_cnt[tid]) atomicAdd(&a.counters[tid], s_cnt[tid]);
 }
 
-// Hub pieces: one wave per piece, persistent over the device-side count.  The last
-// block to finish clears the count and its own ticket for the next launch.
+// Hub pieces: one wave per piece, persistent over the device-side count.  Counters are
+// double-buffered per stream: launch i counts into buffer i%2 and clears buffer (i+1)%2,
+// whose previous user (launch i-1 and its pieces) finished before launch i started.
 template <int NM, int S>
-__global__ __launch_bounds__(BLOCK) void k_pieces(const Piece* __restrict__ pcs, uint32_t* count,
-                                                   uint32_t* ticket, uint32_t cap, const int32_t* __restrict__ edges,
-                                                   DevMeasures M) {
+__global__ __launch_bounds__(BLOCK) void k_pieces(const Piece* __restrict__ pcs, const uint32_t* count,
+                                                   uint32_t cap, const int32_t* __restrict__ edges, DevMeasures M) {
     // the count may exceed cap after an overflow (pieces beyond cap were not written and
     // the overflow counter reports it); never read past the buffer
-    uint32_t cnt = __hip_atomic_load(count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    uint32_t cnt = *count;
     if (cnt > cap) cnt = cap;
     const int lane = lane_id();
     const long long nw = (long long)gridDim.x * NWAVES;
@@ -530,14 +532,6 @@ __global__ __launch_bounds__(BLOCK) void k_pieces(const Piece* __restrict__ pcs,
 #pragma unroll
             for (int k = 0; k < NM; k++)
                 if (pc.p[k] != 0.f) atomic_add_f32(&M.r[(long long)u * S + k], pc.p[k]);
-        }
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        const uint32_t t = atomicAdd(ticket, 1u);
-        if (t == gridDim.x - 1) {
-            atomicExch(count, 0u);
-            atomicExch(ticket, 0u);
         }
     }
 }
@@ -582,12 +576,286 @@ __global__ void k_indegree(uint64_t m, const int32_t* __restrict__ edges, uint32
         atomicAdd(&indeg[edges[e]], 1u);
 }
 
+// ------------------------------------------------------------------ batched PPR
+
+// K personalised PageRank vectors share the graph: x and r are [n][K] rows (K <= 64),
+// one wave per vertex with lane j owning column j.  Every neighbour update is then one
+// wave-wide atomic on a contiguous K*4-byte row (coalesced), and one read of a list
+// serves all K columns.  Columns with nothing pending drop out through `active`.
+struct BatchArgs {
+    float* x;
+    float* r;
+    const float* eps;     // [K]
+    float damp;
+    int K;
+    uint64_t active;      // bit j: column j still has pending vertices
+    uint32_t* colpend;    // [K] snapshot: pending vertices per column
+    float* prows;         // hub p rows [cap][K]
+    uint32_t* prow_count;
+    uint32_t prow_cap;
+};
+
+struct BPiece {          // 16 bytes: a slice of one hub vertex's list
+    uint64_t start;
+    uint32_t len;
+    uint32_t prow;
+};
+
+constexpr uint32_t BHUB = 2048;    // batched: lists longer than this are split
+constexpr uint32_t BPIECE = 2048;
+
+// Snapshot of a [n][K] state: vertex v is pending iff any active column j has
+// |r[v][j]| > eps[j].  Same bitmap / per-chunk counts / hot / trace outputs as
+// k_snapshot, plus per-column pending counts.  One wave per 64-vertex segment.
+__global__ __launch_bounds__(BLOCK) void k_bsnapshot(SnapArgs a, BatchArgs B) {
+    __shared__ uint32_t s_col[NWAVES][WAVE];
+    const int lane = lane_id(), wid = threadIdx.x / WAVE;
+    const long long gwave = (blockIdx.x * (long long)BLOCK + threadIdx.x) / WAVE;
+    const long long nseg = ((long long)a.n + WAVE - 1) / WAVE;
+    const long long words = ((long long)a.n + 31) / 32;
+    const bool col_on = lane < B.K && ((B.active >> lane) & 1ull);
+    const float eps = lane < B.K ? B.eps[lane] : 0.f;
+    uint32_t colcnt = 0;
+    long long s0 = gwave * a.segs_per_wave, s1 = s0 + a.segs_per_wave;
+    if (s1 > nseg) s1 = nseg;
+    int cc = s0 < nseg ? chunk_of((int)(s0 * WAVE), a.cv0, a.nc) : 0;
+    int clo = a.cv0[cc], chi = cc + 1 < a.nc ? a.cv0[cc + 1] : a.n;
+    uint32_t acc_n = 0;
+    uint64_t acc_d = 0, hp = 0;
+    auto flush = [&]() {
+        uint32_t tn = wave_sum_u32(acc_n);
+        uint64_t td = wave_sum_u64(acc_d);
+        if (lane == 0 && tn) {
+            atomicAdd(&a.cnt[2 * cc], (unsigned long long)tn);
+            atomicAdd(&a.cnt[2 * cc + 1], (unsigned long long)td);
+        }
+        acc_n = 0;
+        acc_d = 0;
+    };
+    for (long long seg = s0; seg < s1; seg++) {
+        const long long base = seg * WAVE;
+        uint64_t mask = 0;
+        constexpr int U = 8;
+        for (int i0 = 0; i0 < WAVE; i0 += U) {
+            float row[U];
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                const long long v = base + i0 + u;
+                row[u] = (v < a.n && lane < B.K) ? B.r[v * B.K + lane] : 0.f;
+            }
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                const bool pj = col_on && fabsf(row[u]) > eps && base + i0 + u < a.n;
+                colcnt += pj ? 1u : 0u;
+                if (__ballot(pj)) mask |= 1ull << (i0 + u);
+            }
+        }
+        const long long v = base + lane;
+        const bool in = v < a.n;
+        bool pend = in && ((mask >> lane) & 1ull);
+        if (a.trace && pend) a.trace[v]++;
+        if (a.hot) {
+            bool ph = pend && ((a.hot[v >> 5] >> (v & 31)) & 1u);
+            hp += ph ? 1 : 0;
+            pend = pend && !ph;
+        }
+        const uint64_t bm = __ballot(pend);
+        if (lane == 0) {
+            a.bitmap[seg * 2] = (uint32_t)bm;
+            if (seg * 2 + 1 < words) a.bitmap[seg * 2 + 1] = (uint32_t)(bm >> 32);
+        }
+        if (!bm) continue;
+        const uint64_t d = pend ? a.idx[v + 1] - a.idx[v] : 0;
+        long long vlast = base + WAVE - 1;
+        if (vlast >= a.n) vlast = a.n - 1;
+        if (base >= clo && vlast < chi) {
+            acc_n += pend ? 1u : 0u;
+            acc_d += d;
+        } else {
+            flush();
+            const int c = in ? chunk_of((int)v, a.cv0, a.nc) : -1;
+            cc = chunk_of((int)vlast, a.cv0, a.nc);
+            clo = a.cv0[cc];
+            chi = cc + 1 < a.nc ? a.cv0[cc + 1] : a.n;
+            if (pend) {
+                if (c == cc) { acc_n += 1u; acc_d += d; }
+                else {
+                    atomicAdd(&a.cnt[2 * c], 1ull);
+                    atomicAdd(&a.cnt[2 * c + 1], (unsigned long long)d);
+                }
+            }
+        }
+    }
+    flush();
+    if (a.hot) {
+        uint64_t t = wave_sum_u64(hp);
+        if (lane == 0 && t) atomicAdd(a.hotpend, (unsigned long long)t);
+    }
+    s_col[wid][lane] = colcnt;
+    __syncthreads();
+    if (wid == 0 && lane < B.K) {
+        uint32_t t = 0;
+#pragma unroll
+        for (int w = 0; w < NWAVES; w++) t += s_col[w][lane];
+        if (t) atomicAdd(&B.colpend[lane], t);
+    }
+}
+
+// Push of one vertex by one wave (lanes = columns).  Returns true when it pushed.
+__device__ __forceinline__ void bpush_vertex(int v, uint64_t ptr, uint64_t deg, const PushArgs& a,
+                                             const BatchArgs& B, BPiece* pieces, uint64_t* st) {
+    const int lane = lane_id();
+    const bool col_on = lane < B.K && ((B.active >> lane) & 1ull);
+    float* rp = B.r + (long long)v * B.K + lane;
+    float p = 0.f, d = 0.f;
+    if (col_on && fabsf(*rp) > B.eps[lane]) {
+        d = atomic_exch_f32(rp, 0.f);
+        B.x[(long long)v * B.K + lane] += d;
+        if (d != 0.f && deg) p = B.damp * d / (float)deg;
+    }
+    const uint64_t pm = __ballot(p != 0.f);
+    const uint32_t npush = (uint32_t)__popcll(pm);
+    // a vertex that folded any column counts as pushed (offset pair + its x and r entries)
+    const uint64_t fm = __ballot(d != 0.f);
+    if (lane == 0 && fm) { st[0] += 1; st[1] += (uint32_t)__popcll(fm); }
+    if (!pm || deg == 0) return;
+    if (deg > BHUB) {
+        const uint32_t np = (uint32_t)((deg + BPIECE - 1) / BPIECE);
+        uint32_t base = 0, row = 0;
+        if (lane == 0) {
+            base = atomicAdd(a.piece_count, np);
+            row = atomicAdd(B.prow_count, 1u);
+        }
+        base = __shfl(base, 0, WAVE);
+        row = __shfl(row, 0, WAVE);
+        if (base + np <= a.piece_cap && row < B.prow_cap) {
+            if (lane < B.K) B.prows[(long long)row * B.K + lane] = p;
+            for (uint32_t j = lane; j < np; j += WAVE) {
+                BPiece pc;
+                pc.start = ptr + (uint64_t)j * BPIECE;
+                const uint64_t rem = deg - (uint64_t)j * BPIECE;
+                pc.len = (uint32_t)(rem < BPIECE ? rem : BPIECE);
+                pc.prow = row;
+                pieces[base + j] = pc;
+            }
+        } else if (lane == 0) {
+            atomicAdd(a.overflow, 1ull);
+        }
+        if (lane == 0) { st[4] += deg; st[5] += deg * npush; }
+        return;
+    }
+    if (lane == 0) { st[2] += deg; st[3] += deg * npush; }
+    for (uint64_t e0 = 0; e0 < deg; e0 += WAVE) {
+        const int id = e0 + lane < deg ? a.edges[ptr + e0 + lane] : 0;
+        const int cnt = (int)(deg - e0 < (uint64_t)WAVE ? deg - e0 : WAVE);
+        for (int t = 0; t < cnt; t++) {
+            const int u = __shfl(id, t, WAVE);
+            if (p != 0.f) atomic_add_f32(&B.r[(long long)u * B.K + lane], p);
+        }
+    }
+}
+
+// Batched push.  RANGE / BITMAP: each wave takes 64 consecutive vertices of [v0, v1)
+// (all of them with a live check, or only the snapshot's pending ones); LIST: gathered
+// lists; HOT: the compacted hot list (count on the device).
+template <int SRC>
+__global__ __launch_bounds__(BLOCK) void k_bpush(PushArgs a, BatchArgs B) {
+    const int lane = lane_id();
+    const long long gwave = (blockIdx.x * (long long)BLOCK + threadIdx.x) / WAVE;
+    uint64_t st[6] = {0, 0, 0, 0, 0, 0};   // meaningful in lane 0
+    BPiece* pieces = reinterpret_cast<BPiece*>(a.pieces);
+    if (blockIdx.x == 0 && threadIdx.x == 0) { a.piece_zero[0] = 0u; a.piece_zero[1] = 0u; }
+    if constexpr (SRC == SRC_RANGE || SRC == SRC_BITMAP) {
+        const long long vb = ((long long)a.v0 & ~63ll) + gwave * WAVE;
+        if (vb < a.v1) {
+            uint64_t mask;
+            if constexpr (SRC == SRC_BITMAP) {
+                const long long w = vb >> 5;
+                mask = (uint64_t)a.bitmap[w];
+                if (vb + 32 < a.v1) mask |= (uint64_t)a.bitmap[w + 1] << 32;
+            } else {
+                mask = ~0ull;
+            }
+            if (vb < a.v0) mask &= ~0ull << (a.v0 - vb);
+            if (vb + WAVE > a.v1) mask &= (1ull << (a.v1 - vb)) - 1ull;
+            while (mask) {
+                const int t = __ffsll((long long)mask) - 1;
+                mask &= mask - 1;
+                const long long v = vb + t;
+                if constexpr (SRC == SRC_RANGE)
+                    if (a.hot && ((a.hot[v >> 5] >> (v & 31)) & 1u)) continue;
+                const uint64_t b = a.idx[v], d = a.idx[v + 1] - b;
+                bpush_vertex((int)v, b - a.ebase, d, a, B, pieces, st);
+            }
+        }
+    } else if constexpr (SRC == SRC_LIST) {
+        for (long long i = gwave; i < a.k; i += (long long)gridDim.x * NWAVES) {
+            const uint64_t b = a.off[i], d = a.off[i + 1] - b;
+            bpush_vertex(a.vids[i], b, d, a, B, pieces, st);
+        }
+    } else {
+        const long long k = *a.hcount;
+        for (long long e = gwave; e < k; e += (long long)gridDim.x * NWAVES) {
+            const int i = a.hlist[e];
+            const uint64_t b = a.hoff[i], d = a.hoff[i + 1] - b;
+            bpush_vertex(a.hids[i], b, d, a, B, pieces, st);
+        }
+    }
+    if (lane == 0) {
+        if (st[0]) { atomicAdd(&a.counters[C_VERT], (unsigned long long)st[0]); atomicAdd(&a.counters[C_VERT_NM], (unsigned long long)st[1]); }
+        if (st[2]) { atomicAdd(&a.counters[C_EDGES], (unsigned long long)st[2]); atomicAdd(&a.counters[C_EDGES_NM], (unsigned long long)st[3]); }
+        if (st[4]) { atomicAdd(&a.counters[C_PEDGES], (unsigned long long)st[4]); atomicAdd(&a.counters[C_PEDGES_NM], (unsigned long long)st[5]); }
+    }
+}
+
+__global__ __launch_bounds__(BLOCK) void k_bpieces(const BPiece* __restrict__ pcs, const uint32_t* count,
+                                                    uint32_t cap, const int32_t* __restrict__ edges, BatchArgs B) {
+    uint32_t cnt = *count;
+    if (cnt > cap) cnt = cap;
+    const int lane = lane_id();
+    const long long nw = (long long)gridDim.x * NWAVES;
+    for (long long w = (blockIdx.x * (long long)BLOCK + threadIdx.x) / WAVE; w < cnt; w += nw) {
+        const BPiece pc = pcs[w];
+        const float p = lane < B.K ? B.prows[(long long)pc.prow * B.K + lane] : 0.f;
+        for (uint32_t e0 = 0; e0 < pc.len; e0 += WAVE) {
+            const int id = e0 + lane < pc.len ? edges[pc.start + e0 + lane] : 0;
+            const int c = (int)(pc.len - e0 < (uint32_t)WAVE ? pc.len - e0 : WAVE);
+            for (int t = 0; t < c; t++) {
+                const int u = __shfl(id, t, WAVE);
+                if (p != 0.f) atomic_add_f32(&B.r[(long long)u * B.K + lane], p);
+            }
+        }
+    }
+}
+
+// Hot compaction for a [n][K] state: hot entries in [lo, hi) pending in an active column.
+__global__ __launch_bounds__(BLOCK) void k_bcompact_hot(int lo, int hi, const int32_t* __restrict__ hids, BatchArgs B,
+                                                         int32_t* out, uint32_t* count) {
+    const int lane = lane_id();
+    const long long w0 = lo + ((blockIdx.x * (long long)BLOCK + threadIdx.x) / WAVE) * WAVE;
+    const bool col_on = lane < B.K && ((B.active >> lane) & 1ull);
+    for (int i = 0; i < WAVE; i++) {
+        const long long e = w0 + i;
+        if (e >= hi) break;
+        const int v = hids[e];
+        const bool pj = col_on && fabsf(B.r[(long long)v * B.K + lane]) > B.eps[lane];
+        if (__ballot(pj) && lane == 0) out[atomicAdd(count, 1u)] = (int32_t)e;
+    }
+}
+
+// Sets r[seeds[i]][col] += value (teleport mass of one column).
+__global__ void k_bseed(int cnt, const int32_t* __restrict__ seeds, float* r, int K, int col, float value) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < cnt) atomic_add_f32(&r[(long long)seeds[i] * K + col], value);
+}
+
 // ------------------------------------------------------------------ host-side context
 
 struct PieceBuf {
-    Piece* d = nullptr;
-    uint32_t* cnt = nullptr;     // [0] count, [1] ticket
+    Piece* d = nullptr;          // pieces (Piece, or BPiece for batched contexts)
+    uint32_t* cnt = nullptr;     // [phase*2 + 0] piece count, [phase*2 + 1] batched p-row count
     uint32_t cap = 0;
+    int phase = 0;
 };
 
 struct Slot {
@@ -640,7 +908,55 @@ struct Ctx {
     double t_ms[T_NCLS] = {};
     uint64_t t_launch[T_NCLS] = {};
     double snap_bytes = 0;
+    // batched personalised PageRank (K > 0): x, r are [n][K]
+    int K = 0;
+    float damp = 0.85f;
+    float* d_beps = nullptr;        // [K]
+    uint32_t* d_colpend = nullptr;  // [K]
+    uint64_t active = ~0ull;
+    float* prows[3] = {nullptr, nullptr, nullptr};
+    uint32_t prow_cap[3] = {0, 0, 0};
 };
+
+BatchArgs batch_args(Ctx* c, int s) {
+    BatchArgs B{};
+    B.x = c->d_x;
+    B.r = c->d_r;
+    B.eps = c->d_beps;
+    B.damp = c->damp;
+    B.K = c->K;
+    B.active = c->active;
+    B.colpend = c->d_colpend;
+    B.prows = s >= 0 ? c->prows[s] : nullptr;
+    B.prow_count = s >= 0 ? c->pb[s].cnt + 2 * c->pb[s].phase + 1 : nullptr;
+    B.prow_cap = s >= 0 ? c->prow_cap[s] : 0;
+    return B;
+}
+
+int ensure_bpieces(Ctx* c, int s, uint64_t edges_bound) {
+    uint64_t need64 = 2 * (edges_bound / BPIECE) + 64;
+    uint64_t rows64 = edges_bound / BHUB + 64;
+    if (need64 > 0xffffffffull) need64 = 0xffffffffull;
+    if (rows64 > 0xffffffffull) rows64 = 0xffffffffull;
+    const uint32_t need = (uint32_t)need64, rows = (uint32_t)rows64;
+    PieceBuf& p = c->pb[s];
+    if (p.d && p.cap >= need && c->prows[s] && c->prow_cap[s] >= rows) return EXC_OK;
+    HIPCHK(hipStreamSynchronize(c->st[s]));
+    if (p.d) { (void)hipFree(p.d); p.d = nullptr; p.cap = 0; }
+    if (c->prows[s]) { (void)hipFree(c->prows[s]); c->prows[s] = nullptr; c->prow_cap[s] = 0; }
+    if (hipMalloc(&p.d, (size_t)need * sizeof(BPiece)) != hipSuccess ||
+        hipMalloc(&c->prows[s], (size_t)rows * c->K * sizeof(float)) != hipSuccess) {
+        (void)hipGetLastError();
+        return fail(EXC_ENOMEM, "cannot allocate %u batched hub pieces", need);
+    }
+    p.cap = need;
+    c->prow_cap[s] = rows;
+    if (!p.cnt) {
+        HIPCHK(hipMalloc(&p.cnt, 4 * sizeof(uint32_t)));
+        HIPCHK(hipMemset(p.cnt, 0, 4 * sizeof(uint32_t)));
+    }
+    return EXC_OK;
+}
 
 hipEvent_t get_event(Ctx* c) {
     if (!c->free_events.empty()) { hipEvent_t e = c->free_events.back(); c->free_events.pop_back(); return e; }
@@ -677,6 +993,7 @@ int collect_timers(Ctx* c) {
 }
 
 int ensure_pieces(Ctx* c, int s, uint64_t edges_bound) {
+    if (c->K > 0) return ensure_bpieces(c, s, edges_bound);
     uint64_t need64 = edges_bound / HUB + edges_bound / PIECE + 64;
     if (need64 > 0xffffffffull) need64 = 0xffffffffull;
     uint32_t need = (uint32_t)need64;
@@ -689,8 +1006,8 @@ int ensure_pieces(Ctx* c, int s, uint64_t edges_bound) {
     }
     p.cap = need;
     if (!p.cnt) {
-        HIPCHK(hipMalloc(&p.cnt, 2 * sizeof(uint32_t)));
-        HIPCHK(hipMemset(p.cnt, 0, 2 * sizeof(uint32_t)));
+        HIPCHK(hipMalloc(&p.cnt, 4 * sizeof(uint32_t)));
+        HIPCHK(hipMemset(p.cnt, 0, 4 * sizeof(uint32_t)));
     }
     return EXC_OK;
 }
@@ -729,7 +1046,7 @@ template <int NM, int S>
 void launch_pieces_t(Ctx* c, int s, const int32_t* edges) {
     const unsigned grid = (unsigned)(c->cus * 4);
     hipLaunchKernelGGL((k_pieces<NM, S>), dim3(grid), dim3(BLOCK), 0, c->st[s], c->pb[s].d,
-                       c->pb[s].cnt, c->pb[s].cnt + 1, c->pb[s].cap, edges, c->DM);
+                       c->pb[s].cnt + 2 * c->pb[s].phase, c->pb[s].cap, edges, c->DM);
 }
 
 int norm_group(int g) {
@@ -744,11 +1061,33 @@ int push_launch(Ctx* c, int s, int src, unsigned grid, PushArgs a, int group, ui
     if (grid == 0) return EXC_OK;
     int rc = ensure_pieces(c, s, edges_bound);
     if (rc) return rc;
-    a.pieces = c->pb[s].d;
-    a.piece_count = c->pb[s].cnt;
-    a.piece_cap = c->pb[s].cap;
+    PieceBuf& pb = c->pb[s];
+    a.pieces = pb.d;
+    a.piece_count = pb.cnt + 2 * pb.phase;
+    a.piece_zero = pb.cnt + 2 * (pb.phase ^ 1);
+    a.piece_cap = pb.cap;
     a.counters = c->d_counters + src * C_FIELDS;
     a.overflow = c->d_counters + C_OVERFLOW;
+    if (c->K > 0) {
+        BatchArgs B = batch_args(c, s);
+        {
+            ScopedTimer t(c, c->st[s], T_PUSH + src);
+            switch (src) {
+                case SRC_RANGE: hipLaunchKernelGGL(k_bpush<SRC_RANGE>, dim3(grid), dim3(BLOCK), 0, c->st[s], a, B); break;
+                case SRC_BITMAP: hipLaunchKernelGGL(k_bpush<SRC_BITMAP>, dim3(grid), dim3(BLOCK), 0, c->st[s], a, B); break;
+                case SRC_LIST: hipLaunchKernelGGL(k_bpush<SRC_LIST>, dim3(grid), dim3(BLOCK), 0, c->st[s], a, B); break;
+                default: hipLaunchKernelGGL(k_bpush<SRC_HOT>, dim3(grid), dim3(BLOCK), 0, c->st[s], a, B); break;
+            }
+        }
+        {
+            ScopedTimer t(c, c->st[s], T_PIECE);
+            hipLaunchKernelGGL(k_bpieces, dim3(c->cus * 4), dim3(BLOCK), 0, c->st[s],
+                               reinterpret_cast<const BPiece*>(pb.d), pb.cnt + 2 * pb.phase, pb.cap, a.edges, B);
+        }
+        pb.phase ^= 1;
+        HIPCHK(hipGetLastError());
+        return EXC_OK;
+    }
     const int G = c->push_kernel == EXC_KERNEL_GROUP ? norm_group(group > 0 ? group : 1) : 0;
     {
         ScopedTimer t(c, c->st[s], T_PUSH + src);
@@ -762,6 +1101,7 @@ int push_launch(Ctx* c, int s, int src, unsigned grid, PushArgs a, int group, ui
         else if (c->nm == 2) launch_pieces_t<2, 2>(c, s, a.edges);
         else launch_pieces_t<3, 4>(c, s, a.edges);
     }
+    pb.phase ^= 1;
     HIPCHK(hipGetLastError());
     return EXC_OK;
 }
@@ -775,6 +1115,13 @@ int push_chunk(Ctx* c, int s, int v0, int v1, const int32_t* edges, uint64_t eba
     a.idx = c->d_idx;
     a.edges = edges;
     a.ebase = ebase;
+    if (c->K > 0) {   // batched: one wave per 64 vertices
+        const long long waves = ((long long)v1 - ((long long)v0 & ~63ll) + WAVE - 1) / WAVE;
+        const unsigned grid = (unsigned)((waves + NWAVES - 1) / NWAVES);
+        if (use_list) a.bitmap = c->d_bitmap;
+        else a.hot = hot_skip ? c->d_hot : nullptr;
+        return push_launch(c, s, use_list ? SRC_BITMAP : SRC_RANGE, grid, a, group, edges_bound);
+    }
     if (use_list) {
         a.bitmap = c->d_bitmap;
         const long long w0 = v0 >> 5, w1 = ((long long)v1 + 31) >> 5;
@@ -841,6 +1188,14 @@ int exc_create(const exc_config* cfg, const exc_measures* M, void** out) {
     *out = nullptr;
     c->dev = cfg->device; c->n = cfg->n; c->m = cfg->m; c->nc = cfg->nc;
     c->nm = M->nm; c->S = M->nm == 1 ? 1 : M->nm == 2 ? 2 : 4;
+    if (cfg->batch_k) {
+        if (cfg->batch_k < 1 || cfg->batch_k > 64) { delete c; return fail(EXC_EINVAL, "exc_create: batch_k must be 1..64"); }
+        c->K = cfg->batch_k;
+        c->S = c->K;
+        c->nm = 1;
+        c->damp = M->scale[0];
+        c->active = c->K == 64 ? ~0ull : ((1ull << c->K) - 1);
+    }
     c->M = *M;
     c->zc = cfg->zero_copy != 0;
     c->timing = cfg->timing != 0;
@@ -872,6 +1227,13 @@ int exc_create(const exc_config* cfg, const exc_measures* M, void** out) {
         CK(hipMemset(c->d_trace, 0, n * sizeof(uint16_t)));
     }
     CK(hipMalloc(&c->d_hcount, sizeof(uint32_t)));
+    if (c->K) {
+        CK(hipMalloc(&c->d_beps, (size_t)c->K * sizeof(float)));
+        CK(hipMalloc(&c->d_colpend, (size_t)c->K * sizeof(uint32_t)));
+        CK(hipMemset(c->d_colpend, 0, (size_t)c->K * sizeof(uint32_t)));
+        std::vector<float> e(c->K, M->eps[0]);
+        CK(hipMemcpy(c->d_beps, e.data(), e.size() * sizeof(float), hipMemcpyHostToDevice));
+    }
     for (int s = 0; s < 3; s++) CK(hipStreamCreateWithFlags(&c->st[s], hipStreamNonBlocking));
     for (auto& sl : c->slot) {
         const size_t eb = (c->max_chunk_edges ? c->max_chunk_edges : 1) * sizeof(int32_t);
@@ -922,6 +1284,9 @@ int exc_destroy(void* p) {
     }
     for (auto& pb : c->pb) { if (pb.d) hipFree(pb.d); if (pb.cnt) hipFree(pb.cnt); }
     for (auto s : c->st) if (s) hipStreamDestroy(s);
+    for (auto q : c->prows) if (q) (void)hipFree(q);
+    if (c->d_beps) (void)hipFree(c->d_beps);
+    if (c->d_colpend) (void)hipFree(c->d_colpend);
     void* dev_ptrs[] = {c->d_idx, c->d_x, c->d_r, c->d_x0, c->d_r0, c->d_bitmap, c->d_cnt, c->d_cv0, c->d_trace, c->d_counters,
                         c->d_arena, c->d_hids, c->d_hoff, c->d_hedges, c->d_hot, c->d_hlist, c->d_hcount};
     for (void* q : dev_ptrs) if (q) hipFree(q);
@@ -942,6 +1307,13 @@ int exc_init_measure(void* p, int i, float beta, float lambda) {
     Ctx* c = as_ctx(p);
     if (!c || i < 0 || i >= c->nm) return fail(EXC_EINVAL, "exc_init_measure: bad slot");
     HIPCHK(hipSetDevice(c->dev));
+    if (c->K) {   // batched: every column starts at x = r = 0 (exc_batch_seed adds the teleport mass)
+        const size_t bytes = (size_t)c->n * c->S * sizeof(float);
+        HIPCHK(hipMemsetAsync(c->d_x, 0, bytes, c->st[0]));
+        HIPCHK(hipMemsetAsync(c->d_r, 0, bytes, c->st[0]));
+        HIPCHK(hipStreamSynchronize(c->st[0]));
+        return EXC_OK;
+    }
     ScopedTimer t(c, c->st[0], T_OTHER);
     hipLaunchKernelGGL(k_init, dim3(c->cus * 8), dim3(BLOCK), 0, c->st[0], c->n, c->d_idx, c->d_x, c->d_r,
                        c->S, i, c->M.kind[i], beta, lambda);
@@ -952,7 +1324,7 @@ int exc_init_measure(void* p, int i, float beta, float lambda) {
 
 int exc_set_state(void* p, int i, const float* h_x, const float* h_r) {
     Ctx* c = as_ctx(p);
-    if (!c || i < 0 || i >= c->nm) return fail(EXC_EINVAL, "exc_set_state: bad slot");
+    if (!c || i < 0 || i >= (c->K ? c->K : c->nm)) return fail(EXC_EINVAL, "exc_set_state: bad slot");
     HIPCHK(hipSetDevice(c->dev));
     float* tmp;
     const size_t bytes = (size_t)c->n * sizeof(float);
@@ -972,7 +1344,7 @@ int exc_set_state(void* p, int i, const float* h_x, const float* h_r) {
 
 int exc_read_state(void* p, int i, int which, float* h_out, size_t v0, size_t cnt) {
     Ctx* c = as_ctx(p);
-    if (!c || i < 0 || i >= c->nm || v0 + cnt > (size_t)c->n || !h_out)
+    if (!c || i < 0 || i >= (c->K ? c->K : c->nm) || v0 + cnt > (size_t)c->n || !h_out)
         return fail(EXC_EINVAL, "exc_read_state: bad arguments");
     if (!cnt) return EXC_OK;
     HIPCHK(hipSetDevice(c->dev));
@@ -1039,13 +1411,24 @@ int exc_snapshot(void* p, int hot_skip, uint64_t* h_cnt, uint64_t* h_hotpend) {
     a.segs_per_wave = (int)spw;
     const long long waves = (nseg + spw - 1) / spw;
     const unsigned grid = (unsigned)((waves + NWAVES - 1) / NWAVES);
-    {
+    if (c->K) {
+        hipLaunchKernelGGL(k_zero_u32, dim3((c->K + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, st, c->d_colpend, c->K);
+        const long long bw = (long long)c->cus * 64 * NWAVES;
+        long long bspw = (nseg + bw - 1) / bw;
+        if (bspw < 1) bspw = 1;
+        a.segs_per_wave = (int)bspw;
+        const long long bwaves = (nseg + bspw - 1) / bspw;
+        const unsigned bgrid = (unsigned)((bwaves + NWAVES - 1) / NWAVES);
+        ScopedTimer t(c, st, T_SNAP);
+        hipLaunchKernelGGL(k_bsnapshot, dim3(bgrid), dim3(BLOCK), 0, st, a, batch_args(c, -1));
+        c->snap_bytes += 4.0 * c->K * c->n + 4.0 * (((double)c->n + 31) / 32);
+    } else {
         ScopedTimer t(c, st, T_SNAP);
         if (c->nm == 1) hipLaunchKernelGGL((k_snapshot<1, 1>), dim3(grid), dim3(BLOCK), 0, st, a);
         else if (c->nm == 2) hipLaunchKernelGGL((k_snapshot<2, 2>), dim3(grid), dim3(BLOCK), 0, st, a);
         else hipLaunchKernelGGL((k_snapshot<3, 4>), dim3(grid), dim3(BLOCK), 0, st, a);
+        c->snap_bytes += 4.0 * c->nm * c->n + 4.0 * (((double)c->n + 31) / 32);
     }
-    c->snap_bytes += 4.0 * c->nm * c->n + 4.0 * (((double)c->n + 31) / 32);
     HIPCHK(hipGetLastError());
     HIPCHK(hipMemcpyAsync(c->h_small, c->d_cnt, ncnt * sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
@@ -1076,9 +1459,9 @@ int exc_cache_chunks(void* p, uint64_t resident_edges) {
     Ctx* c = as_ctx(p);
     if (!c || resident_edges > c->m) return fail(EXC_EINVAL, "exc_cache_chunks: bad size");
     HIPCHK(hipSetDevice(c->dev));
-    if (c->d_arena) { hipFree(c->d_arena); c->d_arena = nullptr; c->arena_edges = 0; }
-    if (!resident_edges) return EXC_OK;
-    if (hipMalloc(&c->d_arena, resident_edges * sizeof(int32_t)) != hipSuccess) {
+    if (c->d_arena) { (void)hipFree(c->d_arena); c->d_arena = nullptr; c->arena_edges = 0; }
+    // a prefix of zero-edge chunks is still a cache: keep one element so the pointer is valid
+    if (hipMalloc(&c->d_arena, (resident_edges ? resident_edges : 1) * sizeof(int32_t)) != hipSuccess) {
         (void)hipGetLastError();
         c->d_arena = nullptr;
         return fail(EXC_ENOMEM, "cannot allocate %.2f GB edge cache", resident_edges * 4.0 / 1e9);
@@ -1228,7 +1611,8 @@ int exc_gather_chunk(void* p, int s, int k, int group) {
         a.off = sl.d_off;
         a.k = k;
         a.edges = sl.d_edges;
-        int rc = push_launch(c, 1 + s, SRC_LIST, (unsigned)((k + BLOCK - 1) / BLOCK), a, group, c->max_chunk_edges);
+        const unsigned grid = c->K ? grid_for((long long)k * WAVE, c->cus * 16) : (unsigned)((k + BLOCK - 1) / BLOCK);
+        int rc = push_launch(c, 1 + s, SRC_LIST, grid, a, group, c->max_chunk_edges);
         if (rc) return rc;
     }
     HIPCHK(hipEventRecord(sl.done, st));
@@ -1245,6 +1629,21 @@ int exc_push_hot(void* p, int lo, int hi, int group, int after_slots) {
     if (after_slots)
         for (auto& sl : c->slot) if (sl.busy) HIPCHK(hipStreamWaitEvent(st, sl.done, 0));
     hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, st, c->d_hcount, 1);
+    if (c->K) {
+        {
+            ScopedTimer t(c, st, T_OTHER);
+            const long long waves = ((long long)(hi - lo) + WAVE - 1) / WAVE;
+            hipLaunchKernelGGL(k_bcompact_hot, dim3((unsigned)((waves + NWAVES - 1) / NWAVES)), dim3(BLOCK), 0, st, lo, hi,
+                               c->d_hids, batch_args(c, -1), c->d_hlist, c->d_hcount);
+        }
+        PushArgs a{};
+        a.hlist = c->d_hlist;
+        a.hcount = c->d_hcount;
+        a.hids = c->d_hids;
+        a.hoff = c->d_hoff;
+        a.edges = c->d_hedges;
+        return push_launch(c, 0, SRC_HOT, grid_for((long long)(hi - lo) * WAVE, c->cus * 16), a, group, c->hot_edges);
+    }
     {
         ScopedTimer t(c, st, T_OTHER);
         const unsigned grid = (unsigned)(((long long)(hi - lo) + BLOCK - 1) / BLOCK);
@@ -1259,6 +1658,49 @@ int exc_push_hot(void* p, int lo, int hi, int group, int after_slots) {
     a.hoff = c->d_hoff;
     a.edges = c->d_hedges;
     return push_launch(c, 0, SRC_HOT, (unsigned)(((long long)(hi - lo) + BLOCK - 1) / BLOCK), a, group, c->hot_edges);
+}
+
+int exc_batch_eps(void* p, const float* eps) {
+    Ctx* c = as_ctx(p);
+    if (!c || !c->K || !eps) return fail(EXC_EINVAL, "exc_batch_eps: not a batched context");
+    HIPCHK(hipSetDevice(c->dev));
+    HIPCHK(hipMemcpy(c->d_beps, eps, (size_t)c->K * sizeof(float), hipMemcpyHostToDevice));
+    return EXC_OK;
+}
+
+int exc_batch_seed(void* p, int col, const int32_t* seeds, int count, float value) {
+    Ctx* c = as_ctx(p);
+    if (!c || !c->K || col < 0 || col >= c->K || count < 0 || (count && !seeds))
+        return fail(EXC_EINVAL, "exc_batch_seed: bad arguments");
+    for (int i = 0; i < count; i++)
+        if (seeds[i] < 0 || seeds[i] >= c->n) return fail(EXC_EINVAL, "exc_batch_seed: seed %d out of range", seeds[i]);
+    if (!count) return EXC_OK;
+    HIPCHK(hipSetDevice(c->dev));
+    int32_t* d;
+    HIPCHK(hipMalloc(&d, (size_t)count * sizeof(int32_t)));
+    hipError_t e = hipMemcpy(d, seeds, (size_t)count * sizeof(int32_t), hipMemcpyHostToDevice);
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(k_bseed, dim3((count + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, c->st[0], count, d, c->d_r, c->K, col, value);
+        e = hipStreamSynchronize(c->st[0]);
+    }
+    (void)hipFree(d);
+    if (e != hipSuccess) return fail(EXC_EHIP, "exc_batch_seed: %s", hipGetErrorString(e));
+    return EXC_OK;
+}
+
+int exc_batch_colpend(void* p, uint32_t* out) {
+    Ctx* c = as_ctx(p);
+    if (!c || !c->K || !out) return fail(EXC_EINVAL, "exc_batch_colpend: not a batched context");
+    HIPCHK(hipSetDevice(c->dev));
+    HIPCHK(hipMemcpy(out, c->d_colpend, (size_t)c->K * sizeof(uint32_t), hipMemcpyDeviceToHost));
+    return EXC_OK;
+}
+
+int exc_batch_set_active(void* p, uint64_t mask) {
+    Ctx* c = as_ctx(p);
+    if (!c || !c->K) return fail(EXC_EINVAL, "exc_batch_set_active: not a batched context");
+    c->active = c->K == 64 ? mask : (mask & ((1ull << c->K) - 1));
+    return EXC_OK;
 }
 
 int exc_pass_sync(void* p, double* waited) {

@@ -51,7 +51,7 @@ class Measures(ctypes.Structure):
 class Config(ctypes.Structure):
     _fields_ = [("device", c_int), ("n", c_int), ("m", c_u64), ("nc", c_int), ("cv0", P(c_int)),
                 ("max_chunk_edges", c_u64), ("max_chunk_verts", c_int), ("trace", c_int), ("zero_copy", c_int),
-                ("push_kernel", c_int), ("timing", c_int)]
+                ("push_kernel", c_int), ("timing", c_int), ("batch_k", c_int)]
 
 
 class Chunk(ctypes.Structure):
@@ -121,6 +121,10 @@ HIP_SIGNATURES = {
     "exc_stream_chunk": (c_int, [c_void_p, c_int, P(Chunk), c_int, c_int, c_int, c_int, c_int, c_int]),
     "exc_gather_chunk": (c_int, [c_void_p, c_int, c_int, c_int]),
     "exc_push_hot": (c_int, [c_void_p, c_int, c_int, c_int, c_int]),
+    "exc_batch_eps": (c_int, [c_void_p, c_void_p]),
+    "exc_batch_seed": (c_int, [c_void_p, c_int, c_void_p, c_int, c_float]),
+    "exc_batch_colpend": (c_int, [c_void_p, c_void_p]),
+    "exc_batch_set_active": (c_int, [c_void_p, c_u64]),
     "exc_pass_sync": (c_int, [c_void_p, P(c_double)]),
     "exc_pushed": (c_int, [c_void_p, P(c_u64)]),
     "exc_get_stats": (c_int, [c_void_p, P(Stats)]),
@@ -235,7 +239,7 @@ class Context:
     """One device context (include/excent_hip.h exc_create..exc_destroy)."""
 
     def __init__(self, n, m, cv0, kinds, eps, scale, divdeg, max_chunk_edges=1, max_chunk_verts=1,
-                 trace=False, push_kernel=KERNEL_LB, timing=False, device=0, zero_copy=False):
+                 trace=False, push_kernel=KERNEL_LB, timing=False, device=0, zero_copy=False, batch_k=0):
         self.n, self.m, self.nm = int(n), int(m), len(kinds)
         self.cv0 = np.ascontiguousarray(cv0, dtype=np.int32)
         self.nc = len(self.cv0)
@@ -245,7 +249,8 @@ class Context:
             M.kind[i], M.eps[i], M.scale[i], M.divdeg[i] = kinds[i], eps[i], scale[i], divdeg[i]
         cfg = Config(device, self.n, self.m, self.nc, self.cv0.ctypes.data_as(P(c_int)),
                      max(1, int(max_chunk_edges)), max(1, int(max_chunk_verts)), int(trace), int(zero_copy),
-                     int(push_kernel), int(timing))
+                     int(push_kernel), int(timing), int(batch_k))
+        self.K = int(batch_k)
         self.ctx = c_void_p()
         check(F["exc_create"](ctypes.byref(cfg), ctypes.byref(M), ctypes.byref(self.ctx)), "exc_create")
 
@@ -353,6 +358,22 @@ class Context:
 
     def reset_stats(self):
         check(F["exc_reset_stats"](self.ctx), "reset_stats")
+
+    def batch_eps(self, eps):
+        eps = np.ascontiguousarray(eps, dtype=np.float32)
+        check(F["exc_batch_eps"](self.ctx, ptr(eps)), "batch_eps")
+
+    def batch_seed(self, col, seeds, value):
+        seeds = np.ascontiguousarray(seeds, dtype=np.int32)
+        check(F["exc_batch_seed"](self.ctx, col, ptr(seeds), len(seeds), value), "batch_seed")
+
+    def batch_colpend(self):
+        out = np.zeros(self.K, np.uint32)
+        check(F["exc_batch_colpend"](self.ctx, ptr(out)), "batch_colpend")
+        return out
+
+    def batch_set_active(self, mask):
+        check(F["exc_batch_set_active"](self.ctx, int(mask)), "batch_set_active")
 
     def indegree(self):
         out = np.zeros(self.n, dtype=np.uint32)

@@ -65,6 +65,8 @@ typedef struct exc_config {
     int zero_copy;              /* staging buffers mapped into the device (read in place) */
     int push_kernel;            /* EXC_KERNEL_LB or EXC_KERNEL_GROUP */
     int timing;                 /* record HIP events around kernels (exc_stats) */
+    int batch_k;                /* 0: plain measures; 1..64: batched personalised PageRank with
+                                   K columns ([n][K] state, damping M.scale[0], eps M.eps[0]) */
 } exc_config;
 
 typedef struct exc_chunk { int v0, v1; uint64_t e0, e1; } exc_chunk;
@@ -102,7 +104,8 @@ int exc_destroy(void* ctx);
 /* Offsets upload (src/excent.cu:360-369): host slice [v0, v0+cnt) of the n+1 offsets. */
 int exc_upload_offsets(void* ctx, const uint64_t* h_idx, size_t v0, size_t cnt);
 
-/* k_init (src/excent.cu:41-47, 370-372) for measure slot i of kind M.kind[i]. */
+/* k_init (src/excent.cu:41-47, 370-372) for measure slot i of kind M.kind[i]; on a
+ * batched context it zeroes the whole [n][K] state. */
 int exc_init_measure(void* ctx, int i, float beta, float lambda);
 /* Host-supplied state for measure slot i (any of x, r may be NULL to leave it). */
 int exc_set_state(void* ctx, int i, const float* h_x, const float* h_r);
@@ -167,6 +170,18 @@ int exc_gather_chunk(void* ctx, int slot, int k, int group);
  * with the count kept on the device (no host round trip).  after_slots != 0 makes
  * the resident stream wait for both slots' last work first (746). */
 int exc_push_hot(void* ctx, int lo, int hi, int group, int after_slots);
+
+/* Batched personalised PageRank (no reference counterpart).  K teleport vectors share
+ * every list read: x and r are [n][K] rows, one wave per vertex, lane j = column j, so a
+ * neighbour update is one wave-wide atomic on a contiguous row.  A context created with
+ * batch_k = K starts from x = r = 0; exc_batch_seed adds value to r[seed][col] (the
+ * teleport mass (1-d) s_j); the snapshot sets a vertex pending when any active column is
+ * above its eps; exc_batch_colpend gives the per-column pending counts of the last
+ * snapshot and exc_batch_set_active drops converged columns from the pushes. */
+int exc_batch_eps(void* ctx, const float* eps);
+int exc_batch_seed(void* ctx, int col, const int32_t* seeds, int count, float value);
+int exc_batch_colpend(void* ctx, uint32_t* out);
+int exc_batch_set_active(void* ctx, uint64_t mask);
 
 /* End of pass (src/excent.cu:750-753): waits for all streams; *waited_s gets the wait. */
 int exc_pass_sync(void* ctx, double* waited_s);

@@ -479,6 +479,38 @@ int orc_fp64_reference(int n, const uint64_t* idx, const int32_t* adj, int kind,
     return it;
 }
 
+int orc_fp64_solve_b(int n, const uint64_t* idx, const int32_t* adj, const double* b, double s, int divdeg,
+                     int max_iter, double tol, double* out, double* rel_out) {
+    uint64_t* tidx; int32_t* tadj;
+    transpose(n, idx, adj, &tidx, &tadj);
+    double* x = (double*)malloc((size_t)n * 8);
+    double* y = (double*)malloc((size_t)n * 8);
+    memcpy(x, b, (size_t)n * 8);
+    int it = 0;
+    double rel = 1;
+    for (; it < max_iter && rel > tol; it++) {
+        double a = 0, mx = 0;
+        #pragma omp parallel for schedule(dynamic, 1024) reduction(max : a, mx)
+        for (int w = 0; w < n; w++) {
+            double acc = 0;
+            for (uint64_t e = tidx[w]; e < tidx[w + 1]; e++) {
+                int u = tadj[e];
+                acc += divdeg ? x[u] / (double)(idx[u + 1] - idx[u]) : x[u];
+            }
+            double nv = b[w] + s * acc;
+            y[w] = nv;
+            if (fabs(nv - x[w]) > a) a = fabs(nv - x[w]);
+            if (fabs(nv) > mx) mx = fabs(nv);
+        }
+        rel = mx > 0 ? a / mx : 0;
+        double* t = x; x = y; y = t;
+    }
+    memcpy(out, x, (size_t)n * 8);
+    if (rel_out) *rel_out = rel;
+    free(x); free(y); free(tidx); free(tadj);
+    return it;
+}
+
 /* tools/excent_params.cu:105-163, with the power iteration on the in-neighbour pull. */
 int orc_params(int n, const uint64_t* idx, const int32_t* adj, const double* start, double afac,
                double* lambda, double* alpha, double* mean_katz, double* mean_ev, long* symmetric_bad) {

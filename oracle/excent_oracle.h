@@ -88,6 +88,12 @@ int orc_fp64_reference(int n, const uint64_t* idx, const int32_t* adj, int kind,
                        double alpha, double beta, int max_iter, double tol,
                        double* out, double* rel, double* lambda_out);
 
+/* FP64 Jacobi with a per-vertex constant: y = b + s * sum_{u->w} x[u] (/ outdeg(u) when
+ * divdeg) from x = b, until max|dy|/max|y| < tol.  Personalised PageRank is b = (1-d) t,
+ * s = d, divdeg = 1 (no reference counterpart: the reference has no personalised mode). */
+int orc_fp64_solve_b(int n, const uint64_t* idx, const int32_t* adj, const double* b, double s, int divdeg,
+                     int max_iter, double tol, double* out, double* rel);
+
 /* ---- constants (tools/excent_params.cu:105-170) ----
  * lambda1 by power iteration on the in-neighbour pull from `start`
  * (NULL = all ones, the reference's start), Katz mean with alpha = afac/lambda1,

@@ -1,0 +1,124 @@
+"""Batched personalised PageRank on the GPU ([n][K] rows, one wave per vertex, lanes =
+columns; no reference counterpart): the snapshot against the oracle's multi-measure
+snapshot bit for bit, and every column against the FP64 personalised solution within
+the exact bound ||x - x*||_1 <= ||b + W x - x||_1 / (1 - d), and against a plain
+single-column run."""
+import os
+import re
+
+import numpy as np
+import pytest
+
+import excent_amd.native as nat
+from oracle import oracle as orc
+from tests.conftest import SAMPLE_EGR
+
+pytestmark = pytest.mark.gpu
+
+
+def true_residual(idx, adj, x, b):
+    n = len(idx) - 1
+    deg = np.diff(idx.astype(np.int64))
+    src = np.repeat(np.arange(n), deg)
+    acc = np.zeros(n)
+    np.add.at(acc, adj, 0.85 * x.astype(np.float64)[src] / deg[src])
+    return b + acc - x.astype(np.float64)
+
+
+@pytest.mark.parametrize("K", [1, 7, 64])
+def test_batched_snapshot_bit_exact(K, rmat12):
+    idx, adj = orc.egr_load(rmat12)
+    n = len(idx) - 1
+    rng = np.random.default_rng(K)
+    eps = (rng.random(K) * 0.02 + 0.001).astype(np.float32)
+    rs = [(rng.standard_normal(n) * 0.01 * (rng.random(n) < 0.3)).astype(np.float32) for _ in range(K)]
+    cv0 = orc.plan_chunks(idx, len(adj) // 6)
+    with nat.Context(n, len(adj), cv0, [nat.EXC_PR], [0.01], [0.85], [1], batch_k=K) as ctx:
+        ctx.upload_offsets(idx)
+        ctx.batch_eps(eps)
+        for j in range(K):
+            ctx.set_state(j, r=rs[j])
+        cnt, _ = ctx.snapshot()
+        bm = ctx.bitmap()
+        colpend = ctx.batch_colpend()
+        # inactive columns do not count
+        ctx.batch_set_active((1 << (K // 2)) - 1 if K > 1 else 1)
+        cnt2, _ = ctx.snapshot()
+    obm, ocnt, _ = orc.snapshot(rs, eps, idx, cv0)
+    assert np.array_equal(bm, obm) and np.array_equal(cnt, ocnt)
+    assert np.array_equal(colpend, [(np.abs(r) > e).sum() for r, e in zip(rs, eps)])
+    act = K // 2 if K > 1 else 1
+    _, ocnt2, _ = orc.snapshot(rs[:act], eps[:act], idx, cv0)
+    assert np.array_equal(cnt2, ocnt2)
+
+
+@pytest.mark.parametrize("gname", ["sample", "rmat12"])
+def test_batched_ppr_against_fp64(tmp_path, gname, rmat12):
+    path = SAMPLE_EGR if gname == "sample" else rmat12
+    idx, adj = orc.egr_load(path)
+    n = len(idx) - 1
+    K, S = 16, 4
+    rng = np.random.default_rng(3)
+    seeds = np.stack([rng.choice(n, S, replace=False) for _ in range(K)]).astype(np.int32)
+    sf = str(tmp_path / "seeds.i32")
+    seeds.tofile(sf)
+    eps = 1e-4
+    log = str(tmp_path / "b.log")
+    rc = nat.excent([path, "--measure", "ppr", "--ppr-k", K, "--ppr-seeds", sf, "--eps-ppr", eps, "--dump",
+                     str(tmp_path / "o"), "--stream-ratio", "0.5", "--log", log])
+    text = open(log).read()
+    assert rc == 0, text
+    assert "active columns" in text and re.search(r"16 columns, ppr sum", text)
+    X = np.fromfile(str(tmp_path / "o.ppr.bin"), np.float32).reshape(K, n)
+    mass = np.float32((1 - np.float64(np.float32(0.85))) * n / S)
+    for j in range(K):
+        b = np.zeros(n)
+        np.add.at(b, seeds[j], np.float64(mass))
+        ref, _, _ = orc.fp64_solve_b(idx, adj, b)
+        err = np.abs(X[j] - ref).sum()
+        bound = np.abs(true_residual(idx, adj, X[j], b)).sum() / 0.15
+        assert err <= bound * (1 + 1e-6) + 1e-6, (j, err, bound)
+        assert err <= eps * n / 0.15
+
+
+def test_batched_equals_plain_runs_within_bound(rmat12):
+    """Column j of a batched run and a plain run from the same teleport vector both sit
+    within eps-bounds of the same FP64 solution."""
+    idx, adj = orc.egr_load(rmat12)
+    n = len(idx) - 1
+    K = 4
+    rng = np.random.default_rng(9)
+    seeds = [rng.choice(n, 8, replace=False).astype(np.int32) for _ in range(K)]
+    mass = np.float32(0.15 * n / 8)
+    cv0 = np.array([0], np.int32)
+
+    def run():
+        with nat.Context(n, len(adj), cv0, [nat.EXC_PR], [1e-4], [0.85], [1], batch_k=K) as ctx:
+            ctx.upload_offsets(idx)
+            assert ctx.cache_chunks(len(adj)) == 0
+            ctx.cache_upload(adj)
+            for j in range(K):
+                ctx.batch_seed(j, seeds[j], mass)
+            for it in range(5000):
+                cnt, _ = ctx.snapshot()
+                cp = ctx.batch_colpend()
+                ctx.batch_set_active(sum(1 << j for j in range(K) if cp[j]))
+                if cnt[0] == 0:
+                    break
+                ctx.push_resident(0, n, 0, len(adj), cnt[0] < 0.5 * n)
+                ctx.pass_sync()
+            return [ctx.read_state(j, 0) for j in range(K)], ctx.stats()
+
+    X, st = run()
+    assert st["pushed_edges"] > 0
+    for j in range(K):
+        b = np.zeros(n)
+        np.add.at(b, seeds[j], np.float64(mass))
+        ref, _, _ = orc.fp64_solve_b(idx, adj, b)
+        # plain run of the same column through the oracle's CPU push
+        x = [np.zeros(n, np.float32)]
+        r = [b.astype(np.float32)]
+        orc.run_push(idx, adj, cv0, x, r, [1e-4], [np.float32(0.85)], [1], threads=4)
+        for xx in (X[j], x[0]):
+            err = np.abs(xx - ref).sum()
+            assert err <= np.abs(true_residual(idx, adj, xx, b)).sum() / 0.15 * (1 + 1e-6) + 1e-6

@@ -80,7 +80,9 @@ def test_pagerank_against_fp64(kernel, group, gname, rmat12):
     rl1 = float(np.abs(rs[0].astype(np.float64)).sum())
     assert np.all(np.abs(rs[0]) <= 0.01)
     err = np.abs(x - ref).sum()
-    assert err <= rl1 / 0.15 * 1.001 + 1e-3, (err, rl1)
+    true_l1 = np.abs(true_residual(idx, adj, xs[0], 0.15, 0.85, True)).sum()
+    assert err <= true_l1 / 0.15 * (1 + 1e-6) + 1e-6, (err, true_l1)
+    assert err <= 0.01 * n / 0.15
     # invariant: the GPU residual is the true residual of its x
     tr = true_residual(idx, adj, xs[0], 0.15, 0.85, True)
     assert np.abs(tr - rs[0]).max() < 2e-4 * max(1.0, np.abs(x).max())
@@ -143,5 +145,9 @@ def test_star_hub_pieces(kernel, group):
     leaf = 0.15 + 0.85 * h / L
     ref = np.concatenate([[h], np.full(L, leaf)])
     err = np.abs(xs[0].astype(np.float64) - ref).sum()
-    assert err <= np.abs(rs[0].astype(np.float64)).sum() / 0.15 * 1.01 + 1e-2
+    # exact bound from the FP64 residual of the returned x (float32 accumulation of 20000
+    # adds into a hub of value ~9190 makes the state's own r drift from the true residual)
+    true_l1 = np.abs(true_residual(idx, adj, xs[0], 0.15, 0.85, True)).sum()
+    assert err <= true_l1 / 0.15 * (1 + 1e-6)
+    assert err / ref.sum() < 2e-3
     assert pushed >= 2 * L
