@@ -73,6 +73,7 @@ struct Options {
     // batched personalised PageRank (--measure ppr)
     int ppr_k = 0;
     std::string ppr_seeds, ppr_gen;
+    std::string pull = "auto";   // dense passes by pull over a device transpose: auto | off | threshold
     double eps_ppr = -1;
 };
 
@@ -131,6 +132,7 @@ Options parse(int argc, const char* const* argv) {
         else if (a == "--stats-json") o.stats_json = next();
         else if (a == "--log") o.log = next();
         else if (a == "--device") o.device = atoi(next());
+        else if (a == "--pull") o.pull = next();
         else if (a == "--ppr-k") o.ppr_k = atoi(next());
         else if (a == "--ppr-seeds") o.ppr_seeds = next();
         else if (a == "--ppr-gen") o.ppr_gen = next();
@@ -213,6 +215,8 @@ private:
 
     void init_state();
     void load_ppr_seeds(EgrGraph& g);
+    void maybe_enable_pull();
+    bool pull_tried_ = false, pull_on_ = false;
     bool batched() const { return o_.measure == "ppr"; }
     int K_ = 0;
     std::vector<std::vector<int32_t>> seeds_;
@@ -259,6 +263,7 @@ void Engine::setup() {
     else if (o.push_kernel == "group") push_kernel_ = EXC_KERNEL_GROUP;
     else die("--push-kernel must be lb or group");
     if (o.ev_init != "outdeg" && o.ev_init != "indeg") die("--ev-init must be outdeg or indeg");
+    if (o.pull != "auto" && o.pull != "off" && atof(o.pull.c_str()) <= 0) die("--pull must be auto, off or a positive demand fraction");
     if (o.zero_copy != "auto" && o.zero_copy != "on" && o.zero_copy != "off") die("--zero-copy must be auto, on or off");
     if (o.group_opt < 0 || o.group_opt > 64) die("--group must be 0 (automatic) or 1..64");
     if (o.hot_when != "end" && o.hot_when != "start" && o.hot_when != "both") die("--hot-when must be end, start or both");
@@ -488,6 +493,7 @@ void Engine::setup() {
     } else if (tau_file.empty()) {
         fprintf(out, "fetch rule: tau = %.3f (given)\n", stream_ratio_);
     }
+    maybe_enable_pull();
     ck(exc_pass_sync(ctx, nullptr), "setup sync");
     ck(exc_reset_stats(ctx), "reset stats");
     checkpoint_ = session_ && exc_state_checkpoint(ctx, 0) == EXC_OK;
@@ -554,6 +560,18 @@ void Engine::load_ppr_seeds(EgrGraph& g) {
     for (auto& sj : seeds_)
         for (int v : sj)
             if (v < 0 || v >= n_) die("ppr seed out of range");
+}
+
+// Once the whole graph sits in the chunk cache, dense passes can run as fold + pull over a
+// transpose built on the device (extension; --pull off keeps the push only).
+void Engine::maybe_enable_pull() {
+    if (pull_tried_ || o_.pull == "off" || batched() || !use_cache_ || n_resident_ != nc_) return;
+    for (int c = 0; c < nc_; c++) if (!loaded_[c]) return;
+    pull_tried_ = true;
+    const double thr = o_.pull == "auto" ? 0.05 : atof(o_.pull.c_str());
+    const int rc = exc_enable_pull(ctx_, thr);
+    if (rc == EXC_OK) pull_on_ = true;
+    else if (rc != EXC_ENOMEM) ck(rc, "pull setup");
 }
 
 // k_init for every measure; the eigenvector's residual uses in-degrees with --ev-init indeg.
@@ -703,6 +721,7 @@ void Engine::run_passes() {
         double w = 0;
         ck(exc_pass_sync(ctx, &w), "pass sync");
         t_wait_ += w;
+        maybe_enable_pull();
         const Faults f1 = faults();
         tot_maj_ += f1.maj - f0.maj;
         tot_min_ += f1.min - f0.min;

@@ -73,7 +73,8 @@ struct Piece {           // 32 bytes: a slice of one hub vertex's list
 // Counters kept per context (one atomic per block per field).
 // Per push source (4) x 8 fields, then the overflow flag.
 enum { C_VERT = 0, C_VERT_NM, C_EDGES, C_EDGES_NM, C_PEDGES, C_PEDGES_NM, C_FIELDS = 8 };
-enum { C_OVERFLOW = 4 * C_FIELDS, C_NCOUNTERS = C_OVERFLOW + 1 };
+// Row 4 holds the fold kernel of dense (pull) passes.
+enum { C_OVERFLOW = 5 * C_FIELDS, C_NCOUNTERS = C_OVERFLOW + 1 };
 
 __device__ __forceinline__ void atomic_add_f32(float* p, float v) {
     // No-return hardware float atomic on coarse-grained HBM (global_atomic_add_f32).
@@ -318,6 +319,7 @@ struct PushArgs {
     const uint64_t* hoff;           // HOT: hot list offsets
     const uint64_t* idx;            // CSR offsets (RANGE / BITMAP)
     const int32_t* edges;           // edge array of this launch
+    const float* wts;               // per-edge coefficients aligned with edges, or null
     uint64_t ebase;                 // global edge index of edges[0] (RANGE / BITMAP)
     Piece* pieces;
     uint32_t* piece_count;          // this launch's piece counter
@@ -413,11 +415,13 @@ __device__ __forceinline__ void push_entries(bool valid, int v, uint64_t ptr, ui
                 int mid = (lo + hi + 1) >> 1;
                 if (s_start[mid] <= j) lo = mid; else hi = mid - 1;
             }
-            const int u = a.edges[s_ptr[lo] + (j - s_start[lo])];
+            const uint64_t e = s_ptr[lo] + (j - s_start[lo]);
+            const int u = a.edges[e];
+            const float we = a.wts ? a.wts[e] : 1.f;
 #pragma unroll
             for (int k = 0; k < NM; k++) {
                 const float pk = s_p[k][lo];
-                if (pk != 0.f) atomic_add_f32(&M.r[(long long)u * S + k], pk);
+                if (pk != 0.f) atomic_add_f32(&M.r[(long long)u * S + k], a.wts ? pk * we : pk);
             }
         }
         __syncthreads();
@@ -437,9 +441,10 @@ __device__ __forceinline__ void push_entries(bool valid, int v, uint64_t ptr, ui
             for (int k = 0; k < NM; k++) pk[k] = s_p[k][ent];
             for (uint32_t e = gl; e < d; e += G) {
                 const int u = a.edges[b + e];
+                const float we = a.wts ? a.wts[b + e] : 1.f;
 #pragma unroll
                 for (int k = 0; k < NM; k++)
-                    if (pk[k] != 0.f) atomic_add_f32(&M.r[(long long)u * S + k], pk[k]);
+                    if (pk[k] != 0.f) atomic_add_f32(&M.r[(long long)u * S + k], a.wts ? pk[k] * we : pk[k]);
             }
         }
         __syncthreads();
@@ -518,7 +523,8 @@ __global__ __launch_bounds__(BLOCK) void k_push(PushArgs a, DevMeasures M) {
 // whose previous user (launch i-1 and its pieces) finished before launch i started.
 template <int NM, int S>
 __global__ __launch_bounds__(BLOCK) void k_pieces(const Piece* __restrict__ pcs, const uint32_t* count,
-                                                   uint32_t cap, const int32_t* __restrict__ edges, DevMeasures M) {
+                                                   uint32_t cap, const int32_t* __restrict__ edges,
+                                                   const float* __restrict__ wts, DevMeasures M) {
     // the count may exceed cap after an overflow (pieces beyond cap were not written and
     // the overflow counter reports it); never read past the buffer
     uint32_t cnt = *count;
@@ -529,9 +535,10 @@ __global__ __launch_bounds__(BLOCK) void k_pieces(const Piece* __restrict__ pcs,
         const Piece pc = pcs[w];
         for (uint32_t t = lane; t < pc.len; t += WAVE) {
             const int u = edges[pc.start + t];
+            const float we = wts ? wts[pc.start + t] : 1.f;
 #pragma unroll
             for (int k = 0; k < NM; k++)
-                if (pc.p[k] != 0.f) atomic_add_f32(&M.r[(long long)u * S + k], pc.p[k]);
+                if (pc.p[k] != 0.f) atomic_add_f32(&M.r[(long long)u * S + k], wts ? pc.p[k] * we : pc.p[k]);
         }
     }
 }
@@ -574,6 +581,164 @@ __global__ void k_indegree(uint64_t m, const int32_t* __restrict__ edges, uint32
     for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < (long long)m;
          e += (long long)gridDim.x * blockDim.x)
         atomicAdd(&indeg[edges[e]], 1u);
+}
+
+// ------------------------------------------------------------------ dense passes: fold + pull
+
+// A dense pass over the whole resident graph runs as two kernels: every pending vertex
+// folds its residual and publishes its per-edge weight p (k_fold), then every vertex sums
+// the p of its in-neighbours from the transpose and adds it to its residual (k_pull).  This
+// is the delta push executed with all folds before all adds -- one of the schedules the
+// asynchronous kernel may produce -- so x + r = b + W x holds exactly as after a push, but
+// the scatter's random atomics become gathers plus one plain read-modify-write per vertex.
+
+struct FoldArgs {
+    int n;
+    const uint64_t* idx;
+    float* p;                       // [n][S] weights of this pass (0 when v did not push)
+    unsigned long long* counters;   // C_FIELDS row
+};
+
+template <int NM, int S>
+__global__ __launch_bounds__(BLOCK) void k_fold(FoldArgs a, DevMeasures M) {
+    __shared__ unsigned long long s_cnt[4];
+    const int tid = threadIdx.x;
+    if (tid < 4) s_cnt[tid] = 0;
+    __syncthreads();
+    const long long v = blockIdx.x * (long long)BLOCK + tid;
+    uint32_t nmv = 0;
+    uint64_t deg = 0;
+    if (v < a.n) {
+        deg = a.idx[v + 1] - a.idx[v];
+        Row<S> row = load_row<S>(M.r, v);
+        Row<S> pr;
+#pragma unroll
+        for (int k = 0; k < S; k++) pr.v[k] = 0.f;
+#pragma unroll
+        for (int k = 0; k < NM; k++) {
+            if (fabsf(row.v[k]) > M.eps[k]) {
+                const float d = atomic_exch_f32(&M.r[v * S + k], 0.f);
+                M.x[v * S + k] += d;
+                if (d != 0.f) {
+                    nmv++;
+                    if (deg) pr.v[k] = M.divdeg[k] ? M.scale[k] * d / (float)deg : M.scale[k] * d;
+                }
+            }
+        }
+        if constexpr (S == 1) a.p[v] = pr.v[0];
+        else if constexpr (S == 2) reinterpret_cast<float2*>(a.p)[v] = make_float2(pr.v[0], pr.v[1]);
+        else reinterpret_cast<float4*>(a.p)[v] = make_float4(pr.v[0], pr.v[1], pr.v[2], pr.v[3]);
+    }
+    const uint32_t t0 = wave_sum_u32(nmv ? 1u : 0u), t1 = wave_sum_u32(nmv);
+    const uint64_t de = nmv ? deg : 0;
+    const uint64_t t2 = wave_sum_u64(de), t3 = wave_sum_u64(de * nmv);
+    if (lane_id() == 0 && t0) {
+        atomicAdd(&s_cnt[0], (unsigned long long)t0);
+        atomicAdd(&s_cnt[1], (unsigned long long)t1);
+        atomicAdd(&s_cnt[2], (unsigned long long)t2);
+        atomicAdd(&s_cnt[3], (unsigned long long)t3);
+    }
+    __syncthreads();
+    if (tid < 4 && s_cnt[tid]) atomicAdd(&a.counters[tid], s_cnt[tid]);
+}
+
+struct PullArgs {
+    int n;
+    const uint64_t* tidx;       // transpose offsets [n+1]
+    const int32_t* tadj;        // in-neighbours
+    const float* p;             // [n][S]
+};
+
+constexpr uint32_t PULL_HUB = 4096;   // in-lists longer than this are summed by whole blocks
+
+// Block-level load-balanced pull: the block's 256 destinations' in-lists are laid end to
+// end; lane j takes slot j, j+256, ... and adds the source's p row into its destination's
+// LDS accumulator (LDS float atomics); then each destination adds its sum to r.  In-lists
+// longer than PULL_HUB are summed by k_pull_hub instead.
+template <int NM, int S>
+__global__ __launch_bounds__(BLOCK) void k_pull(PullArgs a, DevMeasures M) {
+    __shared__ uint32_t s_start[BLOCK];
+    __shared__ uint64_t s_ptr[BLOCK];
+    __shared__ float s_acc[NM][BLOCK];
+    __shared__ uint32_t s_wave[NWAVES];
+    const int tid = threadIdx.x;
+    const long long u = blockIdx.x * (long long)BLOCK + tid;
+    uint64_t b = 0, d = 0;
+    if (u < a.n) { b = a.tidx[u]; d = a.tidx[u + 1] - b; }
+    const uint32_t mydeg = d <= PULL_HUB ? (uint32_t)d : 0u;
+    uint32_t total;
+    const uint32_t excl = block_excl_scan(mydeg, s_wave, &total);
+    s_start[tid] = excl;
+    s_ptr[tid] = b;
+#pragma unroll
+    for (int k = 0; k < NM; k++) s_acc[k][tid] = 0.f;
+    __syncthreads();
+    int lo = 0;
+    for (uint32_t j = tid; j < total; j += BLOCK) {
+        int hi = BLOCK - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (s_start[mid] <= j) lo = mid; else hi = mid - 1;
+        }
+        const int v = a.tadj[s_ptr[lo] + (j - s_start[lo])];
+        const Row<S> pv = load_row<S>(a.p, v);
+#pragma unroll
+        for (int k = 0; k < NM; k++)
+            if (pv.v[k] != 0.f) atomicAdd(&s_acc[k][lo], pv.v[k]);
+    }
+    __syncthreads();
+    if (u < a.n && mydeg) {
+#pragma unroll
+        for (int k = 0; k < NM; k++) {
+            const float acc = s_acc[k][tid];
+            if (acc != 0.f) M.r[u * S + k] += acc;
+        }
+    }
+}
+
+// Destinations with more than PULL_HUB in-neighbours: one block each (grid-stride over a
+// host-built list), block reduction, one add.
+template <int NM, int S>
+__global__ __launch_bounds__(BLOCK) void k_pull_hub(PullArgs a, const int32_t* __restrict__ hubs, int nhubs,
+                                                     DevMeasures M) {
+    __shared__ float s_red[NM][NWAVES];
+    for (int h = blockIdx.x; h < nhubs; h += gridDim.x) {
+        const int u = hubs[h];
+        const uint64_t b = a.tidx[u], e = a.tidx[u + 1];
+        float acc[NM];
+#pragma unroll
+        for (int k = 0; k < NM; k++) acc[k] = 0.f;
+        for (uint64_t i = b + threadIdx.x; i < e; i += BLOCK) {
+            const Row<S> pv = load_row<S>(a.p, a.tadj[i]);
+#pragma unroll
+            for (int k = 0; k < NM; k++) acc[k] += pv.v[k];
+        }
+#pragma unroll
+        for (int k = 0; k < NM; k++) {
+            float t = acc[k];
+            for (int o = WAVE / 2; o > 0; o >>= 1) t += __shfl_xor(t, o, WAVE);
+            if (lane_id() == 0) s_red[k][threadIdx.x / WAVE] = t;
+        }
+        __syncthreads();
+        if (threadIdx.x < NM) {
+            float t = 0.f;
+#pragma unroll
+            for (int w = 0; w < NWAVES; w++) t += s_red[threadIdx.x][w];
+            if (t != 0.f) M.r[(long long)u * S + threadIdx.x] += t;
+        }
+        __syncthreads();
+    }
+}
+
+// Transpose build: in-degree counts, then a fill through per-vertex cursors.
+__global__ void k_count_in64(uint64_t m, const int32_t* __restrict__ adj, unsigned long long* cnt) {
+    for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < (long long)m; e += (long long)gridDim.x * blockDim.x)
+        atomicAdd(&cnt[adj[e]], 1ull);
+}
+__global__ void k_fill_in64(int n, const uint64_t* __restrict__ idx, const int32_t* __restrict__ adj,
+                            unsigned long long* cursor, int32_t* tadj) {
+    for (long long v = blockIdx.x * (long long)blockDim.x + threadIdx.x; v < n; v += (long long)gridDim.x * blockDim.x)
+        for (uint64_t e = idx[v]; e < idx[v + 1]; e++) tadj[atomicAdd(&cursor[adj[e]], 1ull)] = (int32_t)v;
 }
 
 // ------------------------------------------------------------------ batched PPR
@@ -871,7 +1036,7 @@ struct Timer {
     int cls;
 };
 
-enum { T_SNAP = 0, T_PUSH, T_PUSH_LAST = T_PUSH + 3, T_PIECE, T_OTHER, T_NCLS };
+enum { T_SNAP = 0, T_PUSH, T_PUSH_LAST = T_PUSH + 3, T_PIECE, T_OTHER, T_FOLD, T_PULL, T_NCLS };
 
 struct Ctx {
     int dev = 0, n = 0, nc = 0, nm = 1, S = 1, push_kernel = EXC_KERNEL_LB;
@@ -908,6 +1073,16 @@ struct Ctx {
     double t_ms[T_NCLS] = {};
     uint64_t t_launch[T_NCLS] = {};
     double snap_bytes = 0;
+    float* d_wts = nullptr;           // per-edge coefficients aligned with the arena (weighted graphs)
+    // dense passes: fold + pull over the transpose (whole graph resident)
+    uint64_t* d_tidx = nullptr;
+    int32_t* d_tadj = nullptr;
+    float* d_p = nullptr;             // [n][S]
+    int32_t* d_pull_hubs = nullptr;
+    int n_pull_hubs = 0;
+    double pull_threshold = -1;       // < 0: off; else pull when demand >= threshold * m
+    uint64_t last_demand = 0;
+    uint64_t pull_passes = 0;
     // batched personalised PageRank (K > 0): x, r are [n][K]
     int K = 0;
     float damp = 0.85f;
@@ -1043,10 +1218,10 @@ int launch_push_src(Ctx* c, hipStream_t st, unsigned grid, const PushArgs& a, in
 }
 
 template <int NM, int S>
-void launch_pieces_t(Ctx* c, int s, const int32_t* edges) {
+void launch_pieces_t(Ctx* c, int s, const int32_t* edges, const float* wts) {
     const unsigned grid = (unsigned)(c->cus * 4);
     hipLaunchKernelGGL((k_pieces<NM, S>), dim3(grid), dim3(BLOCK), 0, c->st[s], c->pb[s].d,
-                       c->pb[s].cnt + 2 * c->pb[s].phase, c->pb[s].cap, edges, c->DM);
+                       c->pb[s].cnt + 2 * c->pb[s].phase, c->pb[s].cap, edges, wts, c->DM);
 }
 
 int norm_group(int g) {
@@ -1097,23 +1272,52 @@ int push_launch(Ctx* c, int s, int src, unsigned grid, PushArgs a, int group, ui
     }
     {
         ScopedTimer t(c, c->st[s], T_PIECE);
-        if (c->nm == 1) launch_pieces_t<1, 1>(c, s, a.edges);
-        else if (c->nm == 2) launch_pieces_t<2, 2>(c, s, a.edges);
-        else launch_pieces_t<3, 4>(c, s, a.edges);
+        if (c->nm == 1) launch_pieces_t<1, 1>(c, s, a.edges, a.wts);
+        else if (c->nm == 2) launch_pieces_t<2, 2>(c, s, a.edges, a.wts);
+        else launch_pieces_t<3, 4>(c, s, a.edges, a.wts);
     }
     pb.phase ^= 1;
     HIPCHK(hipGetLastError());
     return EXC_OK;
 }
 
+template <int NM, int S>
+void launch_fold_pull(Ctx* c, hipStream_t st) {
+    FoldArgs f{c->n, c->d_idx, c->d_p, c->d_counters + 4 * C_FIELDS + 0};
+    PullArgs pa{c->n, c->d_tidx, c->d_tadj, c->d_p};
+    const unsigned grid = (unsigned)(((long long)c->n + BLOCK - 1) / BLOCK);
+    {
+        ScopedTimer t(c, st, T_FOLD);
+        hipLaunchKernelGGL((k_fold<NM, S>), dim3(grid), dim3(BLOCK), 0, st, f, c->DM);
+    }
+    {
+        ScopedTimer t(c, st, T_PULL);
+        hipLaunchKernelGGL((k_pull<NM, S>), dim3(grid), dim3(BLOCK), 0, st, pa, c->DM);
+        if (c->n_pull_hubs)
+            hipLaunchKernelGGL((k_pull_hub<NM, S>), dim3((unsigned)std::min(c->n_pull_hubs, c->cus * 4)), dim3(BLOCK), 0, st,
+                               pa, c->d_pull_hubs, c->n_pull_hubs, c->DM);
+    }
+}
+
+int fold_pull(Ctx* c) {
+    hipStream_t st = c->st[0];
+    if (c->nm == 1) launch_fold_pull<1, 1>(c, st);
+    else if (c->nm == 2) launch_fold_pull<2, 2>(c, st);
+    else launch_fold_pull<3, 4>(c, st);
+    c->pull_passes++;
+    HIPCHK(hipGetLastError());
+    return EXC_OK;
+}
+
 // Range or bitmap push of vertices [v0, v1) whose edges sit at edges[idx[v] - ebase].
 int push_chunk(Ctx* c, int s, int v0, int v1, const int32_t* edges, uint64_t ebase, int use_list,
-               int group, int hot_skip, uint64_t edges_bound) {
+               int group, int hot_skip, uint64_t edges_bound, const float* wts = nullptr) {
     if (v1 <= v0) return EXC_OK;
     PushArgs a{};
     a.v0 = v0; a.v1 = v1;
     a.idx = c->d_idx;
     a.edges = edges;
+    a.wts = wts;
     a.ebase = ebase;
     if (c->K > 0) {   // batched: one wave per 64 vertices
         const long long waves = ((long long)v1 - ((long long)v0 & ~63ll) + WAVE - 1) / WAVE;
@@ -1285,6 +1489,8 @@ int exc_destroy(void* p) {
     for (auto& pb : c->pb) { if (pb.d) hipFree(pb.d); if (pb.cnt) hipFree(pb.cnt); }
     for (auto s : c->st) if (s) hipStreamDestroy(s);
     for (auto q : c->prows) if (q) (void)hipFree(q);
+    void* pull_ptrs[] = {c->d_tidx, c->d_tadj, c->d_p, c->d_pull_hubs, c->d_wts};
+    for (void* q : pull_ptrs) if (q) (void)hipFree(q);
     if (c->d_beps) (void)hipFree(c->d_beps);
     if (c->d_colpend) (void)hipFree(c->d_colpend);
     void* dev_ptrs[] = {c->d_idx, c->d_x, c->d_r, c->d_x0, c->d_r0, c->d_bitmap, c->d_cnt, c->d_cv0, c->d_trace, c->d_counters,
@@ -1434,6 +1640,8 @@ int exc_snapshot(void* p, int hot_skip, uint64_t* h_cnt, uint64_t* h_hotpend) {
     HIPCHK(hipStreamSynchronize(st));
     memcpy(h_cnt, c->h_small, 2 * (size_t)c->nc * sizeof(uint64_t));
     if (h_hotpend) *h_hotpend = c->h_small[2 * c->nc];
+    c->last_demand = 0;
+    for (int i = 0; i < c->nc; i++) c->last_demand += c->h_small[2 * i + 1];
     return EXC_OK;
 }
 
@@ -1555,7 +1763,10 @@ int exc_push_resident(void* p, const exc_chunk* ch, int use_list, int group, int
     if (!c || !ch || ch->v0 < 0 || ch->v1 > c->n || ch->v0 > ch->v1) return fail(EXC_EINVAL, "exc_push_resident: bad chunk");
     if (!c->d_arena || ch->e1 > c->arena_edges) return fail(EXC_ESTATE, "exc_push_resident: chunk not in the edge cache");
     HIPCHK(hipSetDevice(c->dev));
-    return push_chunk(c, 0, ch->v0, ch->v1, c->d_arena, 0, use_list, group, hot_skip, c->arena_edges);
+    if (c->d_tadj && !c->K && !c->d_wts && ch->v0 == 0 && ch->v1 == c->n && !hot_skip &&
+        (double)c->last_demand >= c->pull_threshold * (double)c->m)
+        return fold_pull(c);
+    return push_chunk(c, 0, ch->v0, ch->v1, c->d_arena, 0, use_list, group, hot_skip, c->arena_edges, c->d_wts);
 }
 
 int exc_stream_chunk(void* p, int s, const exc_chunk* ch, int to_resident, int hot_lo, int hot_hi,
@@ -1564,6 +1775,7 @@ int exc_stream_chunk(void* p, int s, const exc_chunk* ch, int to_resident, int h
     if (!c || s < 0 || s > 1 || !ch || ch->v0 < 0 || ch->v1 > c->n || ch->e1 < ch->e0)
         return fail(EXC_EINVAL, "exc_stream_chunk: bad arguments");
     const uint64_t ne = ch->e1 - ch->e0;
+    if (c->d_wts) return fail(EXC_ESTATE, "exc_stream_chunk: weighted graphs must be fully cached");
     if (ne > c->max_chunk_edges) return fail(EXC_EINVAL, "exc_stream_chunk: chunk larger than the staging slot");
     if (to_resident && (!c->d_arena || ch->e1 > c->arena_edges)) return fail(EXC_ESTATE, "exc_stream_chunk: chunk not in the edge cache");
     HIPCHK(hipSetDevice(c->dev));
@@ -1703,6 +1915,17 @@ int exc_batch_set_active(void* p, uint64_t mask) {
     return EXC_OK;
 }
 
+int exc_set_edge_weights(void* p, const float* h_w, uint64_t cnt) {
+    Ctx* c = as_ctx(p);
+    if (!c || c->K || !h_w) return fail(EXC_EINVAL, "exc_set_edge_weights: bad arguments");
+    if (!c->d_arena || c->arena_edges != c->m || cnt != c->m)
+        return fail(EXC_ESTATE, "exc_set_edge_weights: the whole graph must be in the edge cache and cnt == m");
+    HIPCHK(hipSetDevice(c->dev));
+    if (!c->d_wts) HIPCHK(hipMalloc(&c->d_wts, (c->m ? c->m : 1) * sizeof(float)));
+    if (c->m) HIPCHK(hipMemcpy(c->d_wts, h_w, c->m * sizeof(float), hipMemcpyHostToDevice));
+    return EXC_OK;
+}
+
 int exc_pass_sync(void* p, double* waited) {
     Ctx* c = as_ctx(p);
     if (!c) return fail(EXC_EINVAL, "exc_pass_sync: null context");
@@ -1722,7 +1945,7 @@ int exc_pushed(void* p, uint64_t* edges) {
     unsigned long long h[C_NCOUNTERS];
     HIPCHK(hipMemcpy(h, c->d_counters, sizeof h, hipMemcpyDeviceToHost));
     uint64_t e = 0;
-    for (int s = 0; s < 4; s++) e += h[s * C_FIELDS + C_EDGES] + h[s * C_FIELDS + C_PEDGES];
+    for (int s = 0; s < 5; s++) e += h[s * C_FIELDS + C_EDGES] + h[s * C_FIELDS + C_PEDGES];
     *edges = e;
     if (h[C_OVERFLOW]) return fail(EXC_ESTATE, "hub piece buffer overflowed %llu times", h[C_OVERFLOW]);
     return EXC_OK;
@@ -1756,7 +1979,78 @@ int exc_get_stats(void* p, exc_stats* o) {
         o->pushed_edges += r[C_EDGES] + r[C_PEDGES];
         o->pushed_edge_measures += r[C_EDGES_NM] + r[C_PEDGES_NM];
     }
+    {   // dense passes: fold (row 4) and pull
+        const unsigned long long* r = h + 4 * C_FIELDS;
+        const int S = c->S, nm = c->K ? c->K : c->nm;
+        o->fold_ms = c->t_ms[T_FOLD];
+        o->pull_ms = c->t_ms[T_PULL];
+        o->pull_passes = c->pull_passes;
+        // fold: read r and write p for every vertex; offsets, x read + write for folded ones
+        o->fold_bytes = (double)c->pull_passes * c->n * (4.0 * nm + 4.0 * S) + 16.0 * r[C_VERT] + 8.0 * r[C_VERT_NM];
+        // pull: every in-edge (id + source p of nm floats), every vertex (offsets, r read + write)
+        o->pull_bytes = (double)c->pull_passes * ((double)c->m * (4.0 + 4.0 * nm) + (double)c->n * (8.0 + 8.0 * nm));
+        o->pushed_vertices += r[C_VERT];
+        o->pushed_vertex_measures += r[C_VERT_NM];
+        o->pushed_edges += r[C_EDGES];
+        o->pushed_edge_measures += r[C_EDGES_NM];
+    }
     if (h[C_OVERFLOW]) return fail(EXC_ESTATE, "hub piece buffer overflowed %llu times", h[C_OVERFLOW]);
+    return EXC_OK;
+}
+
+int exc_enable_pull(void* p, double threshold) {
+    Ctx* c = as_ctx(p);
+    if (!c || c->K) return fail(EXC_EINVAL, "exc_enable_pull: plain contexts only");
+    if (!c->d_arena || c->arena_edges != c->m) return fail(EXC_ESTATE, "exc_enable_pull: the whole graph must be in the edge cache");
+    HIPCHK(hipSetDevice(c->dev));
+    HIPCHK(hipDeviceSynchronize());
+    c->pull_threshold = threshold;
+    if (c->d_tadj) return EXC_OK;
+    const size_t n = (size_t)c->n;
+    unsigned long long* cnt = nullptr;
+    bool ok = hipMalloc(&c->d_tidx, (n + 1) * sizeof(uint64_t)) == hipSuccess;
+    ok = ok && hipMalloc(&c->d_tadj, (c->m ? c->m : 1) * sizeof(int32_t)) == hipSuccess;
+    ok = ok && hipMalloc(&c->d_p, n * c->S * sizeof(float)) == hipSuccess;
+    ok = ok && hipMalloc(&cnt, (n + 1) * sizeof(unsigned long long)) == hipSuccess;
+    if (!ok) {
+        (void)hipGetLastError();
+        void* q[] = {c->d_tidx, c->d_tadj, c->d_p, cnt};
+        for (void* x : q) if (x) (void)hipFree(x);
+        c->d_tidx = nullptr; c->d_tadj = nullptr; c->d_p = nullptr;
+        c->pull_threshold = -1;
+        return fail(EXC_ENOMEM, "exc_enable_pull: cannot allocate the transpose (%.2f GB)", c->m * 4.0 / 1e9);
+    }
+    auto fail_free = [&](hipError_t e) {
+        (void)hipFree(cnt);
+        return fail(EXC_EHIP, "exc_enable_pull: %s", hipGetErrorString(e));
+    };
+    hipError_t e = hipMemset(cnt, 0, (n + 1) * sizeof(unsigned long long));
+    if (e != hipSuccess) return fail_free(e);
+    hipLaunchKernelGGL(k_count_in64, dim3(c->cus * 16), dim3(BLOCK), 0, 0, c->m, c->d_arena, cnt);
+    std::vector<uint64_t> h(n + 1);
+    e = hipMemcpy(h.data(), cnt, n * sizeof(uint64_t), hipMemcpyDeviceToHost);
+    if (e != hipSuccess) return fail_free(e);
+    std::vector<int32_t> hubs;
+    uint64_t acc = 0;
+    for (size_t v = 0; v < n; v++) {
+        const uint64_t d = h[v];
+        if (d > PULL_HUB) hubs.push_back((int32_t)v);
+        h[v] = acc;
+        acc += d;
+    }
+    h[n] = acc;
+    e = hipMemcpy(c->d_tidx, h.data(), (n + 1) * sizeof(uint64_t), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(cnt, h.data(), (n + 1) * sizeof(uint64_t), hipMemcpyHostToDevice);
+    if (e != hipSuccess) return fail_free(e);
+    hipLaunchKernelGGL(k_fill_in64, dim3(c->cus * 16), dim3(BLOCK), 0, 0, c->n, c->d_idx, c->d_arena, cnt, c->d_tadj);
+    e = hipDeviceSynchronize();
+    if (e != hipSuccess) return fail_free(e);
+    (void)hipFree(cnt);
+    c->n_pull_hubs = (int)hubs.size();
+    if (c->n_pull_hubs) {
+        HIPCHK(hipMalloc(&c->d_pull_hubs, hubs.size() * sizeof(int32_t)));
+        HIPCHK(hipMemcpy(c->d_pull_hubs, hubs.data(), hubs.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+    }
     return EXC_OK;
 }
 
@@ -1770,6 +2064,7 @@ int exc_reset_stats(void* p) {
     HIPCHK(hipMemset(c->d_counters, 0, C_NCOUNTERS * sizeof(unsigned long long)));
     for (int i = 0; i < T_NCLS; i++) { c->t_ms[i] = 0; c->t_launch[i] = 0; }
     c->snap_bytes = 0;
+    c->pull_passes = 0;
     return EXC_OK;
 }
 

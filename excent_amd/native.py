@@ -65,7 +65,9 @@ class Stats(ctypes.Structure):
                 ("other_ms", c_double), ("other_launches", c_u64),
                 ("pushed_vertices", c_u64), ("pushed_edges", c_u64),
                 ("pushed_edge_measures", c_u64), ("pushed_vertex_measures", c_u64),
-                ("src_ms", c_double * 4), ("src_launches", c_u64 * 4), ("src_bytes", c_double * 4)]
+                ("src_ms", c_double * 4), ("src_launches", c_u64 * 4), ("src_bytes", c_double * 4),
+                ("fold_ms", c_double), ("pull_ms", c_double), ("pull_passes", c_u64), ("fold_bytes", c_double),
+                ("pull_bytes", c_double)]
 
 
 class RunStats(ctypes.Structure):
@@ -117,6 +119,8 @@ HIP_SIGNATURES = {
     "exc_slot_buffers": (c_int, [c_void_p, c_int, P(c_void_p), P(c_void_p), P(c_void_p)]),
     "exc_slot_acquire": (c_int, [c_void_p, c_int, P(c_double)]),
     "exc_slot_copy_probe": (c_int, [c_void_p, c_int, c_u64, P(c_double)]),
+    "exc_enable_pull": (c_int, [c_void_p, c_double]),
+    "exc_set_edge_weights": (c_int, [c_void_p, c_void_p, c_u64]),
     "exc_push_resident": (c_int, [c_void_p, P(Chunk), c_int, c_int, c_int]),
     "exc_stream_chunk": (c_int, [c_void_p, c_int, P(Chunk), c_int, c_int, c_int, c_int, c_int, c_int]),
     "exc_gather_chunk": (c_int, [c_void_p, c_int, c_int, c_int]),
@@ -150,6 +154,12 @@ HOST_SIGNATURES = {
     "excent_session_results": (c_int, [c_void_p]),
     "excent_session_close": (c_int, [c_void_p]),
     "excent_session_error": (ctypes.c_char_p, []),
+    "excent_dpr_create": (c_int, [c_int, c_int, c_u64, c_void_p, c_void_p, c_void_p, c_float, c_float, P(c_void_p)]),
+    "excent_dpr_forward": (c_int, [c_void_p, c_void_p, c_void_p]),
+    "excent_dpr_backward": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "excent_dpr_passes": (c_int, [c_void_p, P(c_int), P(c_int)]),
+    "excent_dpr_destroy": (c_int, [c_void_p]),
+    "excent_dpr_error": (ctypes.c_char_p, []),
     "excent_last_error": (ctypes.c_char_p, []),
     "excent_chunk_cap": (c_u64, [c_u64, c_double, c_double, c_double]),
     "excent_plan_chunks": (c_int, [ctypes.c_char_p, c_u64, c_void_p, c_void_p, c_void_p, c_void_p, c_int]),
@@ -319,6 +329,9 @@ class Context:
         ch = Chunk(v0, v1, e0, e1)
         check(F["exc_push_resident"](self.ctx, ctypes.byref(ch), int(use_list), group, int(hot_skip)), "push_resident")
 
+    def enable_pull(self, threshold):
+        return F["exc_enable_pull"](self.ctx, threshold)
+
     def push_hot(self, lo, hi, group=0, after_slots=False):
         check(F["exc_push_hot"](self.ctx, lo, hi, group, int(after_slots)), "push_hot")
 
@@ -410,3 +423,52 @@ def load_egr(path):
     if m.value and F["excent_egr_edges"](path.encode(), 0, m.value, ptr(adj)) != 0:
         raise ExcError(-1, F["excent_last_error"]().decode())
     return idx, adj[: m.value]
+
+
+class DiffPageRank:
+    """Differentiable personalised PageRank through include/excent.h excent_dpr_*."""
+
+    def __init__(self, idx, adj, w_raw, damp=0.85, eps=1e-7, device=0):
+        self.idx = np.ascontiguousarray(idx, np.uint64)
+        self.adj = np.ascontiguousarray(adj, np.int32)
+        self.w = np.ascontiguousarray(w_raw, np.float32)
+        self.n, self.m = len(self.idx) - 1, len(self.adj)
+        self.h = c_void_p()
+        rc = F["excent_dpr_create"](device, self.n, self.m, ptr(self.idx), ptr(self.adj), ptr(self.w), damp, eps,
+                                    ctypes.byref(self.h))
+        if rc != 0:
+            raise ExcError(rc, F["excent_dpr_error"]().decode())
+
+    def forward(self, s):
+        s = np.ascontiguousarray(s, np.float32)
+        x = np.zeros(self.n, np.float32)
+        rc = F["excent_dpr_forward"](self.h, ptr(s), ptr(x))
+        if rc != 0:
+            raise ExcError(rc, F["excent_dpr_error"]().decode())
+        return x
+
+    def backward(self, g):
+        g = np.ascontiguousarray(g, np.float32)
+        gs = np.zeros(self.n, np.float32)
+        gw = np.zeros(self.m, np.float32)
+        gwn = np.zeros(self.m, np.float32)
+        rc = F["excent_dpr_backward"](self.h, ptr(g), ptr(gs), ptr(gw), ptr(gwn))
+        if rc != 0:
+            raise ExcError(rc, F["excent_dpr_error"]().decode())
+        return gs, gw, gwn
+
+    def passes(self):
+        a, b = c_int(0), c_int(0)
+        F["excent_dpr_passes"](self.h, ctypes.byref(a), ctypes.byref(b))
+        return a.value, b.value
+
+    def close(self):
+        if self.h:
+            F["excent_dpr_destroy"](self.h)
+            self.h = c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()

@@ -50,6 +50,20 @@ int excent_reference_main(int argc, const char* const* argv);
 /* bin/excent_params <graph.egr> <out.env> (tools/excent_params.cu:73-174). */
 int excent_params_main(int argc, const char* const* argv);
 
+/* Differentiable personalised PageRank (no reference counterpart).  x = (1-d) s + d A x with
+ * A[u][v] = w_vu / S_v (raw positive weights w in CSR order, S_v = v's out-weight sum).
+ * forward: the weighted delta push over the CSR from r = (1-d) s; backward: for
+ * L = sum g_v x_v, the adjoint delta push over the transpose, then dL/ds = (1-d) lambda,
+ * dL/dw (raw weights, CSR order) and dL/d(w/S) (normalised weights); any output may be
+ * NULL.  Both pushes stop when every |r| <= eps. */
+int excent_dpr_create(int device, int n, uint64_t m, const uint64_t* idx, const int32_t* adj, const float* w_raw,
+                      float damp, float eps, void** handle);
+int excent_dpr_forward(void* handle, const float* s, float* x_out);
+int excent_dpr_backward(void* handle, const float* g, float* grad_s, float* grad_w, float* grad_wn);
+int excent_dpr_passes(void* handle, int* forward, int* backward);
+int excent_dpr_destroy(void* handle);
+const char* excent_dpr_error(void);
+
 /* Last error text of the excent_* helpers below on this thread. */
 const char* excent_last_error(void);
 

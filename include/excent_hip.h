@@ -86,6 +86,10 @@ typedef struct exc_stats {
     double src_ms[4];
     uint64_t src_launches[4];
     double src_bytes[4];
+    /* dense passes run as fold + pull (exc_enable_pull) */
+    double fold_ms, pull_ms;
+    uint64_t pull_passes;
+    double fold_bytes, pull_bytes;
 } exc_stats;
 
 const char* exc_last_error(void);
@@ -146,6 +150,19 @@ int exc_slot_acquire(void* ctx, int slot, double* waited_s);
  * buffer.  tau calibration adds this PCIe leg to the storage timings, which the reference
  * measures storage-only (src/excent.cu:534-576). */
 int exc_slot_copy_probe(void* ctx, int slot, uint64_t bytes, double* seconds);
+
+/* Weighted graphs (no reference counterpart): per-edge coefficients c_e aligned with the
+ * cached edges (the whole graph must be cached).  A push then adds p * c_e along edge e
+ * with p = scale * d (set divdeg = 0 when the coefficients are already normalised). */
+int exc_set_edge_weights(void* ctx, const float* h_w, uint64_t cnt);
+
+/* Dense passes by pull (no reference counterpart): builds the transpose of the cached
+ * graph on the device (the whole graph must be in the edge cache) and a weight buffer.
+ * Afterwards exc_push_resident over all vertices runs as fold + pull when the last
+ * snapshot's demand is at least threshold * m: every pending vertex folds its residual
+ * and publishes its weight, then every vertex adds the weights of its in-neighbours --
+ * the same deltas as the push, all folds before all adds.  EXC_ENOMEM leaves the push. */
+int exc_enable_pull(void* ctx, double threshold);
 
 /* Push a resident chunk run [ch.v0, ch.v1) from the arena on the resident stream
  * (launch_chunk with dst = resident[c], src/excent.cu:634-652, 684-688).  use_list:

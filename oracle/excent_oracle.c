@@ -511,6 +511,44 @@ int orc_fp64_solve_b(int n, const uint64_t* idx, const int32_t* adj, const doubl
     return it;
 }
 
+int orc_fp64_solve_w(int n, const uint64_t* idx, const int32_t* adj, const double* w, const double* b, double s,
+                     int max_iter, double tol, double* out, double* rel_out) {
+    uint64_t m = idx[n];
+    uint64_t* tidx = (uint64_t*)calloc((size_t)n + 1, 8);
+    int32_t* tsrc = (int32_t*)malloc((m ? m : 1) * 4);
+    double* tw = (double*)malloc((m ? m : 1) * 8);
+    for (uint64_t e = 0; e < m; e++) tidx[adj[e] + 1]++;
+    for (int v = 0; v < n; v++) tidx[v + 1] += tidx[v];
+    uint64_t* pos = (uint64_t*)malloc(((size_t)n + 1) * 8);
+    memcpy(pos, tidx, ((size_t)n + 1) * 8);
+    for (int v = 0; v < n; v++)
+        for (uint64_t e = idx[v]; e < idx[v + 1]; e++) { uint64_t q = pos[adj[e]]++; tsrc[q] = v; tw[q] = w[e]; }
+    free(pos);
+    double* x = (double*)malloc((size_t)n * 8);
+    double* y = (double*)malloc((size_t)n * 8);
+    memcpy(x, b, (size_t)n * 8);
+    int it = 0;
+    double rel = 1;
+    for (; it < max_iter && rel > tol; it++) {
+        double a = 0, mx = 0;
+        #pragma omp parallel for schedule(dynamic, 1024) reduction(max : a, mx)
+        for (int u = 0; u < n; u++) {
+            double acc = 0;
+            for (uint64_t e = tidx[u]; e < tidx[u + 1]; e++) acc += tw[e] * x[tsrc[e]];
+            double nv = b[u] + s * acc;
+            y[u] = nv;
+            if (fabs(nv - x[u]) > a) a = fabs(nv - x[u]);
+            if (fabs(nv) > mx) mx = fabs(nv);
+        }
+        rel = mx > 0 ? a / mx : 0;
+        double* t = x; x = y; y = t;
+    }
+    memcpy(out, x, (size_t)n * 8);
+    if (rel_out) *rel_out = rel;
+    free(x); free(y); free(tidx); free(tsrc); free(tw);
+    return it;
+}
+
 /* tools/excent_params.cu:105-163, with the power iteration on the in-neighbour pull. */
 int orc_params(int n, const uint64_t* idx, const int32_t* adj, const double* start, double afac,
                double* lambda, double* alpha, double* mean_katz, double* mean_ev, long* symmetric_bad) {

@@ -94,6 +94,13 @@ int orc_fp64_reference(int n, const uint64_t* idx, const int32_t* adj, int kind,
 int orc_fp64_solve_b(int n, const uint64_t* idx, const int32_t* adj, const double* b, double s, int divdeg,
                      int max_iter, double tol, double* out, double* rel);
 
+/* Weighted FP64 Jacobi: y[u] = b[u] + s * sum over edges e = (v -> u) of w[e] x[v], the
+ * weights aligned with adj (CSR order).  The differentiable personalised PageRank's
+ * forward is (idx, adj, w/S); its adjoint is the same call on the transpose with the
+ * weights carried along.  No reference counterpart. */
+int orc_fp64_solve_w(int n, const uint64_t* idx, const int32_t* adj, const double* w, const double* b, double s,
+                     int max_iter, double tol, double* out, double* rel);
+
 /* ---- constants (tools/excent_params.cu:105-170) ----
  * lambda1 by power iteration on the in-neighbour pull from `start`
  * (NULL = all ones, the reference's start), Katz mean with alpha = afac/lambda1,

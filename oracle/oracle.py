@@ -54,6 +54,8 @@ def lib():
                                            c_void_p, P(c_double), P(c_double)]),
             "orc_fp64_solve_b": (c_int, [c_int, c_void_p, c_void_p, c_void_p, c_double, c_int, c_int, c_double,
                                          c_void_p, P(c_double)]),
+            "orc_fp64_solve_w": (c_int, [c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_double, c_int, c_double,
+                                         c_void_p, P(c_double)]),
             "orc_params": (c_int, [c_int, c_void_p, c_void_p, c_void_p, c_double, P(c_double), P(c_double),
                                    P(c_double), P(c_double), P(ctypes.c_long)]),
             "orc_compare": (None, [c_int, c_void_p, c_void_p, c_int, P(c_double), P(c_double), P(c_double), P(c_int)]),
@@ -220,6 +222,16 @@ def fp64_solve_b(idx, adj, b, s=0.85, divdeg=1, max_iter=20000, tol=1e-12):
     it = lib().orc_fp64_solve_b(n, _p(np.ascontiguousarray(idx, np.uint64)), _p(np.ascontiguousarray(adj, np.int32)),
                                 _p(np.ascontiguousarray(b, np.float64)), s, divdeg, max_iter, tol, _p(out),
                                 ctypes.byref(rel))
+    return out, it, rel.value
+
+
+def fp64_solve_w(idx, adj, w, b, s=0.85, max_iter=100000, tol=1e-14):
+    n = len(idx) - 1
+    out = np.zeros(n, np.float64)
+    rel = c_double(0)
+    it = lib().orc_fp64_solve_w(n, _p(np.ascontiguousarray(idx, np.uint64)), _p(np.ascontiguousarray(adj, np.int32)),
+                                _p(np.ascontiguousarray(w, np.float64)), _p(np.ascontiguousarray(b, np.float64)), s,
+                                max_iter, tol, _p(out), ctypes.byref(rel))
     return out, it, rel.value
 
 

@@ -19,7 +19,7 @@ pytestmark = pytest.mark.gpu
 
 
 def run_passes(idx, adj, kinds, eps, scale, divdeg, inits, cv0=None, kernel=nat.KERNEL_LB, group=0,
-               list_frac=0.5, max_passes=5000):
+               list_frac=0.5, max_passes=5000, pull=None):
     """Resident pass loop: snapshot, per chunk ids-or-range push from the edge cache."""
     n = len(idx) - 1
     m = len(adj)
@@ -33,6 +33,10 @@ def run_passes(idx, adj, kinds, eps, scale, divdeg, inits, cv0=None, kernel=nat.
         if m:
             assert ctx.cache_chunks(m) == 0
             ctx.cache_upload(adj, 0)
+        if pull is not None:
+            assert ctx.enable_pull(pull) == 0
+            cv0 = np.array([0], np.int32)   # one run over all vertices so the pull path is taken
+            bounds = np.append(cv0, n)
         passes = 0
         for passes in range(max_passes):
             cnt, _ = ctx.snapshot()
@@ -51,6 +55,8 @@ def run_passes(idx, adj, kinds, eps, scale, divdeg, inits, cv0=None, kernel=nat.
         xs = [ctx.read_state(i, 0) for i in range(nm)]
         rs = [ctx.read_state(i, 1) for i in range(nm)]
         pushed = ctx.pushed()
+        if pull is not None:
+            assert ctx.stats()["pull_passes"] > 0
     return xs, rs, passes, pushed
 
 
@@ -151,3 +157,43 @@ def test_star_hub_pieces(kernel, group):
     assert err <= true_l1 / 0.15 * (1 + 1e-6)
     assert err / ref.sum() < 2e-3
     assert pushed >= 2 * L
+
+
+@pytest.mark.parametrize("threshold", [0.0, 0.2])
+@pytest.mark.parametrize("gname", ["sample", "rmat12", "star"])
+def test_fold_pull_dense_passes(threshold, gname, rmat12):
+    """Dense passes as fold + pull over the device transpose: same bounds and invariant."""
+    if gname == "star":
+        L = 9000   # in-degree of the hub above the pull hub threshold
+        deg = np.concatenate([[L], np.ones(L, np.int64)])
+        idx = np.concatenate([[0], np.cumsum(deg)]).astype(np.uint64)
+        adj = np.concatenate([np.arange(1, L + 1), np.zeros(L, np.int64)]).astype(np.int32)
+    else:
+        idx, adj = orc.egr_load(SAMPLE_EGR if gname == "sample" else rmat12)
+    n = len(idx) - 1
+    x0, r0 = orc.init(idx, orc.ORC_PR)
+    xs, rs, passes, pushed = run_passes(idx, adj, [nat.EXC_PR], [1e-3], [0.85], [1], [(x0, r0)], pull=threshold)
+    b = np.full(n, np.float64(np.float32(1) - np.float32(0.85)))
+    ref, _, _ = orc.fp64_solve_b(idx, adj, b)
+    err = np.abs(xs[0].astype(np.float64) - ref).sum()
+    true_l1 = np.abs(true_residual(idx, adj, xs[0], b, 0.85, True)).sum()
+    assert err <= true_l1 / 0.15 * (1 + 1e-6) + 1e-6
+    assert np.all(np.abs(rs[0]) <= 1e-3)
+    tr = true_residual(idx, adj, xs[0], b, 0.85, True)
+    assert np.abs(tr - rs[0]).max() < 2e-4 * max(1.0, np.abs(xs[0]).max())
+    assert pushed >= len(adj)
+
+
+def test_fold_pull_all_measures():
+    from tests.conftest import SAMPLE_ENV, read_env
+    env = read_env(SAMPLE_ENV)
+    idx, adj = orc.egr_load(SAMPLE_EGR)
+    lam, alpha = env["LAMBDA"], env["ALPHA"]
+    eps = [0.001, env["EPS_KATZ"] / 10, env["EPS_EV"] / 10]
+    scale = [np.float32(0.85), np.float32(alpha), np.float32(1.0 / lam)]
+    inits = [orc.init(idx, orc.ORC_PR), orc.init(idx, orc.ORC_KATZ, 1.0), orc.init(idx, orc.ORC_EV, lam=np.float32(lam))]
+    xs, rs, _, _ = run_passes(idx, adj, [nat.EXC_PR, nat.EXC_KATZ, nat.EXC_EV], eps, scale, [1, 0, 0], inits, pull=0.0)
+    for i, (bv, w, dd) in enumerate([(0.15, 0.85, True), (1.0, float(scale[1]), False), (0.0, float(scale[2]), False)]):
+        tr = true_residual(idx, adj, xs[i], bv, w, dd)
+        assert np.abs(tr - rs[i]).max() < 1e-3 * max(1.0, np.abs(xs[i]).max()), i
+    assert orc.compare(orc.fp64_reference(idx, adj, orc.ORC_KATZ, alpha=alpha)[0], xs[1])["l1_rel"] < 0.01
