@@ -55,6 +55,7 @@ def main():
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--measure", default="all")
     ap.add_argument("--kernel", default="lb", choices=["lb", "group"])
+    ap.add_argument("--pull", default="auto", help="dense passes by pull: auto | off | demand fraction")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--workdir", default=None)
@@ -90,7 +91,7 @@ def main():
 
     excent_log = os.path.join(work, "excent.log")
     sargs = [graph, "--measure", args.measure, "--cache", "chunks", "--preload", "--stream-ratio", "1",
-             "--timing", "--push-kernel", args.kernel, "--log", excent_log]
+             "--timing", "--push-kernel", args.kernel, "--pull", args.pull, "--log", excent_log]
     if args.measure in ("katz", "all"):
         sargs += ["--alpha", env["ALPHA"], "--beta", 1, "--eps-katz", "1e-6"]
     if args.measure in ("ev", "all"):
@@ -117,27 +118,32 @@ def main():
         log(f"step {i}: {r['passes']} passes, {r['time_passes_s']:.3f}s, {r['pushed_edges'] / 1e9:.3f} G edges")
     pushed = sum(r["pushed_edges"] for r in runs)
     gteps = pushed / elapsed / 1e9
-    # dominant push kernel (by total time) among the vertex sources
+    # dominant kernel (by total time): each entry is (name, total ms, launches, algorithmic bytes)
+    K = lambda key: sum(r["kernel"][key] for r in runs)
     names = ["range", "bitmap", "list", "hot"]
-    tot_ms = [sum(r["kernel"]["src_ms"][i] for r in runs) for i in range(4)]
-    tot_launch = [sum(r["kernel"]["src_launches"][i] for r in runs) for i in range(4)]
-    tot_bytes = [sum(r["kernel"]["src_bytes"][i] for r in runs) for i in range(4)]
-    dom = int(np.argmax(tot_ms))
-    achieved = tot_bytes[dom] / (tot_ms[dom] / 1e3) / 1e9 if tot_ms[dom] > 0 else 0.0
+    tag = "3,4" if args.measure == "all" else ("1,1" if args.measure in ("pr", "katz", "ev") else "")
+    cands = [(f"k_push<{tag},{i},0> ({names[i]} source)", sum(r["kernel"]["src_ms"][i] for r in runs),
+              sum(r["kernel"]["src_launches"][i] for r in runs), sum(r["kernel"]["src_bytes"][i] for r in runs))
+             for i in range(4)]
+    npull = K("pull_passes")
+    cands += [(f"k_fold<{tag}>", K("fold_ms"), npull, K("fold_bytes")),
+              (f"k_pull<{tag}>", K("pull_ms"), npull, K("pull_bytes")),
+              (f"k_pull_hub<{tag}>", K("pull_hub_ms"), npull, K("pull_hub_bytes")),
+              (f"k_snapshot<{tag}>", K("snapshot_ms"), K("snapshot_launches"), K("snapshot_bytes")),
+              (f"k_pieces<{tag}>", K("piece_ms"), K("piece_launches"), K("piece_bytes"))]
+    dom = max(cands, key=lambda c: c[1])
+    achieved = dom[3] / (dom[1] / 1e3) / 1e9 if dom[1] > 0 else 0.0
     traffic = None
     if os.path.exists(args.traffic):
         try:
-            t = json.load(open(args.traffic))
-            traffic = t.get(names[dom])
+            traffic = json.load(open(args.traffic)).get(dom[0].split("<")[0])
         except Exception:
             traffic = None
-    kern = {k: sum(r["kernel"][k] for r in runs) for k in ("snapshot_ms", "push_ms", "piece_ms", "other_ms",
-                                                          "snapshot_bytes", "push_bytes", "piece_bytes")}
+    kern = {c[0]: round(c[1] / len(runs), 3) for c in cands if c[2]}
     roof = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-            "kernel": f"k_push<3,4,{dom},0> ({names[dom]} source)" if args.measure == "all" else f"k_push ({names[dom]})",
-            "launches": tot_launch[dom], "avg_launch_ms": round(tot_ms[dom] / max(1, tot_launch[dom]), 5),
-            "algorithmic_bytes_per_launch": round(tot_bytes[dom] / max(1, tot_launch[dom]), 1)}
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": dom[0],
+            "launches": dom[2], "avg_launch_ms": round(dom[1] / max(1, dom[2]), 5),
+            "algorithmic_bytes_per_launch": round(dom[3] / max(1, dom[2]), 1)}
 
     cpu = None
     if not args.no_cpu:
@@ -184,11 +190,11 @@ def main():
         "config": {"workload": f"R-MAT scale {args.scale} ef {args.edge_factor} --measure {args.measure}, "
                                f"whole graph in HBM, converged to eps", "n": n_v, "m": m_e,
                    "lambda1": env["LAMBDA"], "alpha": env["ALPHA"], "eps_pr": 0.01, "eps_katz": 1e-6,
-                   "eps_ev": 1e-6, "push_kernel": args.kernel, "parallelism": "single GPU"},
+                   "eps_ev": 1e-6, "push_kernel": args.kernel, "pull": args.pull, "parallelism": "single GPU"},
         "time_to_eps_s": round(sum(r["time_passes_s"] for r in runs) / len(runs), 4),
         "passes": [r["passes"] for r in runs],
         "pushed_edges_per_run": runs[-1]["pushed_edges"],
-        "kernel_ms": {k: round(v / len(runs), 3) for k, v in kern.items() if k.endswith("_ms")},
+        "kernel_ms_per_run": kern,
         "roofline": roof,
         "cpu_baseline": cpu,
     }

@@ -87,6 +87,14 @@ __device__ __forceinline__ float atomic_exch_f32(float* p, float v) {
 
 __device__ __forceinline__ int lane_id() { return threadIdx.x & (WAVE - 1); }
 
+// Piece counters: one 256-byte line per counter (count, p-row count), two phases, so a
+// counter being cleared never shares a line with one being incremented.
+constexpr int CNT_STRIDE = 64;   // uint32 slots between counters
+__device__ __forceinline__ void clear_counters(uint32_t* c) {
+    __hip_atomic_exchange(c, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_exchange(c + CNT_STRIDE, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 // Exclusive block scan of one uint32 per thread; *total gets the block sum.
 __device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t* s_wave, uint32_t* total) {
     const int lane = lane_id(), wid = threadIdx.x / WAVE;
@@ -460,7 +468,7 @@ __global__ __launch_bounds__(BLOCK) void k_push(PushArgs a, DevMeasures M) {
     __shared__ unsigned long long s_cnt[6];
     const int tid = threadIdx.x;
     if (tid < 6) s_cnt[tid] = 0;
-    if (blockIdx.x == 0 && tid == 0) { a.piece_zero[0] = 0u; a.piece_zero[1] = 0u; }
+    if (blockIdx.x == 0 && tid == 0) clear_counters(a.piece_zero);
     __syncthreads();
 
     if constexpr (SRC == SRC_RANGE) {
@@ -599,6 +607,15 @@ struct FoldArgs {
     unsigned long long* counters;   // C_FIELDS row
 };
 
+template <int S>
+__device__ __forceinline__ void store_row(float* base, long long v, const Row<S>& r) {
+    if constexpr (S == 1) base[v] = r.v[0];
+    else if constexpr (S == 2) reinterpret_cast<float2*>(base)[v] = make_float2(r.v[0], r.v[1]);
+    else reinterpret_cast<float4*>(base)[v] = make_float4(r.v[0], r.v[1], r.v[2], r.v[3]);
+}
+
+// No other kernel touches r or x while a dense pass runs (it is the only work of the pass,
+// on one stream), so the fold reads and writes whole rows with plain vector accesses.
 template <int NM, int S>
 __global__ __launch_bounds__(BLOCK) void k_fold(FoldArgs a, DevMeasures M) {
     __shared__ unsigned long long s_cnt[4];
@@ -614,20 +631,27 @@ __global__ __launch_bounds__(BLOCK) void k_fold(FoldArgs a, DevMeasures M) {
         Row<S> pr;
 #pragma unroll
         for (int k = 0; k < S; k++) pr.v[k] = 0.f;
+        Row<S> dx;
+#pragma unroll
+        for (int k = 0; k < S; k++) dx.v[k] = 0.f;
 #pragma unroll
         for (int k = 0; k < NM; k++) {
             if (fabsf(row.v[k]) > M.eps[k]) {
-                const float d = atomic_exch_f32(&M.r[v * S + k], 0.f);
-                M.x[v * S + k] += d;
-                if (d != 0.f) {
-                    nmv++;
-                    if (deg) pr.v[k] = M.divdeg[k] ? M.scale[k] * d / (float)deg : M.scale[k] * d;
-                }
+                const float d = row.v[k];
+                row.v[k] = 0.f;
+                dx.v[k] = d;
+                nmv++;
+                if (deg) pr.v[k] = M.divdeg[k] ? M.scale[k] * d / (float)deg : M.scale[k] * d;
             }
         }
-        if constexpr (S == 1) a.p[v] = pr.v[0];
-        else if constexpr (S == 2) reinterpret_cast<float2*>(a.p)[v] = make_float2(pr.v[0], pr.v[1]);
-        else reinterpret_cast<float4*>(a.p)[v] = make_float4(pr.v[0], pr.v[1], pr.v[2], pr.v[3]);
+        if (nmv) {
+            store_row<S>(M.r, v, row);
+            Row<S> xr = load_row<S>(M.x, v);
+#pragma unroll
+            for (int k = 0; k < NM; k++) xr.v[k] += dx.v[k];
+            store_row<S>(M.x, v, xr);
+        }
+        store_row<S>(a.p, v, pr);
     }
     const uint32_t t0 = wave_sum_u32(nmv ? 1u : 0u), t1 = wave_sum_u32(nmv);
     const uint64_t de = nmv ? deg : 0;
@@ -674,17 +698,45 @@ __global__ __launch_bounds__(BLOCK) void k_pull(PullArgs a, DevMeasures M) {
     for (int k = 0; k < NM; k++) s_acc[k][tid] = 0.f;
     __syncthreads();
     int lo = 0;
-    for (uint32_t j = tid; j < total; j += BLOCK) {
+    const int lane = lane_id();
+    const uint32_t rounds = (total + BLOCK - 1) / BLOCK;   // block-uniform trip count
+    for (uint32_t it = 0; it < rounds; it++) {
+        const uint32_t j = it * BLOCK + tid;
+        const bool valid = j < total;
         int hi = BLOCK - 1;
-        while (lo < hi) {
-            const int mid = (lo + hi + 1) >> 1;
-            if (s_start[mid] <= j) lo = mid; else hi = mid - 1;
-        }
-        const int v = a.tadj[s_ptr[lo] + (j - s_start[lo])];
-        const Row<S> pv = load_row<S>(a.p, v);
+        if (valid)
+            while (lo < hi) {
+                const int mid = (lo + hi + 1) >> 1;
+                if (s_start[mid] <= j) lo = mid; else hi = mid - 1;
+            }
+        const int owner = valid ? lo : -1 - lane;   // invalid lanes form their own segments
+        float val[NM];
+        if (valid) {
+            const int v = a.tadj[s_ptr[lo] + (j - s_start[lo])];
+            const Row<S> pv = load_row<S>(a.p, v);
 #pragma unroll
-        for (int k = 0; k < NM; k++)
-            if (pv.v[k] != 0.f) atomicAdd(&s_acc[k][lo], pv.v[k]);
+            for (int k = 0; k < NM; k++) val[k] = pv.v[k];
+        } else {
+#pragma unroll
+            for (int k = 0; k < NM; k++) val[k] = 0.f;
+        }
+        // segmented inclusive sum over lanes with the same owner (owners are non-decreasing
+        // along the wave, so a segment is a run of consecutive lanes)
+#pragma unroll
+        for (int o = 1; o < WAVE; o <<= 1) {
+            const int ow = __shfl_up(owner, o, WAVE);
+#pragma unroll
+            for (int k = 0; k < NM; k++) {
+                const float t = __shfl_up(val[k], o, WAVE);
+                if (lane >= o && ow == owner) val[k] += t;
+            }
+        }
+        const int next = __shfl_down(owner, 1, WAVE);
+        if (valid && (lane == WAVE - 1 || next != owner)) {
+#pragma unroll
+            for (int k = 0; k < NM; k++)
+                if (val[k] != 0.f) atomicAdd(&s_acc[k][owner], val[k]);
+        }
     }
     __syncthreads();
     if (u < a.n && mydeg) {
@@ -696,15 +748,21 @@ __global__ __launch_bounds__(BLOCK) void k_pull(PullArgs a, DevMeasures M) {
     }
 }
 
-// Destinations with more than PULL_HUB in-neighbours: one block each (grid-stride over a
-// host-built list), block reduction, one add.
+// Destinations with more than PULL_HUB in-neighbours: their in-lists are cut into pieces of
+// PULL_PIECE edges at setup; one block per piece reduces and adds its sum to r with one
+// atomic per measure (pieces of one destination may run concurrently).  Runs before
+// k_pull in the pass; the two kernels touch disjoint rows of r.
+struct PullPiece { int32_t u; uint32_t len; uint64_t start; };
+constexpr uint32_t PULL_PIECE = 8192;
+
 template <int NM, int S>
-__global__ __launch_bounds__(BLOCK) void k_pull_hub(PullArgs a, const int32_t* __restrict__ hubs, int nhubs,
+__global__ __launch_bounds__(BLOCK) void k_pull_hub(PullArgs a, const PullPiece* __restrict__ pieces, int npieces,
                                                      DevMeasures M) {
     __shared__ float s_red[NM][NWAVES];
-    for (int h = blockIdx.x; h < nhubs; h += gridDim.x) {
-        const int u = hubs[h];
-        const uint64_t b = a.tidx[u], e = a.tidx[u + 1];
+    for (int h = blockIdx.x; h < npieces; h += gridDim.x) {
+        const PullPiece pc = pieces[h];
+        const int u = pc.u;
+        const uint64_t b = pc.start, e = pc.start + pc.len;
         float acc[NM];
 #pragma unroll
         for (int k = 0; k < NM; k++) acc[k] = 0.f;
@@ -724,7 +782,7 @@ __global__ __launch_bounds__(BLOCK) void k_pull_hub(PullArgs a, const int32_t* _
             float t = 0.f;
 #pragma unroll
             for (int w = 0; w < NWAVES; w++) t += s_red[threadIdx.x][w];
-            if (t != 0.f) M.r[(long long)u * S + threadIdx.x] += t;
+            if (t != 0.f) atomic_add_f32(&M.r[(long long)u * S + threadIdx.x], t);
         }
         __syncthreads();
     }
@@ -929,7 +987,7 @@ __global__ __launch_bounds__(BLOCK) void k_bpush(PushArgs a, BatchArgs B) {
     const long long gwave = (blockIdx.x * (long long)BLOCK + threadIdx.x) / WAVE;
     uint64_t st[6] = {0, 0, 0, 0, 0, 0};   // meaningful in lane 0
     BPiece* pieces = reinterpret_cast<BPiece*>(a.pieces);
-    if (blockIdx.x == 0 && threadIdx.x == 0) { a.piece_zero[0] = 0u; a.piece_zero[1] = 0u; }
+    if (blockIdx.x == 0 && threadIdx.x == 0) clear_counters(a.piece_zero);
     if constexpr (SRC == SRC_RANGE || SRC == SRC_BITMAP) {
         const long long vb = ((long long)a.v0 & ~63ll) + gwave * WAVE;
         if (vb < a.v1) {
@@ -1018,7 +1076,7 @@ __global__ void k_bseed(int cnt, const int32_t* __restrict__ seeds, float* r, in
 
 struct PieceBuf {
     Piece* d = nullptr;          // pieces (Piece, or BPiece for batched contexts)
-    uint32_t* cnt = nullptr;     // [phase*2 + 0] piece count, [phase*2 + 1] batched p-row count
+    uint32_t* cnt = nullptr;     // phase p: piece count at cnt[2p*CNT_STRIDE], p-row count one line later
     uint32_t cap = 0;
     int phase = 0;
 };
@@ -1036,7 +1094,7 @@ struct Timer {
     int cls;
 };
 
-enum { T_SNAP = 0, T_PUSH, T_PUSH_LAST = T_PUSH + 3, T_PIECE, T_OTHER, T_FOLD, T_PULL, T_NCLS };
+enum { T_SNAP = 0, T_PUSH, T_PUSH_LAST = T_PUSH + 3, T_PIECE, T_OTHER, T_FOLD, T_PULL, T_PULL_HUB, T_NCLS };
 
 struct Ctx {
     int dev = 0, n = 0, nc = 0, nm = 1, S = 1, push_kernel = EXC_KERNEL_LB;
@@ -1078,8 +1136,9 @@ struct Ctx {
     uint64_t* d_tidx = nullptr;
     int32_t* d_tadj = nullptr;
     float* d_p = nullptr;             // [n][S]
-    int32_t* d_pull_hubs = nullptr;
+    PullPiece* d_pull_hubs = nullptr;   // pieces of in-lists longer than PULL_HUB
     int n_pull_hubs = 0;
+    uint64_t pull_hub_edges = 0, pull_hub_vertices = 0;
     double pull_threshold = -1;       // < 0: off; else pull when demand >= threshold * m
     uint64_t last_demand = 0;
     uint64_t pull_passes = 0;
@@ -1103,7 +1162,7 @@ BatchArgs batch_args(Ctx* c, int s) {
     B.active = c->active;
     B.colpend = c->d_colpend;
     B.prows = s >= 0 ? c->prows[s] : nullptr;
-    B.prow_count = s >= 0 ? c->pb[s].cnt + 2 * c->pb[s].phase + 1 : nullptr;
+    B.prow_count = s >= 0 ? c->pb[s].cnt + (2 * c->pb[s].phase + 1) * CNT_STRIDE : nullptr;
     B.prow_cap = s >= 0 ? c->prow_cap[s] : 0;
     return B;
 }
@@ -1127,8 +1186,8 @@ int ensure_bpieces(Ctx* c, int s, uint64_t edges_bound) {
     p.cap = need;
     c->prow_cap[s] = rows;
     if (!p.cnt) {
-        HIPCHK(hipMalloc(&p.cnt, 4 * sizeof(uint32_t)));
-        HIPCHK(hipMemset(p.cnt, 0, 4 * sizeof(uint32_t)));
+        HIPCHK(hipMalloc(&p.cnt, 4 * CNT_STRIDE * sizeof(uint32_t)));
+        HIPCHK(hipMemset(p.cnt, 0, 4 * CNT_STRIDE * sizeof(uint32_t)));
     }
     return EXC_OK;
 }
@@ -1181,8 +1240,8 @@ int ensure_pieces(Ctx* c, int s, uint64_t edges_bound) {
     }
     p.cap = need;
     if (!p.cnt) {
-        HIPCHK(hipMalloc(&p.cnt, 4 * sizeof(uint32_t)));
-        HIPCHK(hipMemset(p.cnt, 0, 4 * sizeof(uint32_t)));
+        HIPCHK(hipMalloc(&p.cnt, 4 * CNT_STRIDE * sizeof(uint32_t)));
+        HIPCHK(hipMemset(p.cnt, 0, 4 * CNT_STRIDE * sizeof(uint32_t)));
     }
     return EXC_OK;
 }
@@ -1221,7 +1280,7 @@ template <int NM, int S>
 void launch_pieces_t(Ctx* c, int s, const int32_t* edges, const float* wts) {
     const unsigned grid = (unsigned)(c->cus * 4);
     hipLaunchKernelGGL((k_pieces<NM, S>), dim3(grid), dim3(BLOCK), 0, c->st[s], c->pb[s].d,
-                       c->pb[s].cnt + 2 * c->pb[s].phase, c->pb[s].cap, edges, wts, c->DM);
+                       c->pb[s].cnt + 2 * c->pb[s].phase * CNT_STRIDE, c->pb[s].cap, edges, wts, c->DM);
 }
 
 int norm_group(int g) {
@@ -1238,8 +1297,8 @@ int push_launch(Ctx* c, int s, int src, unsigned grid, PushArgs a, int group, ui
     if (rc) return rc;
     PieceBuf& pb = c->pb[s];
     a.pieces = pb.d;
-    a.piece_count = pb.cnt + 2 * pb.phase;
-    a.piece_zero = pb.cnt + 2 * (pb.phase ^ 1);
+    a.piece_count = pb.cnt + 2 * pb.phase * CNT_STRIDE;
+    a.piece_zero = pb.cnt + 2 * (pb.phase ^ 1) * CNT_STRIDE;
     a.piece_cap = pb.cap;
     a.counters = c->d_counters + src * C_FIELDS;
     a.overflow = c->d_counters + C_OVERFLOW;
@@ -1257,7 +1316,7 @@ int push_launch(Ctx* c, int s, int src, unsigned grid, PushArgs a, int group, ui
         {
             ScopedTimer t(c, c->st[s], T_PIECE);
             hipLaunchKernelGGL(k_bpieces, dim3(c->cus * 4), dim3(BLOCK), 0, c->st[s],
-                               reinterpret_cast<const BPiece*>(pb.d), pb.cnt + 2 * pb.phase, pb.cap, a.edges, B);
+                               reinterpret_cast<const BPiece*>(pb.d), pb.cnt + 2 * pb.phase * CNT_STRIDE, pb.cap, a.edges, B);
         }
         pb.phase ^= 1;
         HIPCHK(hipGetLastError());
@@ -1290,12 +1349,14 @@ void launch_fold_pull(Ctx* c, hipStream_t st) {
         ScopedTimer t(c, st, T_FOLD);
         hipLaunchKernelGGL((k_fold<NM, S>), dim3(grid), dim3(BLOCK), 0, st, f, c->DM);
     }
+    if (c->n_pull_hubs) {
+        ScopedTimer t(c, st, T_PULL_HUB);
+        hipLaunchKernelGGL((k_pull_hub<NM, S>), dim3((unsigned)std::min(c->n_pull_hubs, c->cus * 8)), dim3(BLOCK), 0, st,
+                           pa, c->d_pull_hubs, c->n_pull_hubs, c->DM);
+    }
     {
         ScopedTimer t(c, st, T_PULL);
         hipLaunchKernelGGL((k_pull<NM, S>), dim3(grid), dim3(BLOCK), 0, st, pa, c->DM);
-        if (c->n_pull_hubs)
-            hipLaunchKernelGGL((k_pull_hub<NM, S>), dim3((unsigned)std::min(c->n_pull_hubs, c->cus * 4)), dim3(BLOCK), 0, st,
-                               pa, c->d_pull_hubs, c->n_pull_hubs, c->DM);
     }
 }
 
@@ -1984,11 +2045,16 @@ int exc_get_stats(void* p, exc_stats* o) {
         const int S = c->S, nm = c->K ? c->K : c->nm;
         o->fold_ms = c->t_ms[T_FOLD];
         o->pull_ms = c->t_ms[T_PULL];
+        o->pull_hub_ms = c->t_ms[T_PULL_HUB];
         o->pull_passes = c->pull_passes;
         // fold: read r and write p for every vertex; offsets, x read + write for folded ones
         o->fold_bytes = (double)c->pull_passes * c->n * (4.0 * nm + 4.0 * S) + 16.0 * r[C_VERT] + 8.0 * r[C_VERT_NM];
-        // pull: every in-edge (id + source p of nm floats), every vertex (offsets, r read + write)
-        o->pull_bytes = (double)c->pull_passes * ((double)c->m * (4.0 + 4.0 * nm) + (double)c->n * (8.0 + 8.0 * nm));
+        // pull: every in-edge (id + source p of nm floats), every vertex (offsets, r read + write);
+        // in-lists above the hub threshold go to the hub kernel
+        const double me = (double)(c->m - c->pull_hub_edges), nv = (double)(c->n - c->pull_hub_vertices);
+        o->pull_bytes = (double)c->pull_passes * (me * (4.0 + 4.0 * nm) + nv * (8.0 + 8.0 * nm));
+        o->pull_hub_bytes = (double)c->pull_passes *
+                            ((double)c->pull_hub_edges * (4.0 + 4.0 * nm) + (double)c->pull_hub_vertices * (8.0 + 8.0 * nm));
         o->pushed_vertices += r[C_VERT];
         o->pushed_vertex_measures += r[C_VERT_NM];
         o->pushed_edges += r[C_EDGES];
@@ -2030,11 +2096,16 @@ int exc_enable_pull(void* p, double threshold) {
     std::vector<uint64_t> h(n + 1);
     e = hipMemcpy(h.data(), cnt, n * sizeof(uint64_t), hipMemcpyDeviceToHost);
     if (e != hipSuccess) return fail_free(e);
-    std::vector<int32_t> hubs;
+    std::vector<PullPiece> hubs;
     uint64_t acc = 0;
     for (size_t v = 0; v < n; v++) {
         const uint64_t d = h[v];
-        if (d > PULL_HUB) hubs.push_back((int32_t)v);
+        if (d > PULL_HUB) {
+            c->pull_hub_edges += d;
+            c->pull_hub_vertices++;
+            for (uint64_t o = 0; o < d; o += PULL_PIECE)
+                hubs.push_back({(int32_t)v, (uint32_t)std::min<uint64_t>(PULL_PIECE, d - o), acc + o});
+        }
         h[v] = acc;
         acc += d;
     }
@@ -2048,8 +2119,8 @@ int exc_enable_pull(void* p, double threshold) {
     (void)hipFree(cnt);
     c->n_pull_hubs = (int)hubs.size();
     if (c->n_pull_hubs) {
-        HIPCHK(hipMalloc(&c->d_pull_hubs, hubs.size() * sizeof(int32_t)));
-        HIPCHK(hipMemcpy(c->d_pull_hubs, hubs.data(), hubs.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+        HIPCHK(hipMalloc(&c->d_pull_hubs, hubs.size() * sizeof(PullPiece)));
+        HIPCHK(hipMemcpy(c->d_pull_hubs, hubs.data(), hubs.size() * sizeof(PullPiece), hipMemcpyHostToDevice));
     }
     return EXC_OK;
 }

@@ -87,9 +87,9 @@ typedef struct exc_stats {
     uint64_t src_launches[4];
     double src_bytes[4];
     /* dense passes run as fold + pull (exc_enable_pull) */
-    double fold_ms, pull_ms;
+    double fold_ms, pull_ms, pull_hub_ms;
     uint64_t pull_passes;
-    double fold_bytes, pull_bytes;
+    double fold_bytes, pull_bytes, pull_hub_bytes;
 } exc_stats;
 
 const char* exc_last_error(void);

@@ -30,8 +30,8 @@ def run_passes(idx, adj, kinds, eps, scale, divdeg, inits, cv0=None, kernel=nat.
         ctx.upload_offsets(idx)
         for i, (x0, r0) in enumerate(inits):
             ctx.set_state(i, x=x0, r=r0)
+        assert ctx.cache_chunks(m) == 0   # an edgeless graph still gets a (one-element) cache
         if m:
-            assert ctx.cache_chunks(m) == 0
             ctx.cache_upload(adj, 0)
         if pull is not None:
             assert ctx.enable_pull(pull) == 0
