@@ -59,6 +59,7 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--workdir", default=None)
+    ap.add_argument("--log", default=None, help="bin/excent report of the runs (default: in the workdir)")
     ap.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "push_traffic_rmat24_all.json"),
                     help="JSON with HBM bytes per push launch from rocprofv3 --pmc runs")
     args = ap.parse_args()
@@ -89,7 +90,7 @@ def main():
     env = read_env(envp)
     log(f"constants {env} in {time.time() - t0:.1f}s")
 
-    excent_log = os.path.join(work, "excent.log")
+    excent_log = args.log or os.path.join(work, "excent.log")
     sargs = [graph, "--measure", args.measure, "--cache", "chunks", "--preload", "--stream-ratio", "1",
              "--timing", "--push-kernel", args.kernel, "--pull", args.pull, "--log", excent_log]
     if args.measure in ("katz", "all"):

@@ -323,6 +323,7 @@ struct PushArgs {
     int k;                          // LIST: entries
     const int32_t* hlist;           // HOT: compacted hot entry indices
     const uint32_t* hcount;         // HOT: entry count (device)
+    long long ntiles;               // RANGE / BITMAP / LIST: tiles of the launch (HOT: from hcount)
     const int32_t* hids;            // HOT: hot vertex ids
     const uint64_t* hoff;           // HOT: hot list offsets
     const uint64_t* idx;            // CSR offsets (RANGE / BITMAP)
@@ -471,24 +472,27 @@ __global__ __launch_bounds__(BLOCK) void k_push(PushArgs a, DevMeasures M) {
     if (blockIdx.x == 0 && tid == 0) clear_counters(a.piece_zero);
     __syncthreads();
 
+    // persistent over tiles of BLOCK entries (a window of BITMAP_WINDOW vertices in bitmap
+    // mode), so each block flushes its statistics once
+    long long ntiles = a.ntiles;
+    if constexpr (SRC == SRC_HOT) ntiles = ((long long)*a.hcount + BLOCK - 1) / BLOCK;
+    for (long long tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
     if constexpr (SRC == SRC_RANGE) {
-        const long long v = (long long)a.v0 + (long long)blockIdx.x * BLOCK + tid;
+        const long long v = (long long)a.v0 + tile * BLOCK + tid;
         bool valid = v < a.v1;
         if (valid && a.hot && ((a.hot[v >> 5] >> (v & 31)) & 1u)) valid = false;
         uint64_t b = valid ? a.idx[v] : 0, d = valid ? a.idx[v + 1] - b : 0;
         push_entries<NM, S, G>(valid, (int)(valid ? v : 0), b - (valid ? a.ebase : 0), d, a, M,
                                s_start, s_ptr, s_p, s_wave, s_cnt);
     } else if constexpr (SRC == SRC_LIST) {
-        const long long i = (long long)blockIdx.x * BLOCK + tid;
+        const long long i = tile * BLOCK + tid;
         const bool valid = i < a.k;
         const int v = valid ? a.vids[i] : 0;
         const uint64_t b = valid ? a.off[i] : 0, d = valid ? a.off[i + 1] - b : 0;
         push_entries<NM, S, G>(valid, v, b, d, a, M, s_start, s_ptr, s_p, s_wave, s_cnt);
     } else if constexpr (SRC == SRC_HOT) {
         const long long k = *a.hcount;
-        const long long base = (long long)blockIdx.x * BLOCK;
-        if (base >= k) return;   // block-uniform
-        const long long e = base + tid;
+        const long long e = tile * BLOCK + tid;
         const bool valid = e < k;
         const int i = valid ? a.hlist[e] : 0;
         const int v = valid ? a.hids[i] : 0;
@@ -496,7 +500,7 @@ __global__ __launch_bounds__(BLOCK) void k_push(PushArgs a, DevMeasures M) {
         push_entries<NM, S, G>(valid, v, b, d, a, M, s_start, s_ptr, s_p, s_wave, s_cnt);
     } else {   // SRC_BITMAP: a window of BITMAP_WINDOW vertices, one bitmap word per lane
         __shared__ int32_t s_list[BITMAP_WINDOW];
-        const long long w0 = ((long long)a.v0 >> 5) + (long long)blockIdx.x * BLOCK;
+        const long long w0 = ((long long)a.v0 >> 5) + tile * BLOCK;
         const long long word = w0 + tid;
         uint32_t bits = 0;
         const long long wv = word * 32;
@@ -522,7 +526,8 @@ __global__ __launch_bounds__(BLOCK) void k_push(PushArgs a, DevMeasures M) {
                                    s_p, s_wave, s_cnt);
         }
     }
-    __syncthreads();
+    __syncthreads();   // the tile's LDS (s_list) is reused by the next tile
+    }
     if (tid < 6 && s_cnt[tid]) atomicAdd(&a.counters[tid], s_cnt[tid]);
 }
 
@@ -622,11 +627,11 @@ __global__ __launch_bounds__(BLOCK) void k_fold(FoldArgs a, DevMeasures M) {
     const int tid = threadIdx.x;
     if (tid < 4) s_cnt[tid] = 0;
     __syncthreads();
-    const long long v = blockIdx.x * (long long)BLOCK + tid;
-    uint32_t nmv = 0;
-    uint64_t deg = 0;
-    if (v < a.n) {
-        deg = a.idx[v + 1] - a.idx[v];
+    uint32_t c_vert = 0, c_vnm = 0;
+    uint64_t c_e = 0, c_enm = 0;
+    for (long long v = blockIdx.x * (long long)BLOCK + tid; v < a.n; v += (long long)gridDim.x * BLOCK) {
+        uint32_t nmv = 0;
+        const uint64_t deg = a.idx[v + 1] - a.idx[v];
         Row<S> row = load_row<S>(M.r, v);
         Row<S> pr;
 #pragma unroll
@@ -650,12 +655,15 @@ __global__ __launch_bounds__(BLOCK) void k_fold(FoldArgs a, DevMeasures M) {
 #pragma unroll
             for (int k = 0; k < NM; k++) xr.v[k] += dx.v[k];
             store_row<S>(M.x, v, xr);
+            c_vert++;
+            c_vnm += nmv;
+            c_e += deg;
+            c_enm += deg * nmv;
         }
         store_row<S>(a.p, v, pr);
     }
-    const uint32_t t0 = wave_sum_u32(nmv ? 1u : 0u), t1 = wave_sum_u32(nmv);
-    const uint64_t de = nmv ? deg : 0;
-    const uint64_t t2 = wave_sum_u64(de), t3 = wave_sum_u64(de * nmv);
+    const uint32_t t0 = wave_sum_u32(c_vert), t1 = wave_sum_u32(c_vnm);
+    const uint64_t t2 = wave_sum_u64(c_e), t3 = wave_sum_u64(c_enm);
     if (lane_id() == 0 && t0) {
         atomicAdd(&s_cnt[0], (unsigned long long)t0);
         atomicAdd(&s_cnt[1], (unsigned long long)t1);
@@ -1185,9 +1193,10 @@ int ensure_bpieces(Ctx* c, int s, uint64_t edges_bound) {
     }
     p.cap = need;
     c->prow_cap[s] = rows;
-    if (!p.cnt) {
+    if (!p.cnt) {   // cleared on the launch stream: the streams are non-blocking, so a clear on
+                    // the null stream would not be ordered before the first push
         HIPCHK(hipMalloc(&p.cnt, 4 * CNT_STRIDE * sizeof(uint32_t)));
-        HIPCHK(hipMemset(p.cnt, 0, 4 * CNT_STRIDE * sizeof(uint32_t)));
+        HIPCHK(hipMemsetAsync(p.cnt, 0, 4 * CNT_STRIDE * sizeof(uint32_t), c->st[s]));
     }
     return EXC_OK;
 }
@@ -1239,9 +1248,10 @@ int ensure_pieces(Ctx* c, int s, uint64_t edges_bound) {
         return fail(EXC_ENOMEM, "cannot allocate %u hub pieces", need);
     }
     p.cap = need;
-    if (!p.cnt) {
+    if (!p.cnt) {   // cleared on the launch stream: the streams are non-blocking, so a clear on
+                    // the null stream would not be ordered before the first push
         HIPCHK(hipMalloc(&p.cnt, 4 * CNT_STRIDE * sizeof(uint32_t)));
-        HIPCHK(hipMemset(p.cnt, 0, 4 * CNT_STRIDE * sizeof(uint32_t)));
+        HIPCHK(hipMemsetAsync(p.cnt, 0, 4 * CNT_STRIDE * sizeof(uint32_t), c->st[s]));
     }
     return EXC_OK;
 }
@@ -1293,6 +1303,11 @@ int norm_group(int g) {
 // One push launch + its hub-piece launch on stream index s.
 int push_launch(Ctx* c, int s, int src, unsigned grid, PushArgs a, int group, uint64_t edges_bound) {
     if (grid == 0) return EXC_OK;
+    if (c->K == 0) {   // plain kernels are persistent over tiles: grid = tiles, capped
+        a.ntiles = grid;
+        const unsigned cap = (unsigned)(c->cus * 8);
+        if (grid > cap) grid = cap;
+    }
     int rc = ensure_pieces(c, s, edges_bound);
     if (rc) return rc;
     PieceBuf& pb = c->pb[s];
@@ -1347,7 +1362,7 @@ void launch_fold_pull(Ctx* c, hipStream_t st) {
     const unsigned grid = (unsigned)(((long long)c->n + BLOCK - 1) / BLOCK);
     {
         ScopedTimer t(c, st, T_FOLD);
-        hipLaunchKernelGGL((k_fold<NM, S>), dim3(grid), dim3(BLOCK), 0, st, f, c->DM);
+        hipLaunchKernelGGL((k_fold<NM, S>), dim3(std::min<unsigned>(grid, c->cus * 8)), dim3(BLOCK), 0, st, f, c->DM);
     }
     if (c->n_pull_hubs) {
         ScopedTimer t(c, st, T_PULL_HUB);
@@ -1521,6 +1536,8 @@ int exc_create(const exc_config* cfg, const exc_measures* M, void** out) {
         }
         CK(hipEventCreateWithFlags(&sl.done, hipEventDisableTiming));
     }
+    // the clears above ran on the null stream; the context's streams are non-blocking
+    CK(hipDeviceSynchronize());
 #undef CK
     c->DM.x = c->d_x;
     c->DM.r = c->d_r;
@@ -2133,6 +2150,7 @@ int exc_reset_stats(void* p) {
     int rc = collect_timers(c);
     if (rc) return rc;
     HIPCHK(hipMemset(c->d_counters, 0, C_NCOUNTERS * sizeof(unsigned long long)));
+    HIPCHK(hipDeviceSynchronize());   // order the clear before work on the non-blocking streams
     for (int i = 0; i < T_NCLS; i++) { c->t_ms[i] = 0; c->t_launch[i] = 0; }
     c->snap_bytes = 0;
     c->pull_passes = 0;
