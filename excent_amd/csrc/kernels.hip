@@ -608,7 +608,9 @@ __global__ void k_indegree(uint64_t m, const int32_t* __restrict__ edges, uint32
 struct FoldArgs {
     int n;
     const uint64_t* idx;
-    float* p;                       // [n][S] weights of this pass (0 when v did not push)
+    float* p;                       // [sources][S] weights of this pass (0 when v did not push)
+    const int32_t* rank;            // row of v in p: vertices with out-edges in decreasing
+                                    // out-degree order, so the most-read rows share lines
     unsigned long long* counters;   // C_FIELDS row
 };
 
@@ -660,7 +662,7 @@ __global__ __launch_bounds__(BLOCK) void k_fold(FoldArgs a, DevMeasures M) {
             c_e += deg;
             c_enm += deg * nmv;
         }
-        store_row<S>(a.p, v, pr);
+        if (deg) store_row<S>(a.p, a.rank[v], pr);   // vertices without out-edges are never read
     }
     const uint32_t t0 = wave_sum_u32(c_vert), t1 = wave_sum_u32(c_vnm);
     const uint64_t t2 = wave_sum_u64(c_e), t3 = wave_sum_u64(c_enm);
@@ -677,8 +679,8 @@ __global__ __launch_bounds__(BLOCK) void k_fold(FoldArgs a, DevMeasures M) {
 struct PullArgs {
     int n;
     const uint64_t* tidx;       // transpose offsets [n+1]
-    const int32_t* tadj;        // in-neighbours
-    const float* p;             // [n][S]
+    const int32_t* tadj;        // in-neighbours, as rows of p (FoldArgs::rank)
+    const float* p;             // [sources][S]
 };
 
 constexpr uint32_t PULL_HUB = 4096;   // in-lists longer than this are summed by whole blocks
@@ -802,9 +804,9 @@ __global__ void k_count_in64(uint64_t m, const int32_t* __restrict__ adj, unsign
         atomicAdd(&cnt[adj[e]], 1ull);
 }
 __global__ void k_fill_in64(int n, const uint64_t* __restrict__ idx, const int32_t* __restrict__ adj,
-                            unsigned long long* cursor, int32_t* tadj) {
+                            unsigned long long* cursor, const int32_t* __restrict__ rank, int32_t* tadj) {
     for (long long v = blockIdx.x * (long long)blockDim.x + threadIdx.x; v < n; v += (long long)gridDim.x * blockDim.x)
-        for (uint64_t e = idx[v]; e < idx[v + 1]; e++) tadj[atomicAdd(&cursor[adj[e]], 1ull)] = (int32_t)v;
+        for (uint64_t e = idx[v]; e < idx[v + 1]; e++) tadj[atomicAdd(&cursor[adj[e]], 1ull)] = rank[v];
 }
 
 // ------------------------------------------------------------------ batched PPR
@@ -1144,6 +1146,7 @@ struct Ctx {
     uint64_t* d_tidx = nullptr;
     int32_t* d_tadj = nullptr;
     float* d_p = nullptr;             // [n][S]
+    int32_t* d_rank = nullptr;          // p row of each vertex (out-degree order)
     PullPiece* d_pull_hubs = nullptr;   // pieces of in-lists longer than PULL_HUB
     int n_pull_hubs = 0;
     uint64_t pull_hub_edges = 0, pull_hub_vertices = 0;
@@ -1357,7 +1360,7 @@ int push_launch(Ctx* c, int s, int src, unsigned grid, PushArgs a, int group, ui
 
 template <int NM, int S>
 void launch_fold_pull(Ctx* c, hipStream_t st) {
-    FoldArgs f{c->n, c->d_idx, c->d_p, c->d_counters + 4 * C_FIELDS + 0};
+    FoldArgs f{c->n, c->d_idx, c->d_p, c->d_rank, c->d_counters + 4 * C_FIELDS + 0};
     PullArgs pa{c->n, c->d_tidx, c->d_tadj, c->d_p};
     const unsigned grid = (unsigned)(((long long)c->n + BLOCK - 1) / BLOCK);
     {
@@ -1567,7 +1570,7 @@ int exc_destroy(void* p) {
     for (auto& pb : c->pb) { if (pb.d) hipFree(pb.d); if (pb.cnt) hipFree(pb.cnt); }
     for (auto s : c->st) if (s) hipStreamDestroy(s);
     for (auto q : c->prows) if (q) (void)hipFree(q);
-    void* pull_ptrs[] = {c->d_tidx, c->d_tadj, c->d_p, c->d_pull_hubs, c->d_wts};
+    void* pull_ptrs[] = {c->d_tidx, c->d_tadj, c->d_p, c->d_pull_hubs, c->d_wts, c->d_rank};
     for (void* q : pull_ptrs) if (q) (void)hipFree(q);
     if (c->d_beps) (void)hipFree(c->d_beps);
     if (c->d_colpend) (void)hipFree(c->d_colpend);
@@ -2090,16 +2093,29 @@ int exc_enable_pull(void* p, double threshold) {
     c->pull_threshold = threshold;
     if (c->d_tadj) return EXC_OK;
     const size_t n = (size_t)c->n;
+    // rows of p in decreasing out-degree order (ties by id); vertices without out-edges last
+    std::vector<uint64_t> hidx(n + 1);
+    HIPCHK(hipMemcpy(hidx.data(), c->d_idx, (n + 1) * sizeof(uint64_t), hipMemcpyDeviceToHost));
+    std::vector<int32_t> rank(n);
+    {
+        uint64_t maxd = 0;
+        for (size_t v = 0; v < n; v++) maxd = std::max(maxd, hidx[v + 1] - hidx[v]);
+        std::vector<uint64_t> start(maxd + 2, 0);
+        for (size_t v = 0; v < n; v++) start[maxd - (hidx[v + 1] - hidx[v]) + 1]++;
+        for (uint64_t d = 0; d <= maxd; d++) start[d + 1] += start[d];
+        for (size_t v = 0; v < n; v++) rank[v] = (int32_t)start[maxd - (hidx[v + 1] - hidx[v])]++;
+    }
     unsigned long long* cnt = nullptr;
     bool ok = hipMalloc(&c->d_tidx, (n + 1) * sizeof(uint64_t)) == hipSuccess;
+    ok = ok && hipMalloc(&c->d_rank, n * sizeof(int32_t)) == hipSuccess;
     ok = ok && hipMalloc(&c->d_tadj, (c->m ? c->m : 1) * sizeof(int32_t)) == hipSuccess;
     ok = ok && hipMalloc(&c->d_p, n * c->S * sizeof(float)) == hipSuccess;
     ok = ok && hipMalloc(&cnt, (n + 1) * sizeof(unsigned long long)) == hipSuccess;
     if (!ok) {
         (void)hipGetLastError();
-        void* q[] = {c->d_tidx, c->d_tadj, c->d_p, cnt};
+        void* q[] = {c->d_tidx, c->d_tadj, c->d_p, cnt, c->d_rank};
         for (void* x : q) if (x) (void)hipFree(x);
-        c->d_tidx = nullptr; c->d_tadj = nullptr; c->d_p = nullptr;
+        c->d_tidx = nullptr; c->d_tadj = nullptr; c->d_p = nullptr; c->d_rank = nullptr;
         c->pull_threshold = -1;
         return fail(EXC_ENOMEM, "exc_enable_pull: cannot allocate the transpose (%.2f GB)", c->m * 4.0 / 1e9);
     }
@@ -2107,7 +2123,8 @@ int exc_enable_pull(void* p, double threshold) {
         (void)hipFree(cnt);
         return fail(EXC_EHIP, "exc_enable_pull: %s", hipGetErrorString(e));
     };
-    hipError_t e = hipMemset(cnt, 0, (n + 1) * sizeof(unsigned long long));
+    hipError_t e = hipMemcpy(c->d_rank, rank.data(), n * sizeof(int32_t), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemset(cnt, 0, (n + 1) * sizeof(unsigned long long));
     if (e != hipSuccess) return fail_free(e);
     hipLaunchKernelGGL(k_count_in64, dim3(c->cus * 16), dim3(BLOCK), 0, 0, c->m, c->d_arena, cnt);
     std::vector<uint64_t> h(n + 1);
@@ -2130,7 +2147,7 @@ int exc_enable_pull(void* p, double threshold) {
     e = hipMemcpy(c->d_tidx, h.data(), (n + 1) * sizeof(uint64_t), hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(cnt, h.data(), (n + 1) * sizeof(uint64_t), hipMemcpyHostToDevice);
     if (e != hipSuccess) return fail_free(e);
-    hipLaunchKernelGGL(k_fill_in64, dim3(c->cus * 16), dim3(BLOCK), 0, 0, c->n, c->d_idx, c->d_arena, cnt, c->d_tadj);
+    hipLaunchKernelGGL(k_fill_in64, dim3(c->cus * 16), dim3(BLOCK), 0, 0, c->n, c->d_idx, c->d_arena, cnt, c->d_rank, c->d_tadj);
     e = hipDeviceSynchronize();
     if (e != hipSuccess) return fail_free(e);
     (void)hipFree(cnt);
