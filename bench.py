@@ -60,7 +60,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--workdir", default=None)
     ap.add_argument("--log", default=None, help="bin/excent report of the runs (default: in the workdir)")
-    ap.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "push_traffic_rmat24_all.json"),
+    ap.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "traffic_rmat24_all.json"),
                     help="JSON with HBM bytes per push launch from rocprofv3 --pmc runs")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -136,10 +136,12 @@ def main():
     achieved = dom[3] / (dom[1] / 1e3) / 1e9 if dom[1] > 0 else 0.0
     traffic = None
     if os.path.exists(args.traffic):
-        try:
-            traffic = json.load(open(args.traffic)).get(dom[0].split("<")[0])
-        except Exception:
-            traffic = None
+        # per-launch HBM bytes from rocprofv3 FETCH_SIZE/WRITE_SIZE runs of the same workload
+        tr = json.load(open(args.traffic))
+        want = {"scale": args.scale, "edge_factor": args.edge_factor, "seed": args.seed,
+                "measure": args.measure, "kernel": args.kernel, "pull": args.pull}
+        if tr.get("workload") == want:
+            traffic = tr["bytes_per_launch"].get(dom[0].split("<")[0])
     kern = {c[0]: round(c[1] / len(runs), 3) for c in cands if c[2]}
     roof = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": dom[0],
