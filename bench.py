@@ -45,12 +45,208 @@ def read_env(path):
     return out
 
 
+def fp64_errors(nat, np, graph, work, measure, env):
+    """Errors of the last timed run's values (the --dump files) against the product's FP64
+    solver, and the FP64 global residual of the float32 state (tools/centrality_reference.cu:
+    78-139 metrics; the residual bounds PageRank's L1 error by ||r||_1 / (1 - d))."""
+    t0 = time.time()
+    idx, adj = nat.load_egr(graph)
+    f32 = np.float32
+    out = {}
+    with nat.Fp64Solver(idx, adj) as S:
+        for name in {"pr": ["pr"], "katz": ["katz"], "ev": ["ev"], "all": ["pr", "katz", "ev"]}[measure]:
+            x = np.fromfile(os.path.join(work, f"x.{name}.bin"), dtype=np.float32)
+            if name == "pr":
+                c, s, dd = float(f32(1) - f32(0.85)), float(f32(0.85)), 1
+                ref, rel, _, it = S.solve(nat.EXC_PR, tol=1e-12, max_iter=400)
+            elif name == "katz":
+                c, s, dd = 1.0, float(f32(env["ALPHA"])), 0
+                ref, rel, _, it = S.solve(nat.EXC_KATZ, alpha=env["ALPHA"], beta=1.0, tol=1e-12, max_iter=400)
+            else:
+                c, s, dd = 0.0, float(f32(1.0 / env["LAMBDA"])), 0
+                ref, rel, _, it = S.solve(nat.EXC_EV, tol=1e-10, max_iter=3000)
+            res = S.residual(x, c, s, dd)
+            xv = x.astype(np.float64)
+            if name == "ev":   # unit L2 with the sign fixed by the sum
+                xv = xv / np.linalg.norm(xv) * (1 if xv.sum() >= 0 else -1)
+            d = np.abs(xv - ref)
+            k = min(100, len(x))
+            top = lambda v: set(np.argpartition(-v, k - 1)[:k].tolist())
+            e = {"l1_rel": float(d.sum() / np.abs(ref).sum()), "l2": float(np.sqrt((d * d).sum())),
+                 "max_abs": float(d.max()), "top100_overlap": len(top(xv) & top(ref)),
+                 "residual_l1_fp64": res, "ref_iters": it, "ref_rel_change": rel}
+            if name == "pr":
+                e["l1_bound"] = res / (1 - 0.85)
+                e["l1_abs"] = float(d.sum())
+            out[name] = {k2: (round(v, 9) if isinstance(v, float) else v) for k2, v in e.items()}
+            log(f"fp64 {name}: {out[name]}")
+    log(f"fp64 check {time.time() - t0:.1f}s")
+    return out
+
+
+def graph_size(nat, graph):
+    n, m, wide = nat.c_int(0), nat.c_u64(0), nat.c_int(0)
+    nat.F["excent_egr_header"](graph.encode(), nat.ctypes.byref(n), nat.ctypes.byref(m), nat.ctypes.byref(wide))
+    return n.value, m.value
+
+
+def bench_batched(args, nat, np, work, graph):
+    """K = 64 personalised PageRank columns in one batched run against the same 64 problems
+    solved one after another by the plain solver (--measure pr --ppr-column j), both from
+    the resident graph; FP64 errors on sampled columns."""
+    K, SEEDS, TOP, EPS = 64, 16, 8, 1e-7
+    gen = f"{K}:{SEEDS}:{TOP}:{args.seed + 6}"
+    common = [graph, "--cache", "chunks", "--preload", "--stream-ratio", "1", "--timing", "--ppr-gen", gen,
+              "--eps-ppr", EPS]
+    n, m = graph_size(nat, graph)
+    sample = [0, 21, 42, 63]
+    t0 = time.time()
+    sess = nat.Session(common + ["--measure", "ppr", "--log", os.path.join(work, "batched.log")])
+    log(f"batched setup {time.time() - t0:.1f}s")
+    sess.run(reinit=False)
+    log("batched warmup done")
+    nat.synchronize()
+    t0 = time.perf_counter()
+    rb = sess.run(reinit=True)
+    nat.synchronize()
+    tb = time.perf_counter() - t0
+    xb = {j: sess.values(j) for j in sample}
+    sess.close()
+    log(f"batched: {rb['passes']} passes, {tb:.3f}s, {rb['kernel']['pushed_edge_measures'] / 1e9:.2f} G vector pushes")
+
+    sess = nat.Session(common + ["--measure", "pr", "--ppr-column", 0, "--pull", args.pull,
+                                 "--log", os.path.join(work, "plain.log")])
+    sess.set_column(0)
+    sess.run(reinit=False)   # warmup
+    seq_t, seq_pushes, seq_passes, xp, cvec = [], 0, [], {}, {}
+    for j in range(K):
+        sess.set_column(j)
+        if j in sample:
+            cvec[j] = sess.values(0, which=1).astype(np.float64)   # r0 = (1 - d) * teleport
+        nat.synchronize()
+        t0 = time.perf_counter()
+        r = sess.run(reinit=False)
+        nat.synchronize()
+        seq_t.append(time.perf_counter() - t0)
+        seq_pushes += r["pushed_edges"]
+        seq_passes.append(r["passes"])
+        if j in sample:
+            xp[j] = sess.values(0)
+        log(f"plain column {j}: {r['passes']} passes, {seq_t[-1]:.3f}s")
+    sess.close()
+    ts = sum(seq_t)
+    log(f"plain: 64 runs {ts:.3f}s, {seq_pushes / 1e9:.2f} G pushes")
+
+    fp64 = {}
+    idx, adj = nat.load_egr(graph)
+    with nat.Fp64Solver(idx, adj) as S:
+        d = float(np.float32(0.85))
+        for j in sample:
+            ref, rel, it = S.solve_vec(cvec[j], d, 1, tol=1e-12, max_iter=1000)
+            e = {"ref_iters": it}
+            for tag, x in (("batched", xb[j]), ("plain", xp[j])):
+                dx = np.abs(x.astype(np.float64) - ref)
+                e[tag] = {"l1_rel": round(float(dx.sum() / np.abs(ref).sum()), 9),
+                          "max_abs": round(float(dx.max()), 9),
+                          "l1_bound_rel": round(S.residual(x, cvec[j], d, 1) / (1 - 0.85) / np.abs(ref).sum(), 9)}
+            fp64[f"column {j}"] = e
+            log(f"fp64 column {j}: {e}")
+    vb = rb["kernel"]["pushed_edge_measures"]
+    return {
+        "metric": "batched vs plain personalised PageRank at FP64 error", "unit": "x (wall-time ratio)",
+        "value": round(ts / tb, 3), "higher_is_better": True, "n_gpus": 1, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"R-MAT scale {args.scale} ef {args.edge_factor}, K={K} personalised PageRank "
+                               f"columns ({SEEDS} seeds each, {TOP} from the top 1% by out-degree, mass n/{SEEDS} "
+                               f"per seed), eps {EPS} per vertex and column, whole graph in HBM",
+                   "n": n, "m": m, "ppr_gen": gen},
+        "batched": {"time_s": round(tb, 4), "passes": rb["passes"], "vector_pushes": vb,
+                    "vector_gteps": round(vb / tb / 1e9, 3)},
+        "plain": {"time_s": round(ts, 4), "time_s_per_column": round(ts / K, 4),
+                  "passes_mean": round(sum(seq_passes) / K, 1), "vector_pushes": seq_pushes,
+                  "vector_gteps": round(seq_pushes / ts / 1e9, 3)},
+        "wall_ratio": round(ts / tb, 3),
+        "vector_push_rate_ratio": round((vb / tb) / (seq_pushes / ts), 3),
+        "fp64_err": fp64,
+    }
+
+
+def bench_ooc(args, nat, np, work, graph):
+    """Out-of-core PageRank: the graph file streamed / gathered over PCIe each pass, with a
+    capped hot cache and with no cache; bytes shipped as the reference counts them."""
+    n, m = graph_size(nat, graph)
+    out = {"metric": "converged GTEPS out-of-core (PageRank)", "unit": "GTEPS", "higher_is_better": True,
+           "n_gpus": 1, "dtype": "f32", "data": "synthetic",
+           "config": {"workload": f"R-MAT scale {args.scale} ef {args.edge_factor} --measure pr, eps 0.01, "
+                                  f"edges read from the EGR file (host page cache) every pass",
+                      "n": n, "m": m, "graph_gb": round(m * 4 / 1e9, 3)}, "runs": {}}
+    for tag, extra in ((f"hot {args.cache_mb} MB", ["--cache", "hot", "--cache-mb", args.cache_mb]),
+                       ("no cache", ["--cache", "none"])):
+        t0 = time.time()
+        sess = nat.Session([graph, "--measure", "pr", "--timing", "--log",
+                            os.path.join(work, f"ooc_{tag.split()[0]}.log")] + extra)
+        log(f"ooc {tag}: setup {time.time() - t0:.1f}s")
+        sess.run(reinit=False)
+        nat.synchronize()
+        t0 = time.perf_counter()
+        r = sess.run(reinit=True)
+        nat.synchronize()
+        el = time.perf_counter() - t0
+        sess.close()
+        shipped = r["bytes_stream"] + r["bytes_gather"]
+        out["runs"][tag] = {"time_to_eps_s": round(el, 4), "passes": r["passes"],
+                            "gteps": round(r["pushed_edges"] / el / 1e9, 3),
+                            "bytes_stream_gb": round(r["bytes_stream"] / 1e9, 3),
+                            "bytes_gather_gb": round(r["bytes_gather"] / 1e9, 3),
+                            "shipped_gb_per_s": round(shipped / el / 1e9, 2),
+                            "time_io_s": round(r["time_io_s"], 4), "time_wait_s": round(r["time_wait_s"], 4)}
+        log(f"ooc {tag}: {out['runs'][tag]}")
+    out["value"] = out["runs"][f"hot {args.cache_mb} MB"]["gteps"]
+    return out
+
+
+def bench_adjoint(args, nat, np, work, graph):
+    """Differentiable personalised PageRank on a weighted graph: forward push, adjoint push
+    over the transpose and gradient assembly, timed; the adjoint identity
+    g . x = (1 - d) lambda . s checks the two solves against each other."""
+    idx, adj = nat.load_egr(graph)
+    w = np.fromfile(graph + ".w.f32", dtype=np.float32)
+    n, m = len(idx) - 1, len(adj)
+    rng = np.random.default_rng(args.seed)
+    s = np.ones(n, np.float32)
+    g = rng.standard_normal(n).astype(np.float32)
+    with nat.DiffPageRank(idx, adj, w, eps=1e-7) as P:
+        P.forward(s)
+        P.backward(g)   # warmup
+        t0 = time.perf_counter()
+        x = P.forward(s)
+        tf = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        gs, gw, gwn = P.backward(g)
+        tb = time.perf_counter() - t0
+        pf, pb = P.passes()
+    lhs = float(g.astype(np.float64) @ x.astype(np.float64))
+    rhs = float(gs.astype(np.float64) @ s.astype(np.float64))   # gs = (1 - d) lambda
+    return {"metric": "differentiable personalised PageRank forward + backward", "unit": "s",
+            "value": round(tf + tb, 4), "higher_is_better": False, "n_gpus": 1, "dtype": "f32", "data": "synthetic",
+            "config": {"workload": f"R-MAT scale {args.scale} ef {args.edge_factor}, raw weights uniform(0.5, 1.5), "
+                                   f"s = 1, L = g . x with g ~ N(0, 1), eps 1e-7", "n": n, "m": m},
+            "forward_s": round(tf, 4), "backward_s": round(tb, 4), "passes": [pf, pb],
+            "adjoint_identity_rel_err": abs(lhs - rhs) / max(abs(lhs), 1e-30),
+            "grad_norms": {"s": float(np.linalg.norm(gs)), "w": float(np.linalg.norm(gw)),
+                           "w_normalised": float(np.linalg.norm(gwn))}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--scale", type=int, default=24)
+    ap.add_argument("--config", default="headline", choices=["headline", "batched", "ooc", "adjoint"],
+                    help="headline: the scale-24 all-measure in-HBM run (the bench line); batched: K=64 "
+                         "personalised PageRank vs 64 plain runs; ooc: out-of-core PageRank; adjoint: "
+                         "differentiable personalised PageRank forward + backward")
+    ap.add_argument("--scale", type=int, default=None, help="R-MAT scale (default 24; batched/ooc 26; adjoint 22)")
+    ap.add_argument("--cache-mb", type=int, default=1024, help="ooc: device cache cap for the hot-cache run")
     ap.add_argument("--edge-factor", type=int, default=16)
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--measure", default="all")
@@ -58,6 +254,7 @@ def main():
     ap.add_argument("--pull", default="auto", help="dense passes by pull: auto | off | demand fraction")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 error report of the last run")
     ap.add_argument("--workdir", default=None)
     ap.add_argument("--log", default=None, help="bin/excent report of the runs (default: in the workdir)")
     ap.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "traffic_rmat24_all.json"),
@@ -68,19 +265,29 @@ def main():
               "the multi-GPU design is described, not built", file=sys.stderr)
         sys.exit(2)
 
+    if args.scale is None:
+        args.scale = {"headline": 24, "batched": 26, "ooc": 26, "adjoint": 22}[args.config]
+
     import numpy as np
 
     import excent_amd.native as nat
 
     work = args.workdir or tempfile.mkdtemp(prefix="excent_bench_", dir=os.environ.get("TMPDIR", "/tmp"))
     os.makedirs(work, exist_ok=True)
-    graph = os.path.join(work, f"rmat{args.scale}_ef{args.edge_factor}_s{args.seed}.egr")
+    wtag = "_w" if args.config == "adjoint" else ""
+    graph = os.path.join(work, f"rmat{args.scale}_ef{args.edge_factor}_s{args.seed}{wtag}.egr")
     t0 = time.time()
     if not os.path.exists(graph):
-        rc = nat.run_entry("excent_rmat_main", ["--scale", args.scale, "--edge-factor", args.edge_factor,
-                                                "--seed", args.seed, graph])
+        gargs = ["--scale", args.scale, "--edge-factor", args.edge_factor, "--seed", args.seed]
+        if wtag:
+            gargs.append("--weights")
+        rc = nat.run_entry("excent_rmat_main", gargs + [graph])
         assert rc == 0, "graph generation failed"
     log(f"graph {graph} ready in {time.time() - t0:.1f}s")
+    if args.config != "headline":
+        out = {"batched": bench_batched, "ooc": bench_ooc, "adjoint": bench_adjoint}[args.config](args, nat, np, work, graph)
+        print(json.dumps(out), flush=True)
+        return
     envp = graph + ".env"
     t0 = time.time()
     if not os.path.exists(envp):
@@ -92,7 +299,8 @@ def main():
 
     excent_log = args.log or os.path.join(work, "excent.log")
     sargs = [graph, "--measure", args.measure, "--cache", "chunks", "--preload", "--stream-ratio", "1",
-             "--timing", "--push-kernel", args.kernel, "--pull", args.pull, "--log", excent_log]
+             "--timing", "--push-kernel", args.kernel, "--pull", args.pull, "--log", excent_log,
+             "--dump", os.path.join(work, "x")]
     if args.measure in ("katz", "all"):
         sargs += ["--alpha", env["ALPHA"], "--beta", 1, "--eps-katz", "1e-6"]
     if args.measure in ("ev", "all"):
@@ -148,6 +356,8 @@ def main():
             "launches": dom[2], "avg_launch_ms": round(dom[1] / max(1, dom[2]), 5),
             "algorithmic_bytes_per_launch": round(dom[3] / max(1, dom[2]), 1)}
 
+    fp64 = None if args.no_fp64 else fp64_errors(nat, np, graph, work, args.measure, env)
+
     cpu = None
     if not args.no_cpu:
         from oracle import oracle as orc
@@ -198,6 +408,7 @@ def main():
         "passes": [r["passes"] for r in runs],
         "pushed_edges_per_run": runs[-1]["pushed_edges"],
         "kernel_ms_per_run": kern,
+        "fp64_err": fp64,
         "roofline": roof,
         "cpu_baseline": cpu,
     }

@@ -58,7 +58,8 @@ __global__ void k_prescale(int n, const uint64_t* __restrict__ idx, const double
 // y[w] = c + s * sum_{u -> w} xs[u]; mx[0] = max |y - x|, mx[1] = max |y| (as ordered bits)
 __global__ __launch_bounds__(BLOCK) void k_pull(int n, const uint64_t* __restrict__ tidx, const int32_t* __restrict__ tadj,
                                                  const double* __restrict__ xs, const double* __restrict__ x, double* y,
-                                                 double c, double s, unsigned long long* mx) {
+                                                 double c, const double* __restrict__ cvec, double s,
+                                                 unsigned long long* mx) {
     const long long w = (blockIdx.x * (long long)BLOCK + threadIdx.x) / GL;
     const int gl = threadIdx.x % GL;
     double acc = 0;
@@ -67,7 +68,7 @@ __global__ __launch_bounds__(BLOCK) void k_pull(int n, const uint64_t* __restric
 #pragma unroll
     for (int o = GL / 2; o > 0; o >>= 1) acc += __shfl_xor(acc, o, GL);
     if (w < n && gl == 0) {
-        const double nv = c + s * acc;
+        const double nv = (cvec ? cvec[w] : c) + s * acc;
         y[w] = nv;
         atomicMax(&mx[0], (unsigned long long)__double_as_longlong(fabs(nv - x[w])));
         atomicMax(&mx[1], (unsigned long long)__double_as_longlong(fabs(nv)));
@@ -91,6 +92,13 @@ __global__ void k_scale_diff(int n, double* y, const double* __restrict__ x, dou
     for (int o = 32; o > 0; o >>= 1) md = fmax(md, __shfl_xor(md, o, 64));
     if ((threadIdx.x & 63) == 0) atomicMax(mx, (unsigned long long)__double_as_longlong(md));
 }
+__global__ void k_absdiff_sum(int n, const double* __restrict__ y, const double* __restrict__ x, double* out) {
+    double a = 0;
+    for (long long v = blockIdx.x * (long long)blockDim.x + threadIdx.x; v < n; v += (long long)gridDim.x * blockDim.x)
+        a += fabs(y[v] - x[v]);
+    for (int o = 32; o > 0; o >>= 1) a += __shfl_xor(a, o, 64);
+    if ((threadIdx.x & 63) == 0) atomicAdd(out, a);
+}
 __global__ void k_fill(int n, double* x, double f) {
     for (long long v = blockIdx.x * (long long)blockDim.x + threadIdx.x; v < n; v += (long long)gridDim.x * blockDim.x) x[v] = f;
 }
@@ -102,16 +110,18 @@ struct Solver {
     uint64_t* d_tidx = nullptr;
     int32_t* d_tadj = nullptr;
     double *x = nullptr, *y = nullptr, *xs = nullptr, *red = nullptr;
+    double* cvec = nullptr;   // per-vertex constant of the last vector solve / residual
     unsigned long long* mx = nullptr;
 };
 
 double bits_to_d(unsigned long long b) { double d; memcpy(&d, &b, 8); return d; }
 
-int pull(Solver* S, const double* x, double* y, double c, double s, int divdeg, double* mxd, double* mxy) {
+int pull(Solver* S, const double* x, double* y, double c, double s, int divdeg, double* mxd, double* mxy,
+         const double* cvec = nullptr) {
     HIPCHK(hipMemset(S->mx, 0, 2 * sizeof(unsigned long long)));
     hipLaunchKernelGGL(k_prescale, dim3(S->cus * 8), dim3(BLOCK), 0, 0, S->n, S->d_idx, x, S->xs, divdeg);
     const unsigned grid = (unsigned)(((long long)S->n * GL + BLOCK - 1) / BLOCK);
-    hipLaunchKernelGGL(k_pull, dim3(grid), dim3(BLOCK), 0, 0, S->n, S->d_tidx, S->d_tadj, S->xs, x, y, c, s, S->mx);
+    hipLaunchKernelGGL(k_pull, dim3(grid), dim3(BLOCK), 0, 0, S->n, S->d_tidx, S->d_tadj, S->xs, x, y, c, cvec, s, S->mx);
     unsigned long long h[2];
     HIPCHK(hipMemcpy(h, S->mx, sizeof h, hipMemcpyDeviceToHost));
     *mxd = bits_to_d(h[0]);
@@ -187,7 +197,7 @@ int exc_fp64_create(int device, int n, uint64_t m, const uint64_t* h_idx, const 
 int exc_fp64_destroy(void* p) {
     Solver* S = static_cast<Solver*>(p);
     if (!S) return EXC_OK;
-    void* q[] = {S->d_idx, S->d_tidx, S->d_tadj, S->x, S->y, S->xs, S->red, S->mx};
+    void* q[] = {S->d_idx, S->d_tidx, S->d_tadj, S->x, S->y, S->xs, S->red, S->mx, S->cvec};
     for (void* z : q) if (z) (void)hipFree(z);
     delete S;
     return EXC_OK;
@@ -231,6 +241,63 @@ int exc_fp64_solve(void* p, int kind, double alpha, double beta, int max_iter, d
     HIPCHK(hipMemcpy(h_out, x, (size_t)n * 8, hipMemcpyDeviceToHost));
     if (rel_out) *rel_out = rel;
     if (lambda_out) *lambda_out = lam;
+    if (iters) *iters = it;
+    return EXC_OK;
+}
+
+int upload_cvec(Solver* S, const double* h_c) {
+    if (!h_c) return EXC_OK;
+    if (!S->cvec) HIPCHK(hipMalloc(&S->cvec, (size_t)S->n * 8));
+    HIPCHK(hipMemcpy(S->cvec, h_c, (size_t)S->n * 8, hipMemcpyHostToDevice));
+    return EXC_OK;
+}
+
+// ||c + s W x - x||_1 in FP64 for a float32 state x (W = A^T, scaled by 1/outdeg when
+// divdeg; c a constant or, when h_c is given, a per-vertex vector): the global residual
+// the push's termination bound is stated in (src/excent.cu:100-124 keeps the invariant
+// x + r = c + s W x).
+int exc_fp64_residual(void* p, double c, const double* h_c, double s, int divdeg, const float* h_x, double* l1) {
+    Solver* S = static_cast<Solver*>(p);
+    if (!S || !h_x || !l1) return fail(EXC_EINVAL, "exc_fp64_residual: bad arguments");
+    HIPCHK(hipSetDevice(S->dev));
+    const int n = S->n;
+    int rc = upload_cvec(S, h_c);
+    if (rc) return rc;
+    std::vector<double> hx(h_x, h_x + n);
+    HIPCHK(hipMemcpy(S->x, hx.data(), (size_t)n * 8, hipMemcpyHostToDevice));
+    double mxd, mxy;
+    rc = pull(S, S->x, S->y, c, s, divdeg, &mxd, &mxy, h_c ? S->cvec : nullptr);
+    if (rc) return rc;
+    HIPCHK(hipMemset(S->red, 0, sizeof(double)));
+    hipLaunchKernelGGL(k_absdiff_sum, dim3(S->cus * 4), dim3(BLOCK), 0, 0, n, S->y, S->x, S->red);
+    HIPCHK(hipMemcpy(l1, S->red, sizeof(double), hipMemcpyDeviceToHost));
+    return EXC_OK;
+}
+
+// Jacobi x <- c + s W x from x = c for a per-vertex constant c (personalised PageRank:
+// c = (1 - d) * teleport, s = d, divdeg = 1), until max|x' - x| / max|x'| < tol.
+int exc_fp64_solve_vec(void* p, const double* h_c, double s, int divdeg, int max_iter, double tol, double* h_out,
+                       double* rel_out, int* iters) {
+    Solver* S = static_cast<Solver*>(p);
+    if (!S || !h_c || !h_out) return fail(EXC_EINVAL, "exc_fp64_solve_vec: bad arguments");
+    HIPCHK(hipSetDevice(S->dev));
+    const int n = S->n;
+    int rc = upload_cvec(S, h_c);
+    if (rc) return rc;
+    HIPCHK(hipMemcpy(S->x, S->cvec, (size_t)n * 8, hipMemcpyDeviceToDevice));
+    double* x = S->x;
+    double* y = S->y;
+    double rel = 1;
+    int it = 0;
+    for (; it < max_iter && rel > tol; it++) {
+        double mxd, mxy;
+        rc = pull(S, x, y, 0.0, s, divdeg, &mxd, &mxy, S->cvec);
+        if (rc) return rc;
+        rel = mxy > 0 ? mxd / mxy : 0.0;
+        std::swap(x, y);
+    }
+    HIPCHK(hipMemcpy(h_out, x, (size_t)n * 8, hipMemcpyDeviceToHost));
+    if (rel_out) *rel_out = rel;
     if (iters) *iters = it;
     return EXC_OK;
 }

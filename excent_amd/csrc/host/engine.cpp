@@ -72,6 +72,7 @@ struct Options {
     int device = 0;
     // batched personalised PageRank (--measure ppr)
     int ppr_k = 0;
+    int ppr_col = -1;   // --measure pr with one batched column's teleport set (plain solver)
     std::string ppr_seeds, ppr_gen;
     std::string pull = "auto";   // dense passes by pull over a device transpose: auto | off | threshold
     double eps_ppr = -1;
@@ -137,6 +138,7 @@ Options parse(int argc, const char* const* argv) {
         else if (a == "--ppr-seeds") o.ppr_seeds = next();
         else if (a == "--ppr-gen") o.ppr_gen = next();
         else if (a == "--eps-ppr") o.eps_ppr = atof(next());
+        else if (a == "--ppr-column") o.ppr_col = atoi(next());
         else die("unknown option " + a);
     }
     return o;
@@ -174,6 +176,9 @@ public:
     void setup();
     void run_passes();
     void reinit();
+    void set_column(int j);
+    void values(int col, int which, float* out) { ck(exc_read_state(ctx_, col, which, out, 0, (size_t)n_), "read state"); }
+    int vertices() const { return n_; }
     void results();
     void report();
     void write_stats_json();
@@ -217,9 +222,11 @@ private:
     void load_ppr_seeds(EgrGraph& g);
     void maybe_enable_pull();
     bool pull_tried_ = false, pull_on_ = false;
+    double pull_thr_ = 1.0;   // dense pass (fold + pull) when demand >= pull_thr_ * m
     bool batched() const { return o_.measure == "ppr"; }
     int K_ = 0;
     std::vector<std::vector<int32_t>> seeds_;
+    int ppr_col_ = -1;   // plain PageRank started from this column's teleport set
     std::vector<uint32_t> colpend_;
     int active_cols_ = 0;
     size_t budget_cap(size_t b) const {
@@ -253,6 +260,11 @@ void Engine::setup() {
     if (o.measure == "ppr") {
         if (o.eps_ppr <= 0) die("ppr needs --eps-ppr");
         if (o.ppr_seeds.empty() == o.ppr_gen.empty()) die("ppr needs one of --ppr-seeds FILE or --ppr-gen K:S:TOP:SEED");
+        if (!o.ppr_seeds.empty() && (o.ppr_k < 1 || o.ppr_k > 64)) die("--ppr-k must be 1..64");
+    }
+    if (o.ppr_col >= 0) {
+        if (o.measure != "pr") die("--ppr-column needs --measure pr");
+        if (o.ppr_seeds.empty() == o.ppr_gen.empty()) die("--ppr-column needs one of --ppr-seeds FILE or --ppr-gen K:S:TOP:SEED");
         if (!o.ppr_seeds.empty() && (o.ppr_k < 1 || o.ppr_k > 64)) die("--ppr-k must be 1..64");
     }
     for (int k : kinds_) {
@@ -299,7 +311,8 @@ void Engine::setup() {
     for (int i = 0; i < M_.nm; i++) {
         const int k = kinds_[i];
         M_.kind[i] = k;
-        M_.eps[i] = (float)(batched() ? o.eps_ppr : k == EXC_PR ? o.eps_pr : k == EXC_KATZ ? o.eps_katz : o.eps_ev);
+        M_.eps[i] = (float)(batched() || (o.ppr_col >= 0 && o.eps_ppr > 0) ? o.eps_ppr
+                            : k == EXC_PR ? o.eps_pr : k == EXC_KATZ ? o.eps_katz : o.eps_ev);
         M_.scale[i] = (float)(k == EXC_PR ? (double)0.85f : k == EXC_KATZ ? o.alpha : 1.0 / o.lambda);
         M_.divdeg[i] = k == EXC_PR;
     }
@@ -312,6 +325,10 @@ void Engine::setup() {
     if (batched()) {
         load_ppr_seeds(g);
         cfg.batch_k = K_;
+    } else if (o.ppr_col >= 0) {
+        load_ppr_seeds(g);
+        if (o.ppr_col >= K_) die("--ppr-column must be below the number of columns");
+        ppr_col_ = o.ppr_col;
     }
     ck(exc_create(&cfg, &M_, &ctx_), "create");
     void* ctx = ctx_;
@@ -570,7 +587,7 @@ void Engine::maybe_enable_pull() {
     pull_tried_ = true;
     const double thr = o_.pull == "auto" ? 0.05 : atof(o_.pull.c_str());
     const int rc = exc_enable_pull(ctx_, thr);
-    if (rc == EXC_OK) pull_on_ = true;
+    if (rc == EXC_OK) { pull_on_ = true; pull_thr_ = thr; }
     else if (rc != EXC_ENOMEM) ck(rc, "pull setup");
 }
 
@@ -586,6 +603,14 @@ void Engine::init_state() {
         }
         return;
     }
+    if (ppr_col_ >= 0) {   // one column of the batched problem, on the plain solver
+        const std::vector<int32_t>& sj = seeds_[ppr_col_];
+        const float mass = (float)((1.0 - (double)0.85f) * (double)n_ / (double)sj.size());
+        std::vector<float> x(n_, 0.f), r(n_, 0.f);
+        for (int v : sj) r[v] = mass;
+        ck(exc_set_state(ctx_, 0, x.data(), r.data()), "ppr column init");
+        return;
+    }
     for (int i = 0; i < M_.nm; i++) ck(exc_init_measure(ctx_, i, (float)o_.beta, (float)o_.lambda), "init");
     if (indeg_.empty()) return;
     // r = indeg/lambda - 1 with x = 1: the invariant x + r = W x on a directed graph (the
@@ -598,6 +623,16 @@ void Engine::init_state() {
         for (int v = 0; v < n_; v++) r[v] = (float)((double)indeg_[v] / lam - 1.0);
         ck(exc_set_state(ctx_, i, x.data(), r.data()), "ev init");
     }
+}
+
+// Switches a --ppr-column session to another column: fresh state, statistics cleared.
+void Engine::set_column(int j) {
+    if (ppr_col_ < 0) die("set_column: the session was not opened with --ppr-column");
+    if (j < 0 || j >= K_) die("set_column: column out of range");
+    ppr_col_ = j;
+    init_state();
+    ck(exc_pass_sync(ctx_, nullptr), "column sync");
+    ck(exc_reset_stats(ctx_), "reset stats");
 }
 
 void Engine::reinit() {
@@ -663,6 +698,17 @@ void Engine::run_passes() {
 
         int nres = 0, nstream = 0, ngather = 0;
         if (hot_skip && h_hotpend && o.hot_when != "end") launch_hot(0);
+        uint64_t demand = 0;
+        for (int c = 0; c < nc; c++) demand += h_cnt[2 * c + 1];
+        if (pull_on_ && !hot_skip && (double)demand >= pull_thr_ * (double)m) {
+            // dense pass over the whole cached graph (every chunk is resident once pull is on),
+            // whatever the per-chunk list/range split would have been
+            exc_chunk all{0, n, 0, m};
+            ck(exc_push_resident(ctx, &all, 0, 0, 0), "dense pass");
+            for (int c = 0; c < nc; c++) if (mode[c] == F_RESIDENT) { nres++; n_range_++; }
+            n_launch_++;
+            std::fill(mode.begin(), mode.end(), (int8_t)F_SKIP);
+        }
         for (int c = 0; c < nc;) {   // resident chunks, runs merged into one launch
             if (mode[c] != F_RESIDENT) { c++; continue; }
             const bool ul = use_list(h_cnt[2 * c], chunks_[c].v1 - chunks_[c].v0, o.list_frac);
@@ -924,6 +970,26 @@ extern "C" int excent_session_run(void* session, int reinit, excent_run_stats* s
         if (stats) eng->fill_stats(stats);
         return 0;
     });
+}
+
+extern "C" int excent_session_set_column(void* session, int column) {
+    if (!session) return 1;
+    return guarded([&]() {
+        static_cast<excent::Engine*>(session)->set_column(column);
+        return 0;
+    });
+}
+
+extern "C" int excent_session_values(void* session, int column, int which, float* out) {
+    if (!session || !out) return 1;
+    return guarded([&]() {
+        static_cast<excent::Engine*>(session)->values(column, which, out);
+        return 0;
+    });
+}
+
+extern "C" int excent_session_vertices(void* session) {
+    return session ? static_cast<excent::Engine*>(session)->vertices() : -1;
 }
 
 extern "C" int excent_session_results(void* session) {

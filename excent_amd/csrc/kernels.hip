@@ -74,7 +74,11 @@ struct Piece {           // 32 bytes: a slice of one hub vertex's list
 // Per push source (4) x 8 fields, then the overflow flag.
 enum { C_VERT = 0, C_VERT_NM, C_EDGES, C_EDGES_NM, C_PEDGES, C_PEDGES_NM, C_FIELDS = 8 };
 // Row 4 holds the fold kernel of dense (pull) passes.
-enum { C_OVERFLOW = 5 * C_FIELDS, C_NCOUNTERS = C_OVERFLOW + 1 };
+// rows 0-3: push sources; row 4: dense passes; row 5: dense passes that used the live-row
+// bitmap (their fold counts give the rows the pull read)
+enum { C_DENSE = 4, C_DENSE_LIVE = 5, C_OVERFLOW = 6 * C_FIELDS, C_NCOUNTERS = C_OVERFLOW + 1 };
+// the dense-pass rows have no piece edges; a spare field counts k_pull_hub's row reads
+enum { C_PULL_HUB_LIVE = C_PEDGES };
 
 __device__ __forceinline__ void atomic_add_f32(float* p, float v) {
     // No-return hardware float atomic on coarse-grained HBM (global_atomic_add_f32).
@@ -681,75 +685,117 @@ struct PullArgs {
     const uint64_t* tidx;       // transpose offsets [n+1]
     const int32_t* tadj;        // in-neighbours, as rows of p (FoldArgs::rank)
     const float* p;             // [sources][S]
+    const uint32_t* live;       // bit q set iff row q of p is nonzero this pass (k_pbits);
+                                // nullptr: every row is read
+    unsigned long long* live_count;   // k_pull_hub: rows read
 };
+
+// Bitmap of the p rows the fold made nonzero: one wave per 64 consecutive rows, one ballot,
+// two words.  Rows of sources that did not push (or pushed a zero weight) stay zero, so a
+// gather of them would add nothing; the pull tests the bit and skips the row.  Late dense
+// passes, where a few high-degree sources still push, then read the in-lists and the bitmap
+// but almost no rows.
+template <int NM, int S>
+__global__ __launch_bounds__(BLOCK) void k_pbits(const float* __restrict__ p, int nsrc, uint32_t* bits) {
+    const int lane = lane_id();
+    const long long waves = (long long)gridDim.x * NWAVES;
+    for (long long w = blockIdx.x * (long long)NWAVES + threadIdx.x / WAVE; w * WAVE < nsrc; w += waves) {
+        const long long q = w * WAVE + lane;
+        bool nz = false;
+        if (q < nsrc) {
+            const Row<S> row = load_row<S>(p, q);
+#pragma unroll
+            for (int k = 0; k < NM; k++) nz |= row.v[k] != 0.f;
+        }
+        const uint64_t b = __ballot(nz);
+        if (lane < 2) bits[2 * w + lane] = (uint32_t)(b >> (32 * lane));
+    }
+}
+
+__device__ __forceinline__ bool row_live(const uint32_t* __restrict__ live, int q) {
+    return !live || ((live[q >> 5] >> (q & 31)) & 1u);
+}
 
 constexpr uint32_t PULL_HUB = 4096;   // in-lists longer than this are summed by whole blocks
 
-// Block-level load-balanced pull: the block's 256 destinations' in-lists are laid end to
-// end; lane j takes slot j, j+256, ... and adds the source's p row into its destination's
-// LDS accumulator (LDS float atomics); then each destination adds its sum to r.  In-lists
-// longer than PULL_HUB are summed by k_pull_hub instead.
+// Sequential-slot pull.  A block owns BLOCK consecutive destinations; with hub in-lists
+// stored apart (k_pull_hub), their in-lists are one contiguous run of tadj.  The run is
+// staged through LDS in windows of PULL_WIN slots (coalesced global reads); lane t then
+// takes PULL_SEQ consecutive slots of the window, issues all their p-row loads at once
+// (memory-level parallelism), and sums them in registers, walking the destination
+// boundaries; a partial sum goes to the destination's LDS accumulator when the boundary is
+// crossed (a list spans few lanes, so the LDS float atomics rarely collide).  One binary
+// search per lane per window replaces a search per slot, and no cross-lane reduction is
+// needed.  Finally each destination adds its sum to r with one plain row update.
+// live != nullptr: rows whose bit is clear are zero this pass and are not loaded.
+constexpr int PULL_SEQ = 16;
+constexpr int PULL_WIN = BLOCK * PULL_SEQ;   // 4096 slots, 16 KB of LDS
+constexpr int PULL_LD = BLOCK + 1;           // LDS row pitch: slot j at (j % SEQ) * LD + j / SEQ
+
 template <int NM, int S>
 __global__ __launch_bounds__(BLOCK) void k_pull(PullArgs a, DevMeasures M) {
-    __shared__ uint32_t s_start[BLOCK];
-    __shared__ uint64_t s_ptr[BLOCK];
+    __shared__ uint32_t s_start[BLOCK + 1];
+    __shared__ int32_t s_src[PULL_SEQ * PULL_LD];
     __shared__ float s_acc[NM][BLOCK];
-    __shared__ uint32_t s_wave[NWAVES];
     const int tid = threadIdx.x;
-    const long long u = blockIdx.x * (long long)BLOCK + tid;
-    uint64_t b = 0, d = 0;
-    if (u < a.n) { b = a.tidx[u]; d = a.tidx[u + 1] - b; }
-    const uint32_t mydeg = d <= PULL_HUB ? (uint32_t)d : 0u;
-    uint32_t total;
-    const uint32_t excl = block_excl_scan(mydeg, s_wave, &total);
-    s_start[tid] = excl;
-    s_ptr[tid] = b;
+    const long long u0 = (long long)blockIdx.x * BLOCK;
+    const int nt = (int)((long long)a.n - u0 < BLOCK ? (long long)a.n - u0 : BLOCK);
+    const uint64_t base = a.tidx[u0];
+    for (int i = tid; i <= nt; i += BLOCK) s_start[i] = (uint32_t)(a.tidx[u0 + i] - base);
 #pragma unroll
     for (int k = 0; k < NM; k++) s_acc[k][tid] = 0.f;
     __syncthreads();
-    int lo = 0;
-    const int lane = lane_id();
-    const uint32_t rounds = (total + BLOCK - 1) / BLOCK;   // block-uniform trip count
-    for (uint32_t it = 0; it < rounds; it++) {
-        const uint32_t j = it * BLOCK + tid;
-        const bool valid = j < total;
-        int hi = BLOCK - 1;
-        if (valid)
+    const uint32_t total = s_start[nt];
+    for (uint32_t w0 = 0; w0 < total; w0 += PULL_WIN) {
+        const uint32_t wn = total - w0 < (uint32_t)PULL_WIN ? total - w0 : (uint32_t)PULL_WIN;
+        for (uint32_t j = tid; j < wn; j += BLOCK) s_src[(j % PULL_SEQ) * PULL_LD + j / PULL_SEQ] = a.tadj[base + w0 + j];
+        __syncthreads();
+        const uint32_t j0 = (uint32_t)tid * PULL_SEQ;
+        if (j0 < wn) {
+            const uint32_t jn = wn - j0 < (uint32_t)PULL_SEQ ? wn - j0 : (uint32_t)PULL_SEQ;
+            const uint32_t g = w0 + j0;   // first slot of this lane, relative to base
+            int lo = 0, hi = nt - 1;      // owner: the destination whose range holds slot g
             while (lo < hi) {
                 const int mid = (lo + hi + 1) >> 1;
-                if (s_start[mid] <= j) lo = mid; else hi = mid - 1;
+                if (s_start[mid] <= g) lo = mid; else hi = mid - 1;
             }
-        const int owner = valid ? lo : -1 - lane;   // invalid lanes form their own segments
-        float val[NM];
-        if (valid) {
-            const int v = a.tadj[s_ptr[lo] + (j - s_start[lo])];
-            const Row<S> pv = load_row<S>(a.p, v);
+            Row<S> rows[PULL_SEQ];
 #pragma unroll
-            for (int k = 0; k < NM; k++) val[k] = pv.v[k];
-        } else {
+            for (int q = 0; q < PULL_SEQ; q++) {
 #pragma unroll
-            for (int k = 0; k < NM; k++) val[k] = 0.f;
-        }
-        // segmented inclusive sum over lanes with the same owner (owners are non-decreasing
-        // along the wave, so a segment is a run of consecutive lanes)
-#pragma unroll
-        for (int o = 1; o < WAVE; o <<= 1) {
-            const int ow = __shfl_up(owner, o, WAVE);
-#pragma unroll
-            for (int k = 0; k < NM; k++) {
-                const float t = __shfl_up(val[k], o, WAVE);
-                if (lane >= o && ow == owner) val[k] += t;
+                for (int k = 0; k < S; k++) rows[q].v[k] = 0.f;
+                if ((uint32_t)q < jn) {
+                    const int v = s_src[q * PULL_LD + tid];
+                    if (row_live(a.live, v)) rows[q] = load_row<S>(a.p, v);
+                }
             }
-        }
-        const int next = __shfl_down(owner, 1, WAVE);
-        if (valid && (lane == WAVE - 1 || next != owner)) {
+            int owner = lo;
+            uint32_t next = s_start[owner + 1];
+            float acc[NM];
+#pragma unroll
+            for (int k = 0; k < NM; k++) acc[k] = 0.f;
+#pragma unroll
+            for (int q = 0; q < PULL_SEQ; q++) {
+                if ((uint32_t)q < jn) {
+                    while (g + q >= next) {   // crossing into the next non-empty list
+#pragma unroll
+                        for (int k = 0; k < NM; k++)
+                            if (acc[k] != 0.f) { atomicAdd(&s_acc[k][owner], acc[k]); acc[k] = 0.f; }
+                        owner++;
+                        next = s_start[owner + 1];
+                    }
+#pragma unroll
+                    for (int k = 0; k < NM; k++) acc[k] += rows[q].v[k];
+                }
+            }
 #pragma unroll
             for (int k = 0; k < NM; k++)
-                if (val[k] != 0.f) atomicAdd(&s_acc[k][owner], val[k]);
+                if (acc[k] != 0.f) atomicAdd(&s_acc[k][owner], acc[k]);
         }
+        __syncthreads();   // the next window rewrites s_src
     }
-    __syncthreads();
-    if (u < a.n && mydeg) {
+    if (tid < nt && s_start[tid + 1] > s_start[tid]) {
+        const long long u = u0 + tid;
 #pragma unroll
         for (int k = 0; k < NM; k++) {
             const float acc = s_acc[k][tid];
@@ -769,6 +815,10 @@ template <int NM, int S>
 __global__ __launch_bounds__(BLOCK) void k_pull_hub(PullArgs a, const PullPiece* __restrict__ pieces, int npieces,
                                                      DevMeasures M) {
     __shared__ float s_red[NM][NWAVES];
+    __shared__ unsigned long long s_live;
+    if (threadIdx.x == 0) s_live = 0;
+    __syncthreads();
+    uint32_t taken = 0;
     for (int h = blockIdx.x; h < npieces; h += gridDim.x) {
         const PullPiece pc = pieces[h];
         const int u = pc.u;
@@ -777,7 +827,10 @@ __global__ __launch_bounds__(BLOCK) void k_pull_hub(PullArgs a, const PullPiece*
 #pragma unroll
         for (int k = 0; k < NM; k++) acc[k] = 0.f;
         for (uint64_t i = b + threadIdx.x; i < e; i += BLOCK) {
-            const Row<S> pv = load_row<S>(a.p, a.tadj[i]);
+            const int v = a.tadj[i];
+            if (!row_live(a.live, v)) continue;
+            const Row<S> pv = load_row<S>(a.p, v);
+            taken++;
 #pragma unroll
             for (int k = 0; k < NM; k++) acc[k] += pv.v[k];
         }
@@ -796,6 +849,10 @@ __global__ __launch_bounds__(BLOCK) void k_pull_hub(PullArgs a, const PullPiece*
         }
         __syncthreads();
     }
+    const uint32_t t = wave_sum_u32(taken);
+    if (lane_id() == 0 && t) atomicAdd(&s_live, (unsigned long long)t);
+    __syncthreads();
+    if (threadIdx.x == 0 && s_live) atomicAdd(a.live_count, s_live);
 }
 
 // Transpose build: in-degree counts, then a fill through per-vertex cursors.
@@ -1147,6 +1204,11 @@ struct Ctx {
     int32_t* d_tadj = nullptr;
     float* d_p = nullptr;             // [n][S]
     int32_t* d_rank = nullptr;          // p row of each vertex (out-degree order)
+    uint32_t* d_pbits = nullptr;        // live-row bitmap of p, 2 * ceil(nsrc / 64) words
+    int nsrc = 0;                       // vertices with out-edges (rows of p)
+    int pull_live = 2;                  // live-row bitmap: 0 never, 1 always, 2 when few vertices pend
+    uint64_t last_pending = 0;          // pending vertices of the last snapshot
+    uint64_t pull_passes_live = 0;
     PullPiece* d_pull_hubs = nullptr;   // pieces of in-lists longer than PULL_HUB
     int n_pull_hubs = 0;
     uint64_t pull_hub_edges = 0, pull_hub_vertices = 0;
@@ -1360,12 +1422,23 @@ int push_launch(Ctx* c, int s, int src, unsigned grid, PushArgs a, int group, ui
 
 template <int NM, int S>
 void launch_fold_pull(Ctx* c, hipStream_t st) {
-    FoldArgs f{c->n, c->d_idx, c->d_p, c->d_rank, c->d_counters + 4 * C_FIELDS + 0};
-    PullArgs pa{c->n, c->d_tidx, c->d_tadj, c->d_p};
-    const unsigned grid = (unsigned)(((long long)c->n + BLOCK - 1) / BLOCK);
+    // the bitmap costs a probe per in-edge; it pays once few sources push (late passes,
+    // where the high-degree vertices that converge last keep the demand high)
+    const bool live = c->nsrc && (c->pull_live == 1 || (c->pull_live == 2 && c->last_pending * 16 <= (uint64_t)c->nsrc));
+    unsigned long long* row = c->d_counters + (live ? C_DENSE_LIVE : C_DENSE) * C_FIELDS;
+    FoldArgs f{c->n, c->d_idx, c->d_p, c->d_rank, row};
+    const int ntiles = (int)(((long long)c->n + BLOCK - 1) / BLOCK);
+    PullArgs pa{c->n, c->d_tidx, c->d_tadj, c->d_p, live ? c->d_pbits : nullptr, row + C_PULL_HUB_LIVE};
+    c->pull_passes_live += live;
     {
         ScopedTimer t(c, st, T_FOLD);
-        hipLaunchKernelGGL((k_fold<NM, S>), dim3(std::min<unsigned>(grid, c->cus * 8)), dim3(BLOCK), 0, st, f, c->DM);
+        hipLaunchKernelGGL((k_fold<NM, S>), dim3(std::min(ntiles, c->cus * 8)), dim3(BLOCK), 0, st, f, c->DM);
+    }
+    if (live) {
+        ScopedTimer t(c, st, T_OTHER);
+        const int waves = (c->nsrc + WAVE - 1) / WAVE;
+        hipLaunchKernelGGL((k_pbits<NM, S>), dim3(std::min((waves + NWAVES - 1) / NWAVES, c->cus * 8)), dim3(BLOCK), 0, st,
+                           c->d_p, c->nsrc, c->d_pbits);
     }
     if (c->n_pull_hubs) {
         ScopedTimer t(c, st, T_PULL_HUB);
@@ -1374,7 +1447,7 @@ void launch_fold_pull(Ctx* c, hipStream_t st) {
     }
     {
         ScopedTimer t(c, st, T_PULL);
-        hipLaunchKernelGGL((k_pull<NM, S>), dim3(grid), dim3(BLOCK), 0, st, pa, c->DM);
+        hipLaunchKernelGGL((k_pull<NM, S>), dim3(ntiles), dim3(BLOCK), 0, st, pa, c->DM);
     }
 }
 
@@ -1415,6 +1488,9 @@ int push_chunk(Ctx* c, int s, int v0, int v1, const int32_t* edges, uint64_t eba
     const unsigned grid = (unsigned)(((long long)(v1 - v0) + BLOCK - 1) / BLOCK);
     return push_launch(c, s, SRC_RANGE, grid, a, group, edges_bound);
 }
+
+// every column of a batched context (a state change can make any of them pending again)
+uint64_t all_columns(int K) { return K >= 64 ? ~0ull : ((1ull << K) - 1); }
 
 Ctx* as_ctx(void* p) { return static_cast<Ctx*>(p); }
 
@@ -1477,7 +1553,7 @@ int exc_create(const exc_config* cfg, const exc_measures* M, void** out) {
         c->S = c->K;
         c->nm = 1;
         c->damp = M->scale[0];
-        c->active = c->K == 64 ? ~0ull : ((1ull << c->K) - 1);
+        c->active = all_columns(c->K);
     }
     c->M = *M;
     c->zc = cfg->zero_copy != 0;
@@ -1570,7 +1646,7 @@ int exc_destroy(void* p) {
     for (auto& pb : c->pb) { if (pb.d) hipFree(pb.d); if (pb.cnt) hipFree(pb.cnt); }
     for (auto s : c->st) if (s) hipStreamDestroy(s);
     for (auto q : c->prows) if (q) (void)hipFree(q);
-    void* pull_ptrs[] = {c->d_tidx, c->d_tadj, c->d_p, c->d_pull_hubs, c->d_wts, c->d_rank};
+    void* pull_ptrs[] = {c->d_tidx, c->d_tadj, c->d_p, c->d_pull_hubs, c->d_wts, c->d_rank, c->d_pbits};
     for (void* q : pull_ptrs) if (q) (void)hipFree(q);
     if (c->d_beps) (void)hipFree(c->d_beps);
     if (c->d_colpend) (void)hipFree(c->d_colpend);
@@ -1594,6 +1670,7 @@ int exc_init_measure(void* p, int i, float beta, float lambda) {
     Ctx* c = as_ctx(p);
     if (!c || i < 0 || i >= c->nm) return fail(EXC_EINVAL, "exc_init_measure: bad slot");
     HIPCHK(hipSetDevice(c->dev));
+    if (c->K) c->active = all_columns(c->K);
     if (c->K) {   // batched: every column starts at x = r = 0 (exc_batch_seed adds the teleport mass)
         const size_t bytes = (size_t)c->n * c->S * sizeof(float);
         HIPCHK(hipMemsetAsync(c->d_x, 0, bytes, c->st[0]));
@@ -1613,6 +1690,7 @@ int exc_set_state(void* p, int i, const float* h_x, const float* h_r) {
     Ctx* c = as_ctx(p);
     if (!c || i < 0 || i >= (c->K ? c->K : c->nm)) return fail(EXC_EINVAL, "exc_set_state: bad slot");
     HIPCHK(hipSetDevice(c->dev));
+    if (c->K) c->active = all_columns(c->K);
     float* tmp;
     const size_t bytes = (size_t)c->n * sizeof(float);
     HIPCHK(hipMalloc(&tmp, bytes));
@@ -1650,6 +1728,7 @@ int exc_state_checkpoint(void* p, int restore) {
     Ctx* c = as_ctx(p);
     if (!c) return fail(EXC_EINVAL, "exc_state_checkpoint: null context");
     HIPCHK(hipSetDevice(c->dev));
+    if (c->K) c->active = all_columns(c->K);
     const size_t bytes = (size_t)c->n * c->S * sizeof(float);
     if (restore) {
         if (!c->d_x0) return fail(EXC_ESTATE, "exc_state_checkpoint: nothing saved");
@@ -1722,7 +1801,8 @@ int exc_snapshot(void* p, int hot_skip, uint64_t* h_cnt, uint64_t* h_hotpend) {
     memcpy(h_cnt, c->h_small, 2 * (size_t)c->nc * sizeof(uint64_t));
     if (h_hotpend) *h_hotpend = c->h_small[2 * c->nc];
     c->last_demand = 0;
-    for (int i = 0; i < c->nc; i++) c->last_demand += c->h_small[2 * i + 1];
+    c->last_pending = 0;
+    for (int i = 0; i < c->nc; i++) { c->last_demand += c->h_small[2 * i + 1]; c->last_pending += c->h_small[2 * i]; }
     return EXC_OK;
 }
 
@@ -1969,6 +2049,7 @@ int exc_batch_seed(void* p, int col, const int32_t* seeds, int count, float valu
         if (seeds[i] < 0 || seeds[i] >= c->n) return fail(EXC_EINVAL, "exc_batch_seed: seed %d out of range", seeds[i]);
     if (!count) return EXC_OK;
     HIPCHK(hipSetDevice(c->dev));
+    if (c->K) c->active = all_columns(c->K);
     int32_t* d;
     HIPCHK(hipMalloc(&d, (size_t)count * sizeof(int32_t)));
     hipError_t e = hipMemcpy(d, seeds, (size_t)count * sizeof(int32_t), hipMemcpyHostToDevice);
@@ -2026,7 +2107,7 @@ int exc_pushed(void* p, uint64_t* edges) {
     unsigned long long h[C_NCOUNTERS];
     HIPCHK(hipMemcpy(h, c->d_counters, sizeof h, hipMemcpyDeviceToHost));
     uint64_t e = 0;
-    for (int s = 0; s < 5; s++) e += h[s * C_FIELDS + C_EDGES] + h[s * C_FIELDS + C_PEDGES];
+    for (int s = 0; s < 6; s++) e += h[s * C_FIELDS + C_EDGES] + (s < 4 ? h[s * C_FIELDS + C_PEDGES] : 0ull);
     *edges = e;
     if (h[C_OVERFLOW]) return fail(EXC_ESTATE, "hub piece buffer overflowed %llu times", h[C_OVERFLOW]);
     return EXC_OK;
@@ -2060,25 +2141,36 @@ int exc_get_stats(void* p, exc_stats* o) {
         o->pushed_edges += r[C_EDGES] + r[C_PEDGES];
         o->pushed_edge_measures += r[C_EDGES_NM] + r[C_PEDGES_NM];
     }
-    {   // dense passes: fold (row 4) and pull
-        const unsigned long long* r = h + 4 * C_FIELDS;
+    {   // dense passes: fold (rows 4 and 5) and pull
+        const unsigned long long* r4 = h + C_DENSE * C_FIELDS;
+        const unsigned long long* r5 = h + C_DENSE_LIVE * C_FIELDS;
         const int S = c->S, nm = c->K ? c->K : c->nm;
+        const double P = (double)c->pull_passes, PL = (double)c->pull_passes_live;
         o->fold_ms = c->t_ms[T_FOLD];
         o->pull_ms = c->t_ms[T_PULL];
         o->pull_hub_ms = c->t_ms[T_PULL_HUB];
         o->pull_passes = c->pull_passes;
         // fold: read r and write p for every vertex; offsets, x read + write for folded ones
-        o->fold_bytes = (double)c->pull_passes * c->n * (4.0 * nm + 4.0 * S) + 16.0 * r[C_VERT] + 8.0 * r[C_VERT_NM];
-        // pull: every in-edge (id + source p of nm floats), every vertex (offsets, r read + write);
-        // in-lists above the hub threshold go to the hub kernel
+        o->fold_bytes = P * c->n * (4.0 * nm + 4.0 * S) + 16.0 * (r4[C_VERT] + r5[C_VERT]) +
+                        8.0 * (r4[C_VERT_NM] + r5[C_VERT_NM]);
+        // pull: every in-edge's id, every vertex's offsets and r (read + write), and the p rows
+        // read: every in-edge's in a plain pass; in a pass with the live-row bitmap, the
+        // bitmap plus one row per out-edge of a pushing source (the fold's edge count -- a
+        // pushed weight is zero only on underflow), split between the kernels by
+        // k_pull_hub's own count.  In-lists above the hub threshold go to the hub kernel.
         const double me = (double)(c->m - c->pull_hub_edges), nv = (double)(c->n - c->pull_hub_vertices);
-        o->pull_bytes = (double)c->pull_passes * (me * (4.0 + 4.0 * nm) + nv * (8.0 + 8.0 * nm));
-        o->pull_hub_bytes = (double)c->pull_passes *
-                            ((double)c->pull_hub_edges * (4.0 + 4.0 * nm) + (double)c->pull_hub_vertices * (8.0 + 8.0 * nm));
-        o->pushed_vertices += r[C_VERT];
-        o->pushed_vertex_measures += r[C_VERT_NM];
-        o->pushed_edges += r[C_EDGES];
-        o->pushed_edge_measures += r[C_EDGES_NM];
+        const double bm = 8.0 * ((c->nsrc + 63) / 64);
+        const double hub_rows = (P - PL) * c->pull_hub_edges + (double)r5[C_PULL_HUB_LIVE];
+        const double rows = (P - PL) * me + std::max(0.0, (double)r5[C_EDGES] - (double)r5[C_PULL_HUB_LIVE]);
+        o->pull_bytes = P * (me * 4.0 + nv * (8.0 + 8.0 * nm)) + PL * bm + 4.0 * nm * rows;
+        o->pull_hub_bytes = P * ((double)c->pull_hub_edges * 4.0 + (double)c->pull_hub_vertices * (8.0 + 8.0 * nm)) +
+                            PL * bm + 4.0 * nm * hub_rows;
+        for (const unsigned long long* r : {r4, r5}) {
+            o->pushed_vertices += r[C_VERT];
+            o->pushed_vertex_measures += r[C_VERT_NM];
+            o->pushed_edges += r[C_EDGES];
+            o->pushed_edge_measures += r[C_EDGES_NM];
+        }
     }
     if (h[C_OVERFLOW]) return fail(EXC_ESTATE, "hub piece buffer overflowed %llu times", h[C_OVERFLOW]);
     return EXC_OK;
@@ -2091,6 +2183,10 @@ int exc_enable_pull(void* p, double threshold) {
     HIPCHK(hipSetDevice(c->dev));
     HIPCHK(hipDeviceSynchronize());
     c->pull_threshold = threshold;
+    {   // EXC_PULL_LIVE=0 / 1: never / always use the live-row bitmap (measurements)
+        const char* ev = getenv("EXC_PULL_LIVE");
+        c->pull_live = ev && (ev[0] == '0' || ev[0] == '1') ? ev[0] - '0' : 2;
+    }
     if (c->d_tadj) return EXC_OK;
     const size_t n = (size_t)c->n;
     // rows of p in decreasing out-degree order (ties by id); vertices without out-edges last
@@ -2108,14 +2204,18 @@ int exc_enable_pull(void* p, double threshold) {
     unsigned long long* cnt = nullptr;
     bool ok = hipMalloc(&c->d_tidx, (n + 1) * sizeof(uint64_t)) == hipSuccess;
     ok = ok && hipMalloc(&c->d_rank, n * sizeof(int32_t)) == hipSuccess;
+    c->nsrc = 0;
+    for (size_t v = 0; v < n; v++) c->nsrc += hidx[v + 1] > hidx[v];
+    ok = ok && hipMalloc(&c->d_pbits, (2 * (((size_t)c->nsrc + 63) / 64) + 2) * sizeof(uint32_t)) == hipSuccess;
     ok = ok && hipMalloc(&c->d_tadj, (c->m ? c->m : 1) * sizeof(int32_t)) == hipSuccess;
     ok = ok && hipMalloc(&c->d_p, n * c->S * sizeof(float)) == hipSuccess;
     ok = ok && hipMalloc(&cnt, (n + 1) * sizeof(unsigned long long)) == hipSuccess;
     if (!ok) {
         (void)hipGetLastError();
-        void* q[] = {c->d_tidx, c->d_tadj, c->d_p, cnt, c->d_rank};
+        void* q[] = {c->d_tidx, c->d_tadj, c->d_p, cnt, c->d_rank, c->d_pbits};
         for (void* x : q) if (x) (void)hipFree(x);
-        c->d_tidx = nullptr; c->d_tadj = nullptr; c->d_p = nullptr; c->d_rank = nullptr;
+        c->d_tidx = nullptr; c->d_tadj = nullptr; c->d_p = nullptr; c->d_rank = nullptr; c->d_pbits = nullptr;
+        c->nsrc = 0;
         c->pull_threshold = -1;
         return fail(EXC_ENOMEM, "exc_enable_pull: cannot allocate the transpose (%.2f GB)", c->m * 4.0 / 1e9);
     }
@@ -2130,22 +2230,28 @@ int exc_enable_pull(void* p, double threshold) {
     std::vector<uint64_t> h(n + 1);
     e = hipMemcpy(h.data(), cnt, n * sizeof(uint64_t), hipMemcpyDeviceToHost);
     if (e != hipSuccess) return fail_free(e);
+    // in-lists of at most PULL_HUB entries first, in vertex order (d_tidx; a hub's range there
+    // is empty), then the hub lists (cut into pieces for k_pull_hub); cur = fill cursors
     std::vector<PullPiece> hubs;
+    std::vector<uint64_t> off(n + 1), cur(n);
     uint64_t acc = 0;
     for (size_t v = 0; v < n; v++) {
+        off[v] = cur[v] = acc;
+        if (h[v] <= PULL_HUB) acc += h[v];
+    }
+    off[n] = acc;
+    for (size_t v = 0; v < n; v++) {
         const uint64_t d = h[v];
-        if (d > PULL_HUB) {
-            c->pull_hub_edges += d;
-            c->pull_hub_vertices++;
-            for (uint64_t o = 0; o < d; o += PULL_PIECE)
-                hubs.push_back({(int32_t)v, (uint32_t)std::min<uint64_t>(PULL_PIECE, d - o), acc + o});
-        }
-        h[v] = acc;
+        if (d <= PULL_HUB) continue;
+        c->pull_hub_edges += d;
+        c->pull_hub_vertices++;
+        for (uint64_t o = 0; o < d; o += PULL_PIECE)
+            hubs.push_back({(int32_t)v, (uint32_t)std::min<uint64_t>(PULL_PIECE, d - o), acc + o});
+        cur[v] = acc;
         acc += d;
     }
-    h[n] = acc;
-    e = hipMemcpy(c->d_tidx, h.data(), (n + 1) * sizeof(uint64_t), hipMemcpyHostToDevice);
-    if (e == hipSuccess) e = hipMemcpy(cnt, h.data(), (n + 1) * sizeof(uint64_t), hipMemcpyHostToDevice);
+    e = hipMemcpy(c->d_tidx, off.data(), (n + 1) * sizeof(uint64_t), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(cnt, cur.data(), n * sizeof(uint64_t), hipMemcpyHostToDevice);
     if (e != hipSuccess) return fail_free(e);
     hipLaunchKernelGGL(k_fill_in64, dim3(c->cus * 16), dim3(BLOCK), 0, 0, c->n, c->d_idx, c->d_arena, cnt, c->d_rank, c->d_tadj);
     e = hipDeviceSynchronize();
@@ -2171,6 +2277,7 @@ int exc_reset_stats(void* p) {
     for (int i = 0; i < T_NCLS; i++) { c->t_ms[i] = 0; c->t_launch[i] = 0; }
     c->snap_bytes = 0;
     c->pull_passes = 0;
+    c->pull_passes_live = 0;
     return EXC_OK;
 }
 

@@ -139,6 +139,9 @@ HIP_SIGNATURES = {
     "exc_fp64_solve": (c_int, [c_void_p, c_int, c_double, c_double, c_int, c_double, c_void_p, c_void_p,
                                P(c_double), P(c_double), P(c_int)]),
     "exc_fp64_lambda": (c_int, [c_void_p, c_void_p, c_int, c_double, P(c_double), P(c_double), P(c_int)]),
+    "exc_fp64_residual": (c_int, [c_void_p, c_double, c_void_p, c_double, c_int, c_void_p, P(c_double)]),
+    "exc_fp64_solve_vec": (c_int, [c_void_p, c_void_p, c_double, c_int, c_int, c_double, c_void_p, P(c_double),
+                                   P(c_int)]),
     "exc_fp64_katz_mean": (c_int, [c_void_p, c_double, c_double, P(c_double), P(c_int)]),
 }
 
@@ -152,6 +155,9 @@ HOST_SIGNATURES = {
     "excent_session_open": (c_int, [c_int, P(ctypes.c_char_p), P(c_void_p)]),
     "excent_session_run": (c_int, [c_void_p, c_int, P(RunStats)]),
     "excent_session_results": (c_int, [c_void_p]),
+    "excent_session_set_column": (c_int, [c_void_p, c_int]),
+    "excent_session_values": (c_int, [c_void_p, c_int, c_int, c_void_p]),
+    "excent_session_vertices": (c_int, [c_void_p]),
     "excent_session_close": (c_int, [c_void_p]),
     "excent_session_error": (ctypes.c_char_p, []),
     "excent_dpr_create": (c_int, [c_int, c_int, c_u64, c_void_p, c_void_p, c_void_p, c_float, c_float, P(c_void_p)]),
@@ -217,6 +223,20 @@ class Session:
         out = {k: getattr(st, k) for k, _ in RunStats._fields_ if k != "kernel"}
         out["kernel"] = stats_dict(st.kernel)
         return out
+
+    def set_column(self, j):
+        rc = F["excent_session_set_column"](self.h, int(j))
+        if rc != 0:
+            raise ExcError(rc, F["excent_session_error"]().decode())
+
+    def values(self, column=0, which=0):
+        """x (which = 0) or r (which = 1) of a measure slot or batched column, float32[n]."""
+        n = F["excent_session_vertices"](self.h)
+        out = np.zeros(max(n, 1), dtype=np.float32)
+        rc = F["excent_session_values"](self.h, int(column), int(which), ptr(out))
+        if rc != 0:
+            raise ExcError(rc, F["excent_session_error"]().decode())
+        return out[:n]
 
     def results(self):
         rc = F["excent_session_results"](self.h)
@@ -392,6 +412,60 @@ class Context:
         out = np.zeros(self.n, dtype=np.uint32)
         check(F["exc_indegree"](self.ctx, ptr(out)), "indegree")
         return out
+
+
+class Fp64Solver:
+    """The product's FP64 pull solver on one device (include/excent_hip.h exc_fp64_*), kept
+    across solves and residual evaluations of one graph."""
+
+    def __init__(self, idx, adj, device=0):
+        idx = np.ascontiguousarray(idx, dtype=np.uint64)
+        adj = np.ascontiguousarray(adj, dtype=np.int32)
+        self.n = len(idx) - 1
+        self.h = c_void_p()
+        check(F["exc_fp64_create"](device, self.n, int(idx[-1]), ptr(idx), ptr(adj), ctypes.byref(self.h)),
+              "fp64_create")
+
+    def solve(self, kind, alpha=-1.0, beta=1.0, max_iter=20000, tol=1e-9):
+        out = np.zeros(self.n, dtype=np.float64)
+        rel, lam, it = c_double(0), c_double(0), c_int(0)
+        check(F["exc_fp64_solve"](self.h, kind, alpha, beta, max_iter, tol, None, ptr(out), ctypes.byref(rel),
+                                  ctypes.byref(lam), ctypes.byref(it)), "fp64_solve")
+        return out, rel.value, lam.value, it.value
+
+    def residual(self, x, c, s, divdeg):
+        """||c + s W x - x||_1 in FP64 for a float32 state x; c a scalar or a vector[n]."""
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        assert len(x) == self.n
+        out = c_double(0)
+        cv = None
+        if np.ndim(c):
+            cv = np.ascontiguousarray(c, dtype=np.float64)
+            assert len(cv) == self.n
+        check(F["exc_fp64_residual"](self.h, 0.0 if cv is not None else float(c), None if cv is None else ptr(cv),
+                                     float(s), int(divdeg), ptr(x), ctypes.byref(out)), "fp64_residual")
+        return out.value
+
+    def solve_vec(self, c, s, divdeg, max_iter=20000, tol=1e-12):
+        """Jacobi x = c + s W x for a per-vertex constant c (FP64)."""
+        cv = np.ascontiguousarray(c, dtype=np.float64)
+        assert len(cv) == self.n
+        out = np.zeros(self.n, dtype=np.float64)
+        rel, it = c_double(0), c_int(0)
+        check(F["exc_fp64_solve_vec"](self.h, ptr(cv), float(s), int(divdeg), int(max_iter), float(tol), ptr(out),
+                                      ctypes.byref(rel), ctypes.byref(it)), "fp64_solve_vec")
+        return out, rel.value, it.value
+
+    def close(self):
+        if self.h:
+            F["exc_fp64_destroy"](self.h)
+            self.h = c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
 
 
 def fp64_solve(idx, adj, kind, alpha=-1.0, beta=1.0, max_iter=20000, tol=1e-9, device=0):

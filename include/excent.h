@@ -39,6 +39,14 @@ typedef struct excent_run_stats {
 int excent_session_open(int argc, const char* const* argv, void** session);
 int excent_session_run(void* session, int reinit, excent_run_stats* stats);
 int excent_session_results(void* session);
+/* Sessions opened with --measure pr --ppr-column J: restart from column `column`'s teleport
+ * set (fresh state, statistics cleared); the next run(reinit = 0) solves that column. */
+int excent_session_set_column(void* session, int column);
+/* The session's state after a run: x (which = 0) or r (which = 1) of measure slot / batched
+ * column `column`, n floats into out (what --dump writes, without the file). */
+int excent_session_values(void* session, int column, int which, float* out);
+/* Number of vertices of the session's graph. */
+int excent_session_vertices(void* session);
 int excent_session_close(void* session);
 const char* excent_session_error(void);
 /* bin/excent_rmat: seeded R-MAT graph writer (no reference counterpart). */

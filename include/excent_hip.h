@@ -226,6 +226,17 @@ int exc_fp64_destroy(void* solver);
  * Iterates until rel < tol or max_iter.  Returns iterations via *iters. */
 int exc_fp64_solve(void* solver, int kind, double alpha, double beta, int max_iter, double tol,
                    const double* h_x0, double* h_out, double* rel, double* lambda, int* iters);
+/* FP64 global residual ||c + s W x - x||_1 of a float32 state (W = the transpose's pull,
+ * scaled by 1/outdeg when divdeg; the constant is c, or the vector h_c[n] when non-NULL),
+ * the quantity the PageRank bound ||x - x*||_1 <= ||r||_1 / (1 - d) is stated in
+ * (src/excent.cu:100-124 keeps the invariant x + r = c + s W x;
+ * tools/centrality_reference.cu has no residual). */
+int exc_fp64_residual(void* solver, double c, const double* h_c, double s, int divdeg, const float* h_x, double* l1);
+/* Jacobi x = c + s W x with a per-vertex constant h_c[n] (personalised PageRank: c =
+ * (1 - d) * teleport), from x = c until max|x' - x| / max|x'| < tol or max_iter
+ * (tools/centrality_reference.cu:184-214 with a vector constant). */
+int exc_fp64_solve_vec(void* solver, const double* h_c, double s, int divdeg, int max_iter, double tol, double* h_out,
+                       double* rel, int* iters);
 /* lambda1 by power iteration from h_start (NULL = ones), stop at rel change < tol. */
 int exc_fp64_lambda(void* solver, const double* h_start, int max_iter, double tol,
                     double* lambda, double* mean_ev, int* iters);

@@ -122,3 +122,51 @@ def test_batched_equals_plain_runs_within_bound(rmat12):
         for xx in (X[j], x[0]):
             err = np.abs(xx - ref).sum()
             assert err <= np.abs(true_residual(idx, adj, xx, b)).sum() / 0.15 * (1 + 1e-6) + 1e-6
+
+
+def test_session_column_runs_match_batched_columns(rmat12):
+    """bin/excent --measure pr --ppr-column j (the plain solver on column j's teleport set)
+    and --measure ppr (all K columns at once) through the session API: the plain start is
+    exactly column j's seeds at mass (1 - d) n / S, and both results sit within the exact
+    FP64 bound of the product's vector-constant FP64 solve, itself pinned to the oracle's."""
+    idx, adj = orc.egr_load(rmat12)
+    n = len(idx) - 1
+    gen, eps = "4:6:2:11", 1e-5
+    common = [rmat12, "--cache", "chunks", "--preload", "--ppr-gen", gen, "--eps-ppr", eps]
+    with nat.Session(common + ["--measure", "ppr"]) as S:
+        S.run(reinit=False)
+        xb = [S.values(j) for j in range(4)]
+    mass = np.float32((1.0 - float(np.float32(0.85))) * n / 6)
+    with nat.Session(common + ["--measure", "pr", "--ppr-column", 0, "--pull", "0.001"]) as S, \
+            nat.Fp64Solver(idx, adj) as F:
+        for j in range(4):
+            S.set_column(j)
+            r0 = S.values(0, which=1)
+            assert np.count_nonzero(r0) == 6 and np.all(r0[r0 != 0] == mass)
+            S.run(reinit=False)
+            xp = S.values(0)
+            c = r0.astype(np.float64)
+            ref, _, _ = F.solve_vec(c, float(np.float32(0.85)), 1, tol=1e-13)
+            oref, _, _ = orc.fp64_solve_b(idx, adj, c, s=float(np.float32(0.85)))
+            assert np.allclose(ref, oref, rtol=1e-9, atol=1e-9)
+            for x in (xp, xb[j]):
+                bound = F.residual(x, c, float(np.float32(0.85)), 1) / (1 - 0.85)
+                assert np.abs(x - ref).sum() <= bound * (1 + 1e-6) + 1e-6
+                assert np.abs(x - ref).sum() <= eps * n / (1 - 0.85)
+
+
+def test_batched_session_rerun_after_reinit(rmat12):
+    """A second batched run from the restored initial state pushes again (every column
+    becomes active with the restored state) and lands within the same bound."""
+    idx, adj = orc.egr_load(rmat12)
+    n = len(idx) - 1
+    with nat.Session([rmat12, "--cache", "chunks", "--preload", "--ppr-gen", "3:5:2:4", "--eps-ppr", 1e-5,
+                      "--measure", "ppr"]) as S:
+        r1 = S.run(reinit=False)
+        x1 = [S.values(j) for j in range(3)]
+        r2 = S.run(reinit=True)
+        x2 = [S.values(j) for j in range(3)]
+    assert r1["passes"] > 0 and r2["passes"] > 0
+    assert r2["kernel"]["pushed_edge_measures"] > 0
+    for a, b in zip(x1, x2):
+        assert np.abs(a - b).sum() <= 2 * 1e-5 * n / (1 - 0.85)

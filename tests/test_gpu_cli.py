@@ -126,6 +126,20 @@ def test_reference_tool_matches_oracle(tmp_path, rmat12):
             assert np.allclose(got, ref, rtol=1e-9, atol=1e-12), (path, kind)
 
 
+def test_fp64_residual_matches_oracle(rmat12):
+    """exc_fp64_residual (the bench's FP64 ||r||_1) against the oracle's CPU residual, plus
+    the closed form at x = 0 (every vertex contributes |c|)."""
+    idx, adj = orc.egr_load(rmat12)
+    n = len(idx) - 1
+    x = np.random.default_rng(3).random(n).astype(np.float32) * 2
+    with nat.Fp64Solver(idx, adj) as S:
+        for kind, c, s, dd in ((orc.ORC_PR, 0.15, 0.85, 1), (orc.ORC_KATZ, 1.0, 0.01, 0), (orc.ORC_EV, 0.0, 0.02, 0)):
+            got = S.residual(x, c, s, dd)
+            want = orc.residual_l1(idx, adj, kind, s, c, x)
+            assert abs(got - want) <= 1e-10 * want, (kind, got, want)
+        assert S.residual(np.zeros(n, np.float32), 0.15, 0.85, 1) == pytest.approx(0.15 * n, rel=1e-12)
+
+
 def test_params_tool_reproduces_sample_env(tmp_path):
     out = str(tmp_path / "s.env")
     rc = nat.run_entry("excent_params_main", [SAMPLE_EGR, out])
