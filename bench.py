@@ -35,6 +35,24 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+class native_stdout_to_stderr:
+    """The generator and constants tools print progress with C stdio; keep it off stdout,
+    which carries only the JSON line."""
+
+    def __enter__(self):
+        import ctypes
+        self.libc = ctypes.CDLL(None)
+        sys.stdout.flush()
+        self.libc.fflush(None)
+        self.saved = os.dup(1)
+        os.dup2(2, 1)
+
+    def __exit__(self, *a):
+        self.libc.fflush(None)
+        os.dup2(self.saved, 1)
+        os.close(self.saved)
+
+
 def read_env(path):
     out = {}
     for line in open(path):
@@ -281,7 +299,8 @@ def main():
         gargs = ["--scale", args.scale, "--edge-factor", args.edge_factor, "--seed", args.seed]
         if wtag:
             gargs.append("--weights")
-        rc = nat.run_entry("excent_rmat_main", gargs + [graph])
+        with native_stdout_to_stderr():
+            rc = nat.run_entry("excent_rmat_main", gargs + [graph])
         assert rc == 0, "graph generation failed"
     log(f"graph {graph} ready in {time.time() - t0:.1f}s")
     if args.config != "headline":
@@ -291,8 +310,9 @@ def main():
     envp = graph + ".env"
     t0 = time.time()
     if not os.path.exists(envp):
-        rc = nat.run_entry("excent_params_main", [graph, envp, "--directed", "--alpha-factor", "0.5", "--seed", "7",
-                                                  "--eps-abs", "1e-6"])
+        with native_stdout_to_stderr():
+            rc = nat.run_entry("excent_params_main", [graph, envp, "--directed", "--alpha-factor", "0.5", "--seed",
+                                                      "7", "--eps-abs", "1e-6"])
         assert rc == 0, "constants failed"
     env = read_env(envp)
     log(f"constants {env} in {time.time() - t0:.1f}s")

@@ -688,6 +688,13 @@ struct PullArgs {
     const uint32_t* live;       // bit q set iff row q of p is nonzero this pass (k_pbits);
                                 // nullptr: every row is read
     unsigned long long* live_count;   // k_pull_hub: rows read
+    // k_pull: the next pass's snapshot (bitmap + per-chunk counts), fused into the final
+    // row update; bitmap == nullptr: not fused
+    const uint64_t* idx;
+    const int* cv0;
+    int nc;
+    uint32_t* bitmap;
+    unsigned long long* cnt;
 };
 
 // Bitmap of the p rows the fold made nonzero: one wave per 64 consecutive rows, one ballot,
@@ -794,13 +801,64 @@ __global__ __launch_bounds__(BLOCK) void k_pull(PullArgs a, DevMeasures M) {
         }
         __syncthreads();   // the next window rewrites s_src
     }
-    if (tid < nt && s_start[tid + 1] > s_start[tid]) {
-        const long long u = u0 + tid;
+    if (!a.bitmap) {
+        if (tid < nt && s_start[tid + 1] > s_start[tid]) {
+            const long long u = u0 + tid;
 #pragma unroll
-        for (int k = 0; k < NM; k++) {
-            const float acc = s_acc[k][tid];
-            if (acc != 0.f) M.r[u * S + k] += acc;
+            for (int k = 0; k < NM; k++) {
+                const float acc = s_acc[k][tid];
+                if (acc != 0.f) M.r[u * S + k] += acc;
+            }
         }
+        return;
+    }
+    // Fused snapshot: every destination's row is final here (the hub rows were completed by
+    // k_pull_hub before this kernel), so the pending test of the next pass -- the same
+    // predicate and outputs as k_snapshot without hot set or trace -- runs on the row in
+    // registers: two bitmap words per wave, per-chunk counts reduced per block.
+    __shared__ unsigned long long s_cnt[NWAVES][2];
+    bool pend = false;
+    uint64_t d = 0;
+    if (tid < nt) {
+        const long long u = u0 + tid;
+        Row<S> row = load_row<S>(M.r, u);
+        if (s_start[tid + 1] > s_start[tid]) {
+            bool upd = false;
+#pragma unroll
+            for (int k = 0; k < NM; k++) {
+                const float acc = s_acc[k][tid];
+                if (acc != 0.f) { row.v[k] += acc; upd = true; }
+            }
+            if (upd) store_row<S>(M.r, u, row);
+        }
+#pragma unroll
+        for (int k = 0; k < NM; k++) pend |= fabsf(row.v[k]) > M.eps[k];
+        if (pend) d = a.idx[u + 1] - a.idx[u];
+    }
+    const uint64_t mask = __ballot(pend);
+    const long long wb = u0 + (tid & ~(WAVE - 1));
+    const int lane = lane_id();
+    if (lane == 0 && wb < a.n) {
+        const long long w0 = wb >> 5, words = ((long long)a.n + 31) / 32;
+        a.bitmap[w0] = (uint32_t)mask;
+        if (w0 + 1 < words) a.bitmap[w0 + 1] = (uint32_t)(mask >> 32);
+    }
+    const int c0 = chunk_of((int)u0, a.cv0, a.nc), c1 = chunk_of((int)(u0 + nt - 1), a.cv0, a.nc);
+    if (c0 == c1) {   // the usual case: the tile lies in one chunk
+        const uint32_t tn = wave_sum_u32(pend ? 1u : 0u);
+        const uint64_t td = wave_sum_u64(d);
+        if (lane == 0) { s_cnt[tid / WAVE][0] = tn; s_cnt[tid / WAVE][1] = td; }
+        __syncthreads();
+        if (tid < 2) {   // thread 0: pending vertices, thread 1: their out-degree sum
+            unsigned long long t = 0;
+#pragma unroll
+            for (int w = 0; w < NWAVES; w++) t += s_cnt[w][tid];
+            if (t) atomicAdd(&a.cnt[2 * c0 + tid], t);
+        }
+    } else if (pend) {
+        const int c = chunk_of((int)(u0 + tid), a.cv0, a.nc);
+        atomicAdd(&a.cnt[2 * c], 1ull);
+        atomicAdd(&a.cnt[2 * c + 1], (unsigned long long)d);
     }
 }
 
@@ -810,6 +868,7 @@ __global__ __launch_bounds__(BLOCK) void k_pull(PullArgs a, DevMeasures M) {
 // k_pull in the pass; the two kernels touch disjoint rows of r.
 struct PullPiece { int32_t u; uint32_t len; uint64_t start; };
 constexpr uint32_t PULL_PIECE = 8192;
+constexpr int HUB_UNROLL = 8;
 
 template <int NM, int S>
 __global__ __launch_bounds__(BLOCK) void k_pull_hub(PullArgs a, const PullPiece* __restrict__ pieces, int npieces,
@@ -826,13 +885,26 @@ __global__ __launch_bounds__(BLOCK) void k_pull_hub(PullArgs a, const PullPiece*
         float acc[NM];
 #pragma unroll
         for (int k = 0; k < NM; k++) acc[k] = 0.f;
-        for (uint64_t i = b + threadIdx.x; i < e; i += BLOCK) {
-            const int v = a.tadj[i];
-            if (!row_live(a.live, v)) continue;
-            const Row<S> pv = load_row<S>(a.p, v);
-            taken++;
+        // HUB_UNROLL entries per lane per step: their ids, then their rows, are loaded
+        // together so each lane keeps several gathers in flight
+        for (uint64_t i0 = b + threadIdx.x; i0 < e; i0 += (uint64_t)BLOCK * HUB_UNROLL) {
+            int vs[HUB_UNROLL];
 #pragma unroll
-            for (int k = 0; k < NM; k++) acc[k] += pv.v[k];
+            for (int q = 0; q < HUB_UNROLL; q++) {
+                const uint64_t i = i0 + (uint64_t)q * BLOCK;
+                vs[q] = i < e ? a.tadj[i] : -1;
+            }
+            Row<S> rows[HUB_UNROLL];
+#pragma unroll
+            for (int q = 0; q < HUB_UNROLL; q++) {
+#pragma unroll
+                for (int k = 0; k < S; k++) rows[q].v[k] = 0.f;
+                if (vs[q] >= 0 && row_live(a.live, vs[q])) { rows[q] = load_row<S>(a.p, vs[q]); taken++; }
+            }
+#pragma unroll
+            for (int q = 0; q < HUB_UNROLL; q++)
+#pragma unroll
+                for (int k = 0; k < NM; k++) acc[k] += rows[q].v[k];
         }
 #pragma unroll
         for (int k = 0; k < NM; k++) {
@@ -1207,6 +1279,7 @@ struct Ctx {
     uint32_t* d_pbits = nullptr;        // live-row bitmap of p, 2 * ceil(nsrc / 64) words
     int nsrc = 0;                       // vertices with out-edges (rows of p)
     int pull_live = 2;                  // live-row bitmap: 0 never, 1 always, 2 when few vertices pend
+    bool snap_ready = false;            // the last dense pass computed the next snapshot
     uint64_t last_pending = 0;          // pending vertices of the last snapshot
     uint64_t pull_passes_live = 0;
     PullPiece* d_pull_hubs = nullptr;   // pieces of in-lists longer than PULL_HUB
@@ -1428,8 +1501,16 @@ void launch_fold_pull(Ctx* c, hipStream_t st) {
     unsigned long long* row = c->d_counters + (live ? C_DENSE_LIVE : C_DENSE) * C_FIELDS;
     FoldArgs f{c->n, c->d_idx, c->d_p, c->d_rank, row};
     const int ntiles = (int)(((long long)c->n + BLOCK - 1) / BLOCK);
-    PullArgs pa{c->n, c->d_tidx, c->d_tadj, c->d_p, live ? c->d_pbits : nullptr, row + C_PULL_HUB_LIVE};
+    PullArgs pa{c->n, c->d_tidx, c->d_tadj, c->d_p, live ? c->d_pbits : nullptr, row + C_PULL_HUB_LIVE,
+                c->d_idx, c->d_cv0, c->nc, nullptr, nullptr};
     c->pull_passes_live += live;
+    const bool fuse_snap = !c->d_trace;   // the trace counter stays with k_snapshot
+    if (fuse_snap) {
+        const int ncnt = 2 * c->nc + 1;
+        hipLaunchKernelGGL(k_zero_u64, dim3((ncnt + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, st, c->d_cnt, ncnt);
+        pa.bitmap = c->d_bitmap;
+        pa.cnt = c->d_cnt;
+    }
     {
         ScopedTimer t(c, st, T_FOLD);
         hipLaunchKernelGGL((k_fold<NM, S>), dim3(std::min(ntiles, c->cus * 8)), dim3(BLOCK), 0, st, f, c->DM);
@@ -1449,6 +1530,8 @@ void launch_fold_pull(Ctx* c, hipStream_t st) {
         ScopedTimer t(c, st, T_PULL);
         hipLaunchKernelGGL((k_pull<NM, S>), dim3(ntiles), dim3(BLOCK), 0, st, pa, c->DM);
     }
+    c->snap_ready = fuse_snap;
+    if (fuse_snap) c->snap_bytes += 4.0 * (((double)c->n + 31) / 32);
 }
 
 int fold_pull(Ctx* c) {
@@ -1670,6 +1753,7 @@ int exc_init_measure(void* p, int i, float beta, float lambda) {
     Ctx* c = as_ctx(p);
     if (!c || i < 0 || i >= c->nm) return fail(EXC_EINVAL, "exc_init_measure: bad slot");
     HIPCHK(hipSetDevice(c->dev));
+    c->snap_ready = false;   // the state changes: a fused snapshot no longer holds
     if (c->K) c->active = all_columns(c->K);
     if (c->K) {   // batched: every column starts at x = r = 0 (exc_batch_seed adds the teleport mass)
         const size_t bytes = (size_t)c->n * c->S * sizeof(float);
@@ -1690,6 +1774,7 @@ int exc_set_state(void* p, int i, const float* h_x, const float* h_r) {
     Ctx* c = as_ctx(p);
     if (!c || i < 0 || i >= (c->K ? c->K : c->nm)) return fail(EXC_EINVAL, "exc_set_state: bad slot");
     HIPCHK(hipSetDevice(c->dev));
+    c->snap_ready = false;   // the state changes: a fused snapshot no longer holds
     if (c->K) c->active = all_columns(c->K);
     float* tmp;
     const size_t bytes = (size_t)c->n * sizeof(float);
@@ -1728,6 +1813,7 @@ int exc_state_checkpoint(void* p, int restore) {
     Ctx* c = as_ctx(p);
     if (!c) return fail(EXC_EINVAL, "exc_state_checkpoint: null context");
     HIPCHK(hipSetDevice(c->dev));
+    c->snap_ready = false;   // the state changes: a fused snapshot no longer holds
     if (c->K) c->active = all_columns(c->K);
     const size_t bytes = (size_t)c->n * c->S * sizeof(float);
     if (restore) {
@@ -1751,12 +1837,31 @@ int exc_state_checkpoint(void* p, int restore) {
     return EXC_OK;
 }
 
+// Per-chunk counts (and hot-pending) of the snapshot on the device to the host.
+int snapshot_counts(Ctx* c, uint64_t* h_cnt, uint64_t* h_hotpend) {
+    hipStream_t st = c->st[0];
+    const int ncnt = 2 * c->nc + 1;
+    HIPCHK(hipMemcpyAsync(c->h_small, c->d_cnt, ncnt * sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    memcpy(h_cnt, c->h_small, 2 * (size_t)c->nc * sizeof(uint64_t));
+    if (h_hotpend) *h_hotpend = c->h_small[2 * c->nc];
+    c->last_demand = 0;
+    c->last_pending = 0;
+    for (int i = 0; i < c->nc; i++) { c->last_demand += c->h_small[2 * i + 1]; c->last_pending += c->h_small[2 * i]; }
+    return EXC_OK;
+}
+
 int exc_snapshot(void* p, int hot_skip, uint64_t* h_cnt, uint64_t* h_hotpend) {
     Ctx* c = as_ctx(p);
     if (!c || !h_cnt) return fail(EXC_EINVAL, "exc_snapshot: bad arguments");
     HIPCHK(hipSetDevice(c->dev));
     hipStream_t st = c->st[0];
     const int ncnt = 2 * c->nc + 1;
+    if (c->snap_ready && !hot_skip && !c->K) {   // computed by the dense pass that ended the last pass
+        c->snap_ready = false;
+        return snapshot_counts(c, h_cnt, h_hotpend);
+    }
+    c->snap_ready = false;
     hipLaunchKernelGGL(k_zero_u64, dim3((ncnt + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, st, c->d_cnt, ncnt);
     SnapArgs a{};
     a.n = c->n;
@@ -1796,14 +1901,7 @@ int exc_snapshot(void* p, int hot_skip, uint64_t* h_cnt, uint64_t* h_hotpend) {
         c->snap_bytes += 4.0 * c->nm * c->n + 4.0 * (((double)c->n + 31) / 32);
     }
     HIPCHK(hipGetLastError());
-    HIPCHK(hipMemcpyAsync(c->h_small, c->d_cnt, ncnt * sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
-    HIPCHK(hipStreamSynchronize(st));
-    memcpy(h_cnt, c->h_small, 2 * (size_t)c->nc * sizeof(uint64_t));
-    if (h_hotpend) *h_hotpend = c->h_small[2 * c->nc];
-    c->last_demand = 0;
-    c->last_pending = 0;
-    for (int i = 0; i < c->nc; i++) { c->last_demand += c->h_small[2 * i + 1]; c->last_pending += c->h_small[2 * i]; }
-    return EXC_OK;
+    return snapshot_counts(c, h_cnt, h_hotpend);
 }
 
 int exc_bitmap_to_host(void* p, uint32_t* h_bitmap) {
@@ -1924,6 +2022,7 @@ int exc_push_resident(void* p, const exc_chunk* ch, int use_list, int group, int
     if (!c || !ch || ch->v0 < 0 || ch->v1 > c->n || ch->v0 > ch->v1) return fail(EXC_EINVAL, "exc_push_resident: bad chunk");
     if (!c->d_arena || ch->e1 > c->arena_edges) return fail(EXC_ESTATE, "exc_push_resident: chunk not in the edge cache");
     HIPCHK(hipSetDevice(c->dev));
+    c->snap_ready = false;   // the state changes: a fused snapshot no longer holds
     if (c->d_tadj && !c->K && !c->d_wts && ch->v0 == 0 && ch->v1 == c->n && !hot_skip &&
         (double)c->last_demand >= c->pull_threshold * (double)c->m)
         return fold_pull(c);
@@ -1940,6 +2039,7 @@ int exc_stream_chunk(void* p, int s, const exc_chunk* ch, int to_resident, int h
     if (ne > c->max_chunk_edges) return fail(EXC_EINVAL, "exc_stream_chunk: chunk larger than the staging slot");
     if (to_resident && (!c->d_arena || ch->e1 > c->arena_edges)) return fail(EXC_ESTATE, "exc_stream_chunk: chunk not in the edge cache");
     HIPCHK(hipSetDevice(c->dev));
+    c->snap_ready = false;   // the state changes: a fused snapshot no longer holds
     Slot& sl = c->slot[s];
     hipStream_t st = c->st[1 + s];
     int32_t* dst;
@@ -1972,6 +2072,7 @@ int exc_gather_chunk(void* p, int s, int k, int group) {
     const uint64_t ne = k ? sl.h_off[k] : 0;
     if (ne > c->max_chunk_edges) return fail(EXC_EINVAL, "exc_gather_chunk: gathered lists exceed the slot");
     HIPCHK(hipSetDevice(c->dev));
+    c->snap_ready = false;   // the state changes: a fused snapshot no longer holds
     hipStream_t st = c->st[1 + s];
     if (!c->zc) {
         HIPCHK(hipMemcpyAsync(sl.d_vids, sl.h_vids, (size_t)k * sizeof(int32_t), hipMemcpyHostToDevice, st));
@@ -1998,6 +2099,7 @@ int exc_push_hot(void* p, int lo, int hi, int group, int after_slots) {
     if (!c || lo < 0 || hi > c->nh || lo > hi) return fail(EXC_EINVAL, "exc_push_hot: bad range");
     if (hi == lo) return EXC_OK;
     HIPCHK(hipSetDevice(c->dev));
+    c->snap_ready = false;   // the state changes: a fused snapshot no longer holds
     hipStream_t st = c->st[0];
     if (after_slots)
         for (auto& sl : c->slot) if (sl.busy) HIPCHK(hipStreamWaitEvent(st, sl.done, 0));
@@ -2049,6 +2151,7 @@ int exc_batch_seed(void* p, int col, const int32_t* seeds, int count, float valu
         if (seeds[i] < 0 || seeds[i] >= c->n) return fail(EXC_EINVAL, "exc_batch_seed: seed %d out of range", seeds[i]);
     if (!count) return EXC_OK;
     HIPCHK(hipSetDevice(c->dev));
+    c->snap_ready = false;   // the state changes: a fused snapshot no longer holds
     if (c->K) c->active = all_columns(c->K);
     int32_t* d;
     HIPCHK(hipMalloc(&d, (size_t)count * sizeof(int32_t)));
@@ -2083,6 +2186,7 @@ int exc_set_edge_weights(void* p, const float* h_w, uint64_t cnt) {
     if (!c->d_arena || c->arena_edges != c->m || cnt != c->m)
         return fail(EXC_ESTATE, "exc_set_edge_weights: the whole graph must be in the edge cache and cnt == m");
     HIPCHK(hipSetDevice(c->dev));
+    c->snap_ready = false;   // the state changes: a fused snapshot no longer holds
     if (!c->d_wts) HIPCHK(hipMalloc(&c->d_wts, (c->m ? c->m : 1) * sizeof(float)));
     if (c->m) HIPCHK(hipMemcpy(c->d_wts, h_w, c->m * sizeof(float), hipMemcpyHostToDevice));
     return EXC_OK;

@@ -197,3 +197,46 @@ def test_fold_pull_all_measures():
         tr = true_residual(idx, adj, xs[i], bv, w, dd)
         assert np.abs(tr - rs[i]).max() < 1e-3 * max(1.0, np.abs(xs[i]).max()), i
     assert orc.compare(orc.fp64_reference(idx, adj, orc.ORC_KATZ, alpha=alpha)[0], xs[1])["l1_rel"] < 0.01
+
+
+@pytest.mark.parametrize("measures", ["pr", "all"])
+def test_fused_snapshot_bit_exact(rmat12, measures):
+    """The snapshot a dense pass computes for the next pass inside k_pull (bitmap words,
+    per-chunk pending counts and degree sums) equals the oracle's snapshot of the state
+    read back, bit for bit, on chunk boundaries that cut through the pull's tiles; and
+    equals a second, separately launched snapshot of the same state."""
+    idx, adj = orc.egr_load(rmat12)
+    n, m = len(idx) - 1, len(adj)
+    cv0 = np.array([0, 1000, 1001, 2560, 3333], np.int32)
+    if measures == "pr":
+        kinds, eps, scale, divdeg = [nat.EXC_PR], [1e-3], [0.85], [1]
+        inits = [orc.init(idx, orc.ORC_PR)]
+    else:
+        kinds, eps = [nat.EXC_PR, nat.EXC_KATZ, nat.EXC_EV], [1e-3, 1e-4, 1e-4]
+        # contractive constants (alpha and 1/lambda well below 1/lambda1 of this graph)
+        scale, divdeg = [np.float32(0.85), np.float32(0.001), np.float32(0.001)], [1, 0, 0]
+        inits = [orc.init(idx, orc.ORC_PR), orc.init(idx, orc.ORC_KATZ, 1.0), orc.init(idx, orc.ORC_EV, lam=np.float32(1000))]
+    with nat.Context(n, m, cv0, kinds, eps, scale, divdeg) as ctx:
+        ctx.upload_offsets(idx)
+        for i, (x0, r0) in enumerate(inits):
+            ctx.set_state(i, x=x0, r=r0)
+        assert ctx.cache_chunks(m) == 0
+        ctx.cache_upload(adj, 0)
+        assert ctx.enable_pull(0.0) == 0
+        checked = 0
+        for p in range(200):
+            cnt, _ = ctx.snapshot()
+            bm = ctx.bitmap()
+            rs = [ctx.read_state(i, 1) for i in range(len(kinds))]
+            obm, ocnt, _ = orc.snapshot(rs, np.asarray(eps, np.float32), idx, cv0)
+            assert np.array_equal(bm, obm), p
+            assert np.array_equal(cnt, ocnt), p
+            if p > 0:   # the state is unchanged: a launched snapshot agrees with the fused one
+                cnt2, _ = ctx.snapshot()
+                assert np.array_equal(cnt2, cnt) and np.array_equal(ctx.bitmap(), bm), p
+                checked += 1
+            if cnt[0::2].sum() == 0:
+                break
+            ctx.push_resident(0, n, 0, m, False)
+            ctx.pass_sync()
+        assert checked >= 3
