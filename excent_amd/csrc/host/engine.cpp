@@ -582,10 +582,12 @@ void Engine::load_ppr_seeds(EgrGraph& g) {
 // Once the whole graph sits in the chunk cache, dense passes can run as fold + pull over a
 // transpose built on the device (extension; --pull off keeps the push only).
 void Engine::maybe_enable_pull() {
-    if (pull_tried_ || o_.pull == "off" || batched() || !use_cache_ || n_resident_ != nc_) return;
+    if (pull_tried_ || o_.pull == "off" || !use_cache_ || n_resident_ != nc_) return;
     for (int c = 0; c < nc_; c++) if (!loaded_[c]) return;
     pull_tried_ = true;
-    const double thr = o_.pull == "auto" ? 0.05 : atof(o_.pull.c_str());
+    // auto: 5 % of the edges; batched rows make the scatter dearer per edge and the pull
+    // reads only the pending sources' rows, so the dense pass pays from 2 %
+    const double thr = o_.pull == "auto" ? (batched() ? 0.02 : 0.05) : atof(o_.pull.c_str());
     const int rc = exc_enable_pull(ctx_, thr);
     if (rc == EXC_OK) { pull_on_ = true; pull_thr_ = thr; }
     else if (rc != EXC_ENOMEM) ck(rc, "pull setup");

@@ -935,7 +935,7 @@ __global__ void k_count_in64(uint64_t m, const int32_t* __restrict__ adj, unsign
 __global__ void k_fill_in64(int n, const uint64_t* __restrict__ idx, const int32_t* __restrict__ adj,
                             unsigned long long* cursor, const int32_t* __restrict__ rank, int32_t* tadj) {
     for (long long v = blockIdx.x * (long long)blockDim.x + threadIdx.x; v < n; v += (long long)gridDim.x * blockDim.x)
-        for (uint64_t e = idx[v]; e < idx[v + 1]; e++) tadj[atomicAdd(&cursor[adj[e]], 1ull)] = rank[v];
+        for (uint64_t e = idx[v]; e < idx[v + 1]; e++) tadj[atomicAdd(&cursor[adj[e]], 1ull)] = rank ? rank[v] : (int32_t)v;
 }
 
 // ------------------------------------------------------------------ batched PPR
@@ -1209,6 +1209,156 @@ __global__ __launch_bounds__(BLOCK) void k_bcompact_hot(int lo, int hi, const in
 __global__ void k_bseed(int cnt, const int32_t* __restrict__ seeds, float* r, int K, int col, float value) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < cnt) atomic_add_f32(&r[(long long)seeds[i] * K + col], value);
+}
+
+// ---- batched dense passes: fold + pull over the transpose, [n][K] rows, no atomics
+// (except the hub pieces).  The snapshot bitmap of the pass marks the sources whose p row
+// is written this pass; every other row is stale and skipped.
+
+struct BDenseArgs {
+    int n;
+    const uint64_t* idx;          // out-offsets (fold: degrees)
+    const uint32_t* pending;      // the pass's snapshot bitmap
+    float* p;                     // [n][K] per-edge weights of the pass
+    const uint64_t* tidx;         // transpose (in-lists of at most PULL_HUB entries)
+    const int32_t* tadj;          // in-neighbours (vertex ids)
+    unsigned long long* counters; // C_FIELDS row of the dense pass
+    unsigned long long* live_count;   // k_bpull_hub: rows read
+};
+
+__device__ __forceinline__ bool bit_of(const uint32_t* __restrict__ bm, int v) {
+    return (bm[v >> 5] >> (v & 31)) & 1u;
+}
+
+// Fold: each wave takes 64-vertex segments and folds the snapshot's pending vertices one
+// at a time, lanes = columns: x += r, r = 0, p = d r / deg for every active column over
+// eps, the whole p row written (zeros for the other columns).
+__global__ __launch_bounds__(BLOCK) void k_bfold(BDenseArgs a, BatchArgs B) {
+    const int lane = lane_id();
+    const long long nw = (long long)gridDim.x * NWAVES;
+    const long long words = ((long long)a.n + 31) / 32;
+    const bool col_on = lane < B.K && ((B.active >> lane) & 1ull);
+    uint64_t st[4] = {0, 0, 0, 0};
+    for (long long seg = (blockIdx.x * (long long)BLOCK + threadIdx.x) / WAVE; seg * WAVE < a.n; seg += nw) {
+        uint64_t mask = a.pending[2 * seg];
+        if (2 * seg + 1 < words) mask |= (uint64_t)a.pending[2 * seg + 1] << 32;
+        while (mask) {
+            const int t = __ffsll((long long)mask) - 1;
+            mask &= mask - 1;
+            const long long v = seg * WAVE + t;
+            const uint64_t deg = a.idx[v + 1] - a.idx[v];
+            float d = 0.f, pv = 0.f;
+            if (col_on) {
+                float* rp = B.r + v * B.K + lane;
+                const float rv = *rp;
+                if (fabsf(rv) > B.eps[lane]) {
+                    d = rv;
+                    *rp = 0.f;
+                    B.x[v * B.K + lane] += d;
+                    if (deg) pv = B.damp * d / (float)deg;
+                }
+            }
+            if (deg && lane < B.K) a.p[v * B.K + lane] = pv;
+            const uint64_t fm = __ballot(d != 0.f), pm = __ballot(pv != 0.f);
+            if (lane == 0) {
+                if (fm) { st[0] += 1; st[1] += (uint64_t)__popcll(fm); }
+                if (pm) { st[2] += deg; st[3] += deg * (uint64_t)__popcll(pm); }
+            }
+        }
+    }
+    if (lane == 0) {
+        if (st[0]) { atomicAdd(&a.counters[C_VERT], (unsigned long long)st[0]); atomicAdd(&a.counters[C_VERT_NM], (unsigned long long)st[1]); }
+        if (st[2]) { atomicAdd(&a.counters[C_EDGES], (unsigned long long)st[2]); atomicAdd(&a.counters[C_EDGES_NM], (unsigned long long)st[3]); }
+    }
+}
+
+// Sum of the live p rows of in-list entries [b, e) into acc (lane = column), in slot
+// order; at every owner change `flush(owner, acc)` is called.  Eight rows are loaded before
+// they are added, so a wave keeps eight 256-byte gathers in flight.
+template <typename OwnerOf, typename Flush>
+__device__ __forceinline__ uint32_t bsum_rows(const BDenseArgs& a, const BatchArgs& B, uint64_t b, uint64_t e,
+                                              OwnerOf owner_of, Flush flush, int& cur, float& acc) {
+    const int lane = lane_id();
+    const bool col = lane < B.K;
+    uint32_t taken = 0;
+    for (uint64_t e0 = b; e0 < e; e0 += WAVE) {
+        const uint64_t i = e0 + lane;
+        const int v = i < e ? a.tadj[i] : -1;
+        uint64_t lm = __ballot(v >= 0 && bit_of(a.pending, v));
+        while (lm) {
+            int slot[8];
+#pragma unroll
+            for (int q = 0; q < 8; q++) {
+                slot[q] = lm ? __ffsll((long long)lm) - 1 : -1;
+                if (lm) lm &= lm - 1;
+            }
+            float val[8];
+#pragma unroll
+            for (int q = 0; q < 8; q++) {
+                val[q] = 0.f;
+                if (slot[q] >= 0) {
+                    const int vv = __shfl(v, slot[q], WAVE);
+                    if (col) val[q] = a.p[(long long)vv * B.K + lane];
+                }
+            }
+#pragma unroll
+            for (int q = 0; q < 8; q++) {
+                if (slot[q] < 0) break;
+                const int own = owner_of(e0 + (uint64_t)slot[q]);
+                if (own != cur) {
+                    if (cur >= 0) flush(cur, acc);
+                    acc = 0.f;
+                    cur = own;
+                }
+                acc += val[q];
+                taken++;
+            }
+        }
+    }
+    return taken;
+}
+
+// Pull: a wave owns 64 consecutive destinations, whose in-lists are one contiguous run of
+// the transpose; it walks the run in slot order, reads the live rows (256 bytes each,
+// lanes = columns) and adds each destination's sum to its r row once, with a plain
+// read-modify-write (no other wave touches these rows).  The owner of a slot is the last
+// of the 64 list starts at or before it (one ballot).
+__global__ __launch_bounds__(BLOCK) void k_bpull(BDenseArgs a, BatchArgs B) {
+    const int lane = lane_id();
+    const long long nw = (long long)gridDim.x * NWAVES;
+    const long long ntile = ((long long)a.n + WAVE - 1) / WAVE;
+    for (long long t = (blockIdx.x * (long long)BLOCK + threadIdx.x) / WAVE; t < ntile; t += nw) {
+        const long long u0 = t * WAVE;
+        const int nt = (int)((long long)a.n - u0 < WAVE ? (long long)a.n - u0 : WAVE);
+        const uint64_t lo = a.tidx[u0], hi = a.tidx[u0 + nt];
+        const uint64_t my = lane < nt ? a.tidx[u0 + lane] : hi;   // lanes past the tile never own a slot
+        int cur = -1;
+        float acc = 0.f;
+        auto owner_of = [&](uint64_t pos) { return __popcll(__ballot(my <= pos)) - 1; };
+        auto flush = [&](int own, float s) {
+            if (lane < B.K && s != 0.f) B.r[(u0 + own) * B.K + lane] += s;
+        };
+        bsum_rows(a, B, lo, hi, owner_of, flush, cur, acc);
+        if (cur >= 0) flush(cur, acc);
+    }
+}
+
+// In-lists above PULL_HUB: one wave per piece, one atomic row add per piece.
+__global__ __launch_bounds__(BLOCK) void k_bpull_hub(BDenseArgs a, BatchArgs B, const PullPiece* __restrict__ pieces,
+                                                      int npieces) {
+    const int lane = lane_id();
+    const long long nw = (long long)gridDim.x * NWAVES;
+    uint32_t taken = 0;
+    for (long long h = (blockIdx.x * (long long)BLOCK + threadIdx.x) / WAVE; h < npieces; h += nw) {
+        const PullPiece pc = pieces[h];
+        int cur = -1;
+        float acc = 0.f;
+        auto owner_of = [](uint64_t) { return 0; };
+        auto flush = [](int, float) {};
+        taken += bsum_rows(a, B, pc.start, pc.start + pc.len, owner_of, flush, cur, acc);
+        if (lane < B.K && acc != 0.f) atomic_add_f32(&B.r[(long long)pc.u * B.K + lane], acc);
+    }
+    if (lane == 0 && taken) atomicAdd(a.live_count, (unsigned long long)taken);
 }
 
 // ------------------------------------------------------------------ host-side context
@@ -1534,9 +1684,35 @@ void launch_fold_pull(Ctx* c, hipStream_t st) {
     if (fuse_snap) c->snap_bytes += 4.0 * (((double)c->n + 31) / 32);
 }
 
+// Batched dense pass: fold the snapshot's pending vertices, hub pieces, then the pull.
+void launch_bfold_pull(Ctx* c, hipStream_t st) {
+    unsigned long long* row = c->d_counters + C_DENSE_LIVE * C_FIELDS;
+    BDenseArgs a{c->n, c->d_idx, c->d_bitmap, c->d_p, c->d_tidx, c->d_tadj, row, row + C_PULL_HUB_LIVE};
+    const BatchArgs B = batch_args(c, -1);
+    const long long segs = ((long long)c->n + WAVE - 1) / WAVE;
+    const unsigned cap = (unsigned)c->cus * 8;
+    {
+        ScopedTimer t(c, st, T_FOLD);
+        hipLaunchKernelGGL(k_bfold, dim3((unsigned)std::min<long long>((segs + NWAVES - 1) / NWAVES, cap)), dim3(BLOCK), 0,
+                           st, a, B);
+    }
+    if (c->n_pull_hubs) {
+        ScopedTimer t(c, st, T_PULL_HUB);
+        hipLaunchKernelGGL(k_bpull_hub, dim3((unsigned)std::min<long long>((c->n_pull_hubs + NWAVES - 1) / NWAVES, cap)),
+                           dim3(BLOCK), 0, st, a, B, c->d_pull_hubs, c->n_pull_hubs);
+    }
+    {
+        ScopedTimer t(c, st, T_PULL);
+        hipLaunchKernelGGL(k_bpull, dim3((unsigned)std::min<long long>((segs + NWAVES - 1) / NWAVES, 2 * cap)), dim3(BLOCK),
+                           0, st, a, B);
+    }
+    c->pull_passes_live++;
+}
+
 int fold_pull(Ctx* c) {
     hipStream_t st = c->st[0];
-    if (c->nm == 1) launch_fold_pull<1, 1>(c, st);
+    if (c->K) launch_bfold_pull(c, st);
+    else if (c->nm == 1) launch_fold_pull<1, 1>(c, st);
     else if (c->nm == 2) launch_fold_pull<2, 2>(c, st);
     else launch_fold_pull<3, 4>(c, st);
     c->pull_passes++;
@@ -2023,7 +2199,7 @@ int exc_push_resident(void* p, const exc_chunk* ch, int use_list, int group, int
     if (!c->d_arena || ch->e1 > c->arena_edges) return fail(EXC_ESTATE, "exc_push_resident: chunk not in the edge cache");
     HIPCHK(hipSetDevice(c->dev));
     c->snap_ready = false;   // the state changes: a fused snapshot no longer holds
-    if (c->d_tadj && !c->K && !c->d_wts && ch->v0 == 0 && ch->v1 == c->n && !hot_skip &&
+    if (c->d_tadj && !c->d_wts && ch->v0 == 0 && ch->v1 == c->n && !hot_skip &&
         (double)c->last_demand >= c->pull_threshold * (double)c->m)
         return fold_pull(c);
     return push_chunk(c, 0, ch->v0, ch->v1, c->d_arena, 0, use_list, group, hot_skip, c->arena_edges, c->d_wts);
@@ -2257,13 +2433,17 @@ int exc_get_stats(void* p, exc_stats* o) {
         // fold: read r and write p for every vertex; offsets, x read + write for folded ones
         o->fold_bytes = P * c->n * (4.0 * nm + 4.0 * S) + 16.0 * (r4[C_VERT] + r5[C_VERT]) +
                         8.0 * (r4[C_VERT_NM] + r5[C_VERT_NM]);
+        if (c->K)   // batched: the bitmap, then per pending vertex its offsets, r row and p row,
+                    // per folded column r written and x read + written
+            o->fold_bytes = P * 4.0 * (((double)c->n + 31) / 32) + (double)r5[C_VERT] * (16.0 + 8.0 * nm) +
+                            12.0 * (double)r5[C_VERT_NM];
         // pull: every in-edge's id, every vertex's offsets and r (read + write), and the p rows
         // read: every in-edge's in a plain pass; in a pass with the live-row bitmap, the
         // bitmap plus one row per out-edge of a pushing source (the fold's edge count -- a
         // pushed weight is zero only on underflow), split between the kernels by
         // k_pull_hub's own count.  In-lists above the hub threshold go to the hub kernel.
         const double me = (double)(c->m - c->pull_hub_edges), nv = (double)(c->n - c->pull_hub_vertices);
-        const double bm = 8.0 * ((c->nsrc + 63) / 64);
+        const double bm = c->K ? 4.0 * (((double)c->n + 31) / 32) : 8.0 * ((c->nsrc + 63) / 64);
         const double hub_rows = (P - PL) * c->pull_hub_edges + (double)r5[C_PULL_HUB_LIVE];
         const double rows = (P - PL) * me + std::max(0.0, (double)r5[C_EDGES] - (double)r5[C_PULL_HUB_LIVE]);
         o->pull_bytes = P * (me * 4.0 + nv * (8.0 + 8.0 * nm)) + PL * bm + 4.0 * nm * rows;
@@ -2282,7 +2462,7 @@ int exc_get_stats(void* p, exc_stats* o) {
 
 int exc_enable_pull(void* p, double threshold) {
     Ctx* c = as_ctx(p);
-    if (!c || c->K) return fail(EXC_EINVAL, "exc_enable_pull: plain contexts only");
+    if (!c) return fail(EXC_EINVAL, "exc_enable_pull: null context");
     if (!c->d_arena || c->arena_edges != c->m) return fail(EXC_ESTATE, "exc_enable_pull: the whole graph must be in the edge cache");
     HIPCHK(hipSetDevice(c->dev));
     HIPCHK(hipDeviceSynchronize());
@@ -2293,11 +2473,13 @@ int exc_enable_pull(void* p, double threshold) {
     }
     if (c->d_tadj) return EXC_OK;
     const size_t n = (size_t)c->n;
-    // rows of p in decreasing out-degree order (ties by id); vertices without out-edges last
+    // plain: rows of p in decreasing out-degree order (ties by id), vertices without
+    // out-edges last; batched: p rows by vertex id (a row is 4K bytes, whole lines already)
+    const bool by_id = c->K > 0;
     std::vector<uint64_t> hidx(n + 1);
     HIPCHK(hipMemcpy(hidx.data(), c->d_idx, (n + 1) * sizeof(uint64_t), hipMemcpyDeviceToHost));
-    std::vector<int32_t> rank(n);
-    {
+    std::vector<int32_t> rank(by_id ? 0 : n);
+    if (!by_id) {
         uint64_t maxd = 0;
         for (size_t v = 0; v < n; v++) maxd = std::max(maxd, hidx[v + 1] - hidx[v]);
         std::vector<uint64_t> start(maxd + 2, 0);
@@ -2307,7 +2489,7 @@ int exc_enable_pull(void* p, double threshold) {
     }
     unsigned long long* cnt = nullptr;
     bool ok = hipMalloc(&c->d_tidx, (n + 1) * sizeof(uint64_t)) == hipSuccess;
-    ok = ok && hipMalloc(&c->d_rank, n * sizeof(int32_t)) == hipSuccess;
+    ok = ok && (by_id || hipMalloc(&c->d_rank, n * sizeof(int32_t)) == hipSuccess);
     c->nsrc = 0;
     for (size_t v = 0; v < n; v++) c->nsrc += hidx[v + 1] > hidx[v];
     ok = ok && hipMalloc(&c->d_pbits, (2 * (((size_t)c->nsrc + 63) / 64) + 2) * sizeof(uint32_t)) == hipSuccess;
@@ -2327,7 +2509,7 @@ int exc_enable_pull(void* p, double threshold) {
         (void)hipFree(cnt);
         return fail(EXC_EHIP, "exc_enable_pull: %s", hipGetErrorString(e));
     };
-    hipError_t e = hipMemcpy(c->d_rank, rank.data(), n * sizeof(int32_t), hipMemcpyHostToDevice);
+    hipError_t e = by_id ? hipSuccess : hipMemcpy(c->d_rank, rank.data(), n * sizeof(int32_t), hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemset(cnt, 0, (n + 1) * sizeof(unsigned long long));
     if (e != hipSuccess) return fail_free(e);
     hipLaunchKernelGGL(k_count_in64, dim3(c->cus * 16), dim3(BLOCK), 0, 0, c->m, c->d_arena, cnt);

@@ -170,3 +170,54 @@ def test_batched_session_rerun_after_reinit(rmat12):
     assert r2["kernel"]["pushed_edge_measures"] > 0
     for a, b in zip(x1, x2):
         assert np.abs(a - b).sum() <= 2 * 1e-5 * n / (1 - 0.85)
+
+
+@pytest.mark.parametrize("gname", ["rmat12", "star"])
+def test_batched_dense_passes(gname, rmat12):
+    """Batched fold + pull over the transpose (every pass dense): each column within the
+    exact FP64 bound, per-vertex |r| <= eps at the end, the snapshot after every dense pass
+    bit-exact against the oracle's, and in-lists above the hub threshold (the star's centre
+    has 9000 in-neighbours) summed by pieces."""
+    if gname == "star":
+        L = 9000
+        deg = np.concatenate([[L], np.ones(L, np.int64)])
+        idx = np.concatenate([[0], np.cumsum(deg)]).astype(np.uint64)
+        adj = np.concatenate([np.arange(1, L + 1), np.zeros(L, np.int64)]).astype(np.int32)
+    else:
+        idx, adj = orc.egr_load(rmat12)
+    n, m = len(idx) - 1, len(adj)
+    K, S, eps = 5, 3, 1e-4
+    rng = np.random.default_rng(21)
+    seeds = [rng.choice(n, S, replace=False).astype(np.int32) for _ in range(K)]
+    mass = np.float32(0.15 * n / S)
+    cv0 = np.array([0, n // 3, n // 2], np.int32) if n > 10 else np.array([0], np.int32)
+    with nat.Context(n, m, cv0, [nat.EXC_PR], [eps], [0.85], [1], batch_k=K) as ctx:
+        ctx.upload_offsets(idx)
+        assert ctx.cache_chunks(m) == 0
+        ctx.cache_upload(adj)
+        for j in range(K):
+            ctx.batch_seed(j, seeds[j], mass)
+        assert ctx.enable_pull(0.0) == 0
+        for it in range(5000):
+            cnt, _ = ctx.snapshot()
+            rs = [ctx.read_state(j, 1) for j in range(K)]
+            obm, ocnt, _ = orc.snapshot(rs, np.full(K, eps, np.float32), idx, cv0)
+            assert np.array_equal(ctx.bitmap(), obm) and np.array_equal(cnt, ocnt), it
+            cp = ctx.batch_colpend()
+            ctx.batch_set_active(sum(1 << j for j in range(K) if cp[j]))
+            if cnt[0::2].sum() == 0:
+                break
+            ctx.push_resident(0, n, 0, m, False)
+            ctx.pass_sync()
+        X = [ctx.read_state(j, 0) for j in range(K)]
+        R = [ctx.read_state(j, 1) for j in range(K)]
+        st = ctx.stats()
+    assert st["pull_passes"] > 0 and st["pushed_edge_measures"] > 0
+    for j in range(K):
+        assert np.all(np.abs(R[j]) <= eps)
+        b = np.zeros(n)
+        np.add.at(b, seeds[j], np.float64(mass))
+        ref, _, _ = orc.fp64_solve_b(idx, adj, b)
+        err = np.abs(X[j] - ref).sum()
+        bound = np.abs(true_residual(idx, adj, X[j], b)).sum() / 0.15
+        assert err <= bound * (1 + 1e-6) + 1e-6, (j, err, bound)
