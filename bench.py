@@ -203,10 +203,12 @@ def bench_ooc(args, nat, np, work, graph):
         sess = nat.Session([graph, "--measure", "pr", "--timing", "--log",
                             os.path.join(work, f"ooc_{tag.split()[0]}.log")] + extra)
         log(f"ooc {tag}: setup {time.time() - t0:.1f}s")
-        sess.run(reinit=False)
+        for w in range(args.warmup):
+            sess.run(reinit=w > 0)
+            log(f"ooc {tag}: warmup {w} done")
         nat.synchronize()
         t0 = time.perf_counter()
-        r = sess.run(reinit=True)
+        r = sess.run(reinit=args.warmup > 0)
         nat.synchronize()
         el = time.perf_counter() - t0
         sess.close()
