@@ -165,6 +165,24 @@ __device__ __forceinline__ Row<S> load_row(const float* __restrict__ r, long lon
     return o;
 }
 
+// Rows of the dense pass's weights p hold exactly NM floats (12 bytes for the fused
+// three-measure pass, not the state's padded 16): 10.7 rows share a 128-byte line instead
+// of 8, so more of the most-read rows stay cached.  The unused lanes of the Row read 0.
+template <int NM, int S>
+__device__ __forceinline__ Row<S> load_prow(const float* __restrict__ p, long long q) {
+    Row<S> o;
+    if constexpr (NM == S) {
+        o = load_row<S>(p, q);
+    } else {
+        const float* a = p + q * NM;
+#pragma unroll
+        for (int k = 0; k < NM; k++) o.v[k] = a[k];
+#pragma unroll
+        for (int k = NM; k < S; k++) o.v[k] = 0.f;
+    }
+    return o;
+}
+
 // ------------------------------------------------------------------ init / state
 
 // k_init (src/excent.cu:41-47) for measure slot k of a [n][S] state.
@@ -612,7 +630,7 @@ __global__ void k_indegree(uint64_t m, const int32_t* __restrict__ edges, uint32
 struct FoldArgs {
     int n;
     const uint64_t* idx;
-    float* p;                       // [sources][S] weights of this pass (0 when v did not push)
+    float* p;                       // [sources][NM] weights of this pass (0 when v did not push)
     const int32_t* rank;            // row of v in p: vertices with out-edges in decreasing
                                     // out-degree order, so the most-read rows share lines
     unsigned long long* counters;   // C_FIELDS row
@@ -666,7 +684,14 @@ __global__ __launch_bounds__(BLOCK) void k_fold(FoldArgs a, DevMeasures M) {
             c_e += deg;
             c_enm += deg * nmv;
         }
-        if (deg) store_row<S>(a.p, a.rank[v], pr);   // vertices without out-edges are never read
+        if (deg) {   // vertices without out-edges are never read
+            if constexpr (NM == S) store_row<S>(a.p, a.rank[v], pr);
+            else {
+                float* q = a.p + (long long)a.rank[v] * NM;
+#pragma unroll
+                for (int k = 0; k < NM; k++) q[k] = pr.v[k];
+            }
+        }
     }
     const uint32_t t0 = wave_sum_u32(c_vert), t1 = wave_sum_u32(c_vnm);
     const uint64_t t2 = wave_sum_u64(c_e), t3 = wave_sum_u64(c_enm);
@@ -684,7 +709,7 @@ struct PullArgs {
     int n;
     const uint64_t* tidx;       // transpose offsets [n+1]
     const int32_t* tadj;        // in-neighbours, as rows of p (FoldArgs::rank)
-    const float* p;             // [sources][S]
+    const float* p;             // [sources][NM]
     const uint32_t* live;       // bit q set iff row q of p is nonzero this pass (k_pbits);
                                 // nullptr: every row is read
     unsigned long long* live_count;   // k_pull_hub: rows read
@@ -710,7 +735,7 @@ __global__ __launch_bounds__(BLOCK) void k_pbits(const float* __restrict__ p, in
         const long long q = w * WAVE + lane;
         bool nz = false;
         if (q < nsrc) {
-            const Row<S> row = load_row<S>(p, q);
+            const Row<S> row = load_prow<NM, S>(p, q);
 #pragma unroll
             for (int k = 0; k < NM; k++) nz |= row.v[k] != 0.f;
         }
@@ -735,12 +760,11 @@ constexpr uint32_t PULL_HUB = 4096;   // in-lists longer than this are summed by
 // search per lane per window replaces a search per slot, and no cross-lane reduction is
 // needed.  Finally each destination adds its sum to r with one plain row update.
 // live != nullptr: rows whose bit is clear are zero this pass and are not loaded.
-constexpr int PULL_SEQ = 16;
-constexpr int PULL_WIN = BLOCK * PULL_SEQ;   // 4096 slots, 16 KB of LDS
 constexpr int PULL_LD = BLOCK + 1;           // LDS row pitch: slot j at (j % SEQ) * LD + j / SEQ
 
-template <int NM, int S>
+template <int NM, int S, int PULL_SEQ>
 __global__ __launch_bounds__(BLOCK) void k_pull(PullArgs a, DevMeasures M) {
+    constexpr int PULL_WIN = BLOCK * PULL_SEQ;   // 4096 slots (16 KB of LDS) at PULL_SEQ 16
     __shared__ uint32_t s_start[BLOCK + 1];
     __shared__ int32_t s_src[PULL_SEQ * PULL_LD];
     __shared__ float s_acc[NM][BLOCK];
@@ -773,7 +797,7 @@ __global__ __launch_bounds__(BLOCK) void k_pull(PullArgs a, DevMeasures M) {
                 for (int k = 0; k < S; k++) rows[q].v[k] = 0.f;
                 if ((uint32_t)q < jn) {
                     const int v = s_src[q * PULL_LD + tid];
-                    if (row_live(a.live, v)) rows[q] = load_row<S>(a.p, v);
+                    if (row_live(a.live, v)) rows[q] = load_prow<NM, S>(a.p, v);
                 }
             }
             int owner = lo;
@@ -899,7 +923,7 @@ __global__ __launch_bounds__(BLOCK) void k_pull_hub(PullArgs a, const PullPiece*
             for (int q = 0; q < HUB_UNROLL; q++) {
 #pragma unroll
                 for (int k = 0; k < S; k++) rows[q].v[k] = 0.f;
-                if (vs[q] >= 0 && row_live(a.live, vs[q])) { rows[q] = load_row<S>(a.p, vs[q]); taken++; }
+                if (vs[q] >= 0 && row_live(a.live, vs[q])) { rows[q] = load_prow<NM, S>(a.p, vs[q]); taken++; }
             }
 #pragma unroll
             for (int q = 0; q < HUB_UNROLL; q++)
@@ -1430,6 +1454,7 @@ struct Ctx {
     int nsrc = 0;                       // vertices with out-edges (rows of p)
     int pull_live = 2;                  // live-row bitmap: 0 never, 1 always, 2 when few vertices pend
     bool snap_ready = false;            // the last dense pass computed the next snapshot
+    int pull_seq = 8;                   // in-list entries per lane in k_pull (4, 8, 16 or 32)
     uint64_t last_pending = 0;          // pending vertices of the last snapshot
     uint64_t pull_passes_live = 0;
     PullPiece* d_pull_hubs = nullptr;   // pieces of in-lists longer than PULL_HUB
@@ -1678,7 +1703,10 @@ void launch_fold_pull(Ctx* c, hipStream_t st) {
     }
     {
         ScopedTimer t(c, st, T_PULL);
-        hipLaunchKernelGGL((k_pull<NM, S>), dim3(ntiles), dim3(BLOCK), 0, st, pa, c->DM);
+        if (c->pull_seq == 4) hipLaunchKernelGGL((k_pull<NM, S, 4>), dim3(ntiles), dim3(BLOCK), 0, st, pa, c->DM);
+        else if (c->pull_seq == 32) hipLaunchKernelGGL((k_pull<NM, S, 32>), dim3(ntiles), dim3(BLOCK), 0, st, pa, c->DM);
+        else if (c->pull_seq == 16) hipLaunchKernelGGL((k_pull<NM, S, 16>), dim3(ntiles), dim3(BLOCK), 0, st, pa, c->DM);
+        else hipLaunchKernelGGL((k_pull<NM, S, 8>), dim3(ntiles), dim3(BLOCK), 0, st, pa, c->DM);
     }
     c->snap_ready = fuse_snap;
     if (fuse_snap) c->snap_bytes += 4.0 * (((double)c->n + 31) / 32);
@@ -2424,14 +2452,15 @@ int exc_get_stats(void* p, exc_stats* o) {
     {   // dense passes: fold (rows 4 and 5) and pull
         const unsigned long long* r4 = h + C_DENSE * C_FIELDS;
         const unsigned long long* r5 = h + C_DENSE_LIVE * C_FIELDS;
-        const int S = c->S, nm = c->K ? c->K : c->nm;
+        const int nm = c->K ? c->K : c->nm;
         const double P = (double)c->pull_passes, PL = (double)c->pull_passes_live;
         o->fold_ms = c->t_ms[T_FOLD];
         o->pull_ms = c->t_ms[T_PULL];
         o->pull_hub_ms = c->t_ms[T_PULL_HUB];
         o->pull_passes = c->pull_passes;
-        // fold: read r and write p for every vertex; offsets, x read + write for folded ones
-        o->fold_bytes = P * c->n * (4.0 * nm + 4.0 * S) + 16.0 * (r4[C_VERT] + r5[C_VERT]) +
+        // fold: read r for every vertex, write p for every source; offsets, x read + write for
+        // folded ones
+        o->fold_bytes = P * c->n * 4.0 * nm + P * c->nsrc * 4.0 * nm + 16.0 * (r4[C_VERT] + r5[C_VERT]) +
                         8.0 * (r4[C_VERT_NM] + r5[C_VERT_NM]);
         if (c->K)   // batched: the bitmap, then per pending vertex its offsets, r row and p row,
                     // per folded column r written and x read + written
@@ -2470,6 +2499,8 @@ int exc_enable_pull(void* p, double threshold) {
     {   // EXC_PULL_LIVE=0 / 1: never / always use the live-row bitmap (measurements)
         const char* ev = getenv("EXC_PULL_LIVE");
         c->pull_live = ev && (ev[0] == '0' || ev[0] == '1') ? ev[0] - '0' : 2;
+        const char* sq = getenv("EXC_PULL_SEQ");   // 4 / 8 / 16 / 32 entries per lane (measurements)
+        c->pull_seq = sq ? atoi(sq) : 8;
     }
     if (c->d_tadj) return EXC_OK;
     const size_t n = (size_t)c->n;
@@ -2520,15 +2551,17 @@ int exc_enable_pull(void* p, double threshold) {
     // is empty), then the hub lists (cut into pieces for k_pull_hub); cur = fill cursors
     std::vector<PullPiece> hubs;
     std::vector<uint64_t> off(n + 1), cur(n);
+    uint64_t hub_thr = PULL_HUB;
+    if (const char* ev = getenv("EXC_PULL_HUB")) hub_thr = std::max<uint64_t>(64, strtoull(ev, nullptr, 10));   // measurements
     uint64_t acc = 0;
     for (size_t v = 0; v < n; v++) {
         off[v] = cur[v] = acc;
-        if (h[v] <= PULL_HUB) acc += h[v];
+        if (h[v] <= hub_thr) acc += h[v];
     }
     off[n] = acc;
     for (size_t v = 0; v < n; v++) {
         const uint64_t d = h[v];
-        if (d <= PULL_HUB) continue;
+        if (d <= hub_thr) continue;
         c->pull_hub_edges += d;
         c->pull_hub_vertices++;
         for (uint64_t o = 0; o < d; o += PULL_PIECE)
