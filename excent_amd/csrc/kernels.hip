@@ -1452,10 +1452,10 @@ struct Ctx {
     int32_t* d_rank = nullptr;          // p row of each vertex (out-degree order)
     uint32_t* d_pbits = nullptr;        // live-row bitmap of p, 2 * ceil(nsrc / 64) words
     int nsrc = 0;                       // vertices with out-edges (rows of p)
-    int pull_live = 2;                  // live-row bitmap: 0 never, 1 always, 2 when few vertices pend
+    int pull_live = 2;                  // live-row bitmap: 0 never, 1 always, 2 when demand <= frac * m
+    double pull_live_frac = 0.6;
     bool snap_ready = false;            // the last dense pass computed the next snapshot
     int pull_seq = 8;                   // in-list entries per lane in k_pull (4, 8, 16 or 32)
-    uint64_t last_pending = 0;          // pending vertices of the last snapshot
     uint64_t pull_passes_live = 0;
     PullPiece* d_pull_hubs = nullptr;   // pieces of in-lists longer than PULL_HUB
     int n_pull_hubs = 0;
@@ -1670,9 +1670,10 @@ int push_launch(Ctx* c, int s, int src, unsigned grid, PushArgs a, int group, ui
 
 template <int NM, int S>
 void launch_fold_pull(Ctx* c, hipStream_t st) {
-    // the bitmap costs a probe per in-edge; it pays once few sources push (late passes,
-    // where the high-degree vertices that converge last keep the demand high)
-    const bool live = c->nsrc && (c->pull_live == 1 || (c->pull_live == 2 && c->last_pending * 16 <= (uint64_t)c->nsrc));
+    // the bitmap costs a probe per in-edge and saves the row gathers of the sources that did
+    // not push: it pays once the pass's demand (out-edges of pending sources) is well below m
+    const bool live = c->nsrc && (c->pull_live == 1 ||
+                                  (c->pull_live == 2 && (double)c->last_demand <= c->pull_live_frac * (double)c->m));
     unsigned long long* row = c->d_counters + (live ? C_DENSE_LIVE : C_DENSE) * C_FIELDS;
     FoldArgs f{c->n, c->d_idx, c->d_p, c->d_rank, row};
     const int ntiles = (int)(((long long)c->n + BLOCK - 1) / BLOCK);
@@ -2050,8 +2051,7 @@ int snapshot_counts(Ctx* c, uint64_t* h_cnt, uint64_t* h_hotpend) {
     memcpy(h_cnt, c->h_small, 2 * (size_t)c->nc * sizeof(uint64_t));
     if (h_hotpend) *h_hotpend = c->h_small[2 * c->nc];
     c->last_demand = 0;
-    c->last_pending = 0;
-    for (int i = 0; i < c->nc; i++) { c->last_demand += c->h_small[2 * i + 1]; c->last_pending += c->h_small[2 * i]; }
+    for (int i = 0; i < c->nc; i++) c->last_demand += c->h_small[2 * i + 1];
     return EXC_OK;
 }
 
@@ -2499,6 +2499,7 @@ int exc_enable_pull(void* p, double threshold) {
     {   // EXC_PULL_LIVE=0 / 1: never / always use the live-row bitmap (measurements)
         const char* ev = getenv("EXC_PULL_LIVE");
         c->pull_live = ev && (ev[0] == '0' || ev[0] == '1') ? ev[0] - '0' : 2;
+        if (const char* lf = getenv("EXC_PULL_LIVE_FRAC")) c->pull_live_frac = atof(lf);
         const char* sq = getenv("EXC_PULL_SEQ");   // 4 / 8 / 16 / 32 entries per lane (measurements)
         c->pull_seq = sq ? atoi(sq) : 8;
     }
