@@ -131,6 +131,12 @@ def bench_batched(args, nat, np, work, graph):
     xb = {j: sess.values(j) for j in sample}
     sess.close()
     log(f"batched: {rb['passes']} passes, {tb:.3f}s, {rb['kernel']['pushed_edge_measures'] / 1e9:.2f} G vector pushes")
+    if args.batched_only:
+        k = rb["kernel"]
+        return {"metric": "batched personalised PageRank (no baseline)", "unit": "s", "value": round(tb, 4),
+                "passes": rb["passes"], "dense_passes": k["pull_passes"],
+                "kernel_ms": {x: round(k[x], 2) for x in ("snapshot_ms", "push_ms", "piece_ms", "fold_ms", "pull_ms",
+                                                           "pull_hub_ms")}}
 
     sess = nat.Session(common + ["--measure", "pr", "--ppr-column", 0, "--pull", args.pull,
                                  "--log", os.path.join(work, "plain.log")])
@@ -178,7 +184,10 @@ def bench_batched(args, nat, np, work, graph):
                                f"per seed), eps {EPS} per vertex and column, whole graph in HBM",
                    "n": n, "m": m, "ppr_gen": gen},
         "batched": {"time_s": round(tb, 4), "passes": rb["passes"], "vector_pushes": vb,
-                    "vector_gteps": round(vb / tb / 1e9, 3)},
+                    "vector_gteps": round(vb / tb / 1e9, 3),
+                    "kernel_ms": {k: round(rb["kernel"][k], 2) for k in ("snapshot_ms", "push_ms", "piece_ms",
+                                                                          "fold_ms", "pull_ms", "pull_hub_ms")},
+                    "dense_passes": rb["kernel"]["pull_passes"]},
         "plain": {"time_s": round(ts, 4), "time_s_per_column": round(ts / K, 4),
                   "passes_mean": round(sum(seq_passes) / K, 1), "vector_pushes": seq_pushes,
                   "vector_gteps": round(seq_pushes / ts / 1e9, 3)},
@@ -267,6 +276,7 @@ def main():
                          "differentiable personalised PageRank forward + backward")
     ap.add_argument("--scale", type=int, default=None, help="R-MAT scale (default 24; batched/ooc 26; adjoint 22)")
     ap.add_argument("--cache-mb", type=int, default=1024, help="ooc: device cache cap for the hot-cache run")
+    ap.add_argument("--batched-only", action="store_true", help="batched: time the batched run only (no baseline)")
     ap.add_argument("--edge-factor", type=int, default=16)
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--measure", default="all")

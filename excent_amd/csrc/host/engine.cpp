@@ -702,7 +702,7 @@ void Engine::run_passes() {
         if (hot_skip && h_hotpend && o.hot_when != "end") launch_hot(0);
         uint64_t demand = 0;
         for (int c = 0; c < nc; c++) demand += h_cnt[2 * c + 1];
-        if (pull_on_ && !hot_skip && (double)demand >= pull_thr_ * (double)m) {
+        if (pull_on_ && !hot_skip && ((double)demand >= pull_thr_ * (double)m || exc_dense_required(ctx) == 1)) {
             // dense pass over the whole cached graph (every chunk is resident once pull is on),
             // whatever the per-chunk list/range split would have been
             exc_chunk all{0, n, 0, m};

@@ -76,7 +76,13 @@ enum { C_VERT = 0, C_VERT_NM, C_EDGES, C_EDGES_NM, C_PEDGES, C_PEDGES_NM, C_FIEL
 // Row 4 holds the fold kernel of dense (pull) passes.
 // rows 0-3: push sources; row 4: dense passes; row 5: dense passes that used the live-row
 // bitmap (their fold counts give the rows the pull read)
-enum { C_DENSE = 4, C_DENSE_LIVE = 5, C_OVERFLOW = 6 * C_FIELDS, C_NCOUNTERS = C_OVERFLOW + 1 };
+enum { C_DENSE = 4, C_DENSE_LIVE = 5, C_OVERFLOW = 6 * C_FIELDS };
+// Counters updated once per block by kernels with one block per tile (k_pull): spread over
+// SPREAD_SLOTS slots on separate 128-byte lines (slot = block mod SPREAD_SLOTS), summed on
+// the host.  Fields: rows k_pull read; the fused fold's vertices, vertex-measures, edges and
+// edge-measures.
+enum { SPREAD_SLOTS = 64, SPREAD_STRIDE = 16, S_ROWS = 0, S_VERT, S_VERT_NM, S_EDGES, S_EDGES_NM };
+enum { C_SPREAD = C_OVERFLOW + 16, C_NCOUNTERS = C_SPREAD + SPREAD_SLOTS * SPREAD_STRIDE };
 // the dense-pass rows have no piece edges; a spare field counts k_pull_hub's row reads
 enum { C_PULL_HUB_LIVE = C_PEDGES };
 
@@ -720,6 +726,12 @@ struct PullArgs {
     int nc;
     uint32_t* bitmap;
     unsigned long long* cnt;
+    // k_pull: the next pass's fold, fused after the snapshot test (p_next == nullptr: not
+    // fused): pending vertices move r into x and publish their row of p_next at rank[u]
+    float* p_next;
+    const int32_t* rank;
+    int nsrc;
+    unsigned long long* spread;   // C_SPREAD area of the counters
 };
 
 // Bitmap of the p rows the fold made nonzero: one wave per 64 consecutive rows, one ballot,
@@ -777,6 +789,7 @@ __global__ __launch_bounds__(BLOCK) void k_pull(PullArgs a, DevMeasures M) {
     for (int k = 0; k < NM; k++) s_acc[k][tid] = 0.f;
     __syncthreads();
     const uint32_t total = s_start[nt];
+    uint32_t taken = 0;   // p rows read by this lane
     for (uint32_t w0 = 0; w0 < total; w0 += PULL_WIN) {
         const uint32_t wn = total - w0 < (uint32_t)PULL_WIN ? total - w0 : (uint32_t)PULL_WIN;
         for (uint32_t j = tid; j < wn; j += BLOCK) s_src[(j % PULL_SEQ) * PULL_LD + j / PULL_SEQ] = a.tadj[base + w0 + j];
@@ -797,7 +810,7 @@ __global__ __launch_bounds__(BLOCK) void k_pull(PullArgs a, DevMeasures M) {
                 for (int k = 0; k < S; k++) rows[q].v[k] = 0.f;
                 if ((uint32_t)q < jn) {
                     const int v = s_src[q * PULL_LD + tid];
-                    if (row_live(a.live, v)) rows[q] = load_prow<NM, S>(a.p, v);
+                    if (row_live(a.live, v)) { rows[q] = load_prow<NM, S>(a.p, v); taken++; }
                 }
             }
             int owner = lo;
@@ -825,6 +838,20 @@ __global__ __launch_bounds__(BLOCK) void k_pull(PullArgs a, DevMeasures M) {
         }
         __syncthreads();   // the next window rewrites s_src
     }
+    __shared__ unsigned long long s_red[NWAVES][5];
+    unsigned long long* slot = a.spread + (blockIdx.x % SPREAD_SLOTS) * SPREAD_STRIDE;
+    {
+        const uint32_t t = wave_sum_u32(taken);
+        if (lane_id() == 0) s_red[tid / WAVE][0] = t;
+        __syncthreads();
+        if (tid == 0) {
+            unsigned long long sum = 0;
+#pragma unroll
+            for (int w = 0; w < NWAVES; w++) sum += s_red[w][0];
+            if (sum) atomicAdd(&slot[S_ROWS], sum);
+        }
+        __syncthreads();
+    }
     if (!a.bitmap) {
         if (tid < nt && s_start[tid + 1] > s_start[tid]) {
             const long long u = u0 + tid;
@@ -843,21 +870,50 @@ __global__ __launch_bounds__(BLOCK) void k_pull(PullArgs a, DevMeasures M) {
     __shared__ unsigned long long s_cnt[NWAVES][2];
     bool pend = false;
     uint64_t d = 0;
+    uint32_t nmv = 0;
     if (tid < nt) {
         const long long u = u0 + tid;
         Row<S> row = load_row<S>(M.r, u);
+        bool upd = false;
         if (s_start[tid + 1] > s_start[tid]) {
-            bool upd = false;
 #pragma unroll
             for (int k = 0; k < NM; k++) {
                 const float acc = s_acc[k][tid];
                 if (acc != 0.f) { row.v[k] += acc; upd = true; }
             }
-            if (upd) store_row<S>(M.r, u, row);
         }
 #pragma unroll
         for (int k = 0; k < NM; k++) pend |= fabsf(row.v[k]) > M.eps[k];
         if (pend) d = a.idx[u + 1] - a.idx[u];
+        if (a.p_next) {   // the next pass's fold (k_fold's rule, on the row in registers)
+            Row<S> pr, dx;
+#pragma unroll
+            for (int k = 0; k < S; k++) { pr.v[k] = 0.f; dx.v[k] = 0.f; }
+            if (pend) {
+#pragma unroll
+                for (int k = 0; k < NM; k++) {
+                    if (fabsf(row.v[k]) > M.eps[k]) {
+                        const float dv = row.v[k];
+                        row.v[k] = 0.f;
+                        dx.v[k] = dv;
+                        nmv++;
+                        if (d) pr.v[k] = M.divdeg[k] ? M.scale[k] * dv / (float)d : M.scale[k] * dv;
+                    }
+                }
+                Row<S> xr = load_row<S>(M.x, u);
+#pragma unroll
+                for (int k = 0; k < NM; k++) xr.v[k] += dx.v[k];
+                store_row<S>(M.x, u, xr);
+                upd = true;
+            }
+            const int q = a.rank[u];
+            if (q < a.nsrc) {
+                float* pq = a.p_next + (long long)q * NM;
+#pragma unroll
+                for (int k = 0; k < NM; k++) pq[k] = pr.v[k];
+            }
+        }
+        if (upd) store_row<S>(M.r, u, row);
     }
     const uint64_t mask = __ballot(pend);
     const long long wb = u0 + (tid & ~(WAVE - 1));
@@ -883,6 +939,22 @@ __global__ __launch_bounds__(BLOCK) void k_pull(PullArgs a, DevMeasures M) {
         const int c = chunk_of((int)(u0 + tid), a.cv0, a.nc);
         atomicAdd(&a.cnt[2 * c], 1ull);
         atomicAdd(&a.cnt[2 * c + 1], (unsigned long long)d);
+    }
+    if (a.p_next) {   // fused fold statistics, one atomic per field and block (spread slots)
+        const uint64_t f[4] = {nmv ? 1ull : 0ull, nmv, nmv ? d : 0ull, d * nmv};
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            const uint64_t t = wave_sum_u64(f[i]);
+            if (lane == 0) s_red[tid / WAVE][1 + i] = t;
+        }
+        __syncthreads();
+        if (tid < 4) {
+            unsigned long long sum = 0;
+#pragma unroll
+            for (int w = 0; w < NWAVES; w++) sum += s_red[w][1 + tid];
+            if (sum) atomicAdd(&slot[S_VERT + tid], sum);
+        }
     }
 }
 
@@ -1027,7 +1099,7 @@ __global__ __launch_bounds__(BLOCK) void k_bsnapshot(SnapArgs a, BatchArgs B) {
 #pragma unroll
             for (int u = 0; u < U; u++) {
                 const long long v = base + i0 + u;
-                row[u] = (v < a.n && lane < B.K) ? B.r[v * B.K + lane] : 0.f;
+                row[u] = (v < a.n && col_on) ? B.r[v * B.K + lane] : 0.f;   // inactive columns: no load
             }
 #pragma unroll
             for (int u = 0; u < U; u++) {
@@ -1248,6 +1320,15 @@ struct BDenseArgs {
     const int32_t* tadj;          // in-neighbours (vertex ids)
     unsigned long long* counters; // C_FIELDS row of the dense pass
     unsigned long long* live_count;   // k_bpull_hub: rows read
+    // k_bpull: the next pass's snapshot and fold, fused into the row updates
+    // (p_next == nullptr: not fused)
+    float* p_next;                // [n][K] rows of the next pass
+    uint32_t* bitmap_next;        // its snapshot bitmap
+    const uint32_t* hubbits;      // destinations whose in-lists went to k_bpull_hub
+    const int* cv0;
+    int nc;
+    unsigned long long* cnt;      // [2 nc] per-chunk pending vertices and degree sums
+    unsigned long long* spread;   // C_SPREAD area of the counters
 };
 
 __device__ __forceinline__ bool bit_of(const uint32_t* __restrict__ bm, int v) {
@@ -1282,7 +1363,7 @@ __global__ __launch_bounds__(BLOCK) void k_bfold(BDenseArgs a, BatchArgs B) {
                     if (deg) pv = B.damp * d / (float)deg;
                 }
             }
-            if (deg && lane < B.K) a.p[v * B.K + lane] = pv;
+            if (deg && col_on) a.p[v * B.K + lane] = pv;   // the pull reads active columns only
             const uint64_t fm = __ballot(d != 0.f), pm = __ballot(pv != 0.f);
             if (lane == 0) {
                 if (fm) { st[0] += 1; st[1] += (uint64_t)__popcll(fm); }
@@ -1299,26 +1380,26 @@ __global__ __launch_bounds__(BLOCK) void k_bfold(BDenseArgs a, BatchArgs B) {
 // Sum of the live p rows of in-list entries [b, e) into acc (lane = column), in slot
 // order; at every owner change `flush(owner, acc)` is called.  Eight rows are loaded before
 // they are added, so a wave keeps eight 256-byte gathers in flight.
-template <typename OwnerOf, typename Flush>
+template <int Q, typename OwnerOf, typename Flush>
 __device__ __forceinline__ uint32_t bsum_rows(const BDenseArgs& a, const BatchArgs& B, uint64_t b, uint64_t e,
                                               OwnerOf owner_of, Flush flush, int& cur, float& acc) {
     const int lane = lane_id();
-    const bool col = lane < B.K;
+    const bool col = lane < B.K && ((B.active >> lane) & 1ull);
     uint32_t taken = 0;
     for (uint64_t e0 = b; e0 < e; e0 += WAVE) {
         const uint64_t i = e0 + lane;
         const int v = i < e ? a.tadj[i] : -1;
         uint64_t lm = __ballot(v >= 0 && bit_of(a.pending, v));
         while (lm) {
-            int slot[8];
+            int slot[Q];
 #pragma unroll
-            for (int q = 0; q < 8; q++) {
+            for (int q = 0; q < Q; q++) {
                 slot[q] = lm ? __ffsll((long long)lm) - 1 : -1;
                 if (lm) lm &= lm - 1;
             }
-            float val[8];
+            float val[Q];
 #pragma unroll
-            for (int q = 0; q < 8; q++) {
+            for (int q = 0; q < Q; q++) {
                 val[q] = 0.f;
                 if (slot[q] >= 0) {
                     const int vv = __shfl(v, slot[q], WAVE);
@@ -1326,7 +1407,7 @@ __device__ __forceinline__ uint32_t bsum_rows(const BDenseArgs& a, const BatchAr
                 }
             }
 #pragma unroll
-            for (int q = 0; q < 8; q++) {
+            for (int q = 0; q < Q; q++) {
                 if (slot[q] < 0) break;
                 const int own = owner_of(e0 + (uint64_t)slot[q]);
                 if (own != cur) {
@@ -1347,27 +1428,126 @@ __device__ __forceinline__ uint32_t bsum_rows(const BDenseArgs& a, const BatchAr
 // lanes = columns) and adds each destination's sum to its r row once, with a plain
 // read-modify-write (no other wave touches these rows).  The owner of a slot is the last
 // of the 64 list starts at or before it (one ballot).
+template <int Q>
 __global__ __launch_bounds__(BLOCK) void k_bpull(BDenseArgs a, BatchArgs B) {
-    const int lane = lane_id();
+    __shared__ uint32_t s_col[NWAVES][WAVE];
+    const int lane = lane_id(), wid = threadIdx.x / WAVE;
+    const long long gw = (blockIdx.x * (long long)BLOCK + threadIdx.x) / WAVE;
     const long long nw = (long long)gridDim.x * NWAVES;
     const long long ntile = ((long long)a.n + WAVE - 1) / WAVE;
-    for (long long t = (blockIdx.x * (long long)BLOCK + threadIdx.x) / WAVE; t < ntile; t += nw) {
+    const long long words = ((long long)a.n + 31) / 32;
+    const bool fused = a.p_next != nullptr;
+    const bool col_on = lane < B.K && ((B.active >> lane) & 1ull);
+    const float eps = lane < B.K ? B.eps[lane] : 0.f;
+    // fused: each wave takes a contiguous run of tiles, so the per-chunk counts of the next
+    // snapshot accumulate in registers; otherwise tiles are dealt round robin
+    const long long per = fused ? (ntile + nw - 1) / nw : 1;
+    const long long t0 = fused ? gw * per : gw;
+    const long long t1 = fused ? (t0 + per < ntile ? t0 + per : ntile) : ntile;
+    const long long step = fused ? 1 : nw;
+    uint32_t colcnt = 0;                  // pending entries of this lane's column (next pass)
+    uint64_t fs[4] = {0, 0, 0, 0};        // fused fold statistics (wave-uniform)
+    int cc = -1;
+    uint64_t acc_n = 0, acc_d = 0;
+    auto flush_counts = [&]() {
+        if (cc >= 0 && lane == 0 && acc_n) {
+            atomicAdd(&a.cnt[2 * cc], (unsigned long long)acc_n);
+            atomicAdd(&a.cnt[2 * cc + 1], (unsigned long long)acc_d);
+        }
+        acc_n = 0;
+        acc_d = 0;
+    };
+    for (long long t = t0; t < t1; t += step) {
         const long long u0 = t * WAVE;
         const int nt = (int)((long long)a.n - u0 < WAVE ? (long long)a.n - u0 : WAVE);
         const uint64_t lo = a.tidx[u0], hi = a.tidx[u0 + nt];
         const uint64_t my = lane < nt ? a.tidx[u0 + lane] : hi;   // lanes past the tile never own a slot
+        uint64_t tmask = 0;   // destinations of the tile pending in the next pass
+        // The next pass's fold of destination u (k_bfold's rule on the row just completed):
+        // s = this pass's sum for u (0 for a hub, whose row k_bpull_hub completed).
+        auto fold_row = [&](int own, float s) {
+            const long long u = u0 + own;
+            float rv = col_on ? B.r[u * B.K + lane] + s : 0.f;
+            const bool pend = col_on && fabsf(rv) > eps;
+            const uint64_t pm = __ballot(pend);
+            if (!pm) {
+                if (col_on && s != 0.f) B.r[u * B.K + lane] = rv;
+                return;
+            }
+            const uint64_t deg = a.idx[u + 1] - a.idx[u];
+            float pv = 0.f;
+            if (pend) {
+                B.x[u * B.K + lane] += rv;
+                if (deg) pv = B.damp * rv / (float)deg;
+                rv = 0.f;
+            }
+            if (col_on) {
+                B.r[u * B.K + lane] = rv;
+                if (deg) a.p_next[u * B.K + lane] = pv;
+            }
+            tmask |= 1ull << own;
+            colcnt += pend ? 1u : 0u;
+            const uint64_t vm = __ballot(pv != 0.f);
+            fs[0] += 1;
+            fs[1] += (uint64_t)__popcll(pm);
+            if (vm) { fs[2] += deg; fs[3] += deg * (uint64_t)__popcll(vm); }
+        };
         int cur = -1;
         float acc = 0.f;
         auto owner_of = [&](uint64_t pos) { return __popcll(__ballot(my <= pos)) - 1; };
-        auto flush = [&](int own, float s) {
-            if (lane < B.K && s != 0.f) B.r[(u0 + own) * B.K + lane] += s;
+        auto flush = [&](int own, float sum) {
+            if (fused) {
+                if (__ballot(sum != 0.f)) fold_row(own, sum);
+            } else if (lane < B.K && sum != 0.f) {
+                B.r[(u0 + own) * B.K + lane] += sum;
+            }
         };
-        bsum_rows(a, B, lo, hi, owner_of, flush, cur, acc);
+        bsum_rows<Q>(a, B, lo, hi, owner_of, flush, cur, acc);
         if (cur >= 0) flush(cur, acc);
+        if (!fused) continue;
+        // hub destinations of the tile, then the tile's two bitmap words and counts; a
+        // destination that received nothing stays below eps (it was folded or was not pending)
+        uint64_t hm = __ballot(lane < nt && bit_of(a.hubbits, (int)(u0 + lane)));
+        while (hm) {
+            const int i = __ffsll((long long)hm) - 1;
+            hm &= hm - 1;
+            fold_row(i, 0.f);
+        }
+        if (lane == 0) {
+            a.bitmap_next[2 * t] = (uint32_t)tmask;
+            if (2 * t + 1 < words) a.bitmap_next[2 * t + 1] = (uint32_t)(tmask >> 32);
+        }
+        if (!tmask) continue;
+        const bool pv = (tmask >> lane) & 1ull;
+        const uint64_t dv = pv ? a.idx[u0 + lane + 1] - a.idx[u0 + lane] : 0;
+        const int c0 = chunk_of((int)u0, a.cv0, a.nc), c1 = chunk_of((int)(u0 + nt - 1), a.cv0, a.nc);
+        if (c0 == c1) {
+            if (c0 != cc) { flush_counts(); cc = c0; }
+            acc_n += (uint64_t)__popcll(tmask);
+            acc_d += wave_sum_u64(dv);
+        } else if (pv) {
+            const int c = chunk_of((int)(u0 + lane), a.cv0, a.nc);
+            atomicAdd(&a.cnt[2 * c], 1ull);
+            atomicAdd(&a.cnt[2 * c + 1], (unsigned long long)dv);
+        }
+    }
+    if (!fused) return;
+    flush_counts();
+    s_col[wid][lane] = colcnt;
+    __syncthreads();
+    if (wid == 0 && lane < B.K) {
+        uint32_t tt = 0;
+#pragma unroll
+        for (int w = 0; w < NWAVES; w++) tt += s_col[w][lane];
+        if (tt) atomicAdd(&B.colpend[lane], tt);
+    }
+    if (lane == 0 && fs[0]) {
+        unsigned long long* slot = a.spread + (gw % SPREAD_SLOTS) * SPREAD_STRIDE;
+#pragma unroll
+        for (int i = 0; i < 4; i++) if (fs[i]) atomicAdd(&slot[S_VERT + i], (unsigned long long)fs[i]);
     }
 }
-
-// In-lists above PULL_HUB: one wave per piece, one atomic row add per piece.
+template <int Q>
 __global__ __launch_bounds__(BLOCK) void k_bpull_hub(BDenseArgs a, BatchArgs B, const PullPiece* __restrict__ pieces,
                                                       int npieces) {
     const int lane = lane_id();
@@ -1379,7 +1559,7 @@ __global__ __launch_bounds__(BLOCK) void k_bpull_hub(BDenseArgs a, BatchArgs B, 
         float acc = 0.f;
         auto owner_of = [](uint64_t) { return 0; };
         auto flush = [](int, float) {};
-        taken += bsum_rows(a, B, pc.start, pc.start + pc.len, owner_of, flush, cur, acc);
+        taken += bsum_rows<Q>(a, B, pc.start, pc.start + pc.len, owner_of, flush, cur, acc);
         if (lane < B.K && acc != 0.f) atomic_add_f32(&B.r[(long long)pc.u * B.K + lane], acc);
     }
     if (lane == 0 && taken) atomicAdd(a.live_count, (unsigned long long)taken);
@@ -1455,7 +1635,14 @@ struct Ctx {
     int pull_live = 2;                  // live-row bitmap: 0 never, 1 always, 2 when demand <= frac * m
     double pull_live_frac = 0.6;
     bool snap_ready = false;            // the last dense pass computed the next snapshot
+    bool folded = false;                // ... and the next pass's fold (p holds its rows): the
+                                        // next pass must be a dense pass over all vertices
+    int fuse_fold = 1;                  // fold the next pass inside k_pull when demand is high
+    float* d_p2 = nullptr;              // the second p buffer (rows of the next pass)
+    uint32_t* d_bitmap2 = nullptr;      // batched: the next pass's snapshot bitmap
+    uint32_t* d_hubbits = nullptr;      // batched: destinations summed by k_bpull_hub
     int pull_seq = 8;                   // in-list entries per lane in k_pull (4, 8, 16 or 32)
+    int bpull_q = 8;                    // batched pull: rows in flight per wave (8 or 16)
     uint64_t pull_passes_live = 0;
     PullPiece* d_pull_hubs = nullptr;   // pieces of in-lists longer than PULL_HUB
     int n_pull_hubs = 0;
@@ -1678,16 +1865,22 @@ void launch_fold_pull(Ctx* c, hipStream_t st) {
     FoldArgs f{c->n, c->d_idx, c->d_p, c->d_rank, row};
     const int ntiles = (int)(((long long)c->n + BLOCK - 1) / BLOCK);
     PullArgs pa{c->n, c->d_tidx, c->d_tadj, c->d_p, live ? c->d_pbits : nullptr, row + C_PULL_HUB_LIVE,
-                c->d_idx, c->d_cv0, c->nc, nullptr, nullptr};
+                c->d_idx, c->d_cv0, c->nc, nullptr, nullptr, nullptr, c->d_rank, c->nsrc, c->d_counters + C_SPREAD};
     c->pull_passes_live += live;
     const bool fuse_snap = !c->d_trace;   // the trace counter stays with k_snapshot
+    // fold the next pass here too while the demand is high (the next pass is then a dense pass
+    // whatever its demand; below the threshold the fold stays with k_fold, so the switch to
+    // push passes needs no unfolding)
+    const bool fuse_fold = fuse_snap && c->fuse_fold && c->d_p2 &&
+                           (double)c->last_demand >= std::max(0.1, 2.0 * c->pull_threshold) * (double)c->m;
     if (fuse_snap) {
         const int ncnt = 2 * c->nc + 1;
         hipLaunchKernelGGL(k_zero_u64, dim3((ncnt + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, st, c->d_cnt, ncnt);
         pa.bitmap = c->d_bitmap;
         pa.cnt = c->d_cnt;
+        if (fuse_fold) pa.p_next = c->d_p2;
     }
-    {
+    if (!c->folded) {   // else the last pull already folded this pass into p
         ScopedTimer t(c, st, T_FOLD);
         hipLaunchKernelGGL((k_fold<NM, S>), dim3(std::min(ntiles, c->cus * 8)), dim3(BLOCK), 0, st, f, c->DM);
     }
@@ -1711,31 +1904,51 @@ void launch_fold_pull(Ctx* c, hipStream_t st) {
     }
     c->snap_ready = fuse_snap;
     if (fuse_snap) c->snap_bytes += 4.0 * (((double)c->n + 31) / 32);
+    c->folded = fuse_fold;
+    if (fuse_fold) std::swap(c->d_p, c->d_p2);   // the rows just written serve the next pass
 }
 
 // Batched dense pass: fold the snapshot's pending vertices, hub pieces, then the pull.
 void launch_bfold_pull(Ctx* c, hipStream_t st) {
     unsigned long long* row = c->d_counters + C_DENSE_LIVE * C_FIELDS;
-    BDenseArgs a{c->n, c->d_idx, c->d_bitmap, c->d_p, c->d_tidx, c->d_tadj, row, row + C_PULL_HUB_LIVE};
+    const bool fuse = c->fuse_fold && c->d_p2 && c->d_bitmap2 && !c->d_trace &&
+                      (double)c->last_demand >= std::max(0.1, 2.0 * c->pull_threshold) * (double)c->m;
+    BDenseArgs a{c->n, c->d_idx, c->d_bitmap, c->d_p, c->d_tidx, c->d_tadj, row, row + C_PULL_HUB_LIVE,
+                 fuse ? c->d_p2 : nullptr, c->d_bitmap2, c->d_hubbits, c->d_cv0, c->nc, c->d_cnt,
+                 c->d_counters + C_SPREAD};
     const BatchArgs B = batch_args(c, -1);
     const long long segs = ((long long)c->n + WAVE - 1) / WAVE;
     const unsigned cap = (unsigned)c->cus * 8;
-    {
+    if (!c->folded) {   // else the last pull already folded this pass into p
         ScopedTimer t(c, st, T_FOLD);
         hipLaunchKernelGGL(k_bfold, dim3((unsigned)std::min<long long>((segs + NWAVES - 1) / NWAVES, cap)), dim3(BLOCK), 0,
                            st, a, B);
     }
+    if (fuse) {   // the next snapshot's counts are accumulated by k_bpull
+        const int ncnt = 2 * c->nc + 1;
+        hipLaunchKernelGGL(k_zero_u64, dim3((ncnt + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, st, c->d_cnt, ncnt);
+        hipLaunchKernelGGL(k_zero_u32, dim3((c->K + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, st, c->d_colpend, c->K);
+    }
     if (c->n_pull_hubs) {
         ScopedTimer t(c, st, T_PULL_HUB);
-        hipLaunchKernelGGL(k_bpull_hub, dim3((unsigned)std::min<long long>((c->n_pull_hubs + NWAVES - 1) / NWAVES, cap)),
-                           dim3(BLOCK), 0, st, a, B, c->d_pull_hubs, c->n_pull_hubs);
+        const dim3 g((unsigned)std::min<long long>((c->n_pull_hubs + NWAVES - 1) / NWAVES, cap));
+        if (c->bpull_q == 16) hipLaunchKernelGGL(k_bpull_hub<16>, g, dim3(BLOCK), 0, st, a, B, c->d_pull_hubs, c->n_pull_hubs);
+        else hipLaunchKernelGGL(k_bpull_hub<8>, g, dim3(BLOCK), 0, st, a, B, c->d_pull_hubs, c->n_pull_hubs);
     }
     {
         ScopedTimer t(c, st, T_PULL);
-        hipLaunchKernelGGL(k_bpull, dim3((unsigned)std::min<long long>((segs + NWAVES - 1) / NWAVES, 2 * cap)), dim3(BLOCK),
-                           0, st, a, B);
+        const dim3 g((unsigned)std::min<long long>((segs + NWAVES - 1) / NWAVES, 2 * cap));
+        if (c->bpull_q == 16) hipLaunchKernelGGL(k_bpull<16>, g, dim3(BLOCK), 0, st, a, B);
+        else hipLaunchKernelGGL(k_bpull<8>, g, dim3(BLOCK), 0, st, a, B);
     }
     c->pull_passes_live++;
+    c->folded = fuse;
+    c->snap_ready = fuse;
+    if (fuse) {   // the rows and bitmap just written serve the next pass
+        std::swap(c->d_p, c->d_p2);
+        std::swap(c->d_bitmap, c->d_bitmap2);
+        c->snap_bytes += 4.0 * (((double)c->n + 31) / 32);
+    }
 }
 
 int fold_pull(Ctx* c) {
@@ -1934,7 +2147,8 @@ int exc_destroy(void* p) {
     for (auto& pb : c->pb) { if (pb.d) hipFree(pb.d); if (pb.cnt) hipFree(pb.cnt); }
     for (auto s : c->st) if (s) hipStreamDestroy(s);
     for (auto q : c->prows) if (q) (void)hipFree(q);
-    void* pull_ptrs[] = {c->d_tidx, c->d_tadj, c->d_p, c->d_pull_hubs, c->d_wts, c->d_rank, c->d_pbits};
+    void* pull_ptrs[] = {c->d_tidx, c->d_tadj, c->d_p, c->d_pull_hubs, c->d_wts, c->d_rank, c->d_pbits, c->d_p2,
+                         c->d_bitmap2, c->d_hubbits};
     for (void* q : pull_ptrs) if (q) (void)hipFree(q);
     if (c->d_beps) (void)hipFree(c->d_beps);
     if (c->d_colpend) (void)hipFree(c->d_colpend);
@@ -1959,6 +2173,7 @@ int exc_init_measure(void* p, int i, float beta, float lambda) {
     if (!c || i < 0 || i >= c->nm) return fail(EXC_EINVAL, "exc_init_measure: bad slot");
     HIPCHK(hipSetDevice(c->dev));
     c->snap_ready = false;   // the state changes: a fused snapshot no longer holds
+    c->folded = false;       // the state is replaced: a folded pass no longer applies
     if (c->K) c->active = all_columns(c->K);
     if (c->K) {   // batched: every column starts at x = r = 0 (exc_batch_seed adds the teleport mass)
         const size_t bytes = (size_t)c->n * c->S * sizeof(float);
@@ -1980,6 +2195,7 @@ int exc_set_state(void* p, int i, const float* h_x, const float* h_r) {
     if (!c || i < 0 || i >= (c->K ? c->K : c->nm)) return fail(EXC_EINVAL, "exc_set_state: bad slot");
     HIPCHK(hipSetDevice(c->dev));
     c->snap_ready = false;   // the state changes: a fused snapshot no longer holds
+    c->folded = false;       // the state is replaced: a folded pass no longer applies
     if (c->K) c->active = all_columns(c->K);
     float* tmp;
     const size_t bytes = (size_t)c->n * sizeof(float);
@@ -2022,6 +2238,7 @@ int exc_state_checkpoint(void* p, int restore) {
     if (c->K) c->active = all_columns(c->K);
     const size_t bytes = (size_t)c->n * c->S * sizeof(float);
     if (restore) {
+        c->folded = false;
         if (!c->d_x0) return fail(EXC_ESTATE, "exc_state_checkpoint: nothing saved");
         HIPCHK(hipMemcpyAsync(c->d_x, c->d_x0, bytes, hipMemcpyDeviceToDevice, c->st[0]));
         HIPCHK(hipMemcpyAsync(c->d_r, c->d_r0, bytes, hipMemcpyDeviceToDevice, c->st[0]));
@@ -2061,7 +2278,7 @@ int exc_snapshot(void* p, int hot_skip, uint64_t* h_cnt, uint64_t* h_hotpend) {
     HIPCHK(hipSetDevice(c->dev));
     hipStream_t st = c->st[0];
     const int ncnt = 2 * c->nc + 1;
-    if (c->snap_ready && !hot_skip && !c->K) {   // computed by the dense pass that ended the last pass
+    if ((c->snap_ready || c->folded) && !hot_skip) {   // computed by the dense pass that ended the last pass
         c->snap_ready = false;
         return snapshot_counts(c, h_cnt, h_hotpend);
     }
@@ -2227,8 +2444,10 @@ int exc_push_resident(void* p, const exc_chunk* ch, int use_list, int group, int
     if (!c->d_arena || ch->e1 > c->arena_edges) return fail(EXC_ESTATE, "exc_push_resident: chunk not in the edge cache");
     HIPCHK(hipSetDevice(c->dev));
     c->snap_ready = false;   // the state changes: a fused snapshot no longer holds
-    if (c->d_tadj && !c->d_wts && ch->v0 == 0 && ch->v1 == c->n && !hot_skip &&
-        (double)c->last_demand >= c->pull_threshold * (double)c->m)
+    const bool whole = ch->v0 == 0 && ch->v1 == c->n && !hot_skip;
+    if (c->folded && !whole)
+        return fail(EXC_ESTATE, "exc_push_resident: the last dense pass folded this pass; it must be a dense pass over all vertices");
+    if (c->d_tadj && !c->d_wts && whole && (c->folded || (double)c->last_demand >= c->pull_threshold * (double)c->m))
         return fold_pull(c);
     return push_chunk(c, 0, ch->v0, ch->v1, c->d_arena, 0, use_list, group, hot_skip, c->arena_edges, c->d_wts);
 }
@@ -2244,6 +2463,8 @@ int exc_stream_chunk(void* p, int s, const exc_chunk* ch, int to_resident, int h
     if (to_resident && (!c->d_arena || ch->e1 > c->arena_edges)) return fail(EXC_ESTATE, "exc_stream_chunk: chunk not in the edge cache");
     HIPCHK(hipSetDevice(c->dev));
     c->snap_ready = false;   // the state changes: a fused snapshot no longer holds
+    if (c->folded)
+        return fail(EXC_ESTATE, "exc_stream_chunk: the last dense pass folded this pass; it must be a dense pass over all vertices");
     Slot& sl = c->slot[s];
     hipStream_t st = c->st[1 + s];
     int32_t* dst;
@@ -2277,6 +2498,8 @@ int exc_gather_chunk(void* p, int s, int k, int group) {
     if (ne > c->max_chunk_edges) return fail(EXC_EINVAL, "exc_gather_chunk: gathered lists exceed the slot");
     HIPCHK(hipSetDevice(c->dev));
     c->snap_ready = false;   // the state changes: a fused snapshot no longer holds
+    if (c->folded)
+        return fail(EXC_ESTATE, "exc_gather_chunk: the last dense pass folded this pass; it must be a dense pass over all vertices");
     hipStream_t st = c->st[1 + s];
     if (!c->zc) {
         HIPCHK(hipMemcpyAsync(sl.d_vids, sl.h_vids, (size_t)k * sizeof(int32_t), hipMemcpyHostToDevice, st));
@@ -2304,6 +2527,8 @@ int exc_push_hot(void* p, int lo, int hi, int group, int after_slots) {
     if (hi == lo) return EXC_OK;
     HIPCHK(hipSetDevice(c->dev));
     c->snap_ready = false;   // the state changes: a fused snapshot no longer holds
+    if (c->folded)
+        return fail(EXC_ESTATE, "exc_push_hot: the last dense pass folded this pass; it must be a dense pass over all vertices");
     hipStream_t st = c->st[0];
     if (after_slots)
         for (auto& sl : c->slot) if (sl.busy) HIPCHK(hipStreamWaitEvent(st, sl.done, 0));
@@ -2356,6 +2581,7 @@ int exc_batch_seed(void* p, int col, const int32_t* seeds, int count, float valu
     if (!count) return EXC_OK;
     HIPCHK(hipSetDevice(c->dev));
     c->snap_ready = false;   // the state changes: a fused snapshot no longer holds
+    c->folded = false;       // the state is replaced: a folded pass no longer applies
     if (c->K) c->active = all_columns(c->K);
     int32_t* d;
     HIPCHK(hipMalloc(&d, (size_t)count * sizeof(int32_t)));
@@ -2391,6 +2617,7 @@ int exc_set_edge_weights(void* p, const float* h_w, uint64_t cnt) {
         return fail(EXC_ESTATE, "exc_set_edge_weights: the whole graph must be in the edge cache and cnt == m");
     HIPCHK(hipSetDevice(c->dev));
     c->snap_ready = false;   // the state changes: a fused snapshot no longer holds
+    c->folded = false;       // the state is replaced: a folded pass no longer applies
     if (!c->d_wts) HIPCHK(hipMalloc(&c->d_wts, (c->m ? c->m : 1) * sizeof(float)));
     if (c->m) HIPCHK(hipMemcpy(c->d_wts, h_w, c->m * sizeof(float), hipMemcpyHostToDevice));
     return EXC_OK;
@@ -2416,6 +2643,7 @@ int exc_pushed(void* p, uint64_t* edges) {
     HIPCHK(hipMemcpy(h, c->d_counters, sizeof h, hipMemcpyDeviceToHost));
     uint64_t e = 0;
     for (int s = 0; s < 6; s++) e += h[s * C_FIELDS + C_EDGES] + (s < 4 ? h[s * C_FIELDS + C_PEDGES] : 0ull);
+    for (int q = 0; q < SPREAD_SLOTS; q++) e += h[C_SPREAD + q * SPREAD_STRIDE + S_EDGES];
     *edges = e;
     if (h[C_OVERFLOW]) return fail(EXC_ESTATE, "hub piece buffer overflowed %llu times", h[C_OVERFLOW]);
     return EXC_OK;
@@ -2473,8 +2701,13 @@ int exc_get_stats(void* p, exc_stats* o) {
         // k_pull_hub's own count.  In-lists above the hub threshold go to the hub kernel.
         const double me = (double)(c->m - c->pull_hub_edges), nv = (double)(c->n - c->pull_hub_vertices);
         const double bm = c->K ? 4.0 * (((double)c->n + 31) / 32) : 8.0 * ((c->nsrc + 63) / 64);
+        unsigned long long sp[5] = {0, 0, 0, 0, 0};
+        for (int q = 0; q < SPREAD_SLOTS; q++)
+            for (int f = 0; f < 5; f++) sp[f] += h[C_SPREAD + q * SPREAD_STRIDE + f];
         const double hub_rows = (P - PL) * c->pull_hub_edges + (double)r5[C_PULL_HUB_LIVE];
-        const double rows = (P - PL) * me + std::max(0.0, (double)r5[C_EDGES] - (double)r5[C_PULL_HUB_LIVE]);
+        // k_pull counts the rows it reads (batched passes count them through the fold)
+        const double rows = c->K ? std::max(0.0, (double)(r5[C_EDGES] + sp[S_EDGES]) - (double)r5[C_PULL_HUB_LIVE])
+                                 : (double)sp[S_ROWS];
         o->pull_bytes = P * (me * 4.0 + nv * (8.0 + 8.0 * nm)) + PL * bm + 4.0 * nm * rows;
         o->pull_hub_bytes = P * ((double)c->pull_hub_edges * 4.0 + (double)c->pull_hub_vertices * (8.0 + 8.0 * nm)) +
                             PL * bm + 4.0 * nm * hub_rows;
@@ -2484,6 +2717,11 @@ int exc_get_stats(void* p, exc_stats* o) {
             o->pushed_edges += r[C_EDGES];
             o->pushed_edge_measures += r[C_EDGES_NM];
         }
+        // folds done inside k_pull for the following pass
+        o->pushed_vertices += sp[S_VERT];
+        o->pushed_vertex_measures += sp[S_VERT_NM];
+        o->pushed_edges += sp[S_EDGES];
+        o->pushed_edge_measures += sp[S_EDGES_NM];
     }
     if (h[C_OVERFLOW]) return fail(EXC_ESTATE, "hub piece buffer overflowed %llu times", h[C_OVERFLOW]);
     return EXC_OK;
@@ -2500,6 +2738,8 @@ int exc_enable_pull(void* p, double threshold) {
         const char* ev = getenv("EXC_PULL_LIVE");
         c->pull_live = ev && (ev[0] == '0' || ev[0] == '1') ? ev[0] - '0' : 2;
         if (const char* lf = getenv("EXC_PULL_LIVE_FRAC")) c->pull_live_frac = atof(lf);
+        if (const char* ff = getenv("EXC_FUSE_FOLD")) c->fuse_fold = atoi(ff);   // 0: fold with k_fold always
+        if (const char* bq = getenv("EXC_BPULL_Q")) c->bpull_q = atoi(bq);   // 8 / 16 rows per wave
         const char* sq = getenv("EXC_PULL_SEQ");   // 4 / 8 / 16 / 32 entries per lane (measurements)
         c->pull_seq = sq ? atoi(sq) : 8;
     }
@@ -2527,12 +2767,18 @@ int exc_enable_pull(void* p, double threshold) {
     ok = ok && hipMalloc(&c->d_pbits, (2 * (((size_t)c->nsrc + 63) / 64) + 2) * sizeof(uint32_t)) == hipSuccess;
     ok = ok && hipMalloc(&c->d_tadj, (c->m ? c->m : 1) * sizeof(int32_t)) == hipSuccess;
     ok = ok && hipMalloc(&c->d_p, n * c->S * sizeof(float)) == hipSuccess;
+    if (ok && c->fuse_fold) ok = hipMalloc(&c->d_p2, n * c->S * sizeof(float)) == hipSuccess;
+    if (ok && by_id && c->fuse_fold) {
+        ok = hipMalloc(&c->d_bitmap2, ((n + 63) / 64) * 2 * sizeof(uint32_t)) == hipSuccess;
+        ok = ok && hipMalloc(&c->d_hubbits, ((n + 63) / 64) * 2 * sizeof(uint32_t)) == hipSuccess;
+    }
     ok = ok && hipMalloc(&cnt, (n + 1) * sizeof(unsigned long long)) == hipSuccess;
     if (!ok) {
         (void)hipGetLastError();
-        void* q[] = {c->d_tidx, c->d_tadj, c->d_p, cnt, c->d_rank, c->d_pbits};
+        void* q[] = {c->d_tidx, c->d_tadj, c->d_p, cnt, c->d_rank, c->d_pbits, c->d_p2, c->d_bitmap2, c->d_hubbits};
         for (void* x : q) if (x) (void)hipFree(x);
         c->d_tidx = nullptr; c->d_tadj = nullptr; c->d_p = nullptr; c->d_rank = nullptr; c->d_pbits = nullptr;
+        c->d_p2 = nullptr; c->d_bitmap2 = nullptr; c->d_hubbits = nullptr;
         c->nsrc = 0;
         c->pull_threshold = -1;
         return fail(EXC_ENOMEM, "exc_enable_pull: cannot allocate the transpose (%.2f GB)", c->m * 4.0 / 1e9);
@@ -2577,12 +2823,23 @@ int exc_enable_pull(void* p, double threshold) {
     e = hipDeviceSynchronize();
     if (e != hipSuccess) return fail_free(e);
     (void)hipFree(cnt);
+    if (c->d_hubbits) {   // batched: which destinations k_bpull_hub completes
+        std::vector<uint32_t> hb(((n + 63) / 64) * 2, 0);
+        for (const PullPiece& pc : hubs) hb[pc.u >> 5] |= 1u << (pc.u & 31);
+        HIPCHK(hipMemcpy(c->d_hubbits, hb.data(), hb.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+    }
     c->n_pull_hubs = (int)hubs.size();
     if (c->n_pull_hubs) {
         HIPCHK(hipMalloc(&c->d_pull_hubs, hubs.size() * sizeof(PullPiece)));
         HIPCHK(hipMemcpy(c->d_pull_hubs, hubs.data(), hubs.size() * sizeof(PullPiece), hipMemcpyHostToDevice));
     }
     return EXC_OK;
+}
+
+int exc_dense_required(void* p) {
+    Ctx* c = as_ctx(p);
+    if (!c) return fail(EXC_EINVAL, "exc_dense_required: null context");
+    return c->folded ? 1 : 0;
 }
 
 int exc_reset_stats(void* p) {

@@ -139,6 +139,7 @@ HIP_SIGNATURES = {
     "exc_fp64_solve": (c_int, [c_void_p, c_int, c_double, c_double, c_int, c_double, c_void_p, c_void_p,
                                P(c_double), P(c_double), P(c_int)]),
     "exc_fp64_lambda": (c_int, [c_void_p, c_void_p, c_int, c_double, P(c_double), P(c_double), P(c_int)]),
+    "exc_dense_required": (c_int, [c_void_p]),
     "exc_fp64_residual": (c_int, [c_void_p, c_double, c_void_p, c_double, c_int, c_void_p, P(c_double)]),
     "exc_fp64_solve_vec": (c_int, [c_void_p, c_void_p, c_double, c_int, c_int, c_double, c_void_p, P(c_double),
                                    P(c_int)]),
@@ -351,6 +352,11 @@ class Context:
 
     def enable_pull(self, threshold):
         return F["exc_enable_pull"](self.ctx, threshold)
+
+    def dense_required(self):
+        rc = F["exc_dense_required"](self.ctx)
+        check(min(rc, 0), "dense_required")
+        return rc == 1
 
     def push_hot(self, lo, hi, group=0, after_slots=False):
         check(F["exc_push_hot"](self.ctx, lo, hi, group, int(after_slots)), "push_hot")

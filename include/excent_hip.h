@@ -163,6 +163,11 @@ int exc_set_edge_weights(void* ctx, const float* h_w, uint64_t cnt);
  * and publishes its weight, then every vertex adds the weights of its in-neighbours --
  * the same deltas as the push, all folds before all adds.  EXC_ENOMEM leaves the push. */
 int exc_enable_pull(void* ctx, double threshold);
+/* 1 when the last dense pass also folded the next one (its pending vertices have moved r into
+ * x and published their weights): the next pass must then be exc_push_resident over the
+ * whole vertex range, which runs as a dense pass whatever its demand; 0 otherwise.  No
+ * reference counterpart (the reference folds inside each push). */
+int exc_dense_required(void* ctx);
 
 /* Push a resident chunk run [ch.v0, ch.v1) from the arena on the resident stream
  * (launch_chunk with dst = resident[c], src/excent.cu:634-652, 684-688).  use_list:

@@ -172,12 +172,16 @@ def test_batched_session_rerun_after_reinit(rmat12):
         assert np.abs(a - b).sum() <= 2 * 1e-5 * n / (1 - 0.85)
 
 
+@pytest.mark.parametrize("fuse", [0, 1])
 @pytest.mark.parametrize("gname", ["rmat12", "star"])
-def test_batched_dense_passes(gname, rmat12):
+def test_batched_dense_passes(gname, fuse, rmat12, monkeypatch):
     """Batched fold + pull over the transpose (every pass dense): each column within the
-    exact FP64 bound, per-vertex |r| <= eps at the end, the snapshot after every dense pass
-    bit-exact against the oracle's, and in-lists above the hub threshold (the star's centre
-    has 9000 in-neighbours) summed by pieces."""
+    exact FP64 bound, per-vertex |r| <= eps at the end, and in-lists above the hub threshold
+    (the star's centre has 9000 in-neighbours) summed by pieces.  fuse = 0: the fold stays
+    in k_bfold and the snapshot after every pass is checked bit for bit against the
+    oracle's; fuse = 1: k_bpull also computes the next snapshot and fold (the state read
+    back after such a pass is already folded, so only the end results are checked)."""
+    monkeypatch.setenv("EXC_FUSE_FOLD", str(fuse))
     if gname == "star":
         L = 9000
         deg = np.concatenate([[L], np.ones(L, np.int64)])
@@ -198,11 +202,15 @@ def test_batched_dense_passes(gname, rmat12):
         for j in range(K):
             ctx.batch_seed(j, seeds[j], mass)
         assert ctx.enable_pull(0.0) == 0
+        folded_passes = 0
         for it in range(5000):
             cnt, _ = ctx.snapshot()
-            rs = [ctx.read_state(j, 1) for j in range(K)]
-            obm, ocnt, _ = orc.snapshot(rs, np.full(K, eps, np.float32), idx, cv0)
-            assert np.array_equal(ctx.bitmap(), obm) and np.array_equal(cnt, ocnt), it
+            if ctx.dense_required():
+                folded_passes += 1
+            else:
+                rs = [ctx.read_state(j, 1) for j in range(K)]
+                obm, ocnt, _ = orc.snapshot(rs, np.full(K, eps, np.float32), idx, cv0)
+                assert np.array_equal(ctx.bitmap(), obm) and np.array_equal(cnt, ocnt), it
             cp = ctx.batch_colpend()
             ctx.batch_set_active(sum(1 << j for j in range(K) if cp[j]))
             if cnt[0::2].sum() == 0:
@@ -213,6 +221,7 @@ def test_batched_dense_passes(gname, rmat12):
         R = [ctx.read_state(j, 1) for j in range(K)]
         st = ctx.stats()
     assert st["pull_passes"] > 0 and st["pushed_edge_measures"] > 0
+    assert (folded_passes > 0) == bool(fuse)
     for j in range(K):
         assert np.all(np.abs(R[j]) <= eps)
         b = np.zeros(n)

@@ -200,11 +200,13 @@ def test_fold_pull_all_measures():
 
 
 @pytest.mark.parametrize("measures", ["pr", "all"])
-def test_fused_snapshot_bit_exact(rmat12, measures):
+def test_fused_snapshot_bit_exact(rmat12, measures, monkeypatch):
     """The snapshot a dense pass computes for the next pass inside k_pull (bitmap words,
     per-chunk pending counts and degree sums) equals the oracle's snapshot of the state
     read back, bit for bit, on chunk boundaries that cut through the pull's tiles; and
-    equals a second, separately launched snapshot of the same state."""
+    equals a second, separately launched snapshot of the same state.  (The next pass's
+    fold is kept out of k_pull here, so the state read back is the one the snapshot saw.)"""
+    monkeypatch.setenv("EXC_FUSE_FOLD", "0")
     idx, adj = orc.egr_load(rmat12)
     n, m = len(idx) - 1, len(adj)
     cv0 = np.array([0, 1000, 1001, 2560, 3333], np.int32)
@@ -240,3 +242,42 @@ def test_fused_snapshot_bit_exact(rmat12, measures):
             ctx.push_resident(0, n, 0, m, False)
             ctx.pass_sync()
         assert checked >= 3
+
+
+def test_fused_fold_state_and_results(rmat12):
+    """With the next pass's fold inside k_pull: the context reports that a dense pass is
+    required, refuses a partial push, and the run ends within the same FP64 bound and
+    invariant as with the separate fold; both runs push the same vertices' edges per pass
+    count to within the atomic-order noise of the pass structure."""
+    import os
+    idx, adj = orc.egr_load(rmat12)
+    n, m = len(idx) - 1, len(adj)
+    x0, r0 = orc.init(idx, orc.ORC_PR)
+    cv0 = np.array([0], np.int32)
+    with nat.Context(n, m, cv0, [nat.EXC_PR], [1e-4], [0.85], [1]) as ctx:
+        ctx.upload_offsets(idx)
+        ctx.set_state(0, x=x0, r=r0)
+        assert ctx.cache_chunks(m) == 0
+        ctx.cache_upload(adj, 0)
+        assert ctx.enable_pull(0.0) == 0
+        saw_folded = False
+        for p in range(3000):
+            cnt, _ = ctx.snapshot()
+            if cnt[0::2].sum() == 0:
+                break
+            if ctx.dense_required():
+                saw_folded = True
+                with pytest.raises(nat.ExcError, match="folded"):
+                    ctx.push_resident(0, n // 2, 0, int(idx[n // 2]), False)
+            ctx.push_resident(0, n, 0, m, False)
+            ctx.pass_sync()
+        assert saw_folded
+        x, r = ctx.read_state(0, 0), ctx.read_state(0, 1)
+        st = ctx.stats()
+    b = np.full(n, np.float64(np.float32(1) - np.float32(0.85)))
+    ref, _, _ = orc.fp64_solve_b(idx, adj, b)
+    tr = true_residual(idx, adj, x, b, 0.85, True)
+    assert np.abs(x.astype(np.float64) - ref).sum() <= np.abs(tr).sum() / 0.15 * (1 + 1e-6) + 1e-6
+    assert np.all(np.abs(r) <= 1e-4)
+    assert np.abs(tr - r).max() < 2e-4 * max(1.0, np.abs(x).max())
+    assert st["pushed_edges"] >= m and st["pull_passes"] > 0
