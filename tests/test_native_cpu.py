@@ -148,3 +148,13 @@ def test_cli_bad_files(tmp_path, capfd):
 def test_cli_binary_usage():
     r = subprocess.run([os.path.join(nat.BINDIR, "excent")], capture_output=True, text=True)
     assert r.returncode == 1 and "usage:" in r.stderr
+
+
+def test_bench_refuses_more_than_one_gpu():
+    """bench.py runs on one GPU only: --gpus N > 1 exits with a message, before any GPU work."""
+    import subprocess
+    import sys
+    from tests.conftest import ROOT
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "4"], capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 2 and "one GPU only" in r.stderr and r.stdout == ""
