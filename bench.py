@@ -121,11 +121,12 @@ def bench_batched(args, nat, np, work, graph):
     t0 = time.time()
     sess = nat.Session(common + ["--measure", "ppr", "--log", os.path.join(work, "batched.log")])
     log(f"batched setup {time.time() - t0:.1f}s")
-    sess.run(reinit=False)
-    log("batched warmup done")
+    for w in range(args.warmup):
+        sess.run(reinit=w > 0)
+        log(f"batched warmup {w} done")
     nat.synchronize()
     t0 = time.perf_counter()
-    rb = sess.run(reinit=True)
+    rb = sess.run(reinit=args.warmup > 0)
     nat.synchronize()
     tb = time.perf_counter() - t0
     xb = {j: sess.values(j) for j in sample}

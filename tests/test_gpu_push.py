@@ -281,3 +281,14 @@ def test_fused_fold_state_and_results(rmat12):
     assert np.all(np.abs(r) <= 1e-4)
     assert np.abs(tr - r).max() < 2e-4 * max(1.0, np.abs(x).max())
     assert st["pushed_edges"] >= m and st["pull_passes"] > 0
+
+
+@pytest.mark.parametrize("n", [1, 255, 257, 1000])
+def test_dense_passes_edgeless_and_ragged(n):
+    """Dense passes on an edgeless graph (no sources, empty transpose, no p rows): x = b in
+    one pass, with tile counts that are not multiples of the 256-vertex tile."""
+    idx = np.zeros(n + 1, np.uint64)
+    adj = np.zeros(0, np.int32)
+    x0, r0 = orc.init(idx, orc.ORC_PR)
+    xs, rs, passes, pushed = run_passes(idx, adj, [nat.EXC_PR], [0.01], [0.85], [1], [(x0, r0)], pull=0.0)
+    assert np.all(xs[0] == np.float32(1) - np.float32(0.85)) and np.all(rs[0] == 0) and pushed == 0 and passes == 1
