@@ -7,8 +7,9 @@
 //             in-neighbour lists) with the coefficient of edge v -> u moved to u's list;
 //   grads   : dL/ds = (1 - d) lambda;  dL/d(w_vu/S_v) = d lambda_u x_v;
 //             dL/dw_vu = (d x_v / S_v) (lambda_u - sum_{u'} (w_vu'/S_v) lambda_u').
-// Both pushes run through the device C ABI on fully cached graphs; the transpose and the
-// gradient assembly run on the host with OpenMP.
+// Both solves run through the device C ABI on fully cached graphs, as pushes and, while the
+// demand is high, as dense passes (fold + pull weighted by the edge coefficients); the
+// transpose and the gradient assembly run on the host with OpenMP.
 #include <omp.h>
 
 #include <cmath>
@@ -57,7 +58,11 @@ int make_ctx(Dpr* D, const std::vector<uint64_t>& off, const std::vector<int32_t
     if ((rc = exc_upload_offsets(*out, off.data(), 0, off.size()))) return rc;
     if ((rc = exc_cache_chunks(*out, D->m))) return rc;
     if (D->m && (rc = exc_cache_upload(*out, lists.data(), 0, D->m))) return rc;
-    return exc_set_edge_weights(*out, w.data(), D->m);
+    if ((rc = exc_set_edge_weights(*out, w.data(), D->m))) return rc;
+    // dense passes (fold + weighted pull over the device transpose) while the demand is high;
+    // without the memory for the transpose the push alone solves it
+    rc = exc_enable_pull(*out, 0.05);
+    return rc == EXC_ENOMEM ? EXC_OK : rc;
 }
 
 // Runs passes (snapshot, push over all vertices) from x = 0, r = r0 until nothing is pending.

@@ -719,6 +719,7 @@ struct PullArgs {
     const uint32_t* live;       // bit q set iff row q of p is nonzero this pass (k_pbits);
                                 // nullptr: every row is read
     unsigned long long* live_count;   // k_pull_hub: rows read
+    const float* tw;            // weighted graphs: each in-edge's coefficient, aligned with tadj
     // k_pull: the next pass's snapshot (bitmap + per-chunk counts), fused into the final
     // row update; bitmap == nullptr: not fused
     const uint64_t* idx;
@@ -774,10 +775,12 @@ constexpr uint32_t PULL_HUB = 4096;   // in-lists longer than this are summed by
 // live != nullptr: rows whose bit is clear are zero this pass and are not loaded.
 constexpr int PULL_LD = BLOCK + 1;           // LDS row pitch: slot j at (j % SEQ) * LD + j / SEQ
 
-template <int NM, int S, int PULL_SEQ>
+// W: weighted graph -- each row is scaled by its in-edge's coefficient (staged with the ids).
+template <int NM, int S, int PULL_SEQ, bool W = false>
 __global__ __launch_bounds__(BLOCK) void k_pull(PullArgs a, DevMeasures M) {
-    constexpr int PULL_WIN = BLOCK * PULL_SEQ;   // 4096 slots (16 KB of LDS) at PULL_SEQ 16
+    constexpr int PULL_WIN = BLOCK * PULL_SEQ;   // 2048 slots (8 KB of LDS) at PULL_SEQ 8
     __shared__ uint32_t s_start[BLOCK + 1];
+    __shared__ float s_w[W ? PULL_SEQ * PULL_LD : 1];
     __shared__ int32_t s_src[PULL_SEQ * PULL_LD];
     __shared__ float s_acc[NM][BLOCK];
     const int tid = threadIdx.x;
@@ -792,7 +795,10 @@ __global__ __launch_bounds__(BLOCK) void k_pull(PullArgs a, DevMeasures M) {
     uint32_t taken = 0;   // p rows read by this lane
     for (uint32_t w0 = 0; w0 < total; w0 += PULL_WIN) {
         const uint32_t wn = total - w0 < (uint32_t)PULL_WIN ? total - w0 : (uint32_t)PULL_WIN;
-        for (uint32_t j = tid; j < wn; j += BLOCK) s_src[(j % PULL_SEQ) * PULL_LD + j / PULL_SEQ] = a.tadj[base + w0 + j];
+        for (uint32_t j = tid; j < wn; j += BLOCK) {
+            s_src[(j % PULL_SEQ) * PULL_LD + j / PULL_SEQ] = a.tadj[base + w0 + j];
+            if constexpr (W) s_w[(j % PULL_SEQ) * PULL_LD + j / PULL_SEQ] = a.tw[base + w0 + j];
+        }
         __syncthreads();
         const uint32_t j0 = (uint32_t)tid * PULL_SEQ;
         if (j0 < wn) {
@@ -810,7 +816,15 @@ __global__ __launch_bounds__(BLOCK) void k_pull(PullArgs a, DevMeasures M) {
                 for (int k = 0; k < S; k++) rows[q].v[k] = 0.f;
                 if ((uint32_t)q < jn) {
                     const int v = s_src[q * PULL_LD + tid];
-                    if (row_live(a.live, v)) { rows[q] = load_prow<NM, S>(a.p, v); taken++; }
+                    if (row_live(a.live, v)) {
+                        rows[q] = load_prow<NM, S>(a.p, v);
+                        taken++;
+                        if constexpr (W) {
+                            const float w = s_w[q * PULL_LD + tid];
+#pragma unroll
+                            for (int k = 0; k < NM; k++) rows[q].v[k] *= w;
+                        }
+                    }
                 }
             }
             int owner = lo;
@@ -985,17 +999,26 @@ __global__ __launch_bounds__(BLOCK) void k_pull_hub(PullArgs a, const PullPiece*
         // together so each lane keeps several gathers in flight
         for (uint64_t i0 = b + threadIdx.x; i0 < e; i0 += (uint64_t)BLOCK * HUB_UNROLL) {
             int vs[HUB_UNROLL];
+            float ws[HUB_UNROLL];
 #pragma unroll
             for (int q = 0; q < HUB_UNROLL; q++) {
                 const uint64_t i = i0 + (uint64_t)q * BLOCK;
                 vs[q] = i < e ? a.tadj[i] : -1;
+                ws[q] = a.tw && i < e ? a.tw[i] : 1.f;
             }
             Row<S> rows[HUB_UNROLL];
 #pragma unroll
             for (int q = 0; q < HUB_UNROLL; q++) {
 #pragma unroll
                 for (int k = 0; k < S; k++) rows[q].v[k] = 0.f;
-                if (vs[q] >= 0 && row_live(a.live, vs[q])) { rows[q] = load_prow<NM, S>(a.p, vs[q]); taken++; }
+                if (vs[q] >= 0 && row_live(a.live, vs[q])) {
+                    rows[q] = load_prow<NM, S>(a.p, vs[q]);
+                    taken++;
+                    if (a.tw) {
+#pragma unroll
+                        for (int k = 0; k < NM; k++) rows[q].v[k] *= ws[q];
+                    }
+                }
             }
 #pragma unroll
             for (int q = 0; q < HUB_UNROLL; q++)
@@ -1029,9 +1052,14 @@ __global__ void k_count_in64(uint64_t m, const int32_t* __restrict__ adj, unsign
         atomicAdd(&cnt[adj[e]], 1ull);
 }
 __global__ void k_fill_in64(int n, const uint64_t* __restrict__ idx, const int32_t* __restrict__ adj,
-                            unsigned long long* cursor, const int32_t* __restrict__ rank, int32_t* tadj) {
+                            unsigned long long* cursor, const int32_t* __restrict__ rank, int32_t* tadj,
+                            const float* __restrict__ w, float* tw) {
     for (long long v = blockIdx.x * (long long)blockDim.x + threadIdx.x; v < n; v += (long long)gridDim.x * blockDim.x)
-        for (uint64_t e = idx[v]; e < idx[v + 1]; e++) tadj[atomicAdd(&cursor[adj[e]], 1ull)] = rank ? rank[v] : (int32_t)v;
+        for (uint64_t e = idx[v]; e < idx[v + 1]; e++) {
+            const unsigned long long q = atomicAdd(&cursor[adj[e]], 1ull);
+            tadj[q] = rank ? rank[v] : (int32_t)v;
+            if (w) tw[q] = w[e];
+        }
 }
 
 // ------------------------------------------------------------------ batched PPR
@@ -1625,6 +1653,7 @@ struct Ctx {
     uint64_t t_launch[T_NCLS] = {};
     double snap_bytes = 0;
     float* d_wts = nullptr;           // per-edge coefficients aligned with the arena (weighted graphs)
+    float* d_twts = nullptr;          // the same coefficients in transpose order (dense passes)
     // dense passes: fold + pull over the transpose (whole graph resident)
     uint64_t* d_tidx = nullptr;
     int32_t* d_tadj = nullptr;
@@ -1864,7 +1893,7 @@ void launch_fold_pull(Ctx* c, hipStream_t st) {
     unsigned long long* row = c->d_counters + (live ? C_DENSE_LIVE : C_DENSE) * C_FIELDS;
     FoldArgs f{c->n, c->d_idx, c->d_p, c->d_rank, row};
     const int ntiles = (int)(((long long)c->n + BLOCK - 1) / BLOCK);
-    PullArgs pa{c->n, c->d_tidx, c->d_tadj, c->d_p, live ? c->d_pbits : nullptr, row + C_PULL_HUB_LIVE,
+    PullArgs pa{c->n, c->d_tidx, c->d_tadj, c->d_p, live ? c->d_pbits : nullptr, row + C_PULL_HUB_LIVE, c->d_twts,
                 c->d_idx, c->d_cv0, c->nc, nullptr, nullptr, nullptr, c->d_rank, c->nsrc, c->d_counters + C_SPREAD};
     c->pull_passes_live += live;
     const bool fuse_snap = !c->d_trace;   // the trace counter stays with k_snapshot
@@ -1897,7 +1926,15 @@ void launch_fold_pull(Ctx* c, hipStream_t st) {
     }
     {
         ScopedTimer t(c, st, T_PULL);
-        if (c->pull_seq == 4) hipLaunchKernelGGL((k_pull<NM, S, 4>), dim3(ntiles), dim3(BLOCK), 0, st, pa, c->DM);
+        bool weighted = false;
+        if constexpr (NM == 1) {
+            if (c->d_twts) {   // weighted graphs (one measure)
+                hipLaunchKernelGGL((k_pull<NM, S, 8, true>), dim3(ntiles), dim3(BLOCK), 0, st, pa, c->DM);
+                weighted = true;
+            }
+        }
+        if (weighted) {
+        } else if (c->pull_seq == 4) hipLaunchKernelGGL((k_pull<NM, S, 4>), dim3(ntiles), dim3(BLOCK), 0, st, pa, c->DM);
         else if (c->pull_seq == 32) hipLaunchKernelGGL((k_pull<NM, S, 32>), dim3(ntiles), dim3(BLOCK), 0, st, pa, c->DM);
         else if (c->pull_seq == 16) hipLaunchKernelGGL((k_pull<NM, S, 16>), dim3(ntiles), dim3(BLOCK), 0, st, pa, c->DM);
         else hipLaunchKernelGGL((k_pull<NM, S, 8>), dim3(ntiles), dim3(BLOCK), 0, st, pa, c->DM);
@@ -2147,7 +2184,7 @@ int exc_destroy(void* p) {
     for (auto& pb : c->pb) { if (pb.d) hipFree(pb.d); if (pb.cnt) hipFree(pb.cnt); }
     for (auto s : c->st) if (s) hipStreamDestroy(s);
     for (auto q : c->prows) if (q) (void)hipFree(q);
-    void* pull_ptrs[] = {c->d_tidx, c->d_tadj, c->d_p, c->d_pull_hubs, c->d_wts, c->d_rank, c->d_pbits, c->d_p2,
+    void* pull_ptrs[] = {c->d_tidx, c->d_tadj, c->d_p, c->d_pull_hubs, c->d_wts, c->d_twts, c->d_rank, c->d_pbits, c->d_p2,
                          c->d_bitmap2, c->d_hubbits};
     for (void* q : pull_ptrs) if (q) (void)hipFree(q);
     if (c->d_beps) (void)hipFree(c->d_beps);
@@ -2447,7 +2484,8 @@ int exc_push_resident(void* p, const exc_chunk* ch, int use_list, int group, int
     const bool whole = ch->v0 == 0 && ch->v1 == c->n && !hot_skip;
     if (c->folded && !whole)
         return fail(EXC_ESTATE, "exc_push_resident: the last dense pass folded this pass; it must be a dense pass over all vertices");
-    if (c->d_tadj && !c->d_wts && whole && (c->folded || (double)c->last_demand >= c->pull_threshold * (double)c->m))
+    if (c->d_tadj && (!c->d_wts || c->d_twts) && whole &&
+        (c->folded || (double)c->last_demand >= c->pull_threshold * (double)c->m))
         return fold_pull(c);
     return push_chunk(c, 0, ch->v0, ch->v1, c->d_arena, 0, use_list, group, hot_skip, c->arena_edges, c->d_wts);
 }
@@ -2615,6 +2653,7 @@ int exc_set_edge_weights(void* p, const float* h_w, uint64_t cnt) {
     if (!c || c->K || !h_w) return fail(EXC_EINVAL, "exc_set_edge_weights: bad arguments");
     if (!c->d_arena || c->arena_edges != c->m || cnt != c->m)
         return fail(EXC_ESTATE, "exc_set_edge_weights: the whole graph must be in the edge cache and cnt == m");
+    if (c->d_tadj) return fail(EXC_ESTATE, "exc_set_edge_weights: set the weights before exc_enable_pull");
     HIPCHK(hipSetDevice(c->dev));
     c->snap_ready = false;   // the state changes: a fused snapshot no longer holds
     c->folded = false;       // the state is replaced: a folded pass no longer applies
@@ -2766,6 +2805,7 @@ int exc_enable_pull(void* p, double threshold) {
     for (size_t v = 0; v < n; v++) c->nsrc += hidx[v + 1] > hidx[v];
     ok = ok && hipMalloc(&c->d_pbits, (2 * (((size_t)c->nsrc + 63) / 64) + 2) * sizeof(uint32_t)) == hipSuccess;
     ok = ok && hipMalloc(&c->d_tadj, (c->m ? c->m : 1) * sizeof(int32_t)) == hipSuccess;
+    if (ok && c->d_wts) ok = hipMalloc(&c->d_twts, (c->m ? c->m : 1) * sizeof(float)) == hipSuccess;
     ok = ok && hipMalloc(&c->d_p, n * c->S * sizeof(float)) == hipSuccess;
     if (ok && c->fuse_fold) ok = hipMalloc(&c->d_p2, n * c->S * sizeof(float)) == hipSuccess;
     if (ok && by_id && c->fuse_fold) {
@@ -2775,10 +2815,10 @@ int exc_enable_pull(void* p, double threshold) {
     ok = ok && hipMalloc(&cnt, (n + 1) * sizeof(unsigned long long)) == hipSuccess;
     if (!ok) {
         (void)hipGetLastError();
-        void* q[] = {c->d_tidx, c->d_tadj, c->d_p, cnt, c->d_rank, c->d_pbits, c->d_p2, c->d_bitmap2, c->d_hubbits};
+        void* q[] = {c->d_tidx, c->d_tadj, c->d_p, cnt, c->d_rank, c->d_pbits, c->d_p2, c->d_bitmap2, c->d_hubbits, c->d_twts};
         for (void* x : q) if (x) (void)hipFree(x);
         c->d_tidx = nullptr; c->d_tadj = nullptr; c->d_p = nullptr; c->d_rank = nullptr; c->d_pbits = nullptr;
-        c->d_p2 = nullptr; c->d_bitmap2 = nullptr; c->d_hubbits = nullptr;
+        c->d_p2 = nullptr; c->d_bitmap2 = nullptr; c->d_hubbits = nullptr; c->d_twts = nullptr;
         c->nsrc = 0;
         c->pull_threshold = -1;
         return fail(EXC_ENOMEM, "exc_enable_pull: cannot allocate the transpose (%.2f GB)", c->m * 4.0 / 1e9);
@@ -2819,7 +2859,8 @@ int exc_enable_pull(void* p, double threshold) {
     e = hipMemcpy(c->d_tidx, off.data(), (n + 1) * sizeof(uint64_t), hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(cnt, cur.data(), n * sizeof(uint64_t), hipMemcpyHostToDevice);
     if (e != hipSuccess) return fail_free(e);
-    hipLaunchKernelGGL(k_fill_in64, dim3(c->cus * 16), dim3(BLOCK), 0, 0, c->n, c->d_idx, c->d_arena, cnt, c->d_rank, c->d_tadj);
+    hipLaunchKernelGGL(k_fill_in64, dim3(c->cus * 16), dim3(BLOCK), 0, 0, c->n, c->d_idx, c->d_arena, cnt, c->d_rank, c->d_tadj,
+                       c->d_wts, c->d_twts);
     e = hipDeviceSynchronize();
     if (e != hipSuccess) return fail_free(e);
     (void)hipFree(cnt);

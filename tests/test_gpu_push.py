@@ -292,3 +292,50 @@ def test_dense_passes_edgeless_and_ragged(n):
     x0, r0 = orc.init(idx, orc.ORC_PR)
     xs, rs, passes, pushed = run_passes(idx, adj, [nat.EXC_PR], [0.01], [0.85], [1], [(x0, r0)], pull=0.0)
     assert np.all(xs[0] == np.float32(1) - np.float32(0.85)) and np.all(rs[0] == 0) and pushed == 0 and passes == 1
+
+
+@pytest.mark.parametrize("gname", ["rmat12", "star"])
+def test_weighted_dense_passes(gname, rmat12):
+    """Dense passes on a weighted graph (the pull scales each in-edge's row by its
+    coefficient, staged with the ids; hub pieces too): the result sits within the exact
+    FP64 bound of the weighted solve, per-vertex |r| <= eps at the end, and weights set
+    after the pull is enabled are refused."""
+    if gname == "star":
+        L = 9000
+        deg = np.concatenate([[L], np.ones(L, np.int64)])
+        idx = np.concatenate([[0], np.cumsum(deg)]).astype(np.uint64)
+        adj = np.concatenate([np.arange(1, L + 1), np.zeros(L, np.int64)]).astype(np.int32)
+    else:
+        idx, adj = orc.egr_load(rmat12)
+    n, m = len(idx) - 1, len(adj)
+    rng = np.random.default_rng(8)
+    w = rng.uniform(0.5, 1.5, m)
+    src = np.repeat(np.arange(n), np.diff(idx.astype(np.int64)))
+    S = np.bincount(src, weights=w, minlength=n)
+    wn = (w / S[src]).astype(np.float32)
+    eps = 1e-5
+    b = np.full(n, np.float64(np.float32(1) - np.float32(0.85)))
+    with nat.Context(n, m, np.array([0], np.int32), [nat.EXC_PR], [eps], [0.85], [0]) as ctx:
+        ctx.upload_offsets(idx)
+        ctx.set_state(0, x=np.zeros(n, np.float32), r=b.astype(np.float32))
+        assert ctx.cache_chunks(m) == 0
+        ctx.cache_upload(adj, 0)
+        check = nat.F["exc_set_edge_weights"](ctx.ctx, nat.ptr(wn), m)
+        assert check == 0
+        assert ctx.enable_pull(0.0) == 0
+        assert nat.F["exc_set_edge_weights"](ctx.ctx, nat.ptr(wn), m) != 0   # after the pull: refused
+        for p in range(5000):
+            cnt, _ = ctx.snapshot()
+            if cnt[0::2].sum() == 0:
+                break
+            ctx.push_resident(0, n, 0, m, False)
+            ctx.pass_sync()
+        x, r = ctx.read_state(0, 0), ctx.read_state(0, 1)
+        assert ctx.stats()["pull_passes"] > 0
+    sf = float(np.float32(0.85))   # the scale the GPU multiplies by
+    ref, _, _ = orc.fp64_solve_w(idx, adj, wn.astype(np.float64), b, s=sf)
+    acc = np.zeros(n)
+    np.add.at(acc, adj, sf * wn.astype(np.float64) * x.astype(np.float64)[src])
+    tr = b + acc - x.astype(np.float64)
+    assert np.abs(x.astype(np.float64) - ref).sum() <= np.abs(tr).sum() / (1 - sf) * (1 + 1e-6) + 1e-6
+    assert np.all(np.abs(r) <= eps)
