@@ -228,8 +228,20 @@ def bench_ooc(args, nat, np, work, graph):
                             "bytes_stream_gb": round(r["bytes_stream"] / 1e9, 3),
                             "bytes_gather_gb": round(r["bytes_gather"] / 1e9, 3),
                             "shipped_gb_per_s": round(shipped / el / 1e9, 2),
-                            "time_io_s": round(r["time_io_s"], 4), "time_wait_s": round(r["time_wait_s"], 4)}
+                            "time_io_s": round(r["time_io_s"], 4), "time_wait_s": round(r["time_wait_s"], 4),
+                            "kernel_ms": {k: round(r["kernel"][k], 1) for k in ("snapshot_ms", "push_ms", "piece_ms",
+                                                                                "other_ms")}}
         log(f"ooc {tag}: {out['runs'][tag]}")
+    # the link peak: synchronous copies of a 256 MB pinned staging buffer to the device
+    chunk = 64 << 20
+    with nat.Context(1, 1, np.array([0], np.int32), [nat.EXC_PR], [0.01], [0.85], [1], max_chunk_edges=chunk,
+                     max_chunk_verts=1) as ctx:
+        best = min(ctx.slot_copy_probe(0, chunk * 4) for _ in range(5))
+    link = chunk * 4 / best / 1e9
+    out["link_h2d_gb_per_s"] = round(link, 2)
+    for r in out["runs"].values():
+        r["link_fraction"] = round(r["shipped_gb_per_s"] / link, 3)
+    log(f"ooc: pinned H2D {link:.1f} GB/s")
     out["value"] = out["runs"][f"hot {args.cache_mb} MB"]["gteps"]
     return out
 

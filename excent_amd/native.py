@@ -361,6 +361,12 @@ class Context:
     def push_hot(self, lo, hi, group=0, after_slots=False):
         check(F["exc_push_hot"](self.ctx, lo, hi, group, int(after_slots)), "push_hot")
 
+    def slot_copy_probe(self, s, nbytes):
+        """Seconds of one synchronous H2D copy of nbytes from slot s's pinned edge buffer."""
+        t = c_double(0)
+        check(F["exc_slot_copy_probe"](self.ctx, s, int(nbytes), ctypes.byref(t)), "slot_copy_probe")
+        return t.value
+
     def slot_buffers(self, s, max_edges, max_verts):
         e, v, o = c_void_p(), c_void_p(), c_void_p()
         check(F["exc_slot_buffers"](self.ctx, s, ctypes.byref(e), ctypes.byref(v), ctypes.byref(o)), "slot_buffers")
