@@ -63,6 +63,18 @@ def read_env(path):
     return out
 
 
+def solve_logged(S, kind, tol, max_iter, seg=20, **kw):
+    """S.solve in segments of seg iterations, each resumed from the last one's vector and
+    logged, so that a long FP64 solve on a large graph reports its progress."""
+    x, it = None, 0
+    while True:
+        x, rel, lam, k = S.solve(kind, tol=tol, max_iter=min(seg, max_iter - it), x0=x, **kw)
+        it += k
+        if it >= max_iter or rel <= tol:
+            return x, rel, lam, it
+        log(f"fp64 solve kind {kind}: {it} iterations, relative change {rel:.3g}")
+
+
 def fp64_errors(nat, np, graph, work, measure, env):
     """Errors of the last timed run's values (the --dump files) against the product's FP64
     solver, and the FP64 global residual of the float32 state (tools/centrality_reference.cu:
@@ -76,13 +88,13 @@ def fp64_errors(nat, np, graph, work, measure, env):
             x = np.fromfile(os.path.join(work, f"x.{name}.bin"), dtype=np.float32)
             if name == "pr":
                 c, s, dd = float(f32(1) - f32(0.85)), float(f32(0.85)), 1
-                ref, rel, _, it = S.solve(nat.EXC_PR, tol=1e-12, max_iter=400)
+                ref, rel, _, it = solve_logged(S, nat.EXC_PR, 1e-12, 400)
             elif name == "katz":
                 c, s, dd = 1.0, float(f32(env["ALPHA"])), 0
-                ref, rel, _, it = S.solve(nat.EXC_KATZ, alpha=env["ALPHA"], beta=1.0, tol=1e-12, max_iter=400)
+                ref, rel, _, it = solve_logged(S, nat.EXC_KATZ, 1e-12, 400, alpha=env["ALPHA"], beta=1.0)
             else:
                 c, s, dd = 0.0, float(f32(1.0 / env["LAMBDA"])), 0
-                ref, rel, _, it = S.solve(nat.EXC_EV, tol=1e-10, max_iter=3000)
+                ref, rel, _, it = solve_logged(S, nat.EXC_EV, 1e-10, 3000)
             res = S.residual(x, c, s, dd)
             xv = x.astype(np.float64)
             if name == "ev":   # unit L2 with the sign fixed by the sum
@@ -100,6 +112,65 @@ def fp64_errors(nat, np, graph, work, measure, env):
             log(f"fp64 {name}: {out[name]}")
     log(f"fp64 check {time.time() - t0:.1f}s")
     return out
+
+
+def constants(nat, graph):
+    """LAMBDA / ALPHA of the graph in the reference's .env format (tools/excent_params.cu),
+    computed once per graph file by the product's FP64 power iteration."""
+    envp = graph + ".env"
+    t0 = time.time()
+    if not os.path.exists(envp):
+        with native_stdout_to_stderr():
+            rc = nat.run_entry("excent_params_main", [graph, envp, "--directed", "--alpha-factor", "0.5", "--seed",
+                                                      "7", "--eps-abs", "1e-6"])
+        assert rc == 0, "constants failed"
+    env = read_env(envp)
+    log(f"constants {env} in {time.time() - t0:.1f}s")
+    return env
+
+
+def measure_args(measure, env):
+    """Katz alpha/beta/eps and eigenvector lambda/eps flags of the all-measure workloads."""
+    a = []
+    if measure in ("katz", "all"):
+        a += ["--alpha", env["ALPHA"], "--beta", 1, "--eps-katz", "1e-6"]
+    if measure in ("ev", "all"):
+        a += ["--lambda", env["LAMBDA"], "--eps-ev", "1e-6", "--ev-init", "indeg"]
+    return a
+
+
+def evict(path):
+    """Drop the file's pages from the host page cache (what tools/evict_file.c does); dirty
+    pages are written back first, since only clean ones can be dropped."""
+    fd = os.open(path, os.O_RDONLY)
+    try:
+        os.fsync(fd)
+        os.posix_fadvise(fd, 0, 0, os.POSIX_FADV_DONTNEED)
+    finally:
+        os.close(fd)
+
+
+def page_cache_fraction(path, np):
+    """Fraction of the file's pages in the host page cache (mincore over a read-only map)."""
+    import ctypes
+    libc = ctypes.CDLL(None, use_errno=True)
+    libc.mmap.restype = ctypes.c_void_p
+    libc.mmap.argtypes = [ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_long]
+    libc.mincore.argtypes = [ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p]
+    libc.munmap.argtypes = [ctypes.c_void_p, ctypes.c_size_t]
+    size = os.path.getsize(path)
+    page = os.sysconf("SC_PAGE_SIZE")
+    fd = os.open(path, os.O_RDONLY)
+    try:
+        addr = libc.mmap(None, size, 1, 1, fd, 0)   # PROT_READ, MAP_SHARED
+        if addr in (None, ctypes.c_void_p(-1).value):
+            return None
+        vec = np.zeros((size + page - 1) // page, np.uint8)
+        rc = libc.mincore(addr, size, vec.ctypes.data)
+        libc.munmap(addr, size)
+        return None if rc else round(float((vec & 1).mean()), 4)
+    finally:
+        os.close(fd)
 
 
 def graph_size(nat, graph):
@@ -199,28 +270,47 @@ def bench_batched(args, nat, np, work, graph):
 
 
 def bench_ooc(args, nat, np, work, graph):
-    """Out-of-core PageRank: the graph file streamed / gathered over PCIe each pass, with a
-    capped hot cache and with no cache; bytes shipped as the reference counts them."""
+    """Out-of-core run: the graph file streamed / gathered over PCIe each pass, with a capped
+    hot cache and with no cache (and, with --cold, the hot-cache run again after the file
+    was dropped from the host page cache); bytes shipped as the reference counts them; FP64
+    errors of the hot-cache run's result."""
     n, m = graph_size(nat, graph)
-    out = {"metric": "converged GTEPS out-of-core (PageRank)", "unit": "GTEPS", "higher_is_better": True,
+    measure = args.measure
+    env = constants(nat, graph) if measure in ("katz", "ev", "all") else {}
+    eps = {"pr": "eps 0.01", "katz": "eps 1e-6", "ev": "eps 1e-6", "all": "eps 0.01 / 1e-6 / 1e-6"}[measure]
+    out = {"metric": f"converged GTEPS out-of-core (--measure {measure})", "unit": "GTEPS", "higher_is_better": True,
            "n_gpus": 1, "dtype": "f32", "data": "synthetic",
-           "config": {"workload": f"R-MAT scale {args.scale} ef {args.edge_factor} --measure pr, eps 0.01, "
-                                  f"edges read from the EGR file (host page cache) every pass",
+           "config": {"workload": f"R-MAT scale {args.scale} ef {args.edge_factor} --measure {measure}, {eps}, "
+                                  f"edges read from the EGR file every pass",
                       "n": n, "m": m, "graph_gb": round(m * 4 / 1e9, 3)}, "runs": {}}
-    for tag, extra in ((f"hot {args.cache_mb} MB", ["--cache", "hot", "--cache-mb", args.cache_mb]),
-                       ("no cache", ["--cache", "none"])):
+    hot = f"hot {args.cache_mb} MB"
+    modes = [(hot, ["--cache", "hot", "--cache-mb", args.cache_mb], False), ("no cache", ["--cache", "none"], False)]
+    if args.cold:
+        modes.append((hot + ", cold page cache", ["--cache", "hot", "--cache-mb", args.cache_mb], True))
+    for tag, extra, cold in modes:
+        sargs = [graph, "--measure", measure, "--timing", "--log",
+                 os.path.join(work, f"ooc_{tag.split()[0]}{'_cold' if cold else ''}.log")] + extra
+        sargs += measure_args(measure, env)
+        if tag == hot:
+            sargs += ["--dump", os.path.join(work, "x")]
+        resident = None
+        if cold:
+            evict(graph)
+            resident = page_cache_fraction(graph, np)
+            log(f"ooc {tag}: {resident} of the file in the page cache after eviction")
         t0 = time.time()
-        sess = nat.Session([graph, "--measure", "pr", "--timing", "--log",
-                            os.path.join(work, f"ooc_{tag.split()[0]}.log")] + extra)
+        sess = nat.Session(sargs)
         log(f"ooc {tag}: setup {time.time() - t0:.1f}s")
-        for w in range(args.warmup):
+        for w in range(0 if cold else args.warmup):
             sess.run(reinit=w > 0)
             log(f"ooc {tag}: warmup {w} done")
         nat.synchronize()
         t0 = time.perf_counter()
-        r = sess.run(reinit=args.warmup > 0)
+        r = sess.run(reinit=args.warmup > 0 and not cold)
         nat.synchronize()
         el = time.perf_counter() - t0
+        if tag == hot:
+            sess.results()
         sess.close()
         shipped = r["bytes_stream"] + r["bytes_gather"]
         out["runs"][tag] = {"time_to_eps_s": round(el, 4), "passes": r["passes"],
@@ -231,6 +321,8 @@ def bench_ooc(args, nat, np, work, graph):
                             "time_io_s": round(r["time_io_s"], 4), "time_wait_s": round(r["time_wait_s"], 4),
                             "kernel_ms": {k: round(r["kernel"][k], 1) for k in ("snapshot_ms", "push_ms", "piece_ms",
                                                                                 "other_ms")}}
+        if cold:
+            out["runs"][tag]["page_cache_fraction_at_start"] = resident
         log(f"ooc {tag}: {out['runs'][tag]}")
     # the link peak: synchronous copies of a 256 MB pinned staging buffer to the device
     chunk = 64 << 20
@@ -242,7 +334,10 @@ def bench_ooc(args, nat, np, work, graph):
     for r in out["runs"].values():
         r["link_fraction"] = round(r["shipped_gb_per_s"] / link, 3)
     log(f"ooc: pinned H2D {link:.1f} GB/s")
-    out["value"] = out["runs"][f"hot {args.cache_mb} MB"]["gteps"]
+    out["value"] = out["runs"][hot]["gteps"]
+    out["time_to_eps_s"] = out["runs"][hot]["time_to_eps_s"]
+    if not args.no_fp64:
+        out["fp64_error"] = fp64_errors(nat, np, graph, work, measure, env)
     return out
 
 
@@ -292,7 +387,10 @@ def main():
     ap.add_argument("--batched-only", action="store_true", help="batched: time the batched run only (no baseline)")
     ap.add_argument("--edge-factor", type=int, default=16)
     ap.add_argument("--seed", type=int, default=1)
-    ap.add_argument("--measure", default="all")
+    ap.add_argument("--measure", default=None, choices=["pr", "katz", "ev", "all"],
+                    help="measures of the plain runs (default: all; ooc: pr)")
+    ap.add_argument("--cold", action="store_true",
+                    help="ooc: also time the hot-cache run after dropping the file from the host page cache")
     ap.add_argument("--kernel", default="lb", choices=["lb", "group"])
     ap.add_argument("--pull", default="auto", help="dense passes by pull: auto | off | demand fraction")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
@@ -310,6 +408,8 @@ def main():
 
     if args.scale is None:
         args.scale = {"headline": 24, "batched": 26, "ooc": 26, "adjoint": 22}[args.config]
+    if args.measure is None:
+        args.measure = "pr" if args.config == "ooc" else "all"
 
     import numpy as np
 
@@ -332,24 +432,12 @@ def main():
         out = {"batched": bench_batched, "ooc": bench_ooc, "adjoint": bench_adjoint}[args.config](args, nat, np, work, graph)
         print(json.dumps(out), flush=True)
         return
-    envp = graph + ".env"
-    t0 = time.time()
-    if not os.path.exists(envp):
-        with native_stdout_to_stderr():
-            rc = nat.run_entry("excent_params_main", [graph, envp, "--directed", "--alpha-factor", "0.5", "--seed",
-                                                      "7", "--eps-abs", "1e-6"])
-        assert rc == 0, "constants failed"
-    env = read_env(envp)
-    log(f"constants {env} in {time.time() - t0:.1f}s")
+    env = constants(nat, graph)
 
     excent_log = args.log or os.path.join(work, "excent.log")
     sargs = [graph, "--measure", args.measure, "--cache", "chunks", "--preload", "--stream-ratio", "1",
              "--timing", "--push-kernel", args.kernel, "--pull", args.pull, "--log", excent_log,
-             "--dump", os.path.join(work, "x")]
-    if args.measure in ("katz", "all"):
-        sargs += ["--alpha", env["ALPHA"], "--beta", 1, "--eps-katz", "1e-6"]
-    if args.measure in ("ev", "all"):
-        sargs += ["--lambda", env["LAMBDA"], "--eps-ev", "1e-6", "--ev-init", "indeg"]
+             "--dump", os.path.join(work, "x")] + measure_args(args.measure, env)
     t0 = time.time()
     sess = nat.Session(sargs)
     log(f"session setup {time.time() - t0:.1f}s")

@@ -438,11 +438,16 @@ class Fp64Solver:
         check(F["exc_fp64_create"](device, self.n, int(idx[-1]), ptr(idx), ptr(adj), ctypes.byref(self.h)),
               "fp64_create")
 
-    def solve(self, kind, alpha=-1.0, beta=1.0, max_iter=20000, tol=1e-9):
+    def solve(self, kind, alpha=-1.0, beta=1.0, max_iter=20000, tol=1e-9, x0=None):
+        """Jacobi (PageRank, Katz) or power iteration (eigenvector) from x0 (default: the
+        reference's start vector) until the relative change is at most tol."""
         out = np.zeros(self.n, dtype=np.float64)
+        if x0 is not None:
+            x0 = np.ascontiguousarray(x0, dtype=np.float64)
+            assert len(x0) == self.n
         rel, lam, it = c_double(0), c_double(0), c_int(0)
-        check(F["exc_fp64_solve"](self.h, kind, alpha, beta, max_iter, tol, None, ptr(out), ctypes.byref(rel),
-                                  ctypes.byref(lam), ctypes.byref(it)), "fp64_solve")
+        check(F["exc_fp64_solve"](self.h, kind, alpha, beta, max_iter, tol, None if x0 is None else ptr(x0), ptr(out),
+                                  ctypes.byref(rel), ctypes.byref(lam), ctypes.byref(it)), "fp64_solve")
         return out, rel.value, lam.value, it.value
 
     def residual(self, x, c, s, divdeg):

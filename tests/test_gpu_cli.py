@@ -140,6 +140,29 @@ def test_fp64_residual_matches_oracle(rmat12):
         assert S.residual(np.zeros(n, np.float32), 0.15, 0.85, 1) == pytest.approx(0.15 * n, rel=1e-12)
 
 
+def test_fp64_solve_resumes_from_x0(rmat12):
+    """A solve resumed from an earlier solve's output equals one solve of the summed
+    iterations (PageRank and Katz Jacobi; eigenvector power iteration, whose resumed run
+    starts from the unit vector the first one returned), and the resumed PageRank matches
+    the oracle's FP64 reference."""
+    idx, adj = orc.egr_load(rmat12)
+    with nat.Fp64Solver(idx, adj) as S:
+        for kind, kw in ((nat.EXC_PR, {}), (nat.EXC_KATZ, {"alpha": 0.002, "beta": 1.0}), (nat.EXC_EV, {})):
+            whole, _, lam_w, it_w = S.solve(kind, tol=0.0, max_iter=14, **kw)
+            part, _, _, it_a = S.solve(kind, tol=0.0, max_iter=6, **kw)
+            rest, _, lam_r, it_b = S.solve(kind, tol=0.0, max_iter=8, x0=part, **kw)
+            assert (it_w, it_a, it_b) == (14, 6, 8)
+            np.testing.assert_allclose(rest, whole, rtol=1e-11, atol=1e-300)
+            if kind == nat.EXC_EV:
+                assert lam_r == pytest.approx(lam_w, rel=1e-11)
+        x, rel, _, it = S.solve(nat.EXC_PR, tol=1e-12, max_iter=400)
+        part, _, _, _ = S.solve(nat.EXC_PR, tol=0.0, max_iter=30)
+        y, rel2, _, it2 = S.solve(nat.EXC_PR, tol=1e-12, max_iter=400, x0=part)
+        assert it2 < it and rel2 <= 1e-12
+        ref = orc.fp64_reference(idx, adj, orc.ORC_PR, tol=1e-12, max_iter=400)[0]
+        np.testing.assert_allclose(y, ref, rtol=1e-9)
+
+
 def test_params_tool_reproduces_sample_env(tmp_path):
     out = str(tmp_path / "s.env")
     rc = nat.run_entry("excent_params_main", [SAMPLE_EGR, out])
@@ -147,3 +170,34 @@ def test_params_tool_reproduces_sample_env(tmp_path):
     got, want = read_env(out), read_env(SAMPLE_ENV)
     for k in ("LAMBDA", "ALPHA", "EPS_KATZ", "EPS_EV"):
         assert abs(got[k] - want[k]) <= 1e-6 * abs(want[k]), k
+
+
+def test_run_sh_end_to_end(tmp_path, rmat12):
+    """run.sh as the reference's users call it: the sample with its shipped constants
+    (--measure=all), and a graph without a .env, which the wrapper computes next to the
+    graph with bin/excent_params and then passes to bin/excent (--measure=katz)."""
+    import shutil
+    import subprocess
+    from tests.conftest import ROOT
+    sh = os.path.join(ROOT, "run.sh")
+    print("case run.sh sample all", flush=True)
+    r = subprocess.run([sh, "--measure=all", "--sample", "--", "--dump", str(tmp_path / "a"), "--stream-ratio", "0.1"],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    assert re.search(r"measure all \(n=2000\)", r.stdout), r.stdout
+    idx, adj = orc.egr_load(SAMPLE_EGR)
+    pr_check(idx, adj, np.fromfile(str(tmp_path / "a.pr.bin"), np.float32))
+    env = read_env(SAMPLE_ENV)
+    kref = orc.fp64_reference(idx, adj, orc.ORC_KATZ, alpha=env["ALPHA"])[0]
+    assert orc.compare(kref, np.fromfile(str(tmp_path / "a.katz.bin"), np.float32))["l1_rel"] < 0.05
+    g = str(tmp_path / "g.egr")
+    shutil.copy(rmat12, g)
+    print("case run.sh rmat12 katz (constants computed)", flush=True)
+    r = subprocess.run([sh, "--measure=katz", f"--graph={g}", "--", "--dump", str(tmp_path / "k")],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    assert os.path.exists(str(tmp_path / "g.env")) and "Computing the constants" in r.stdout
+    genv = read_env(str(tmp_path / "g.env"))
+    idx, adj = orc.egr_load(g)
+    kref = orc.fp64_reference(idx, adj, orc.ORC_KATZ, alpha=genv["ALPHA"])[0]
+    assert orc.compare(kref, np.fromfile(str(tmp_path / "k.katz.bin"), np.float32))["l1_rel"] < 0.05

@@ -158,3 +158,19 @@ def test_bench_refuses_more_than_one_gpu():
     r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "4"], capture_output=True, text=True,
                        timeout=120)
     assert r.returncode == 2 and "one GPU only" in r.stderr and r.stdout == ""
+
+
+@pytest.mark.parametrize("args,msg", [
+    ([], "--measure=pr|katz|ev|all is required"),
+    (["--measure=foo", "--sample"], "--measure must be pr, katz, ev, or all, got 'foo'"),
+    (["--measure=pr"], "--graph is required (or use --sample)"),
+    (["--measure=pr", "--graph=/nonexistent/g.egr"], "graph file not found: /nonexistent/g.egr"),
+    (["--measure=pr", "--sample", "--bogus"], "unrecognized argument: --bogus"),
+])
+def test_run_sh_validation(args, msg):
+    """run.sh rejects what the reference's run.sh rejects (run.sh:42-67), with exit 1,
+    before building or touching a GPU."""
+    r = subprocess.run([os.path.join(ROOT, "run.sh"), *args], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 1 and f"ERROR: {msg}" in r.stderr, r.stderr
+    r = subprocess.run([os.path.join(ROOT, "run.sh"), "--help"], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0 and "usage:" in r.stderr
