@@ -180,15 +180,15 @@ def graph_size(nat, graph):
 
 
 def bench_batched(args, nat, np, work, graph):
-    """K = 64 personalised PageRank columns in one batched run against the same 64 problems
-    solved one after another by the plain solver (--measure pr --ppr-column j), both from
-    the resident graph; FP64 errors on sampled columns."""
-    K, SEEDS, TOP, EPS = 64, 16, 8, 1e-7
+    """K (default 64) personalised PageRank columns in one batched run against the same K
+    problems solved one after another by the plain solver (--measure pr --ppr-column j),
+    both from the resident graph; FP64 errors on sampled columns."""
+    K, SEEDS, TOP, EPS = args.k, 16, 8, 1e-7
     gen = f"{K}:{SEEDS}:{TOP}:{args.seed + 6}"
     common = [graph, "--cache", "chunks", "--preload", "--stream-ratio", "1", "--timing", "--ppr-gen", gen,
               "--eps-ppr", EPS]
     n, m = graph_size(nat, graph)
-    sample = [0, 21, 42, 63]
+    sample = sorted({0, K // 3, (2 * K) // 3, K - 1})
     t0 = time.time()
     sess = nat.Session(common + ["--measure", "ppr", "--log", os.path.join(work, "batched.log")])
     log(f"batched setup {time.time() - t0:.1f}s")
@@ -206,7 +206,8 @@ def bench_batched(args, nat, np, work, graph):
     if args.batched_only:
         k = rb["kernel"]
         return {"metric": "batched personalised PageRank (no baseline)", "unit": "s", "value": round(tb, 4),
-                "passes": rb["passes"], "dense_passes": k["pull_passes"],
+                "k": K, "passes": rb["passes"], "dense_passes": k["pull_passes"],
+                "vector_pushes": k["pushed_edge_measures"], "vector_gteps": round(k["pushed_edge_measures"] / tb / 1e9, 3),
                 "kernel_ms": {x: round(k[x], 2) for x in ("snapshot_ms", "push_ms", "piece_ms", "fold_ms", "pull_ms",
                                                            "pull_hub_ms")}}
 
@@ -231,7 +232,7 @@ def bench_batched(args, nat, np, work, graph):
         log(f"plain column {j}: {r['passes']} passes, {seq_t[-1]:.3f}s")
     sess.close()
     ts = sum(seq_t)
-    log(f"plain: 64 runs {ts:.3f}s, {seq_pushes / 1e9:.2f} G pushes")
+    log(f"plain: {K} runs {ts:.3f}s, {seq_pushes / 1e9:.2f} G pushes")
 
     fp64 = {}
     idx, adj = nat.load_egr(graph)
@@ -385,6 +386,7 @@ def main():
     ap.add_argument("--scale", type=int, default=None, help="R-MAT scale (default 24; batched/ooc 26; adjoint 22)")
     ap.add_argument("--cache-mb", type=int, default=1024, help="ooc: device cache cap for the hot-cache run")
     ap.add_argument("--batched-only", action="store_true", help="batched: time the batched run only (no baseline)")
+    ap.add_argument("--k", type=int, default=64, help="batched: personalisation columns (1..64)")
     ap.add_argument("--edge-factor", type=int, default=16)
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--measure", default=None, choices=["pr", "katz", "ev", "all"],

@@ -8,6 +8,7 @@
 // the push semantics (push to out-neighbours) on directed ones.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -55,23 +56,26 @@ __global__ void k_prescale(int n, const uint64_t* __restrict__ idx, const double
         xs[v] = divdeg ? (d ? x[v] / (double)d : 0.0) : x[v];
     }
 }
-// y[w] = c + s * sum_{u -> w} xs[u]; mx[0] = max |y - x|, mx[1] = max |y| (as ordered bits)
+// y[w] = c + s * sum_{u -> w} xs[u]; mx[0] = max |y - x|, mx[1] = max |y| (as ordered bits).
+// GL lanes per vertex, vertex groups strided over the grid (the whole group takes the same
+// trip count, so the shuffles see every lane).
 __global__ __launch_bounds__(BLOCK) void k_pull(int n, const uint64_t* __restrict__ tidx, const int32_t* __restrict__ tadj,
                                                  const double* __restrict__ xs, const double* __restrict__ x, double* y,
                                                  double c, const double* __restrict__ cvec, double s,
                                                  unsigned long long* mx) {
-    const long long w = (blockIdx.x * (long long)BLOCK + threadIdx.x) / GL;
     const int gl = threadIdx.x % GL;
-    double acc = 0;
-    if (w < n)
+    const long long groups = (long long)gridDim.x * (BLOCK / GL);
+    for (long long w = (blockIdx.x * (long long)BLOCK + threadIdx.x) / GL; w < n; w += groups) {
+        double acc = 0;
         for (uint64_t e = tidx[w] + gl; e < tidx[w + 1]; e += GL) acc += xs[tadj[e]];
 #pragma unroll
-    for (int o = GL / 2; o > 0; o >>= 1) acc += __shfl_xor(acc, o, GL);
-    if (w < n && gl == 0) {
-        const double nv = (cvec ? cvec[w] : c) + s * acc;
-        y[w] = nv;
-        atomicMax(&mx[0], (unsigned long long)__double_as_longlong(fabs(nv - x[w])));
-        atomicMax(&mx[1], (unsigned long long)__double_as_longlong(fabs(nv)));
+        for (int o = GL / 2; o > 0; o >>= 1) acc += __shfl_xor(acc, o, GL);
+        if (gl == 0) {
+            const double nv = (cvec ? cvec[w] : c) + s * acc;
+            y[w] = nv;
+            atomicMax(&mx[0], (unsigned long long)__double_as_longlong(fabs(nv - x[w])));
+            atomicMax(&mx[1], (unsigned long long)__double_as_longlong(fabs(nv)));
+        }
     }
 }
 __global__ void k_sums(int n, const double* __restrict__ x, double* out) {   // out[0] += x^2, out[1] += x
@@ -120,8 +124,13 @@ int pull(Solver* S, const double* x, double* y, double c, double s, int divdeg, 
          const double* cvec = nullptr) {
     HIPCHK(hipMemset(S->mx, 0, 2 * sizeof(unsigned long long)));
     hipLaunchKernelGGL(k_prescale, dim3(S->cus * 8), dim3(BLOCK), 0, 0, S->n, S->d_idx, x, S->xs, divdeg);
-    const unsigned grid = (unsigned)(((long long)S->n * GL + BLOCK - 1) / BLOCK);
+    HIPCHK(hipGetLastError());
+    // at most 64 blocks per CU (the kernel strides over vertex groups): a grid of one group
+    // per vertex would pass 2^32 threads at n = 2^28
+    const long long want = ((long long)S->n * GL + BLOCK - 1) / BLOCK;
+    const unsigned grid = (unsigned)std::min<long long>(want, (long long)S->cus * 64);
     hipLaunchKernelGGL(k_pull, dim3(grid), dim3(BLOCK), 0, 0, S->n, S->d_tidx, S->d_tadj, S->xs, x, y, c, cvec, s, S->mx);
+    HIPCHK(hipGetLastError());
     unsigned long long h[2];
     HIPCHK(hipMemcpy(h, S->mx, sizeof h, hipMemcpyDeviceToHost));
     *mxd = bits_to_d(h[0]);
@@ -132,6 +141,7 @@ int pull(Solver* S, const double* x, double* y, double c, double s, int divdeg, 
 int sums(Solver* S, const double* x, double* sq, double* sm) {
     HIPCHK(hipMemset(S->red, 0, 2 * sizeof(double)));
     hipLaunchKernelGGL(k_sums, dim3(S->cus * 4), dim3(BLOCK), 0, 0, S->n, x, S->red);
+    HIPCHK(hipGetLastError());
     double h[2];
     HIPCHK(hipMemcpy(h, S->red, sizeof h, hipMemcpyDeviceToHost));
     *sq = h[0];
@@ -142,6 +152,7 @@ int sums(Solver* S, const double* x, double* sq, double* sm) {
 int scale_diff(Solver* S, double* y, const double* x, double f, double* md) {
     HIPCHK(hipMemset(S->mx, 0, sizeof(unsigned long long)));
     hipLaunchKernelGGL(k_scale_diff, dim3(S->cus * 4), dim3(BLOCK), 0, 0, S->n, y, x, f, S->mx);
+    HIPCHK(hipGetLastError());
     unsigned long long h;
     HIPCHK(hipMemcpy(&h, S->mx, sizeof h, hipMemcpyDeviceToHost));
     *md = bits_to_d(h);
@@ -173,6 +184,7 @@ int exc_fp64_create(int device, int n, uint64_t m, const uint64_t* h_idx, const 
     CK(hipMalloc(&cnt, ((size_t)n + 1) * 8));
     CK(hipMemset(cnt, 0, ((size_t)n + 1) * 8));
     hipLaunchKernelGGL(k_count_in, dim3(S->cus * 16), dim3(BLOCK), 0, 0, m, d_adj, cnt);
+    CK(hipGetLastError());
     std::vector<unsigned long long> h((size_t)n + 1);
     CK(hipMemcpy(h.data(), cnt, h.size() * 8, hipMemcpyDeviceToHost));
     for (size_t v = 0; v < (size_t)n; v++) h[v + 1] += h[v];
@@ -181,6 +193,7 @@ int exc_fp64_create(int device, int n, uint64_t m, const uint64_t* h_idx, const 
     CK(hipMemcpy(cnt, h.data(), h.size() * 8, hipMemcpyHostToDevice));
     CK(hipMalloc(&S->d_tadj, (m ? m : 1) * 4));
     hipLaunchKernelGGL(k_fill_in, dim3(S->cus * 8), dim3(BLOCK), 0, 0, n, S->d_idx, d_adj, cnt, S->d_tadj);
+    CK(hipGetLastError());
     CK(hipDeviceSynchronize());
     hipFree(d_adj);
     hipFree(cnt);
@@ -212,6 +225,7 @@ int exc_fp64_solve(void* p, int kind, double alpha, double beta, int max_iter, d
     const int n = S->n;
     if (h_x0) HIPCHK(hipMemcpy(S->x, h_x0, (size_t)n * 8, hipMemcpyHostToDevice));
     else hipLaunchKernelGGL(k_fill, dim3(S->cus * 4), dim3(BLOCK), 0, 0, n, S->x, kind == EXC_PR ? 0.15 : 1.0);
+    HIPCHK(hipGetLastError());
     double c = 0, s = 1;
     int divdeg = 0;
     if (kind == EXC_PR) { c = 0.15; s = 0.85; divdeg = 1; }
@@ -270,6 +284,7 @@ int exc_fp64_residual(void* p, double c, const double* h_c, double s, int divdeg
     if (rc) return rc;
     HIPCHK(hipMemset(S->red, 0, sizeof(double)));
     hipLaunchKernelGGL(k_absdiff_sum, dim3(S->cus * 4), dim3(BLOCK), 0, 0, n, S->y, S->x, S->red);
+    HIPCHK(hipGetLastError());
     HIPCHK(hipMemcpy(l1, S->red, sizeof(double), hipMemcpyDeviceToHost));
     return EXC_OK;
 }
@@ -310,6 +325,7 @@ int exc_fp64_lambda(void* p, const double* h_start, int max_iter, double tol, do
     const int n = S->n;
     if (h_start) HIPCHK(hipMemcpy(S->x, h_start, (size_t)n * 8, hipMemcpyHostToDevice));
     else hipLaunchKernelGGL(k_fill, dim3(S->cus * 4), dim3(BLOCK), 0, 0, n, S->x, 1.0);
+    HIPCHK(hipGetLastError());
     double sq, sm, md;
     int rc = sums(S, S->x, &sq, &sm);
     if (rc) return rc;
@@ -347,6 +363,7 @@ int exc_fp64_katz_mean(void* p, double alpha, double beta, double* mean, int* it
     HIPCHK(hipSetDevice(S->dev));
     const int n = S->n;
     hipLaunchKernelGGL(k_fill, dim3(S->cus * 4), dim3(BLOCK), 0, 0, n, S->x, beta);
+    HIPCHK(hipGetLastError());
     double* x = S->x;
     double* y = S->y;
     double meanK = 0, a, b, sq, sm;

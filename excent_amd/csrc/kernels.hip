@@ -610,12 +610,13 @@ __global__ __launch_bounds__(BLOCK) void k_fill_hot(int lo, int hi, const int32_
                                                      const uint64_t* __restrict__ hoff,
                                                      const uint64_t* __restrict__ idx, uint64_t e0,
                                                      const int32_t* __restrict__ src, int32_t* hedges) {
-    const long long w = lo + (blockIdx.x * (long long)BLOCK + threadIdx.x) / WAVE;
-    if (w >= hi) return;
-    const int32_t* from = src + (idx[hids[w]] - e0);
-    int32_t* to = hedges + hoff[w];
-    const uint64_t d = hoff[w + 1] - hoff[w];
-    for (uint64_t i = lane_id(); i < d; i += WAVE) to[i] = from[i];
+    const long long nw = (long long)gridDim.x * NWAVES;
+    for (long long w = lo + (blockIdx.x * (long long)BLOCK + threadIdx.x) / WAVE; w < hi; w += nw) {
+        const int32_t* from = src + (idx[hids[w]] - e0);
+        int32_t* to = hedges + hoff[w];
+        const uint64_t d = hoff[w + 1] - hoff[w];
+        for (uint64_t i = lane_id(); i < d; i += WAVE) to[i] = from[i];
+    }
 }
 
 __global__ void k_indegree(uint64_t m, const int32_t* __restrict__ edges, uint32_t* indeg) {
@@ -1672,6 +1673,16 @@ struct Ctx {
     uint32_t* d_hubbits = nullptr;      // batched: destinations summed by k_bpull_hub
     int pull_seq = 8;                   // in-list entries per lane in k_pull (4, 8, 16 or 32)
     int bpull_q = 8;                    // batched pull: rows in flight per wave (8 or 16)
+    // Batched pull tuner: the first two dense passes whose demand is at least half the edges
+    // run with 8 and then 16 rows in flight and are timed with events; later passes use the
+    // faster one.  The rows are added in slot order either way, so the choice changes the
+    // time and not the sums.  EXC_BPULL_Q fixes the value and turns the tuner off.
+    bool bq_tune = false;
+    int bq_trial = 0;                   // next candidate (BQ_CAND index)
+    bool bq_recorded = false;           // a trial pass's events are recorded and not yet read
+    double bq_demand = 0;               // demand of that pass
+    double bq_ns_edge[2] = {0, 0};      // each candidate's time per demand edge
+    hipEvent_t bq_ev[2] = {nullptr, nullptr};
     uint64_t pull_passes_live = 0;
     PullPiece* d_pull_hubs = nullptr;   // pieces of in-lists longer than PULL_HUB
     int n_pull_hubs = 0;
@@ -1945,8 +1956,30 @@ void launch_fold_pull(Ctx* c, hipStream_t st) {
     if (fuse_fold) std::swap(c->d_p, c->d_p2);   // the rows just written serve the next pass
 }
 
+constexpr int BQ_CAND[2] = {8, 16};
+
+// Reads the timed trial pass (every pass ends with a device synchronisation, so its
+// events are complete) and, after the last candidate, keeps the fastest.
+void bq_collect(Ctx* c) {
+    if (!c->bq_recorded) return;
+    c->bq_recorded = false;
+    float ms = 0.f;
+    if (hipEventSynchronize(c->bq_ev[1]) != hipSuccess || hipEventElapsedTime(&ms, c->bq_ev[0], c->bq_ev[1]) != hipSuccess) {
+        c->bq_tune = false;   // no timing: keep the current value
+        return;
+    }
+    c->bq_ns_edge[c->bq_trial] = (double)ms * 1e6 / std::max(1.0, c->bq_demand);
+    if (++c->bq_trial == 2) {
+        c->bpull_q = c->bq_ns_edge[1] < c->bq_ns_edge[0] ? BQ_CAND[1] : BQ_CAND[0];
+        c->bq_tune = false;
+    }
+}
+
 // Batched dense pass: fold the snapshot's pending vertices, hub pieces, then the pull.
 void launch_bfold_pull(Ctx* c, hipStream_t st) {
+    bq_collect(c);
+    const bool trial = c->bq_tune && (double)c->last_demand >= 0.5 * (double)c->m;
+    if (trial) c->bpull_q = BQ_CAND[c->bq_trial];
     unsigned long long* row = c->d_counters + C_DENSE_LIVE * C_FIELDS;
     const bool fuse = c->fuse_fold && c->d_p2 && c->d_bitmap2 && !c->d_trace &&
                       (double)c->last_demand >= std::max(0.1, 2.0 * c->pull_threshold) * (double)c->m;
@@ -1966,6 +1999,7 @@ void launch_bfold_pull(Ctx* c, hipStream_t st) {
         hipLaunchKernelGGL(k_zero_u64, dim3((ncnt + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, st, c->d_cnt, ncnt);
         hipLaunchKernelGGL(k_zero_u32, dim3((c->K + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, st, c->d_colpend, c->K);
     }
+    if (trial) hipEventRecord(c->bq_ev[0], st);
     if (c->n_pull_hubs) {
         ScopedTimer t(c, st, T_PULL_HUB);
         const dim3 g((unsigned)std::min<long long>((c->n_pull_hubs + NWAVES - 1) / NWAVES, cap));
@@ -1977,6 +2011,11 @@ void launch_bfold_pull(Ctx* c, hipStream_t st) {
         const dim3 g((unsigned)std::min<long long>((segs + NWAVES - 1) / NWAVES, 2 * cap));
         if (c->bpull_q == 16) hipLaunchKernelGGL(k_bpull<16>, g, dim3(BLOCK), 0, st, a, B);
         else hipLaunchKernelGGL(k_bpull<8>, g, dim3(BLOCK), 0, st, a, B);
+    }
+    if (trial) {
+        hipEventRecord(c->bq_ev[1], st);
+        c->bq_recorded = true;
+        c->bq_demand = (double)c->last_demand;
     }
     c->pull_passes_live++;
     c->folded = fuse;
@@ -2174,6 +2213,7 @@ int exc_destroy(void* p) {
     hipDeviceSynchronize();
     for (auto& t : c->pending_timers) { hipEventDestroy(t.a); hipEventDestroy(t.b); }
     for (auto e : c->free_events) hipEventDestroy(e);
+    for (auto e : c->bq_ev) if (e) hipEventDestroy(e);
     for (auto& sl : c->slot) {
         if (sl.h_edges) hipHostFree(sl.h_edges);
         if (sl.h_vids) hipHostFree(sl.h_vids);
@@ -2516,8 +2556,8 @@ int exc_stream_chunk(void* p, int s, const exc_chunk* ch, int to_resident, int h
     if (hot_hi > hot_lo && c->d_hedges) {
         ScopedTimer t(c, st, T_OTHER);
         const int32_t* src = to_resident ? dst + ch->e0 : dst;
-        const long long items = (long long)(hot_hi - hot_lo) * WAVE;
-        hipLaunchKernelGGL(k_fill_hot, dim3((unsigned)((items + BLOCK - 1) / BLOCK)), dim3(BLOCK), 0, st,
+        const long long items = (long long)(hot_hi - hot_lo) * WAVE;   // one wave per hot entry, strided
+        hipLaunchKernelGGL(k_fill_hot, dim3(grid_for(items, c->cus * 16)), dim3(BLOCK), 0, st,
                            hot_lo, hot_hi, c->d_hids, c->d_hoff, c->d_idx, ch->e0, src, c->d_hedges);
     }
     int rc = push_chunk(c, 1 + s, ch->v0, ch->v1, dst, ebase, use_list, group, hot_skip,
@@ -2762,6 +2802,9 @@ int exc_get_stats(void* p, exc_stats* o) {
         o->pushed_edges += sp[S_EDGES];
         o->pushed_edge_measures += sp[S_EDGES_NM];
     }
+    o->pull_rows_in_flight = c->K ? c->bpull_q : 0;
+    o->pull_tune_ns_per_edge[0] = c->bq_ns_edge[0];
+    o->pull_tune_ns_per_edge[1] = c->bq_ns_edge[1];
     if (h[C_OVERFLOW]) return fail(EXC_ESTATE, "hub piece buffer overflowed %llu times", h[C_OVERFLOW]);
     return EXC_OK;
 }
@@ -2778,7 +2821,13 @@ int exc_enable_pull(void* p, double threshold) {
         c->pull_live = ev && (ev[0] == '0' || ev[0] == '1') ? ev[0] - '0' : 2;
         if (const char* lf = getenv("EXC_PULL_LIVE_FRAC")) c->pull_live_frac = atof(lf);
         if (const char* ff = getenv("EXC_FUSE_FOLD")) c->fuse_fold = atoi(ff);   // 0: fold with k_fold always
-        if (const char* bq = getenv("EXC_BPULL_Q")) c->bpull_q = atoi(bq);   // 8 / 16 rows per wave
+        if (const char* bq = getenv("EXC_BPULL_Q")) {   // 8 / 16 rows per wave, no tuning
+            c->bpull_q = atoi(bq) == 16 ? 16 : 8;
+        } else if (c->K && !c->bq_ev[0] && c->bq_trial == 0) {
+            HIPCHK(hipEventCreate(&c->bq_ev[0]));
+            HIPCHK(hipEventCreate(&c->bq_ev[1]));
+            c->bq_tune = true;
+        }
         const char* sq = getenv("EXC_PULL_SEQ");   // 4 / 8 / 16 / 32 entries per lane (measurements)
         c->pull_seq = sq ? atoi(sq) : 8;
     }

@@ -67,7 +67,8 @@ class Stats(ctypes.Structure):
                 ("pushed_edge_measures", c_u64), ("pushed_vertex_measures", c_u64),
                 ("src_ms", c_double * 4), ("src_launches", c_u64 * 4), ("src_bytes", c_double * 4),
                 ("fold_ms", c_double), ("pull_ms", c_double), ("pull_hub_ms", c_double), ("pull_passes", c_u64),
-                ("fold_bytes", c_double), ("pull_bytes", c_double), ("pull_hub_bytes", c_double)]
+                ("fold_bytes", c_double), ("pull_bytes", c_double), ("pull_hub_bytes", c_double),
+                ("pull_rows_in_flight", c_int), ("pull_tune_ns_per_edge", c_double * 2)]
 
 
 class RunStats(ctypes.Structure):

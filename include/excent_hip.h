@@ -90,6 +90,10 @@ typedef struct exc_stats {
     double fold_ms, pull_ms, pull_hub_ms;
     uint64_t pull_passes;
     double fold_bytes, pull_bytes, pull_hub_bytes;
+    /* batched pull: rows in flight per wave in use, and the tuner's time per demand edge
+     * (ns) with 8 and with 16 (0 until that trial pass ran) */
+    int pull_rows_in_flight;
+    double pull_tune_ns_per_edge[2];
 } exc_stats;
 
 const char* exc_last_error(void);

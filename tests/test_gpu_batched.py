@@ -172,6 +172,42 @@ def test_batched_session_rerun_after_reinit(rmat12):
         assert np.abs(a - b).sum() <= 2 * 1e-5 * n / (1 - 0.85)
 
 
+def test_batched_pull_rows_fixed_by_env(rmat12, monkeypatch):
+    """EXC_BPULL_Q fixes the batched pull's rows in flight and turns the tuner off; the
+    columns still meet their FP64 bound."""
+    monkeypatch.setenv("EXC_BPULL_Q", "16")
+    idx, adj = orc.egr_load(rmat12)
+    n, m = len(idx) - 1, len(adj)
+    K, S, eps = 3, 2, 1e-4
+    rng = np.random.default_rng(5)
+    seeds = [rng.choice(n, S, replace=False).astype(np.int32) for _ in range(K)]
+    mass = np.float32(0.15 * n / S)
+    with nat.Context(n, m, np.array([0], np.int32), [nat.EXC_PR], [eps], [0.85], [1], batch_k=K) as ctx:
+        ctx.upload_offsets(idx)
+        assert ctx.cache_chunks(m) == 0
+        ctx.cache_upload(adj)
+        for j in range(K):
+            ctx.batch_seed(j, seeds[j], mass)
+        assert ctx.enable_pull(0.0) == 0
+        for it in range(5000):
+            cnt, _ = ctx.snapshot()
+            cp = ctx.batch_colpend()
+            ctx.batch_set_active(sum(1 << j for j in range(K) if cp[j]))
+            if cnt[0::2].sum() == 0:
+                break
+            ctx.push_resident(0, n, 0, m, False)
+            ctx.pass_sync()
+        X = [ctx.read_state(j, 0) for j in range(K)]
+        st = ctx.stats()
+    assert st["pull_rows_in_flight"] == 16 and list(st["pull_tune_ns_per_edge"]) == [0.0, 0.0]
+    for j in range(K):
+        b = np.zeros(n)
+        np.add.at(b, seeds[j], np.float64(mass))
+        ref, _, _ = orc.fp64_solve_b(idx, adj, b)
+        bound = np.abs(true_residual(idx, adj, X[j], b)).sum() / 0.15
+        assert np.abs(X[j] - ref).sum() <= bound * (1 + 1e-6) + 1e-6, j
+
+
 @pytest.mark.parametrize("fuse", [0, 1])
 @pytest.mark.parametrize("gname", ["rmat12", "star"])
 def test_batched_dense_passes(gname, fuse, rmat12, monkeypatch):
@@ -222,6 +258,13 @@ def test_batched_dense_passes(gname, fuse, rmat12, monkeypatch):
         st = ctx.stats()
     assert st["pull_passes"] > 0 and st["pushed_edge_measures"] > 0
     assert (folded_passes > 0) == bool(fuse)
+    # the rows-in-flight tuner timed 8 and 16 on the first two passes over half the edges
+    # (rmat12's early passes) and kept the faster; without such passes it keeps 8
+    t8, t16 = st["pull_tune_ns_per_edge"]
+    if gname == "rmat12":
+        assert t8 > 0 and t16 > 0 and st["pull_rows_in_flight"] == (16 if t16 < t8 else 8), st
+    else:
+        assert st["pull_rows_in_flight"] in (8, 16)
     for j in range(K):
         assert np.all(np.abs(R[j]) <= eps)
         b = np.zeros(n)

@@ -201,3 +201,23 @@ def test_run_sh_end_to_end(tmp_path, rmat12):
     idx, adj = orc.egr_load(g)
     kref = orc.fp64_reference(idx, adj, orc.ORC_KATZ, alpha=genv["ALPHA"])[0]
     assert orc.compare(kref, np.fromfile(str(tmp_path / "k.katz.bin"), np.float32))["l1_rel"] < 0.05
+
+
+def test_fp64_solver_strided_grid(tmp_path_factory):
+    """A graph with more vertices than the FP64 pull's capped grid has 16-lane groups
+    (2^19 > 256 CUs x 64 blocks x 16 groups), so every group takes several vertices: the
+    solves still equal the oracle's FP64 restatement, and the residual its CPU sum."""
+    from tests.conftest import rmat_graph
+    g = rmat_graph(tmp_path_factory, 19, seed=5)
+    idx, adj = orc.egr_load(g)
+    n = len(idx) - 1
+    print("case fp64 strided grid, n =", n, flush=True)
+    with nat.Fp64Solver(idx, adj) as S:
+        for kind, kw in ((nat.EXC_PR, {}), (nat.EXC_KATZ, {"alpha": 1e-3}), (nat.EXC_EV, {})):
+            got, _, _, it = S.solve(kind, tol=1e-10, max_iter=300, **kw)
+            ref, it_ref, _, _ = orc.fp64_reference(idx, adj, kind, max_iter=300, tol=1e-10, **kw)
+            assert abs(it - it_ref) <= 1, (kind, it, it_ref)
+            np.testing.assert_allclose(got, ref, rtol=1e-8, atol=1e-12)
+        x = np.random.default_rng(4).random(n).astype(np.float32)
+        want = orc.residual_l1(idx, adj, orc.ORC_PR, 0.85, 0.15, x)
+        assert abs(S.residual(x, 0.15, 0.85, 1) - want) <= 1e-10 * want
