@@ -190,7 +190,7 @@ def bench_batched(args, nat, np, work, graph):
     n, m = graph_size(nat, graph)
     sample = sorted({0, K // 3, (2 * K) // 3, K - 1})
     t0 = time.time()
-    sess = nat.Session(common + ["--measure", "ppr", "--log", os.path.join(work, "batched.log")])
+    sess = nat.Session(common + ["--measure", "ppr", "--log", os.path.join(args.logdir or work, "batched.log")])
     log(f"batched setup {time.time() - t0:.1f}s")
     for w in range(args.warmup):
         sess.run(reinit=w > 0)
@@ -207,12 +207,14 @@ def bench_batched(args, nat, np, work, graph):
         k = rb["kernel"]
         return {"metric": "batched personalised PageRank (no baseline)", "unit": "s", "value": round(tb, 4),
                 "k": K, "passes": rb["passes"], "dense_passes": k["pull_passes"],
+                "pull_rows_in_flight": k["pull_rows_in_flight"],
+                "pull_tune_ns_per_edge": [round(v, 4) for v in k["pull_tune_ns_per_edge"]],
                 "vector_pushes": k["pushed_edge_measures"], "vector_gteps": round(k["pushed_edge_measures"] / tb / 1e9, 3),
                 "kernel_ms": {x: round(k[x], 2) for x in ("snapshot_ms", "push_ms", "piece_ms", "fold_ms", "pull_ms",
                                                            "pull_hub_ms")}}
 
     sess = nat.Session(common + ["--measure", "pr", "--ppr-column", 0, "--pull", args.pull,
-                                 "--log", os.path.join(work, "plain.log")])
+                                 "--log", os.path.join(args.logdir or work, "plain.log")])
     sess.set_column(0)
     sess.run(reinit=False)   # warmup
     seq_t, seq_pushes, seq_passes, xp, cvec = [], 0, [], {}, {}
@@ -260,7 +262,9 @@ def bench_batched(args, nat, np, work, graph):
                     "vector_gteps": round(vb / tb / 1e9, 3),
                     "kernel_ms": {k: round(rb["kernel"][k], 2) for k in ("snapshot_ms", "push_ms", "piece_ms",
                                                                           "fold_ms", "pull_ms", "pull_hub_ms")},
-                    "dense_passes": rb["kernel"]["pull_passes"]},
+                    "dense_passes": rb["kernel"]["pull_passes"],
+                    "pull_rows_in_flight": rb["kernel"]["pull_rows_in_flight"],
+                    "pull_tune_ns_per_edge": [round(v, 4) for v in rb["kernel"]["pull_tune_ns_per_edge"]]},
         "plain": {"time_s": round(ts, 4), "time_s_per_column": round(ts / K, 4),
                   "passes_mean": round(sum(seq_passes) / K, 1), "vector_pushes": seq_pushes,
                   "vector_gteps": round(seq_pushes / ts / 1e9, 3)},
@@ -290,7 +294,7 @@ def bench_ooc(args, nat, np, work, graph):
         modes.append((hot + ", cold page cache", ["--cache", "hot", "--cache-mb", args.cache_mb], True))
     for tag, extra, cold in modes:
         sargs = [graph, "--measure", measure, "--timing", "--log",
-                 os.path.join(work, f"ooc_{tag.split()[0]}{'_cold' if cold else ''}.log")] + extra
+                 os.path.join(args.logdir or work, f"ooc_{tag.split()[0]}{'_cold' if cold else ''}.log")] + extra
         sargs += measure_args(measure, env)
         if tag == hot:
             sargs += ["--dump", os.path.join(work, "x")]
@@ -400,6 +404,8 @@ def main():
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 error report of the last run")
     ap.add_argument("--workdir", default=None)
     ap.add_argument("--log", default=None, help="bin/excent report of the runs (default: in the workdir)")
+    ap.add_argument("--logdir", default=None, help="batched / ooc: directory of the runs' bin/excent reports "
+                                                   "(default: the workdir)")
     ap.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "traffic_rmat24_all.json"),
                     help="JSON with HBM bytes per push launch from rocprofv3 --pmc runs")
     args = ap.parse_args()

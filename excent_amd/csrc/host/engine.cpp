@@ -782,6 +782,7 @@ void Engine::run_passes() {
                 (unsigned long long)pass_runs);
         if (batched()) fprintf(out, " | active columns %d", active_cols_);
         fprintf(out, "\n");
+        fflush(out);   // a long run shows its progress pass by pass
     }
     T_iter_ = Clock::now();
     T_setup_ = T_begin;

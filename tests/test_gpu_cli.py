@@ -106,6 +106,46 @@ def test_out_of_core_regimes(tmp_path, rmat14, mode):
         assert " resident " in lines[0] and "stream 0" in lines[0]
 
 
+@pytest.fixture(scope="module")
+def rmat14_env(rmat14, tmp_path_factory):
+    """rmat14's constants: lambda1 by FP64 power iteration, alpha = 0.5 / lambda1."""
+    out = str(tmp_path_factory.mktemp("env") / "rmat14.env")
+    assert nat.run_entry("excent_params_main", [rmat14, out, "--directed", "--alpha-factor", "0.5", "--seed", "7"]) == 0
+    return read_env(out)
+
+
+@pytest.mark.parametrize("mode", ["stream_only", "gather_only", "mixed", "hot", "zero_copy"])
+def test_out_of_core_regimes_all_measures(tmp_path, rmat14, rmat14_env, mode):
+    """The fused three-measure push (16-byte rows) in every out-of-core regime: PageRank,
+    Katz and eigenvector against FP64, every |r| under its eps.  --max-passes bounds the
+    run, so a regime that stopped converging fails here instead of running on."""
+    idx, adj = orc.egr_load(rmat14)
+    env = rmat14_env
+    eps = 1e-5
+    base = [rmat14, "--measure", "all", "--alpha", env["ALPHA"], "--beta", 1, "--lambda", env["LAMBDA"],
+            "--eps-katz", eps, "--eps-ev", eps, "--ev-init", "indeg", "--dump", str(tmp_path / "o"),
+            "--chunk-medges", "0.02", "--max-passes", "2000"]
+    extra = {"stream_only": ["--cache", "none", "--stream-ratio", "0"],
+             "gather_only": ["--cache", "none", "--stream-ratio", "2"],
+             "mixed": ["--cache", "none", "--stream-ratio", "0.2", "--dedup", "off"],
+             "zero_copy": ["--cache", "none", "--zero-copy", "on", "--stream-ratio", "0.3"]}.get(mode)
+    if mode == "hot":
+        free, total = nat.ctypes.c_size_t(0), nat.ctypes.c_size_t(0)
+        nat.F["exc_mem_info"](0, nat.ctypes.byref(free), nat.ctypes.byref(total))
+        extra = ["--cache", "hot", "--reserve-mb", f"{free.value / (1 << 20) - 64:.3f}", "--stream-ratio", "0.3"]
+    print("case all measures", mode, flush=True)
+    rc, text = run(base + extra, tmp_path, mode)
+    assert rc == 0, text
+    passes = int(re.search(r"passes (\d+)  measure all", text).group(1))
+    assert passes < 2000, "did not converge"
+    x = {k: np.fromfile(str(tmp_path / f"o.{k}.bin"), np.float32) for k in ("pr", "katz", "ev")}
+    pr_check(idx, adj, x["pr"])
+    kref = orc.fp64_reference(idx, adj, orc.ORC_KATZ, alpha=env["ALPHA"], tol=1e-12, max_iter=3000)[0]
+    assert orc.compare(kref, x["katz"])["l1_rel"] < 1e-3
+    eref = orc.fp64_reference(idx, adj, orc.ORC_EV, tol=1e-12, max_iter=3000)[0]
+    assert orc.compare(eref, x["ev"], unit=True)["l1_rel"] < 1e-2
+
+
 def test_trace_pending(tmp_path, rmat12):
     tf = str(tmp_path / "trace.u16")
     rc, text = run([rmat12, "--measure", "pr", "--trace-pending", tf, "--stream-ratio", "0.1"], tmp_path)
@@ -172,10 +212,11 @@ def test_params_tool_reproduces_sample_env(tmp_path):
         assert abs(got[k] - want[k]) <= 1e-6 * abs(want[k]), k
 
 
-def test_run_sh_end_to_end(tmp_path, rmat12):
+def test_run_sh_end_to_end(tmp_path):
     """run.sh as the reference's users call it: the sample with its shipped constants
-    (--measure=all), and a graph without a .env, which the wrapper computes next to the
-    graph with bin/excent_params and then passes to bin/excent (--measure=katz)."""
+    (--measure=all), and a copy of the sample without a .env, for which the wrapper writes
+    the constants next to the graph with bin/excent_params (the sample's own .env digits)
+    and passes them to bin/excent (--measure=katz)."""
     import shutil
     import subprocess
     from tests.conftest import ROOT
@@ -191,24 +232,25 @@ def test_run_sh_end_to_end(tmp_path, rmat12):
     kref = orc.fp64_reference(idx, adj, orc.ORC_KATZ, alpha=env["ALPHA"])[0]
     assert orc.compare(kref, np.fromfile(str(tmp_path / "a.katz.bin"), np.float32))["l1_rel"] < 0.05
     g = str(tmp_path / "g.egr")
-    shutil.copy(rmat12, g)
-    print("case run.sh rmat12 katz (constants computed)", flush=True)
+    shutil.copy(SAMPLE_EGR, g)
+    print("case run.sh sample copy, katz (constants computed)", flush=True)
     r = subprocess.run([sh, "--measure=katz", f"--graph={g}", "--", "--dump", str(tmp_path / "k")],
                        capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stderr
     assert os.path.exists(str(tmp_path / "g.env")) and "Computing the constants" in r.stdout
     genv = read_env(str(tmp_path / "g.env"))
-    idx, adj = orc.egr_load(g)
+    for k in ("LAMBDA", "ALPHA", "EPS_KATZ", "EPS_EV"):
+        assert abs(genv[k] - env[k]) <= 1e-6 * abs(env[k]), k
     kref = orc.fp64_reference(idx, adj, orc.ORC_KATZ, alpha=genv["ALPHA"])[0]
     assert orc.compare(kref, np.fromfile(str(tmp_path / "k.katz.bin"), np.float32))["l1_rel"] < 0.05
 
 
-def test_fp64_solver_strided_grid(tmp_path_factory):
+def test_fp64_solver_strided_grid(tmpdir_factory):
     """A graph with more vertices than the FP64 pull's capped grid has 16-lane groups
     (2^19 > 256 CUs x 64 blocks x 16 groups), so every group takes several vertices: the
     solves still equal the oracle's FP64 restatement, and the residual its CPU sum."""
     from tests.conftest import rmat_graph
-    g = rmat_graph(tmp_path_factory, 19, seed=5)
+    g = rmat_graph(tmpdir_factory, 19, seed=5)
     idx, adj = orc.egr_load(g)
     n = len(idx) - 1
     print("case fp64 strided grid, n =", n, flush=True)
