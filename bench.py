@@ -296,6 +296,8 @@ def bench_ooc(args, nat, np, work, graph):
         sargs = [graph, "--measure", measure, "--timing", "--log",
                  os.path.join(args.logdir or work, f"ooc_{tag.split()[0]}{'_cold' if cold else ''}.log")] + extra
         sargs += measure_args(measure, env)
+        if args.max_passes:
+            sargs += ["--max-passes", args.max_passes]
         if tag == hot:
             sargs += ["--dump", os.path.join(work, "x")]
         resident = None
@@ -319,6 +321,7 @@ def bench_ooc(args, nat, np, work, graph):
         sess.close()
         shipped = r["bytes_stream"] + r["bytes_gather"]
         out["runs"][tag] = {"time_to_eps_s": round(el, 4), "passes": r["passes"],
+                            "converged": not args.max_passes or r["passes"] < args.max_passes,
                             "gteps": round(r["pushed_edges"] / el / 1e9, 3),
                             "bytes_stream_gb": round(r["bytes_stream"] / 1e9, 3),
                             "bytes_gather_gb": round(r["bytes_gather"] / 1e9, 3),
@@ -395,6 +398,8 @@ def main():
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--measure", default=None, choices=["pr", "katz", "ev", "all"],
                     help="measures of the plain runs (default: all; ooc: pr)")
+    ap.add_argument("--max-passes", type=int, default=0,
+                    help="ooc: stop a run after this many passes (it is then reported as not converged)")
     ap.add_argument("--cold", action="store_true",
                     help="ooc: also time the hot-cache run after dropping the file from the host page cache")
     ap.add_argument("--kernel", default="lb", choices=["lb", "group"])
