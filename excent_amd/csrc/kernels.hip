@@ -1340,18 +1340,55 @@ __global__ void k_bseed(int cnt, const int32_t* __restrict__ seeds, float* r, in
 // (except the hub pieces).  The snapshot bitmap of the pass marks the sources whose p row
 // is written this pass; every other row is stale and skipped.
 
+// Batched dense-pass weight rows are stored as PT: float, or bf16 (uint16_t: the high half
+// of the float's bits, rounded to nearest even), which makes a 64-column row one 128-byte
+// line.  A bf16 weight w is exact for the source's effective delta d' = w deg / damp; the fold
+// moves d' into x and leaves the rounding remainder d - d' (at most 2^-9 |d|) in r, where it
+// is pushed again once it exceeds eps (error feedback), so x + r = b + W x holds as with
+// float rows.
+__device__ __forceinline__ uint16_t f32_to_bf16(float f) {
+    const uint32_t u = __float_as_uint(f);
+    return (uint16_t)((u + 0x7fffu + ((u >> 16) & 1u)) >> 16);
+}
+__device__ __forceinline__ float bf16_to_f32(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
+// bf16 rows have an even stride (K rounded up) so that a row is whole 4-byte words: the
+// gather of a bf16 row is one word (two columns) per lane over 32 lanes, two rows per wave
+// instruction (BAcc<uint16_t>).
+__host__ __device__ constexpr int prow_stride(int K, bool bf16) { return bf16 ? K + (K & 1) : K; }
+// Folds delta d of a vertex with deg out-edges: stores its weight at p[i] (deg > 0) and
+// returns the part of d it accounts for (d itself for float rows and for deg == 0).
+__device__ __forceinline__ float prow_fold(float* p, long long i, float d, uint64_t deg, float damp) {
+    if (deg) p[i] = damp * d / (float)deg;
+    return d;
+}
+__device__ __forceinline__ float prow_fold(uint16_t* p, long long i, float d, uint64_t deg, float damp) {
+    if (!deg) return d;
+    const uint16_t h = f32_to_bf16(damp * d / (float)deg);
+    p[i] = h;
+    return bf16_to_f32(h) * (float)deg / damp;
+}
+__device__ __forceinline__ void prow_zero(float* p, long long i) { p[i] = 0.f; }
+__device__ __forceinline__ void prow_zero(uint16_t* p, long long i) { p[i] = 0; }
+
+template <typename PT>
 struct BDenseArgs {
     int n;
     const uint64_t* idx;          // out-offsets (fold: degrees)
     const uint32_t* pending;      // the pass's snapshot bitmap
-    float* p;                     // [n][K] per-edge weights of the pass
+    PT* p;                        // [n][pk] per-edge weights of the pass (columns 0..K-1), the
+                                  // row of vertex v at rank[v] (decreasing out-degree order)
+    int pk;                       // row stride of p: prow_stride(K, PT is bf16)
+    const int32_t* rank;          // p row of each vertex
+    const int32_t* inv;           // vertex of each p row (rows below nsrc)
+    uint32_t* live;               // bit q: row q of p is live this pass (k_blive)
+    int nsrc;                     // rows of vertices with out-edges
     const uint64_t* tidx;         // transpose (in-lists of at most PULL_HUB entries)
     const int32_t* tadj;          // in-neighbours (vertex ids)
     unsigned long long* counters; // C_FIELDS row of the dense pass
     unsigned long long* live_count;   // k_bpull_hub: rows read
     // k_bpull: the next pass's snapshot and fold, fused into the row updates
     // (p_next == nullptr: not fused)
-    float* p_next;                // [n][K] rows of the next pass
+    PT* p_next;                   // [n][pk] rows of the next pass
     uint32_t* bitmap_next;        // its snapshot bitmap
     const uint32_t* hubbits;      // destinations whose in-lists went to k_bpull_hub
     const int* cv0;
@@ -1367,7 +1404,8 @@ __device__ __forceinline__ bool bit_of(const uint32_t* __restrict__ bm, int v) {
 // Fold: each wave takes 64-vertex segments and folds the snapshot's pending vertices one
 // at a time, lanes = columns: x += r, r = 0, p = d r / deg for every active column over
 // eps, the whole p row written (zeros for the other columns).
-__global__ __launch_bounds__(BLOCK) void k_bfold(BDenseArgs a, BatchArgs B) {
+template <typename PT>
+__global__ __launch_bounds__(BLOCK) void k_bfold(BDenseArgs<PT> a, BatchArgs B) {
     const int lane = lane_id();
     const long long nw = (long long)gridDim.x * NWAVES;
     const long long words = ((long long)a.n + 31) / 32;
@@ -1381,19 +1419,22 @@ __global__ __launch_bounds__(BLOCK) void k_bfold(BDenseArgs a, BatchArgs B) {
             mask &= mask - 1;
             const long long v = seg * WAVE + t;
             const uint64_t deg = a.idx[v + 1] - a.idx[v];
-            float d = 0.f, pv = 0.f;
-            if (col_on) {
+            float d = 0.f;
+            bool pw = false;   // a weight row entry of this column is live
+            if (col_on) {      // the pull reads active columns only
                 float* rp = B.r + v * B.K + lane;
                 const float rv = *rp;
                 if (fabsf(rv) > B.eps[lane]) {
                     d = rv;
-                    *rp = 0.f;
-                    B.x[v * B.K + lane] += d;
-                    if (deg) pv = B.damp * d / (float)deg;
+                    const float moved = prow_fold(a.p, (long long)a.rank[v] * a.pk + lane, rv, deg, B.damp);
+                    *rp = rv - moved;
+                    B.x[v * B.K + lane] += moved;
+                    pw = deg != 0;
+                } else if (deg) {
+                    prow_zero(a.p, (long long)a.rank[v] * a.pk + lane);
                 }
             }
-            if (deg && col_on) a.p[v * B.K + lane] = pv;   // the pull reads active columns only
-            const uint64_t fm = __ballot(d != 0.f), pm = __ballot(pv != 0.f);
+            const uint64_t fm = __ballot(d != 0.f), pm = __ballot(pw);
             if (lane == 0) {
                 if (fm) { st[0] += 1; st[1] += (uint64_t)__popcll(fm); }
                 if (pm) { st[2] += deg; st[3] += deg * (uint64_t)__popcll(pm); }
@@ -1406,19 +1447,65 @@ __global__ __launch_bounds__(BLOCK) void k_bfold(BDenseArgs a, BatchArgs B) {
     }
 }
 
-// Sum of the live p rows of in-list entries [b, e) into acc (lane = column), in slot
-// order; at every owner change `flush(owner, acc)` is called.  Eight rows are loaded before
-// they are added, so a wave keeps eight 256-byte gathers in flight.
-template <int Q, typename OwnerOf, typename Flush>
-__device__ __forceinline__ uint32_t bsum_rows(const BDenseArgs& a, const BatchArgs& B, uint64_t b, uint64_t e,
-                                              OwnerOf owner_of, Flush flush, int& cur, float& acc) {
+// The pass's live rows in p-row order (the pull tests in-list entries, which are row
+// numbers): bit q = the snapshot bit of inv[q].  One ballot per 64 rows.
+template <typename PT>
+__global__ __launch_bounds__(BLOCK) void k_blive(BDenseArgs<PT> a) {
+    const int lane = lane_id();
+    const long long waves = (long long)gridDim.x * NWAVES;
+    for (long long w = blockIdx.x * (long long)NWAVES + threadIdx.x / WAVE; w * WAVE < a.nsrc; w += waves) {
+        const long long q = w * WAVE + lane;
+        const uint64_t b = __ballot(q < a.nsrc && bit_of(a.pending, a.inv[q]));
+        if (lane < 2) a.live[2 * w + lane] = (uint32_t)(b >> (32 * lane));
+    }
+}
+
+// Per-owner sums of gathered rows.  BAcc<float>: lane = column.  BAcc<uint16_t>: the two
+// halves of the wave hold different rows' partial sums, lane j of a half columns 2j and
+// 2j + 1 (lo, hi); cols() adds the halves and hands column c to lane c (four shuffles, once
+// per owner).
+template <typename PT> struct BAcc;
+template <> struct BAcc<float> {
+    float v = 0.f;
+    __device__ void clear() { v = 0.f; }
+    __device__ float cols() const { return v; }
+};
+template <> struct BAcc<uint16_t> {
+    float lo = 0.f, hi = 0.f;
+    __device__ void clear() { lo = 0.f; hi = 0.f; }
+    __device__ float cols() const {
+        const int lane = lane_id();
+        const float l = lo + __shfl_xor(lo, 32, WAVE), h = hi + __shfl_xor(hi, 32, WAVE);
+        const float a = __shfl(l, lane >> 1, WAVE), b = __shfl(h, lane >> 1, WAVE);
+        return (lane & 1) ? b : a;
+    }
+};
+
+// Sum of the live p rows of in-list entries [b, e) into acc, in slot order; at every owner
+// change `flush(owner, sums)` is called with lane = column.  owners(e0) gives, lane-parallel,
+// the owner of slot e0 + lane; per row the wave reads that and the row's id with readlane
+// (scalar), so the row's address is scalar and a row costs a few instructions besides its
+// load.  Float rows: one 4K-byte row per wave instruction (lane = column), Q rows loaded
+// before they are added, so a wave keeps Q gathers in flight.  Every load lands in its own
+// register and is used only after the batch is issued (a use inside the load's branch makes
+// the compiler wait for each load in turn).
+template <int Q, typename Owners, typename Start, typename Flush>
+__device__ __forceinline__ uint32_t bsum_rows(const BDenseArgs<float>& a, const BatchArgs& B, uint64_t b, uint64_t e,
+                                              Owners owners, Start start, Flush flush, int& cur, BAcc<float>& acc) {
     const int lane = lane_id();
     const bool col = lane < B.K && ((B.active >> lane) & 1ull);
     uint32_t taken = 0;
+    // windows of 64 entries; the next window's ids and the live-bitmap words of the window
+    // after the current one are loaded while the current window's rows are summed
+    int v = b + lane < e ? a.tadj[b + lane] : -1;
+    uint32_t w = v >= 0 ? a.live[v >> 5] : 0u;
+    int vn = b + WAVE + lane < e ? a.tadj[b + WAVE + lane] : -1;
     for (uint64_t e0 = b; e0 < e; e0 += WAVE) {
-        const uint64_t i = e0 + lane;
-        const int v = i < e ? a.tadj[i] : -1;
-        uint64_t lm = __ballot(v >= 0 && bit_of(a.pending, v));
+        uint64_t lm = __ballot(v >= 0 && ((w >> (v & 31)) & 1u));
+        const uint32_t wn = vn >= 0 ? a.live[vn >> 5] : 0u;
+        const uint64_t e2 = e0 + 2 * WAVE;
+        const int vnn = e2 + lane < e ? a.tadj[e2 + lane] : -1;
+        const int ownv = lm ? owners(e0) : 0;
         while (lm) {
             int slot[Q];
 #pragma unroll
@@ -1431,23 +1518,96 @@ __device__ __forceinline__ uint32_t bsum_rows(const BDenseArgs& a, const BatchAr
             for (int q = 0; q < Q; q++) {
                 val[q] = 0.f;
                 if (slot[q] >= 0) {
-                    const int vv = __shfl(v, slot[q], WAVE);
-                    if (col) val[q] = a.p[(long long)vv * B.K + lane];
+                    const int vv = __builtin_amdgcn_readlane(v, slot[q]);
+                    const float* row = a.p + (long long)vv * a.pk;
+                    if (col) val[q] = row[lane];
                 }
             }
 #pragma unroll
             for (int q = 0; q < Q; q++) {
                 if (slot[q] < 0) break;
-                const int own = owner_of(e0 + (uint64_t)slot[q]);
+                const int own = __builtin_amdgcn_readlane(ownv, slot[q]);
                 if (own != cur) {
-                    if (cur >= 0) flush(cur, acc);
-                    acc = 0.f;
+                    if (cur >= 0) flush(cur, acc.cols());
+                    acc.clear();
                     cur = own;
+                    start(own);
                 }
-                acc += val[q];
+                acc.v += val[q];
                 taken++;
             }
         }
+        v = vn;
+        w = wn;
+        vn = vnn;
+    }
+    return taken;
+}
+
+// bf16 rows: a wave instruction gathers two rows, lanes 0-31 the first (one 4-byte word =
+// two columns per lane) and lanes 32-63 the second, so it has the shape of a float-row
+// gather (64 words, two lines) and carries twice the rows.  Q rows in flight = Q / 2
+// instructions; each half adds its rows to its own partial sums.
+template <int Q, typename Owners, typename Start, typename Flush>
+__device__ __forceinline__ uint32_t bsum_rows(const BDenseArgs<uint16_t>& a, const BatchArgs& B, uint64_t b,
+                                              uint64_t e, Owners owners, Start start, Flush flush, int& cur,
+                                              BAcc<uint16_t>& acc) {
+    const int lane = lane_id(), half = lane >> 5, j = lane & 31;
+    // halves of inactive columns (and of the padding column of an odd K) hold stale weights
+    const bool clo = 2 * j < B.K && ((B.active >> (2 * j)) & 1ull);
+    const bool chi = 2 * j + 1 < B.K && ((B.active >> (2 * j + 1)) & 1ull);
+    const uint32_t wmask = (clo ? 0x0000ffffu : 0u) | (chi ? 0xffff0000u : 0u);
+    const bool col = wmask != 0;
+    uint32_t taken = 0;
+    int v = b + lane < e ? a.tadj[b + lane] : -1;
+    uint32_t w = v >= 0 ? a.live[v >> 5] : 0u;
+    int vn = b + WAVE + lane < e ? a.tadj[b + WAVE + lane] : -1;
+    for (uint64_t e0 = b; e0 < e; e0 += WAVE) {
+        uint64_t lm = __ballot(v >= 0 && ((w >> (v & 31)) & 1u));
+        const uint32_t wn = vn >= 0 ? a.live[vn >> 5] : 0u;
+        const uint64_t e2 = e0 + 2 * WAVE;
+        const int vnn = e2 + lane < e ? a.tadj[e2 + lane] : -1;
+        const int ownv = lm ? owners(e0) : 0;
+        while (lm) {
+            int slot[Q];
+#pragma unroll
+            for (int q = 0; q < Q; q++) {
+                slot[q] = lm ? __ffsll((long long)lm) - 1 : -1;
+                if (lm) lm &= lm - 1;
+            }
+            uint32_t raw[Q / 2];
+#pragma unroll
+            for (int q = 0; q < Q / 2; q++) {
+                raw[q] = 0;
+                const int s0 = slot[2 * q], s1 = slot[2 * q + 1];
+                const int v0 = s0 >= 0 ? __builtin_amdgcn_readlane(v, s0) : 0;
+                const int v1 = s1 >= 0 ? __builtin_amdgcn_readlane(v, s1) : 0;
+                const int sl = half ? s1 : s0;
+                const int vv = half ? v1 : v0;
+                if (sl >= 0 && col)
+                    raw[q] = *reinterpret_cast<const uint32_t*>(a.p + (long long)vv * a.pk + 2 * j);
+            }
+#pragma unroll
+            for (int q = 0; q < Q; q++) {
+                if (slot[q] < 0) break;
+                const int own = __builtin_amdgcn_readlane(ownv, slot[q]);
+                if (own != cur) {
+                    if (cur >= 0) flush(cur, acc.cols());
+                    acc.clear();
+                    cur = own;
+                    start(own);
+                }
+                if (half == (q & 1)) {
+                    const uint32_t wv = raw[q >> 1] & wmask;
+                    acc.lo += __uint_as_float(wv << 16);
+                    acc.hi += __uint_as_float(wv & 0xffff0000u);
+                }
+                taken++;
+            }
+        }
+        v = vn;
+        w = wn;
+        vn = vnn;
     }
     return taken;
 }
@@ -1457,8 +1617,8 @@ __device__ __forceinline__ uint32_t bsum_rows(const BDenseArgs& a, const BatchAr
 // lanes = columns) and adds each destination's sum to its r row once, with a plain
 // read-modify-write (no other wave touches these rows).  The owner of a slot is the last
 // of the 64 list starts at or before it (one ballot).
-template <int Q>
-__global__ __launch_bounds__(BLOCK) void k_bpull(BDenseArgs a, BatchArgs B) {
+template <int Q, typename PT>
+__global__ __launch_bounds__(BLOCK, 6) void k_bpull(BDenseArgs<PT> a, BatchArgs B) {
     __shared__ uint32_t s_col[NWAVES][WAVE];
     const int lane = lane_id(), wid = threadIdx.x / WAVE;
     const long long gw = (blockIdx.x * (long long)BLOCK + threadIdx.x) / WAVE;
@@ -1492,55 +1652,90 @@ __global__ __launch_bounds__(BLOCK) void k_bpull(BDenseArgs a, BatchArgs B) {
         const uint64_t lo = a.tidx[u0], hi = a.tidx[u0 + nt];
         const uint64_t my = lane < nt ? a.tidx[u0 + lane] : hi;   // lanes past the tile never own a slot
         uint64_t tmask = 0;   // destinations of the tile pending in the next pass
+        uint64_t vis = 0;     // destinations of the tile whose row fold_row has tested
+        // lane d: destination d's out-degree and p row, loaded with the tile's list starts so
+        // that a fold waits only for its r and x rows (issued together)
+        const uint64_t degv = fused && lane < nt ? a.idx[u0 + lane + 1] - a.idx[u0 + lane] : 0;
+        const int rankv = fused && lane < nt ? a.rank[u0 + lane] : 0;
         // The next pass's fold of destination u (k_bfold's rule on the row just completed):
         // s = this pass's sum for u (0 for a hub, whose row k_bpull_hub completed).
-        auto fold_row = [&](int own, float s) {
+        // r0, x0: u's r and x rows (lane = column; read by the caller)
+        auto fold_row = [&](int own, float s, float r0, float x0) {
             const long long u = u0 + own;
-            float rv = col_on ? B.r[u * B.K + lane] + s : 0.f;
+            vis |= 1ull << own;
+            float rv = r0 + s;
+            const float xv = x0;
             const bool pend = col_on && fabsf(rv) > eps;
             const uint64_t pm = __ballot(pend);
             if (!pm) {
                 if (col_on && s != 0.f) B.r[u * B.K + lane] = rv;
                 return;
             }
-            const uint64_t deg = a.idx[u + 1] - a.idx[u];
-            float pv = 0.f;
+            const uint64_t deg = (uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)degv, own) |
+                                 (uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(degv >> 32), own) << 32;
+            const long long prow = (long long)__builtin_amdgcn_readlane(rankv, own) * a.pk + lane;
+            bool pw = false;
             if (pend) {
-                B.x[u * B.K + lane] += rv;
-                if (deg) pv = B.damp * rv / (float)deg;
-                rv = 0.f;
+                const float moved = prow_fold(a.p_next, prow, rv, deg, B.damp);
+                B.x[u * B.K + lane] = xv + moved;
+                rv -= moved;
+                pw = deg != 0;
+            } else if (col_on && deg) {
+                prow_zero(a.p_next, prow);
             }
-            if (col_on) {
-                B.r[u * B.K + lane] = rv;
-                if (deg) a.p_next[u * B.K + lane] = pv;
-            }
+            if (col_on) B.r[u * B.K + lane] = rv;
             tmask |= 1ull << own;
             colcnt += pend ? 1u : 0u;
-            const uint64_t vm = __ballot(pv != 0.f);
+            const uint64_t vm = __ballot(pw);
             fs[0] += 1;
             fs[1] += (uint64_t)__popcll(pm);
             if (vm) { fs[2] += deg; fs[3] += deg * (uint64_t)__popcll(vm); }
         };
         int cur = -1;
-        float acc = 0.f;
-        auto owner_of = [&](uint64_t pos) { return __popcll(__ballot(my <= pos)) - 1; };
+        BAcc<PT> acc;
+        // owner of slot e0 + lane: the destinations whose lists start at or before e0, plus
+        // one per list that starts inside the window at or before the lane's slot
+        auto owners = [&](uint64_t e0) {
+            int ow = __popcll(__ballot(my <= e0)) - 1;
+            uint64_t sm = __ballot(my > e0 && my < e0 + WAVE);
+            while (sm) {
+                const int d = __ffsll((long long)sm) - 1;
+                sm &= sm - 1;
+                const int off = __builtin_amdgcn_readlane((int)(my - e0), d);
+                ow += lane >= off ? 1 : 0;
+            }
+            return ow;
+        };
+        // fused: the r and x rows of a destination are loaded when its first row is added
+        // (start), so they have arrived when its sum is complete (flush)
+        float rpre = 0.f, xpre = 0.f;
+        auto start = [&](int own) {
+            if (fused && col_on) {
+                rpre = B.r[(u0 + own) * B.K + lane];
+                xpre = B.x[(u0 + own) * B.K + lane];
+            }
+        };
         auto flush = [&](int own, float sum) {
             if (fused) {
-                if (__ballot(sum != 0.f)) fold_row(own, sum);
+                if (__ballot(sum != 0.f)) fold_row(own, sum, rpre, xpre);
             } else if (lane < B.K && sum != 0.f) {
                 B.r[(u0 + own) * B.K + lane] += sum;
             }
         };
-        bsum_rows<Q>(a, B, lo, hi, owner_of, flush, cur, acc);
-        if (cur >= 0) flush(cur, acc);
+        bsum_rows<Q>(a, B, lo, hi, owners, start, flush, cur, acc);
+        if (cur >= 0) flush(cur, acc.cols());
         if (!fused) continue;
         // hub destinations of the tile, then the tile's two bitmap words and counts; a
-        // destination that received nothing stays below eps (it was folded or was not pending)
+        // destination that received nothing keeps its row: below eps if it was not folded
+        // before this pass; with bf16 weights a vertex folded before this pass may hold a
+        // rounding remainder above eps, so those are tested too
         uint64_t hm = __ballot(lane < nt && bit_of(a.hubbits, (int)(u0 + lane)));
+        if (sizeof(PT) == 2) hm |= __ballot(lane < nt && bit_of(a.pending, (int)(u0 + lane))) & ~vis;
         while (hm) {
             const int i = __ffsll((long long)hm) - 1;
             hm &= hm - 1;
-            fold_row(i, 0.f);
+            const long long u = u0 + i;
+            fold_row(i, 0.f, col_on ? B.r[u * B.K + lane] : 0.f, col_on ? B.x[u * B.K + lane] : 0.f);
         }
         if (lane == 0) {
             a.bitmap_next[2 * t] = (uint32_t)tmask;
@@ -1548,7 +1743,7 @@ __global__ __launch_bounds__(BLOCK) void k_bpull(BDenseArgs a, BatchArgs B) {
         }
         if (!tmask) continue;
         const bool pv = (tmask >> lane) & 1ull;
-        const uint64_t dv = pv ? a.idx[u0 + lane + 1] - a.idx[u0 + lane] : 0;
+        const uint64_t dv = pv ? degv : 0;
         const int c0 = chunk_of((int)u0, a.cv0, a.nc), c1 = chunk_of((int)(u0 + nt - 1), a.cv0, a.nc);
         if (c0 == c1) {
             if (c0 != cc) { flush_counts(); cc = c0; }
@@ -1576,8 +1771,8 @@ __global__ __launch_bounds__(BLOCK) void k_bpull(BDenseArgs a, BatchArgs B) {
         for (int i = 0; i < 4; i++) if (fs[i]) atomicAdd(&slot[S_VERT + i], (unsigned long long)fs[i]);
     }
 }
-template <int Q>
-__global__ __launch_bounds__(BLOCK) void k_bpull_hub(BDenseArgs a, BatchArgs B, const PullPiece* __restrict__ pieces,
+template <int Q, typename PT>
+__global__ __launch_bounds__(BLOCK) void k_bpull_hub(BDenseArgs<PT> a, BatchArgs B, const PullPiece* __restrict__ pieces,
                                                       int npieces) {
     const int lane = lane_id();
     const long long nw = (long long)gridDim.x * NWAVES;
@@ -1585,11 +1780,13 @@ __global__ __launch_bounds__(BLOCK) void k_bpull_hub(BDenseArgs a, BatchArgs B, 
     for (long long h = (blockIdx.x * (long long)BLOCK + threadIdx.x) / WAVE; h < npieces; h += nw) {
         const PullPiece pc = pieces[h];
         int cur = -1;
-        float acc = 0.f;
-        auto owner_of = [](uint64_t) { return 0; };
+        BAcc<PT> acc;
+        auto owners = [](uint64_t) { return 0; };
+        auto start = [](int) {};
         auto flush = [](int, float) {};
-        taken += bsum_rows<Q>(a, B, pc.start, pc.start + pc.len, owner_of, flush, cur, acc);
-        if (lane < B.K && acc != 0.f) atomic_add_f32(&B.r[(long long)pc.u * B.K + lane], acc);
+        taken += bsum_rows<Q>(a, B, pc.start, pc.start + pc.len, owners, start, flush, cur, acc);
+        const float sum = acc.cols();
+        if (lane < B.K && sum != 0.f) atomic_add_f32(&B.r[(long long)pc.u * B.K + lane], sum);
     }
     if (lane == 0 && taken) atomicAdd(a.live_count, (unsigned long long)taken);
 }
@@ -1660,6 +1857,7 @@ struct Ctx {
     int32_t* d_tadj = nullptr;
     float* d_p = nullptr;             // [n][S]
     int32_t* d_rank = nullptr;          // p row of each vertex (out-degree order)
+    int32_t* d_inv = nullptr;           // batched: vertex of each p row
     uint32_t* d_pbits = nullptr;        // live-row bitmap of p, 2 * ceil(nsrc / 64) words
     int nsrc = 0;                       // vertices with out-edges (rows of p)
     int pull_live = 2;                  // live-row bitmap: 0 never, 1 always, 2 when demand <= frac * m
@@ -1673,6 +1871,8 @@ struct Ctx {
     uint32_t* d_hubbits = nullptr;      // batched: destinations summed by k_bpull_hub
     int pull_seq = 8;                   // in-list entries per lane in k_pull (4, 8, 16 or 32)
     int bpull_q = 8;                    // batched pull: rows in flight per wave (8 or 16)
+    // batched dense passes: p rows in bf16 (2 bytes per column; EXC_BATCH_ROWS=f32: float)
+    bool brows16 = true;
     // Batched pull tuner: the first two dense passes whose demand is at least half the edges
     // run with 8 and then 16 rows in flight and are timed with events; later passes use the
     // faster one.  The rows are added in slot order either way, so the choice changes the
@@ -1976,23 +2176,26 @@ void bq_collect(Ctx* c) {
 }
 
 // Batched dense pass: fold the snapshot's pending vertices, hub pieces, then the pull.
-void launch_bfold_pull(Ctx* c, hipStream_t st) {
-    bq_collect(c);
-    const bool trial = c->bq_tune && (double)c->last_demand >= 0.5 * (double)c->m;
-    if (trial) c->bpull_q = BQ_CAND[c->bq_trial];
+template <typename PT>
+void launch_bfold_pull_rows(Ctx* c, hipStream_t st, bool fuse, bool trial) {
     unsigned long long* row = c->d_counters + C_DENSE_LIVE * C_FIELDS;
-    const bool fuse = c->fuse_fold && c->d_p2 && c->d_bitmap2 && !c->d_trace &&
-                      (double)c->last_demand >= std::max(0.1, 2.0 * c->pull_threshold) * (double)c->m;
-    BDenseArgs a{c->n, c->d_idx, c->d_bitmap, c->d_p, c->d_tidx, c->d_tadj, row, row + C_PULL_HUB_LIVE,
-                 fuse ? c->d_p2 : nullptr, c->d_bitmap2, c->d_hubbits, c->d_cv0, c->nc, c->d_cnt,
-                 c->d_counters + C_SPREAD};
+    BDenseArgs<PT> a{c->n, c->d_idx, c->d_bitmap, reinterpret_cast<PT*>(c->d_p), prow_stride(c->K, sizeof(PT) == 2),
+                     c->d_rank, c->d_inv, c->d_pbits, c->nsrc, c->d_tidx, c->d_tadj, row,
+                     row + C_PULL_HUB_LIVE, fuse ? reinterpret_cast<PT*>(c->d_p2) : nullptr, c->d_bitmap2,
+                     c->d_hubbits, c->d_cv0, c->nc, c->d_cnt, c->d_counters + C_SPREAD};
     const BatchArgs B = batch_args(c, -1);
     const long long segs = ((long long)c->n + WAVE - 1) / WAVE;
     const unsigned cap = (unsigned)c->cus * 8;
     if (!c->folded) {   // else the last pull already folded this pass into p
         ScopedTimer t(c, st, T_FOLD);
-        hipLaunchKernelGGL(k_bfold, dim3((unsigned)std::min<long long>((segs + NWAVES - 1) / NWAVES, cap)), dim3(BLOCK), 0,
-                           st, a, B);
+        hipLaunchKernelGGL(k_bfold<PT>, dim3((unsigned)std::min<long long>((segs + NWAVES - 1) / NWAVES, cap)),
+                           dim3(BLOCK), 0, st, a, B);
+    }
+    {
+        ScopedTimer t(c, st, T_FOLD);
+        const long long rw = ((long long)c->nsrc + WAVE - 1) / WAVE;
+        hipLaunchKernelGGL(k_blive<PT>, dim3((unsigned)std::max<long long>(1, std::min<long long>((rw + NWAVES - 1) / NWAVES, cap))),
+                           dim3(BLOCK), 0, st, a);
     }
     if (fuse) {   // the next snapshot's counts are accumulated by k_bpull
         const int ncnt = 2 * c->nc + 1;
@@ -2003,15 +2206,29 @@ void launch_bfold_pull(Ctx* c, hipStream_t st) {
     if (c->n_pull_hubs) {
         ScopedTimer t(c, st, T_PULL_HUB);
         const dim3 g((unsigned)std::min<long long>((c->n_pull_hubs + NWAVES - 1) / NWAVES, cap));
-        if (c->bpull_q == 16) hipLaunchKernelGGL(k_bpull_hub<16>, g, dim3(BLOCK), 0, st, a, B, c->d_pull_hubs, c->n_pull_hubs);
-        else hipLaunchKernelGGL(k_bpull_hub<8>, g, dim3(BLOCK), 0, st, a, B, c->d_pull_hubs, c->n_pull_hubs);
+        // bf16 hub pieces: 8 rows in flight (16 measured 2.4x slower there, while the pull
+        // itself gains from 16)
+        if (c->bpull_q == 16 && sizeof(PT) == 4)
+            hipLaunchKernelGGL((k_bpull_hub<16, PT>), g, dim3(BLOCK), 0, st, a, B, c->d_pull_hubs, c->n_pull_hubs);
+        else
+            hipLaunchKernelGGL((k_bpull_hub<8, PT>), g, dim3(BLOCK), 0, st, a, B, c->d_pull_hubs, c->n_pull_hubs);
     }
     {
         ScopedTimer t(c, st, T_PULL);
         const dim3 g((unsigned)std::min<long long>((segs + NWAVES - 1) / NWAVES, 2 * cap));
-        if (c->bpull_q == 16) hipLaunchKernelGGL(k_bpull<16>, g, dim3(BLOCK), 0, st, a, B);
-        else hipLaunchKernelGGL(k_bpull<8>, g, dim3(BLOCK), 0, st, a, B);
+        if (c->bpull_q == 16) hipLaunchKernelGGL((k_bpull<16, PT>), g, dim3(BLOCK), 0, st, a, B);
+        else hipLaunchKernelGGL((k_bpull<8, PT>), g, dim3(BLOCK), 0, st, a, B);
     }
+}
+
+void launch_bfold_pull(Ctx* c, hipStream_t st) {
+    bq_collect(c);
+    const bool trial = c->bq_tune && (double)c->last_demand >= 0.5 * (double)c->m;
+    if (trial) c->bpull_q = BQ_CAND[c->bq_trial];
+    const bool fuse = c->fuse_fold && c->d_p2 && c->d_bitmap2 && !c->d_trace &&
+                      (double)c->last_demand >= std::max(0.1, 2.0 * c->pull_threshold) * (double)c->m;
+    if (c->brows16) launch_bfold_pull_rows<uint16_t>(c, st, fuse, trial);
+    else launch_bfold_pull_rows<float>(c, st, fuse, trial);
     if (trial) {
         hipEventRecord(c->bq_ev[1], st);
         c->bq_recorded = true;
@@ -2225,7 +2442,7 @@ int exc_destroy(void* p) {
     for (auto s : c->st) if (s) hipStreamDestroy(s);
     for (auto q : c->prows) if (q) (void)hipFree(q);
     void* pull_ptrs[] = {c->d_tidx, c->d_tadj, c->d_p, c->d_pull_hubs, c->d_wts, c->d_twts, c->d_rank, c->d_pbits, c->d_p2,
-                         c->d_bitmap2, c->d_hubbits};
+                         c->d_bitmap2, c->d_hubbits, c->d_inv};
     for (void* q : pull_ptrs) if (q) (void)hipFree(q);
     if (c->d_beps) (void)hipFree(c->d_beps);
     if (c->d_colpend) (void)hipFree(c->d_colpend);
@@ -2760,6 +2977,7 @@ int exc_get_stats(void* p, exc_stats* o) {
         const unsigned long long* r4 = h + C_DENSE * C_FIELDS;
         const unsigned long long* r5 = h + C_DENSE_LIVE * C_FIELDS;
         const int nm = c->K ? c->K : c->nm;
+        const double pb = c->K && c->brows16 ? 2.0 : 4.0;   // bytes per column of a p row
         const double P = (double)c->pull_passes, PL = (double)c->pull_passes_live;
         o->fold_ms = c->t_ms[T_FOLD];
         o->pull_ms = c->t_ms[T_PULL];
@@ -2769,17 +2987,18 @@ int exc_get_stats(void* p, exc_stats* o) {
         // folded ones
         o->fold_bytes = P * c->n * 4.0 * nm + P * c->nsrc * 4.0 * nm + 16.0 * (r4[C_VERT] + r5[C_VERT]) +
                         8.0 * (r4[C_VERT_NM] + r5[C_VERT_NM]);
-        if (c->K)   // batched: the bitmap, then per pending vertex its offsets, r row and p row,
-                    // per folded column r written and x read + written
-            o->fold_bytes = P * 4.0 * (((double)c->n + 31) / 32) + (double)r5[C_VERT] * (16.0 + 8.0 * nm) +
-                            12.0 * (double)r5[C_VERT_NM];
+        if (c->K)   // batched: the bitmap, then per pending vertex its offsets, p-row number, r row
+                    // and p row, per folded column r written and x read + written; per pass the
+                    // live-row bitmap from the row->vertex map
+            o->fold_bytes = P * 4.0 * (((double)c->n + 31) / 32) + (double)r5[C_VERT] * (20.0 + (4.0 + pb) * nm) +
+                            12.0 * (double)r5[C_VERT_NM] + P * (4.0 * c->nsrc + c->nsrc / 8.0);
         // pull: every in-edge's id, every vertex's offsets and r (read + write), and the p rows
         // read: every in-edge's in a plain pass; in a pass with the live-row bitmap, the
         // bitmap plus one row per out-edge of a pushing source (the fold's edge count -- a
         // pushed weight is zero only on underflow), split between the kernels by
         // k_pull_hub's own count.  In-lists above the hub threshold go to the hub kernel.
         const double me = (double)(c->m - c->pull_hub_edges), nv = (double)(c->n - c->pull_hub_vertices);
-        const double bm = c->K ? 4.0 * (((double)c->n + 31) / 32) : 8.0 * ((c->nsrc + 63) / 64);
+        const double bm = 8.0 * ((c->nsrc + 63) / 64);
         unsigned long long sp[5] = {0, 0, 0, 0, 0};
         for (int q = 0; q < SPREAD_SLOTS; q++)
             for (int f = 0; f < 5; f++) sp[f] += h[C_SPREAD + q * SPREAD_STRIDE + f];
@@ -2787,9 +3006,9 @@ int exc_get_stats(void* p, exc_stats* o) {
         // k_pull counts the rows it reads (batched passes count them through the fold)
         const double rows = c->K ? std::max(0.0, (double)(r5[C_EDGES] + sp[S_EDGES]) - (double)r5[C_PULL_HUB_LIVE])
                                  : (double)sp[S_ROWS];
-        o->pull_bytes = P * (me * 4.0 + nv * (8.0 + 8.0 * nm)) + PL * bm + 4.0 * nm * rows;
+        o->pull_bytes = P * (me * 4.0 + nv * (8.0 + 8.0 * nm)) + PL * bm + pb * nm * rows;
         o->pull_hub_bytes = P * ((double)c->pull_hub_edges * 4.0 + (double)c->pull_hub_vertices * (8.0 + 8.0 * nm)) +
-                            PL * bm + 4.0 * nm * hub_rows;
+                            PL * bm + pb * nm * hub_rows;
         for (const unsigned long long* r : {r4, r5}) {
             o->pushed_vertices += r[C_VERT];
             o->pushed_vertex_measures += r[C_VERT_NM];
@@ -2833,9 +3052,9 @@ int exc_enable_pull(void* p, double threshold) {
     }
     if (c->d_tadj) return EXC_OK;
     const size_t n = (size_t)c->n;
-    // plain: rows of p in decreasing out-degree order (ties by id), vertices without
-    // out-edges last; batched: p rows by vertex id (a row is 4K bytes, whole lines already)
-    const bool by_id = c->K > 0;
+    // rows of p in decreasing out-degree order (ties by id), vertices without out-edges
+    // last: the most-read rows share cache lines (plain) and stay in the caches (batched)
+    const bool by_id = false;
     std::vector<uint64_t> hidx(n + 1);
     HIPCHK(hipMemcpy(hidx.data(), c->d_idx, (n + 1) * sizeof(uint64_t), hipMemcpyDeviceToHost));
     std::vector<int32_t> rank(by_id ? 0 : n);
@@ -2855,18 +3074,29 @@ int exc_enable_pull(void* p, double threshold) {
     ok = ok && hipMalloc(&c->d_pbits, (2 * (((size_t)c->nsrc + 63) / 64) + 2) * sizeof(uint32_t)) == hipSuccess;
     ok = ok && hipMalloc(&c->d_tadj, (c->m ? c->m : 1) * sizeof(int32_t)) == hipSuccess;
     if (ok && c->d_wts) ok = hipMalloc(&c->d_twts, (c->m ? c->m : 1) * sizeof(float)) == hipSuccess;
-    ok = ok && hipMalloc(&c->d_p, n * c->S * sizeof(float)) == hipSuccess;
-    if (ok && c->fuse_fold) ok = hipMalloc(&c->d_p2, n * c->S * sizeof(float)) == hipSuccess;
-    if (ok && by_id && c->fuse_fold) {
+    if (c->K) {   // EXC_BATCH_ROWS=f32: float weight rows in batched dense passes (measurements)
+        const char* br = getenv("EXC_BATCH_ROWS");
+        c->brows16 = !(br && !strcmp(br, "f32"));
+    }
+    const bool b16 = c->K && c->brows16;
+    const size_t prow_bytes = (size_t)(c->K ? prow_stride(c->K, b16) : c->S) * (b16 ? sizeof(uint16_t) : sizeof(float));
+    ok = ok && hipMalloc(&c->d_p, n * prow_bytes) == hipSuccess;
+    ok = ok && hipMemset(c->d_p, 0, n * prow_bytes) == hipSuccess;
+    if (ok && c->fuse_fold) ok = hipMalloc(&c->d_p2, n * prow_bytes) == hipSuccess;
+    if (ok && c->fuse_fold) ok = hipMemset(c->d_p2, 0, n * prow_bytes) == hipSuccess;
+    if (ok && c->K) ok = hipMalloc(&c->d_inv, std::max<size_t>(1, (size_t)c->nsrc) * sizeof(int32_t)) == hipSuccess;
+    if (ok && c->K && c->fuse_fold) {
         ok = hipMalloc(&c->d_bitmap2, ((n + 63) / 64) * 2 * sizeof(uint32_t)) == hipSuccess;
         ok = ok && hipMalloc(&c->d_hubbits, ((n + 63) / 64) * 2 * sizeof(uint32_t)) == hipSuccess;
     }
     ok = ok && hipMalloc(&cnt, (n + 1) * sizeof(unsigned long long)) == hipSuccess;
     if (!ok) {
         (void)hipGetLastError();
-        void* q[] = {c->d_tidx, c->d_tadj, c->d_p, cnt, c->d_rank, c->d_pbits, c->d_p2, c->d_bitmap2, c->d_hubbits, c->d_twts};
+        void* q[] = {c->d_tidx, c->d_tadj, c->d_p, cnt, c->d_rank, c->d_pbits, c->d_p2, c->d_bitmap2, c->d_hubbits, c->d_twts,
+                     c->d_inv};
         for (void* x : q) if (x) (void)hipFree(x);
         c->d_tidx = nullptr; c->d_tadj = nullptr; c->d_p = nullptr; c->d_rank = nullptr; c->d_pbits = nullptr;
+        c->d_inv = nullptr;
         c->d_p2 = nullptr; c->d_bitmap2 = nullptr; c->d_hubbits = nullptr; c->d_twts = nullptr;
         c->nsrc = 0;
         c->pull_threshold = -1;
@@ -2877,6 +3107,12 @@ int exc_enable_pull(void* p, double threshold) {
         return fail(EXC_EHIP, "exc_enable_pull: %s", hipGetErrorString(e));
     };
     hipError_t e = by_id ? hipSuccess : hipMemcpy(c->d_rank, rank.data(), n * sizeof(int32_t), hipMemcpyHostToDevice);
+    if (e == hipSuccess && c->d_inv) {
+        std::vector<int32_t> inv(std::max<size_t>(1, (size_t)c->nsrc), 0);
+        for (size_t v = 0; v < n; v++)
+            if (rank[v] < c->nsrc) inv[rank[v]] = (int32_t)v;
+        e = hipMemcpy(c->d_inv, inv.data(), inv.size() * sizeof(int32_t), hipMemcpyHostToDevice);
+    }
     if (e == hipSuccess) e = hipMemset(cnt, 0, (n + 1) * sizeof(unsigned long long));
     if (e != hipSuccess) return fail_free(e);
     hipLaunchKernelGGL(k_count_in64, dim3(c->cus * 16), dim3(BLOCK), 0, 0, c->m, c->d_arena, cnt);

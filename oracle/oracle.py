@@ -267,3 +267,42 @@ def residual_l1(idx, adj, kind, scale, beta, x):
     n = len(idx) - 1
     return lib().orc_residual_l1(n, _p(np.ascontiguousarray(idx, np.uint64)), _p(np.ascontiguousarray(adj, np.int32)),
                                  kind, scale, beta, _p(np.ascontiguousarray(x, np.float32)))
+
+
+def bf16_round(a):
+    """float32 -> the nearest bfloat16 (ties to even), returned as float32: the high half of
+    the float's bits after adding 0x7fff plus the lowest kept bit."""
+    u = np.ascontiguousarray(a, np.float32).view(np.uint32).astype(np.uint64)
+    u = ((u + 0x7FFF + ((u >> 16) & 1)) >> 16) << 16
+    return u.astype(np.uint32).view(np.float32)
+
+
+def dense_pass_rows(idx, adj, x, r, eps, damp=0.85, rows="bf16", feedback=True):
+    """One dense personalised-PageRank pass (fold of every pending vertex, then every vertex
+    adds the weights of its in-neighbours) on one column, in place on float32 x and r.
+
+    The delta push of src/excent.cu:100-124 (pending iff |r| > eps, x += d, r = 0, weight
+    damp d / deg to each out-neighbour, nothing pushed for deg = 0) with all folds before all
+    adds.  rows="bf16": the weight is stored in bfloat16 and, with feedback, the fold moves
+    only the delta the stored weight accounts for, bf16(w) deg / damp, into x and leaves the
+    rounding remainder in r; feedback=False moves all of d (the remainder is lost).  Returns
+    the number of pending vertices folded."""
+    deg = np.diff(np.asarray(idx, np.int64))
+    n = len(deg)
+    damp = np.float32(damp)
+    pend = np.abs(r) > np.float32(eps)
+    d = np.where(pend, r, np.float32(0)).astype(np.float32)
+    nz = pend & (deg > 0)
+    w = np.zeros(n, np.float32)
+    w[nz] = damp * d[nz] / deg[nz].astype(np.float32)
+    moved = d.copy()
+    if rows == "bf16":
+        w = bf16_round(w)
+        if feedback:
+            moved[nz] = w[nz] * deg[nz].astype(np.float32) / damp
+    x += moved
+    r -= moved
+    acc = np.zeros(n, np.float32)
+    np.add.at(acc, np.asarray(adj), w[np.repeat(np.arange(n), deg)])
+    r += acc
+    return int(pend.sum())

@@ -208,16 +208,23 @@ def test_batched_pull_rows_fixed_by_env(rmat12, monkeypatch):
         assert np.abs(X[j] - ref).sum() <= bound * (1 + 1e-6) + 1e-6, j
 
 
+@pytest.mark.parametrize("rows", ["bf16", "f32"])
 @pytest.mark.parametrize("fuse", [0, 1])
 @pytest.mark.parametrize("gname", ["rmat12", "star"])
-def test_batched_dense_passes(gname, fuse, rmat12, monkeypatch):
+def test_batched_dense_passes(gname, fuse, rows, rmat12, monkeypatch):
     """Batched fold + pull over the transpose (every pass dense): each column within the
-    exact FP64 bound, per-vertex |r| <= eps at the end, and in-lists above the hub threshold
-    (the star's centre has 9000 in-neighbours) summed by pieces.  fuse = 0: the fold stays
-    in k_bfold and the snapshot after every pass is checked bit for bit against the
-    oracle's; fuse = 1: k_bpull also computes the next snapshot and fold (the state read
-    back after such a pass is already folded, so only the end results are checked)."""
+    exact FP64 bound, per-vertex |r| <= eps at the end, the residual invariant
+    r = b + W x - x per vertex within 1e-4 |x| + eps (float rounding reaches 2e-5 |x| at the
+    star's centre; bf16 weight rows keep their rounding remainder in r, and without that the
+    oracle's run misses this tolerance by 7x or more), and in-lists above the hub
+    threshold (the star's centre has 9000 in-neighbours) summed by pieces.  fuse = 0: the
+    fold stays in k_bfold and the snapshot after every pass is checked bit for bit against
+    the oracle's; fuse = 1: k_bpull also computes the next snapshot and fold (the state read
+    back after such a pass is already folded, so only the end results are checked).
+    rows: the weight rows' storage (EXC_BATCH_ROWS)."""
     monkeypatch.setenv("EXC_FUSE_FOLD", str(fuse))
+    monkeypatch.setenv("EXC_BATCH_ROWS", rows)
+    print(f"case batched dense passes {gname} fuse={fuse} rows={rows}", flush=True)
     if gname == "star":
         L = 9000
         deg = np.concatenate([[L], np.ones(L, np.int64)])
@@ -271,5 +278,8 @@ def test_batched_dense_passes(gname, fuse, rmat12, monkeypatch):
         np.add.at(b, seeds[j], np.float64(mass))
         ref, _, _ = orc.fp64_solve_b(idx, adj, b)
         err = np.abs(X[j] - ref).sum()
-        bound = np.abs(true_residual(idx, adj, X[j], b)).sum() / 0.15
+        true = true_residual(idx, adj, X[j], b)
+        bound = np.abs(true).sum() / 0.15
         assert err <= bound * (1 + 1e-6) + 1e-6, (j, err, bound)
+        dev = np.abs(R[j] - true) - (1e-4 * np.abs(X[j]) + eps)
+        assert dev.max() <= 0, (j, int(dev.argmax()), dev.max())
