@@ -778,7 +778,7 @@ constexpr int PULL_LD = BLOCK + 1;           // LDS row pitch: slot j at (j % SE
 
 // W: weighted graph -- each row is scaled by its in-edge's coefficient (staged with the ids).
 template <int NM, int S, int PULL_SEQ, bool W = false>
-__global__ __launch_bounds__(BLOCK) void k_pull(PullArgs a, DevMeasures M) {
+__global__ __launch_bounds__(BLOCK, 8) void k_pull(PullArgs a, DevMeasures M) {
     constexpr int PULL_WIN = BLOCK * PULL_SEQ;   // 2048 slots (8 KB of LDS) at PULL_SEQ 8
     __shared__ uint32_t s_start[BLOCK + 1];
     __shared__ float s_w[W ? PULL_SEQ * PULL_LD : 1];
@@ -794,13 +794,33 @@ __global__ __launch_bounds__(BLOCK) void k_pull(PullArgs a, DevMeasures M) {
     __syncthreads();
     const uint32_t total = s_start[nt];
     uint32_t taken = 0;   // p rows read by this lane
+    // Thread tid stages slots tid + k BLOCK (k < PULL_SEQ) of each window.  The next window's
+    // ids (and coefficients) are loaded into registers while this window's rows are gathered,
+    // so a window waits for one round trip to memory, not two.
+    static_assert(BLOCK % PULL_SEQ == 0, "slot layout");
+    int32_t nid[PULL_SEQ];
+    float nwt[W ? PULL_SEQ : 1];
+    auto fetch = [&](uint32_t w0) {
+#pragma unroll
+        for (int k = 0; k < PULL_SEQ; k++) {
+            const uint32_t j = w0 + (uint32_t)tid + (uint32_t)k * BLOCK;
+            nid[k] = j < total ? a.tadj[base + j] : 0;
+            if constexpr (W) nwt[k] = j < total ? a.tw[base + j] : 0.f;
+        }
+    };
+    fetch(0);
     for (uint32_t w0 = 0; w0 < total; w0 += PULL_WIN) {
         const uint32_t wn = total - w0 < (uint32_t)PULL_WIN ? total - w0 : (uint32_t)PULL_WIN;
-        for (uint32_t j = tid; j < wn; j += BLOCK) {
-            s_src[(j % PULL_SEQ) * PULL_LD + j / PULL_SEQ] = a.tadj[base + w0 + j];
-            if constexpr (W) s_w[(j % PULL_SEQ) * PULL_LD + j / PULL_SEQ] = a.tw[base + w0 + j];
+#pragma unroll
+        for (int k = 0; k < PULL_SEQ; k++) {
+            const uint32_t j = (uint32_t)tid + (uint32_t)k * BLOCK;
+            if (j < wn) {
+                s_src[(j % PULL_SEQ) * PULL_LD + j / PULL_SEQ] = nid[k];
+                if constexpr (W) s_w[(j % PULL_SEQ) * PULL_LD + j / PULL_SEQ] = nwt[k];
+            }
         }
         __syncthreads();
+        if (w0 + PULL_WIN < total) fetch(w0 + PULL_WIN);
         const uint32_t j0 = (uint32_t)tid * PULL_SEQ;
         if (j0 < wn) {
             const uint32_t jn = wn - j0 < (uint32_t)PULL_SEQ ? wn - j0 : (uint32_t)PULL_SEQ;
