@@ -999,6 +999,9 @@ __global__ __launch_bounds__(BLOCK, 8) void k_pull(PullArgs a, DevMeasures M) {
 // k_pull in the pass; the two kernels touch disjoint rows of r.
 struct PullPiece { int32_t u; uint32_t len; uint64_t start; };
 constexpr uint32_t PULL_PIECE = 8192;
+// Batched contexts sum a piece with one wave (k_bpull_hub), so their pieces are shorter: a
+// wave's rows are a serial chain, and more pieces keep more waves busy.
+constexpr uint32_t BPULL_PIECE = 1024;
 constexpr int HUB_UNROLL = 8;
 
 template <int NM, int S>
@@ -1415,6 +1418,7 @@ struct BDenseArgs {
     int nc;
     unsigned long long* cnt;      // [2 nc] per-chunk pending vertices and degree sums
     unsigned long long* spread;   // C_SPREAD area of the counters
+    unsigned long long* tile_next;   // k_bpull: next unclaimed tile (zeroed before the launch)
 };
 
 __device__ __forceinline__ bool bit_of(const uint32_t* __restrict__ bm, int v) {
@@ -1637,6 +1641,8 @@ __device__ __forceinline__ uint32_t bsum_rows(const BDenseArgs<uint16_t>& a, con
 // lanes = columns) and adds each destination's sum to its r row once, with a plain
 // read-modify-write (no other wave touches these rows).  The owner of a slot is the last
 // of the 64 list starts at or before it (one ballot).
+constexpr long long BPULL_CLAIM = 4;   // tiles of 64 destinations a wave claims at a time
+
 template <int Q, typename PT>
 __global__ __launch_bounds__(BLOCK, 6) void k_bpull(BDenseArgs<PT> a, BatchArgs B) {
     __shared__ uint32_t s_col[NWAVES][WAVE];
@@ -1648,12 +1654,10 @@ __global__ __launch_bounds__(BLOCK, 6) void k_bpull(BDenseArgs<PT> a, BatchArgs 
     const bool fused = a.p_next != nullptr;
     const bool col_on = lane < B.K && ((B.active >> lane) & 1ull);
     const float eps = lane < B.K ? B.eps[lane] : 0.f;
-    // fused: each wave takes a contiguous run of tiles, so the per-chunk counts of the next
-    // snapshot accumulate in registers; otherwise tiles are dealt round robin
-    const long long per = fused ? (ntile + nw - 1) / nw : 1;
-    const long long t0 = fused ? gw * per : gw;
-    const long long t1 = fused ? (t0 + per < ntile ? t0 + per : ntile) : ntile;
-    const long long step = fused ? 1 : nw;
+    // waves claim runs of BPULL_CLAIM consecutive tiles from a counter (in-degree sums vary
+    // widely between tiles); within a run the per-chunk counts of the next snapshot
+    // accumulate in registers
+    (void)nw;
     uint32_t colcnt = 0;                  // pending entries of this lane's column (next pass)
     uint64_t fs[4] = {0, 0, 0, 0};        // fused fold statistics (wave-uniform)
     int cc = -1;
@@ -1666,7 +1670,13 @@ __global__ __launch_bounds__(BLOCK, 6) void k_bpull(BDenseArgs<PT> a, BatchArgs 
         acc_n = 0;
         acc_d = 0;
     };
-    for (long long t = t0; t < t1; t += step) {
+    for (;;) {
+    unsigned long long claim = 0;
+    if (lane == 0) claim = atomicAdd(a.tile_next, (unsigned long long)BPULL_CLAIM);
+    const long long t0 = (long long)__shfl(claim, 0, WAVE);
+    if (t0 >= ntile) break;
+    const long long t1 = t0 + BPULL_CLAIM < ntile ? t0 + BPULL_CLAIM : ntile;
+    for (long long t = t0; t < t1; t++) {
         const long long u0 = t * WAVE;
         const int nt = (int)((long long)a.n - u0 < WAVE ? (long long)a.n - u0 : WAVE);
         const uint64_t lo = a.tidx[u0], hi = a.tidx[u0 + nt];
@@ -1775,6 +1785,7 @@ __global__ __launch_bounds__(BLOCK, 6) void k_bpull(BDenseArgs<PT> a, BatchArgs 
             atomicAdd(&a.cnt[2 * c + 1], (unsigned long long)dv);
         }
     }
+    }
     if (!fused) return;
     flush_counts();
     s_col[wid][lane] = colcnt;
@@ -1878,6 +1889,7 @@ struct Ctx {
     float* d_p = nullptr;             // [n][S]
     int32_t* d_rank = nullptr;          // p row of each vertex (out-degree order)
     int32_t* d_inv = nullptr;           // batched: vertex of each p row
+    unsigned long long* d_tile_next = nullptr;   // batched: k_bpull's tile counter
     uint32_t* d_pbits = nullptr;        // live-row bitmap of p, 2 * ceil(nsrc / 64) words
     int nsrc = 0;                       // vertices with out-edges (rows of p)
     int pull_live = 2;                  // live-row bitmap: 0 never, 1 always, 2 when demand <= frac * m
@@ -2202,7 +2214,7 @@ void launch_bfold_pull_rows(Ctx* c, hipStream_t st, bool fuse, bool trial) {
     BDenseArgs<PT> a{c->n, c->d_idx, c->d_bitmap, reinterpret_cast<PT*>(c->d_p), prow_stride(c->K, sizeof(PT) == 2),
                      c->d_rank, c->d_inv, c->d_pbits, c->nsrc, c->d_tidx, c->d_tadj, row,
                      row + C_PULL_HUB_LIVE, fuse ? reinterpret_cast<PT*>(c->d_p2) : nullptr, c->d_bitmap2,
-                     c->d_hubbits, c->d_cv0, c->nc, c->d_cnt, c->d_counters + C_SPREAD};
+                     c->d_hubbits, c->d_cv0, c->nc, c->d_cnt, c->d_counters + C_SPREAD, c->d_tile_next};
     const BatchArgs B = batch_args(c, -1);
     const long long segs = ((long long)c->n + WAVE - 1) / WAVE;
     const unsigned cap = (unsigned)c->cus * 8;
@@ -2233,9 +2245,10 @@ void launch_bfold_pull_rows(Ctx* c, hipStream_t st, bool fuse, bool trial) {
         else
             hipLaunchKernelGGL((k_bpull_hub<8, PT>), g, dim3(BLOCK), 0, st, a, B, c->d_pull_hubs, c->n_pull_hubs);
     }
+    hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(BLOCK), 0, st, c->d_tile_next, 1);
     {
         ScopedTimer t(c, st, T_PULL);
-        const dim3 g((unsigned)std::min<long long>((segs + NWAVES - 1) / NWAVES, 2 * cap));
+        const dim3 g((unsigned)std::min<long long>((segs + NWAVES - 1) / NWAVES, cap));
         if (c->bpull_q == 16) hipLaunchKernelGGL((k_bpull<16, PT>), g, dim3(BLOCK), 0, st, a, B);
         else hipLaunchKernelGGL((k_bpull<8, PT>), g, dim3(BLOCK), 0, st, a, B);
     }
@@ -2462,7 +2475,7 @@ int exc_destroy(void* p) {
     for (auto s : c->st) if (s) hipStreamDestroy(s);
     for (auto q : c->prows) if (q) (void)hipFree(q);
     void* pull_ptrs[] = {c->d_tidx, c->d_tadj, c->d_p, c->d_pull_hubs, c->d_wts, c->d_twts, c->d_rank, c->d_pbits, c->d_p2,
-                         c->d_bitmap2, c->d_hubbits, c->d_inv};
+                         c->d_bitmap2, c->d_hubbits, c->d_inv, c->d_tile_next};
     for (void* q : pull_ptrs) if (q) (void)hipFree(q);
     if (c->d_beps) (void)hipFree(c->d_beps);
     if (c->d_colpend) (void)hipFree(c->d_colpend);
@@ -3105,6 +3118,7 @@ int exc_enable_pull(void* p, double threshold) {
     if (ok && c->fuse_fold) ok = hipMalloc(&c->d_p2, n * prow_bytes) == hipSuccess;
     if (ok && c->fuse_fold) ok = hipMemset(c->d_p2, 0, n * prow_bytes) == hipSuccess;
     if (ok && c->K) ok = hipMalloc(&c->d_inv, std::max<size_t>(1, (size_t)c->nsrc) * sizeof(int32_t)) == hipSuccess;
+    if (ok && c->K) ok = hipMalloc(&c->d_tile_next, sizeof(unsigned long long)) == hipSuccess;
     if (ok && c->K && c->fuse_fold) {
         ok = hipMalloc(&c->d_bitmap2, ((n + 63) / 64) * 2 * sizeof(uint32_t)) == hipSuccess;
         ok = ok && hipMalloc(&c->d_hubbits, ((n + 63) / 64) * 2 * sizeof(uint32_t)) == hipSuccess;
@@ -3113,10 +3127,11 @@ int exc_enable_pull(void* p, double threshold) {
     if (!ok) {
         (void)hipGetLastError();
         void* q[] = {c->d_tidx, c->d_tadj, c->d_p, cnt, c->d_rank, c->d_pbits, c->d_p2, c->d_bitmap2, c->d_hubbits, c->d_twts,
-                     c->d_inv};
+                     c->d_inv, c->d_tile_next};
         for (void* x : q) if (x) (void)hipFree(x);
         c->d_tidx = nullptr; c->d_tadj = nullptr; c->d_p = nullptr; c->d_rank = nullptr; c->d_pbits = nullptr;
         c->d_inv = nullptr;
+        c->d_tile_next = nullptr;
         c->d_p2 = nullptr; c->d_bitmap2 = nullptr; c->d_hubbits = nullptr; c->d_twts = nullptr;
         c->nsrc = 0;
         c->pull_threshold = -1;
@@ -3156,8 +3171,9 @@ int exc_enable_pull(void* p, double threshold) {
         if (d <= hub_thr) continue;
         c->pull_hub_edges += d;
         c->pull_hub_vertices++;
-        for (uint64_t o = 0; o < d; o += PULL_PIECE)
-            hubs.push_back({(int32_t)v, (uint32_t)std::min<uint64_t>(PULL_PIECE, d - o), acc + o});
+        const uint64_t piece = c->K ? BPULL_PIECE : PULL_PIECE;
+        for (uint64_t o = 0; o < d; o += piece)
+            hubs.push_back({(int32_t)v, (uint32_t)std::min<uint64_t>(piece, d - o), acc + o});
         cur[v] = acc;
         acc += d;
     }
