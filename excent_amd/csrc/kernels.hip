@@ -825,28 +825,47 @@ __global__ __launch_bounds__(BLOCK, 8) void k_pull(PullArgs a, DevMeasures M) {
         if (j0 < wn) {
             const uint32_t jn = wn - j0 < (uint32_t)PULL_SEQ ? wn - j0 : (uint32_t)PULL_SEQ;
             const uint32_t g = w0 + j0;   // first slot of this lane, relative to base
-            int lo = 0, hi = nt - 1;      // owner: the destination whose range holds slot g
-            while (lo < hi) {
-                const int mid = (lo + hi + 1) >> 1;
-                if (s_start[mid] <= g) lo = mid; else hi = mid - 1;
+            // ids of the lane's slots, then (live-row passes) their bitmap words, then the
+            // rows: each stage is issued as one batch and used only after it, so a lane waits
+            // once per stage, not once per slot
+            int vid[PULL_SEQ];
+#pragma unroll
+            for (int q = 0; q < PULL_SEQ; q++) vid[q] = (uint32_t)q < jn ? s_src[q * PULL_LD + tid] : -1;
+            bool lv[PULL_SEQ];
+            if (a.live) {
+                uint32_t wd[PULL_SEQ];
+#pragma unroll
+                for (int q = 0; q < PULL_SEQ; q++) wd[q] = vid[q] >= 0 ? a.live[vid[q] >> 5] : 0u;
+#pragma unroll
+                for (int q = 0; q < PULL_SEQ; q++) lv[q] = vid[q] >= 0 && ((wd[q] >> (vid[q] & 31)) & 1u);
+            } else {
+#pragma unroll
+                for (int q = 0; q < PULL_SEQ; q++) lv[q] = vid[q] >= 0;
             }
             Row<S> rows[PULL_SEQ];
 #pragma unroll
             for (int q = 0; q < PULL_SEQ; q++) {
 #pragma unroll
                 for (int k = 0; k < S; k++) rows[q].v[k] = 0.f;
-                if ((uint32_t)q < jn) {
-                    const int v = s_src[q * PULL_LD + tid];
-                    if (row_live(a.live, v)) {
-                        rows[q] = load_prow<NM, S>(a.p, v);
-                        taken++;
-                        if constexpr (W) {
-                            const float w = s_w[q * PULL_LD + tid];
-#pragma unroll
-                            for (int k = 0; k < NM; k++) rows[q].v[k] *= w;
-                        }
-                    }
+                if (lv[q]) {
+                    rows[q] = load_prow<NM, S>(a.p, vid[q]);
+                    taken++;
                 }
+            }
+            if constexpr (W) {   // rows of non-live slots are zero
+#pragma unroll
+                for (int q = 0; q < PULL_SEQ; q++) {
+                    const float w = (uint32_t)q < jn ? s_w[q * PULL_LD + tid] : 0.f;
+#pragma unroll
+                    for (int k = 0; k < NM; k++) rows[q].v[k] *= w;
+                }
+            }
+            // owner: the destination whose range holds slot g (searched while the rows above
+            // are in flight)
+            int lo = 0, hi = nt - 1;
+            while (lo < hi) {
+                const int mid = (lo + hi + 1) >> 1;
+                if (s_start[mid] <= g) lo = mid; else hi = mid - 1;
             }
             int owner = lo;
             uint32_t next = s_start[owner + 1];
@@ -1030,19 +1049,33 @@ __global__ __launch_bounds__(BLOCK) void k_pull_hub(PullArgs a, const PullPiece*
                 vs[q] = i < e ? a.tadj[i] : -1;
                 ws[q] = a.tw && i < e ? a.tw[i] : 1.f;
             }
+            // the live-bitmap words, then the rows, each as one batch (see k_pull)
+            bool lv[HUB_UNROLL];
+            if (a.live) {
+                uint32_t wd[HUB_UNROLL];
+#pragma unroll
+                for (int q = 0; q < HUB_UNROLL; q++) wd[q] = vs[q] >= 0 ? a.live[vs[q] >> 5] : 0u;
+#pragma unroll
+                for (int q = 0; q < HUB_UNROLL; q++) lv[q] = vs[q] >= 0 && ((wd[q] >> (vs[q] & 31)) & 1u);
+            } else {
+#pragma unroll
+                for (int q = 0; q < HUB_UNROLL; q++) lv[q] = vs[q] >= 0;
+            }
             Row<S> rows[HUB_UNROLL];
 #pragma unroll
             for (int q = 0; q < HUB_UNROLL; q++) {
 #pragma unroll
                 for (int k = 0; k < S; k++) rows[q].v[k] = 0.f;
-                if (vs[q] >= 0 && row_live(a.live, vs[q])) {
+                if (lv[q]) {
                     rows[q] = load_prow<NM, S>(a.p, vs[q]);
                     taken++;
-                    if (a.tw) {
-#pragma unroll
-                        for (int k = 0; k < NM; k++) rows[q].v[k] *= ws[q];
-                    }
                 }
+            }
+            if (a.tw) {   // rows of non-live slots are zero
+#pragma unroll
+                for (int q = 0; q < HUB_UNROLL; q++)
+#pragma unroll
+                    for (int k = 0; k < NM; k++) rows[q].v[k] *= ws[q];
             }
 #pragma unroll
             for (int q = 0; q < HUB_UNROLL; q++)
