@@ -349,6 +349,72 @@ def bench_ooc(args, nat, np, work, graph):
     return out
 
 
+def adjoint_fp64_checks(nat, np, idx, adj, w, s, g, x, gs, gw, gwn, seed):
+    """The float32 results against the product's FP64 solves with per-edge coefficients
+    (exc_fp64_solve_coef, pinned to the oracle's weighted solve in the tests): the forward
+    x, the gradients against the FP64 adjoint solve lambda = (I - d W^T)^-1 g
+    (dL/ds = (1 - d) lambda; dL/dw of a raw weight by the chain rule through the per-source
+    normalisation), and central differences of L = g . x in FP64 on 32 seeded directions
+    (even: raw edge weights, odd: teleport), step 1e-3."""
+    D = 0.85
+    t0 = time.time()
+    n, m = len(idx) - 1, len(adj)
+    deg = np.diff(idx.astype(np.int64))
+    src = np.repeat(np.arange(n, dtype=np.int64), deg)
+    wd = w.astype(np.float64)
+    s64, g64 = s.astype(np.float64), g.astype(np.float64)
+
+    def coefs(wr):
+        S = np.bincount(src, weights=wr, minlength=n)
+        return wr / S[src], S
+
+    wn, S = coefs(wd)
+    rel = lambda a, b: float(np.linalg.norm(a.astype(np.float64) - b) / max(np.linalg.norm(b), 1e-300))
+    out = {}
+    with nat.Fp64Solver(idx, adj) as F:
+        F.set_coef(adj, wn)
+        xr, _, itf = F.solve_coef((1 - D) * s64, D)
+        lam, _, itb = F.solve_coef(g64, D, transpose=True)
+        gs_ref = (1 - D) * lam
+        t = np.bincount(src, weights=wn * lam[adj], minlength=n)
+        gw_ref = D * xr[src] / S[src] * (lam[adj] - t[src])
+        gwn_ref = D * lam[adj] * xr[src]
+        out["x_l1_rel"] = float(np.abs(x.astype(np.float64) - xr).sum() / np.abs(xr).sum())
+        out["grad_s_l2_rel"] = rel(gs, gs_ref)
+        out["grad_w_l2_rel"] = rel(gw, gw_ref)
+        out["grad_w_normalised_l2_rel"] = rel(gwn, gwn_ref)
+        out["fp64_iters"] = [itf, itb]
+        log(f"adjoint fp64: forward {itf} / adjoint {itb} iterations, {out}")
+        h = 1e-3
+        worst_gpu = worst_fp64 = 0.0
+        for k in range(32):
+            dr = np.random.default_rng(seed + 100 + k)
+            if k % 2 == 0:   # raw edge weights (stay positive: uniform(0.5, 1.5) +- 1e-3 N(0, 1))
+                dw = dr.standard_normal(m)
+                F.set_coef(adj, coefs(wd + h * dw)[0])
+                lp = float(g64 @ F.solve_coef((1 - D) * s64, D)[0])
+                F.set_coef(adj, coefs(wd - h * dw)[0])
+                lm = float(g64 @ F.solve_coef((1 - D) * s64, D)[0])
+                an, af = float(gw.astype(np.float64) @ dw), float(gw_ref @ dw)
+            else:            # teleport vector
+                ds = dr.standard_normal(n)
+                F.set_coef(adj, wn)
+                lp = float(g64 @ F.solve_coef((1 - D) * (s64 + h * ds), D)[0])
+                lm = float(g64 @ F.solve_coef((1 - D) * (s64 - h * ds), D)[0])
+                an, af = float(gs.astype(np.float64) @ ds), float(gs_ref @ ds)
+            fd = (lp - lm) / (2 * h)
+            worst_gpu = max(worst_gpu, abs(an - fd) / max(abs(fd), 1e-300))
+            worst_fp64 = max(worst_fp64, abs(af - fd) / max(abs(fd), 1e-300))
+            log(f"adjoint fd direction {k} ({'w' if k % 2 == 0 else 's'}): fd {fd:.9g}, gpu {an:.9g}, "
+                f"fp64 {af:.9g}, {time.time() - t0:.1f}s")
+        out["fd_directions"] = 32
+        out["fd_max_rel_err_gpu"] = worst_gpu
+        out["fd_max_rel_err_fp64_adjoint"] = worst_fp64
+    out = {k: (float(f"{v:.4g}") if isinstance(v, float) else v) for k, v in out.items()}
+    log(f"adjoint fp64 checks {time.time() - t0:.1f}s: {out}")
+    return out
+
+
 def bench_adjoint(args, nat, np, work, graph):
     """Differentiable personalised PageRank on a weighted graph: forward push, adjoint push
     over the transpose and gradient assembly, timed; the adjoint identity
@@ -371,6 +437,7 @@ def bench_adjoint(args, nat, np, work, graph):
         pf, pb = P.passes()
     lhs = float(g.astype(np.float64) @ x.astype(np.float64))
     rhs = float(gs.astype(np.float64) @ s.astype(np.float64))   # gs = (1 - d) lambda
+    fp64 = None if args.no_fp64 else adjoint_fp64_checks(nat, np, idx, adj, w, s, g, x, gs, gw, gwn, args.seed)
     return {"metric": "differentiable personalised PageRank forward + backward", "unit": "s",
             "value": round(tf + tb, 4), "higher_is_better": False, "n_gpus": 1, "dtype": "f32", "data": "synthetic",
             "config": {"workload": f"R-MAT scale {args.scale} ef {args.edge_factor}, raw weights uniform(0.5, 1.5), "
@@ -378,7 +445,8 @@ def bench_adjoint(args, nat, np, work, graph):
             "forward_s": round(tf, 4), "backward_s": round(tb, 4), "passes": [pf, pb],
             "adjoint_identity_rel_err": abs(lhs - rhs) / max(abs(lhs), 1e-30),
             "grad_norms": {"s": float(np.linalg.norm(gs)), "w": float(np.linalg.norm(gw)),
-                           "w_normalised": float(np.linalg.norm(gwn))}}
+                           "w_normalised": float(np.linalg.norm(gwn))},
+            "fp64_check": fp64}
 
 
 def main():

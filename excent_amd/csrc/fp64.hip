@@ -103,6 +103,39 @@ __global__ void k_absdiff_sum(int n, const double* __restrict__ y, const double*
     for (int o = 32; o > 0; o >>= 1) a += __shfl_xor(a, o, 64);
     if ((threadIdx.x & 63) == 0) atomicAdd(out, a);
 }
+// Transpose refill that carries each edge's coefficient with its in-list entry.
+__global__ void k_fill_in_coef(int n, const uint64_t* __restrict__ idx, const int32_t* __restrict__ adj,
+                               const double* __restrict__ coef, unsigned long long* pos, int32_t* tadj, double* tcoef) {
+    for (long long v = blockIdx.x * (long long)blockDim.x + threadIdx.x; v < n; v += (long long)gridDim.x * blockDim.x)
+        for (uint64_t e = idx[v]; e < idx[v + 1]; e++) {
+            const unsigned long long q = atomicAdd(&pos[adj[e]], 1ull);
+            tadj[q] = (int32_t)v;
+            tcoef[q] = coef[e];
+        }
+}
+// y[w] = cvec[w] + s * sum_{e in [lidx[w], lidx[w+1])} lcoef[e] x[ladj[e]] over lists given
+// as offsets / neighbours / coefficients: the transpose's in-lists (x = c + s W x) or the
+// graph's out-lists (x = c + s W^T x); mx as in k_pull.
+__global__ __launch_bounds__(BLOCK) void k_pull_coef(int n, const uint64_t* __restrict__ lidx,
+                                                      const int32_t* __restrict__ ladj,
+                                                      const double* __restrict__ lcoef, const double* __restrict__ x,
+                                                      double* y, const double* __restrict__ cvec, double s,
+                                                      unsigned long long* mx) {
+    const int gl = threadIdx.x % GL;
+    const long long groups = (long long)gridDim.x * (BLOCK / GL);
+    for (long long w = (blockIdx.x * (long long)BLOCK + threadIdx.x) / GL; w < n; w += groups) {
+        double acc = 0;
+        for (uint64_t e = lidx[w] + gl; e < lidx[w + 1]; e += GL) acc += lcoef[e] * x[ladj[e]];
+#pragma unroll
+        for (int o = GL / 2; o > 0; o >>= 1) acc += __shfl_xor(acc, o, GL);
+        if (gl == 0) {
+            const double nv = cvec[w] + s * acc;
+            y[w] = nv;
+            atomicMax(&mx[0], (unsigned long long)__double_as_longlong(fabs(nv - x[w])));
+            atomicMax(&mx[1], (unsigned long long)__double_as_longlong(fabs(nv)));
+        }
+    }
+}
 __global__ void k_fill(int n, double* x, double f) {
     for (long long v = blockIdx.x * (long long)blockDim.x + threadIdx.x; v < n; v += (long long)gridDim.x * blockDim.x) x[v] = f;
 }
@@ -116,6 +149,11 @@ struct Solver {
     double *x = nullptr, *y = nullptr, *xs = nullptr, *red = nullptr;
     double* cvec = nullptr;   // per-vertex constant of the last vector solve / residual
     unsigned long long* mx = nullptr;
+    // per-edge coefficients (exc_fp64_set_coef): out-lists in the graph's order, and the
+    // transpose (refilled so that tcoef is aligned with tadj)
+    int32_t* d_adj = nullptr;
+    double* d_coef = nullptr;
+    double* d_tcoef = nullptr;
 };
 
 double bits_to_d(unsigned long long b) { double d; memcpy(&d, &b, 8); return d; }
@@ -210,7 +248,7 @@ int exc_fp64_create(int device, int n, uint64_t m, const uint64_t* h_idx, const 
 int exc_fp64_destroy(void* p) {
     Solver* S = static_cast<Solver*>(p);
     if (!S) return EXC_OK;
-    void* q[] = {S->d_idx, S->d_tidx, S->d_tadj, S->x, S->y, S->xs, S->red, S->mx, S->cvec};
+    void* q[] = {S->d_idx, S->d_tidx, S->d_tadj, S->x, S->y, S->xs, S->red, S->mx, S->cvec, S->d_adj, S->d_coef, S->d_tcoef};
     for (void* z : q) if (z) (void)hipFree(z);
     delete S;
     return EXC_OK;
@@ -353,6 +391,72 @@ int exc_fp64_lambda(void* p, const double* h_start, int max_iter, double tol, do
     if (mean_ev) *mean_ev = sm * sm / n;
     if (iters) *iters = pit;
     if (x != S->x) std::swap(S->x, S->y);
+    return EXC_OK;
+}
+
+// Per-edge coefficients for exc_fp64_solve_coef: h_adj and h_coef are the graph's out-lists
+// (the CSR given to exc_fp64_create) and one coefficient per edge.
+int exc_fp64_set_coef(void* p, const int32_t* h_adj, const double* h_coef) {
+    Solver* S = static_cast<Solver*>(p);
+    if (!S || (S->m && (!h_adj || !h_coef))) return fail(EXC_EINVAL, "exc_fp64_set_coef: bad arguments");
+    HIPCHK(hipSetDevice(S->dev));
+    const size_t m = S->m ? S->m : 1;
+    if (!S->d_adj) HIPCHK(hipMalloc(&S->d_adj, m * 4));
+    if (!S->d_coef) HIPCHK(hipMalloc(&S->d_coef, m * 8));
+    if (!S->d_tcoef) HIPCHK(hipMalloc(&S->d_tcoef, m * 8));
+    if (S->m) {
+        HIPCHK(hipMemcpy(S->d_adj, h_adj, S->m * 4, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(S->d_coef, h_coef, S->m * 8, hipMemcpyHostToDevice));
+    }
+    unsigned long long* pos = nullptr;
+    HIPCHK(hipMalloc(&pos, ((size_t)S->n + 1) * 8));
+    hipError_t e = hipMemcpy(pos, S->d_tidx, ((size_t)S->n + 1) * 8, hipMemcpyDeviceToDevice);
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(k_fill_in_coef, dim3(S->cus * 8), dim3(BLOCK), 0, 0, S->n, S->d_idx, S->d_adj, S->d_coef, pos,
+                           S->d_tadj, S->d_tcoef);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipDeviceSynchronize();
+    (void)hipFree(pos);
+    if (e != hipSuccess) return fail(EXC_EHIP, "exc_fp64_set_coef: %s", hipGetErrorString(e));
+    return EXC_OK;
+}
+
+// Jacobi x = c + s W x (transpose = 0: W[u][v] = coef of edge v -> u, the pull over in-lists)
+// or x = c + s W^T x (transpose = 1: the pull over out-lists, the adjoint system) from x = c
+// until max|x' - x| / max|x'| < tol or max_iter.
+int exc_fp64_solve_coef(void* p, int transpose, const double* h_c, double s, int max_iter, double tol, double* h_out,
+                        double* rel_out, int* iters) {
+    Solver* S = static_cast<Solver*>(p);
+    if (!S || !h_c || !h_out) return fail(EXC_EINVAL, "exc_fp64_solve_coef: bad arguments");
+    if (!S->d_coef) return fail(EXC_ESTATE, "exc_fp64_solve_coef: no coefficients (exc_fp64_set_coef)");
+    HIPCHK(hipSetDevice(S->dev));
+    const int n = S->n;
+    int rc = upload_cvec(S, h_c);
+    if (rc) return rc;
+    HIPCHK(hipMemcpy(S->x, S->cvec, (size_t)n * 8, hipMemcpyDeviceToDevice));
+    const uint64_t* lidx = transpose ? S->d_idx : S->d_tidx;
+    const int32_t* ladj = transpose ? S->d_adj : S->d_tadj;
+    const double* lcoef = transpose ? S->d_coef : S->d_tcoef;
+    const long long want = ((long long)n * GL + BLOCK - 1) / BLOCK;
+    const unsigned grid = (unsigned)std::min<long long>(want, (long long)S->cus * 64);
+    double* x = S->x;
+    double* y = S->y;
+    double rel = 1;
+    int it = 0;
+    for (; it < max_iter && rel > tol; it++) {
+        HIPCHK(hipMemset(S->mx, 0, 2 * sizeof(unsigned long long)));
+        hipLaunchKernelGGL(k_pull_coef, dim3(grid), dim3(BLOCK), 0, 0, n, lidx, ladj, lcoef, x, y, S->cvec, s, S->mx);
+        HIPCHK(hipGetLastError());
+        unsigned long long h[2];
+        HIPCHK(hipMemcpy(h, S->mx, sizeof h, hipMemcpyDeviceToHost));
+        const double mxd = bits_to_d(h[0]), mxy = bits_to_d(h[1]);
+        rel = mxy > 0 ? mxd / mxy : 0.0;
+        std::swap(x, y);
+    }
+    HIPCHK(hipMemcpy(h_out, x, (size_t)n * 8, hipMemcpyDeviceToHost));
+    if (rel_out) *rel_out = rel;
+    if (iters) *iters = it;
     return EXC_OK;
 }
 

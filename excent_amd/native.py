@@ -145,6 +145,9 @@ HIP_SIGNATURES = {
     "exc_fp64_solve_vec": (c_int, [c_void_p, c_void_p, c_double, c_int, c_int, c_double, c_void_p, P(c_double),
                                    P(c_int)]),
     "exc_fp64_katz_mean": (c_int, [c_void_p, c_double, c_double, P(c_double), P(c_int)]),
+    "exc_fp64_set_coef": (c_int, [c_void_p, c_void_p, c_void_p]),
+    "exc_fp64_solve_coef": (c_int, [c_void_p, c_int, c_void_p, c_double, c_int, c_double, c_void_p, P(c_double),
+                                    P(c_int)]),
 }
 
 # include/excent.h
@@ -472,6 +475,23 @@ class Fp64Solver:
         rel, it = c_double(0), c_int(0)
         check(F["exc_fp64_solve_vec"](self.h, ptr(cv), float(s), int(divdeg), int(max_iter), float(tol), ptr(out),
                                       ctypes.byref(rel), ctypes.byref(it)), "fp64_solve_vec")
+        return out, rel.value, it.value
+
+    def set_coef(self, adj, coef):
+        """Per-edge FP64 coefficients (adj: the adjacency given at construction)."""
+        adj = np.ascontiguousarray(adj, dtype=np.int32)
+        coef = np.ascontiguousarray(coef, dtype=np.float64)
+        assert len(adj) == len(coef)
+        check(F["exc_fp64_set_coef"](self.h, ptr(adj), ptr(coef)), "fp64_set_coef")
+
+    def solve_coef(self, c, s, transpose=False, max_iter=20000, tol=1e-14):
+        """Jacobi x = c + s W x (transpose: x = c + s W^T x) with the per-edge coefficients."""
+        cv = np.ascontiguousarray(c, dtype=np.float64)
+        assert len(cv) == self.n
+        out = np.zeros(self.n, dtype=np.float64)
+        rel, it = c_double(0), c_int(0)
+        check(F["exc_fp64_solve_coef"](self.h, int(bool(transpose)), ptr(cv), float(s), int(max_iter), float(tol),
+                                       ptr(out), ctypes.byref(rel), ctypes.byref(it)), "fp64_solve_coef")
         return out, rel.value, it.value
 
     def close(self):

@@ -251,6 +251,16 @@ int exc_fp64_lambda(void* solver, const double* h_start, int max_iter, double to
                     double* lambda, double* mean_ev, int* iters);
 /* Katz fixed-point mean with alpha (tools/excent_params.cu:149-163). */
 int exc_fp64_katz_mean(void* solver, double alpha, double beta, double* mean, int* iters);
+/* Per-edge FP64 coefficients (no reference counterpart; the FP64 check of the weighted
+ * forward and adjoint solves): h_adj is the graph's adjacency as given to exc_fp64_create,
+ * h_coef one coefficient per edge in the same order. */
+int exc_fp64_set_coef(void* solver, const int32_t* h_adj, const double* h_coef);
+/* Jacobi with the coefficients, from x = c until max|x' - x| / max|x'| < tol or max_iter:
+ * transpose = 0: x = c + s W x, (W x)[u] = sum over edges v -> u of coef * x[v] (the
+ * forward weighted personalised PageRank); transpose = 1: x = c + s W^T x, the sum over the
+ * out-edges u -> w of coef * x[w] (its adjoint). */
+int exc_fp64_solve_coef(void* solver, int transpose, const double* h_c, double s, int max_iter, double tol,
+                        double* h_out, double* rel, int* iters);
 
 #ifdef __cplusplus
 }

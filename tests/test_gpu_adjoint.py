@@ -50,6 +50,31 @@ def transpose(idx, adj, vals):
     return tidx, src[order].astype(np.int32), vals[order]
 
 
+def test_fp64_coefficient_solves_against_oracle(wgraph):
+    """The product's FP64 Jacobi with per-edge coefficients (exc_fp64_solve_coef), used to
+    check the full-size gradients, against the oracle's weighted FP64 solve: the forward
+    system over the in-lists, and the adjoint system over the out-lists against the oracle
+    on a host transpose."""
+    idx, adj, w = wgraph
+    n = len(idx) - 1
+    wn, _, _ = normalise(idx, w)
+    rng = np.random.default_rng(3)
+    c = rng.random(n)
+    g = rng.standard_normal(n)
+    with nat.Fp64Solver(idx, adj) as F:
+        F.set_coef(adj, wn)
+        x, rel, it = F.solve_coef(c, D)
+        lam, rel2, _ = F.solve_coef(g, D, transpose=True)
+    assert rel <= 1e-14 and rel2 <= 1e-14 and it > 10
+    xo, _, _ = orc.fp64_solve_w(idx, adj, wn, c, s=D)
+    tidx, tadj, twn = transpose(idx, adj, wn)
+    lo, _, _ = orc.fp64_solve_w(tidx, tadj, twn, g, s=D)
+    assert np.abs(x - xo).max() <= 1e-10 * np.abs(xo).max()
+    assert np.abs(lam - lo).max() <= 1e-10 * np.abs(lo).max()
+    # the two systems are adjoint: g . x(c) = c . lambda(g)
+    assert abs(g @ x - c @ lam) <= 1e-10 * abs(g @ x)
+
+
 def test_forward_against_fp64(wgraph):
     idx, adj, w = wgraph
     n = len(idx) - 1
