@@ -1681,7 +1681,6 @@ __global__ __launch_bounds__(BLOCK, 6) void k_bpull(BDenseArgs<PT> a, BatchArgs 
     __shared__ uint32_t s_col[NWAVES][WAVE];
     const int lane = lane_id(), wid = threadIdx.x / WAVE;
     const long long gw = (blockIdx.x * (long long)BLOCK + threadIdx.x) / WAVE;
-    const long long nw = (long long)gridDim.x * NWAVES;
     const long long ntile = ((long long)a.n + WAVE - 1) / WAVE;
     const long long words = ((long long)a.n + 31) / 32;
     const bool fused = a.p_next != nullptr;
@@ -1690,7 +1689,7 @@ __global__ __launch_bounds__(BLOCK, 6) void k_bpull(BDenseArgs<PT> a, BatchArgs 
     // waves claim runs of BPULL_CLAIM consecutive tiles from a counter (in-degree sums vary
     // widely between tiles); within a run the per-chunk counts of the next snapshot
     // accumulate in registers
-    (void)nw;
+
     uint32_t colcnt = 0;                  // pending entries of this lane's column (next pass)
     uint64_t fs[4] = {0, 0, 0, 0};        // fused fold statistics (wave-uniform)
     int cc = -1;
