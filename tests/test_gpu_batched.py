@@ -283,3 +283,49 @@ def test_batched_dense_passes(gname, fuse, rows, rmat12, monkeypatch):
         assert err <= bound * (1 + 1e-6) + 1e-6, (j, err, bound)
         dev = np.abs(R[j] - true) - (1e-4 * np.abs(X[j]) + eps)
         assert dev.max() <= 0, (j, int(dev.argmax()), dev.max())
+
+
+@pytest.mark.parametrize("mode", ["stream_only", "gather_only", "mixed", "hot"])
+def test_batched_out_of_core_regimes(tmp_path, rmat12, mode):
+    """Batched personalised PageRank with the edges read per chunk from the EGR file -- every
+    chunk streamed, every chunk's pending lists gathered, or both -- or partly held in the
+    hot-vertex cache: one read of a chunk serves all K columns, and every column lands within
+    its exact FP64 bound."""
+    idx, adj = orc.egr_load(rmat12)
+    n = len(idx) - 1
+    K, S, eps = 8, 4, 1e-4
+    rng = np.random.default_rng(11)
+    seeds = np.stack([rng.choice(n, S, replace=False) for _ in range(K)]).astype(np.int32)
+    sf = str(tmp_path / "seeds.i32")
+    seeds.tofile(sf)
+    extra = {"stream_only": ["--cache", "none", "--stream-ratio", "0"],
+             "gather_only": ["--cache", "none", "--stream-ratio", "2"],
+             "mixed": ["--cache", "none", "--stream-ratio", "0.2", "--dedup", "off"]}.get(mode)
+    if mode == "hot":
+        free, total = nat.ctypes.c_size_t(0), nat.ctypes.c_size_t(0)
+        nat.F["exc_mem_info"](0, nat.ctypes.byref(free), nat.ctypes.byref(total))
+        extra = ["--cache", "hot", "--reserve-mb", f"{free.value / (1 << 20) - 64:.3f}", "--stream-ratio", "0.3"]
+    print(f"case batched out of core {mode}", flush=True)
+    log = str(tmp_path / "b.log")
+    rc = nat.excent([rmat12, "--measure", "ppr", "--ppr-k", K, "--ppr-seeds", sf, "--eps-ppr", eps, "--dump",
+                     str(tmp_path / "o"), "--chunk-medges", "0.01", "--max-passes", "5000", "--log", log] + extra)
+    text = open(log).read()
+    assert rc == 0, text
+    lines = [l for l in text.splitlines() if l.startswith("pass ")]
+    assert lines
+    if mode == "stream_only":
+        assert all(" gather 0 " in l for l in lines)
+    if mode == "gather_only":
+        assert all(" stream 0 " in l for l in lines)
+    if mode == "hot":
+        assert "GPU degree cache:" in text
+    X = np.fromfile(str(tmp_path / "o.ppr.bin"), np.float32).reshape(K, n)
+    mass = np.float32((1 - np.float64(np.float32(0.85))) * n / S)
+    for j in range(K):
+        b = np.zeros(n)
+        np.add.at(b, seeds[j], np.float64(mass))
+        ref, _, _ = orc.fp64_solve_b(idx, adj, b)
+        err = np.abs(X[j] - ref).sum()
+        bound = np.abs(true_residual(idx, adj, X[j], b)).sum() / 0.15
+        assert err <= bound * (1 + 1e-6) + 1e-6, (j, err, bound)
+        assert err <= eps * n / 0.15
