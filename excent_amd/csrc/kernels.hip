@@ -1017,7 +1017,7 @@ __global__ __launch_bounds__(BLOCK, 8) void k_pull(PullArgs a, DevMeasures M) {
 // atomic per measure (pieces of one destination may run concurrently).  Runs before
 // k_pull in the pass; the two kernels touch disjoint rows of r.
 struct PullPiece { int32_t u; uint32_t len; uint64_t start; };
-constexpr uint32_t PULL_PIECE = 8192;
+constexpr uint32_t PULL_PIECE = 2048;   // one step of BLOCK x HUB_UNROLL entries
 // Batched contexts sum a piece with one wave (k_bpull_hub), so their pieces are shorter: a
 // wave's rows are a serial chain, and more pieces keep more waves busy.
 constexpr uint32_t BPULL_PIECE = 1024;
