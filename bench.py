@@ -75,6 +75,13 @@ def solve_logged(S, kind, tol, max_iter, seg=20, **kw):
         log(f"fp64 solve kind {kind}: {it} iterations, relative change {rel:.3g}")
 
 
+def top_overlap(np, a, b, k):
+    """Vertices shared by the k largest entries of a and of b (k capped at the length)."""
+    k = min(k, len(a))
+    top = lambda v: set(np.argpartition(-np.asarray(v), k - 1)[:k].tolist())
+    return len(top(a) & top(b))
+
+
 def fp64_errors(nat, np, graph, work, measure, env):
     """Errors of the last timed run's values (the --dump files) against the product's FP64
     solver, and the FP64 global residual of the float32 state (tools/centrality_reference.cu:
@@ -100,10 +107,9 @@ def fp64_errors(nat, np, graph, work, measure, env):
             if name == "ev":   # unit L2 with the sign fixed by the sum
                 xv = xv / np.linalg.norm(xv) * (1 if xv.sum() >= 0 else -1)
             d = np.abs(xv - ref)
-            k = min(100, len(x))
-            top = lambda v: set(np.argpartition(-v, k - 1)[:k].tolist())
             e = {"l1_rel": float(d.sum() / np.abs(ref).sum()), "l2": float(np.sqrt((d * d).sum())),
-                 "max_abs": float(d.max()), "top100_overlap": len(top(xv) & top(ref)),
+                 "max_abs": float(d.max()), "top100_overlap": top_overlap(np, xv, ref, 100),
+                 "top1000_overlap": top_overlap(np, xv, ref, 1000),
                  "residual_l1_fp64": res, "ref_iters": it, "ref_rel_change": rel}
             if name == "pr":
                 e["l1_bound"] = res / (1 - 0.85)
@@ -247,7 +253,8 @@ def bench_batched(args, nat, np, work, graph):
                 dx = np.abs(x.astype(np.float64) - ref)
                 e[tag] = {"l1_rel": round(float(dx.sum() / np.abs(ref).sum()), 9),
                           "max_abs": round(float(dx.max()), 9),
-                          "l1_bound_rel": round(S.residual(x, cvec[j], d, 1) / (1 - 0.85) / np.abs(ref).sum(), 9)}
+                          "l1_bound_rel": round(S.residual(x, cvec[j], d, 1) / (1 - 0.85) / np.abs(ref).sum(), 9),
+                          "top1000_overlap": top_overlap(np, x.astype(np.float64), ref, 1000)}
             fp64[f"column {j}"] = e
             log(f"fp64 column {j}: {e}")
     vb = rb["kernel"]["pushed_edge_measures"]

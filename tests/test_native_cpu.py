@@ -174,3 +174,15 @@ def test_run_sh_validation(args, msg):
     assert r.returncode == 1 and f"ERROR: {msg}" in r.stderr, r.stderr
     r = subprocess.run([os.path.join(ROOT, "run.sh"), "--help"], capture_output=True, text=True, timeout=60)
     assert r.returncode == 0 and "usage:" in r.stderr
+
+
+def test_bench_top_overlap_known_answers():
+    """bench.py's top-k overlap (reported at k = 100 and 1000 beside the FP64 errors)."""
+    import bench
+    a = np.arange(5000, dtype=np.float64)
+    assert bench.top_overlap(np, a, a, 1000) == 1000
+    assert bench.top_overlap(np, a, -a, 1000) == 0          # reversed ranking
+    b = a.copy()
+    b[[4999, 10]] = b[[10, 4999]]                            # the largest swapped out of the top
+    assert bench.top_overlap(np, a, b, 1000) == 999
+    assert bench.top_overlap(np, a[:50], a[:50], 1000) == 50   # k capped at the length
