@@ -76,6 +76,7 @@ struct Options {
     std::string ppr_seeds, ppr_gen;
     std::string pull = "auto";   // dense passes by pull over a device transpose: auto | off | threshold
     double eps_ppr = -1;
+    std::string stop = "vertex";  // vertex: nothing pending; global: also when sum |r_k| <= n eps_k
 };
 
 const char* USAGE =
@@ -139,6 +140,7 @@ Options parse(int argc, const char* const* argv) {
         else if (a == "--ppr-gen") o.ppr_gen = next();
         else if (a == "--eps-ppr") o.eps_ppr = atof(next());
         else if (a == "--ppr-column") o.ppr_col = atoi(next());
+        else if (a == "--stop") o.stop = next();
         else die("unknown option " + a);
     }
     return o;
@@ -217,6 +219,9 @@ private:
     uint64_t tot_pages_ = 0, tot_runs_ = 0;
     std::vector<double> sum_;
     std::vector<uint32_t> indeg_;
+    double l1_[EXC_MAXM] = {0, 0, 0};   // FP64 sum of |r| per measure, last snapshot
+    bool have_l1_ = false;
+    int stop_reason_ = 0;               // 0 nothing pending, 1 global L1 bound, 2 pass limit
 
     void init_state();
     void load_ppr_seeds(EgrGraph& g);
@@ -280,6 +285,8 @@ void Engine::setup() {
     if (o.group_opt < 0 || o.group_opt > 64) die("--group must be 0 (automatic) or 1..64");
     if (o.hot_when != "end" && o.hot_when != "start" && o.hot_when != "both") die("--hot-when must be end, start or both");
     if (o.hot_order != "sweep" && o.hot_order != "ranges") die("--hot-order must be sweep or ranges");
+    if (o.stop != "vertex" && o.stop != "global") die("--stop must be vertex or global");
+    if (o.stop == "global" && o.measure == "ppr") die("--stop global needs plain measures (pr, katz, ev, all)");
 
     try {
         g_.reset(new EgrGraph(o.graph, 16, /*offsets_in_ram=*/false));
@@ -659,6 +666,11 @@ void Engine::run_passes() {
     n_list_ = n_range_ = n_launch_ = 0;
     tot_maj_ = tot_min_ = 0;
     tot_pages_ = tot_runs_ = 0;
+    stop_reason_ = 2;
+    have_l1_ = false;
+    const bool stop_global = o.stop == "global";
+    if (pull_on_) ck(exc_fold_ahead(ctx, 1), "fold ahead");
+    bool fold_ahead_off = false;   // set once a folded state met the global bound
     const Clock::time_point T_begin = Clock::now();
     const size_t words = ((size_t)n + 31) / 32;
     std::vector<uint64_t> h_cnt(2 * (size_t)nc);
@@ -694,9 +706,25 @@ void Engine::run_passes() {
             ck(exc_batch_set_active(ctx, mask), "active columns");
         }
         const PlanOut po = fetch_plan(nc, h_cnt.data(), resident_, loaded_, stream_ratio_, chunk_edges, hot_mode_, pass_, hlo_, hhi_, mode.data());
-        if (po.need_bitmap) ck(exc_bitmap_to_host(ctx, h_bitmap.data()), "bitmap readback");
+        // FP64 global residual of the snapshot state (plain measures)
+        have_l1_ = !batched() && exc_residual_l1(ctx, l1_) == EXC_OK;
+        bool global_met = false;
+        if (stop_global && have_l1_) {
+            global_met = true;
+            for (int i = 0; i < M_.nm; i++) global_met = global_met && l1_[i] <= (double)n * (double)M_.eps[i];
+            if (global_met && exc_dense_required(ctx) == 1) {
+                // the last dense pass folded this state: its deltas are still in flight, so the
+                // state cannot be returned; deliver them with no further fold and test again
+                global_met = false;
+                fold_ahead_off = true;
+            }
+        }
+        if (po.need_bitmap && !global_met) ck(exc_bitmap_to_host(ctx, h_bitmap.data()), "bitmap readback");
         t_snap_ += secs(a, Clock::now());
-        if (po.pend_v == 0 && h_hotpend == 0) break;
+        if (po.pend_v == 0 && h_hotpend == 0) { stop_reason_ = 0; break; }
+        if (global_met) { stop_reason_ = 1; break; }
+        // a dense pass folds the pass after it only when the loop will go on from its state
+        if (pull_on_) ck(exc_fold_ahead(ctx, !fold_ahead_off && pass_ + 1 < o.max_passes), "fold ahead");
 
         int nres = 0, nstream = 0, ngather = 0;
         if (hot_skip && h_hotpend && o.hot_when != "end") launch_hot(0);
@@ -781,11 +809,19 @@ void Engine::run_passes() {
                 (unsigned long long)h_hotpend, f1.maj - f0.maj, f1.min - f0.min, (unsigned long long)pass_pages,
                 (unsigned long long)pass_runs);
         if (batched()) fprintf(out, " | active columns %d", active_cols_);
+        if (have_l1_) {
+            fprintf(out, " | residual l1");
+            for (int i = 0; i < M_.nm; i++) fprintf(out, " %s %.4e", MNAME[kinds_[i]], l1_[i]);
+        }
         fprintf(out, "\n");
         fflush(out);   // a long run shows its progress pass by pass
     }
     T_iter_ = Clock::now();
     T_setup_ = T_begin;
+    if (stop_reason_ == 2 && !batched()) {   // stopped by the pass limit: the residual of the final state
+        ck(exc_snapshot(ctx, 0, h_cnt.data(), &h_hotpend), "final snapshot");
+        have_l1_ = exc_residual_l1(ctx, l1_) == EXC_OK;
+    }
 }
 
 void Engine::results() {
@@ -848,6 +884,11 @@ void Engine::report() {
         for (int i = 0; i < M_.nm; i++) fprintf(out, "  %s sum %.6e", MNAME[kinds_[i]], sum_[i]);
     }
     fprintf(out, "\n");
+    if (have_l1_) {   // extension: the FP64 global residual and the rule that ended the loop
+        fprintf(out, "residual l1 (fp64):");
+        for (int i = 0; i < M_.nm; i++) fprintf(out, "  %s %.6e (n*eps %.6e)", MNAME[kinds_[i]], l1_[i], (double)n_ * (double)M_.eps[i]);
+        fprintf(out, " | stop: %s\n", stop_reason_ == 0 ? "nothing pending" : stop_reason_ == 1 ? "global l1 <= n*eps" : "pass limit");
+    }
     if (o.count_pushes) fprintf(out, "edges pushed: %.2f x |E|\n", m_ ? (double)pushed / m_ : 0.0);
     fprintf(out, "launches from GPU memory: %llu pending-only, %llu whole-range\n", (unsigned long long)n_list_, (unsigned long long)n_range_);
     fprintf(out, "bytes shipped: stream %.2f GB, gather %.2f GB (graph edges %.2f GB)\n", bytes_stream_ / 1e9, bytes_gather_ / 1e9, m_ * 4.0 / 1e9);
@@ -873,6 +914,8 @@ void Engine::fill_stats(excent_run_stats* s) {
     s->bytes_gather = bytes_gather_;
     s->launches = n_launch_;
     s->kernel = st;
+    for (int i = 0; i < EXC_MAXM; i++) s->residual_l1[i] = have_l1_ && i < M_.nm ? l1_[i] : 0.0;
+    s->stop_reason = stop_reason_;
 }
 
 void Engine::write_stats_json() {
@@ -897,6 +940,12 @@ void Engine::write_stats_json() {
     fprintf(j, " \"kernel\": {\"snapshot_ms\": %.6f, \"snapshot_launches\": %llu, \"snapshot_bytes\": %.1f, \"push_ms\": %.6f, \"push_launches\": %llu, \"push_bytes\": %.1f, \"piece_ms\": %.6f, \"piece_launches\": %llu, \"piece_bytes\": %.1f, \"other_ms\": %.6f},\n",
             st.snapshot_ms, (unsigned long long)st.snapshot_launches, st.snapshot_bytes, st.push_ms, (unsigned long long)st.push_launches,
             st.push_bytes, st.piece_ms, (unsigned long long)st.piece_launches, st.piece_bytes, st.other_ms);
+    if (have_l1_) {
+        fprintf(j, " \"residual_l1\": {");
+        for (int i = 0; i < M_.nm; i++) fprintf(j, "%s\"%s\": %.9e", i ? ", " : "", MNAME[kinds_[i]], l1_[i]);
+        fprintf(j, "},\n");
+    }
+    fprintf(j, " \"stop\": \"%s\",\n", stop_reason_ == 0 ? "pending" : stop_reason_ == 1 ? "global" : "passes");
     fprintf(j, " \"sums\": {");
     if (batched()) {
         fprintf(j, "\"ppr\": [");

@@ -146,6 +146,22 @@ __device__ __forceinline__ uint32_t wave_sum_u32(uint32_t v) {
     return v;
 }
 
+__device__ __forceinline__ double wave_sum_f64(double v) {
+#pragma unroll
+    for (int o = WAVE / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, WAVE);
+    return v;
+}
+
+// FP64 global residual sums sum_v |r_k[v]| of a snapshot: each wave (k_snapshot) or block
+// (k_pull's fused snapshot) adds its partial sums into one of L1_SLOTS slots on separate
+// 128-byte lines (slot = wave or block index mod L1_SLOTS), so no address takes more than
+// a few thousand FP64 atomics per pass; the host adds the slots.  The slots sit behind the
+// per-chunk counts in the counts buffer and are cleared with them.
+constexpr int L1_SLOTS = 64, L1_STRIDE = 16;   // doubles per slot line
+__device__ __forceinline__ void l1_flush(double* l1, long long who, int k, double v) {
+    if (v != 0.0) __hip_atomic_fetch_add(l1 + (who % L1_SLOTS) * L1_STRIDE + k, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 // src/excent.cu:49-56: last chunk c with cv0[c] <= v.
 __device__ __forceinline__ int chunk_of(int v, const int* __restrict__ cv0, int nc) {
     int lo = 0, hi = nc - 1;
@@ -239,6 +255,7 @@ struct SnapArgs {
     unsigned long long* hotpend;
     uint16_t* trace;            // or null
     int segs_per_wave;          // 64-vertex segments per wave
+    double* l1;                 // FP64 residual sum slots (L1_SLOTS x L1_STRIDE), or null
 };
 
 // k_snapshot (src/excent.cu:69-98).  One wave takes segs_per_wave consecutive 64-vertex
@@ -262,6 +279,9 @@ __global__ __launch_bounds__(BLOCK) void k_snapshot(SnapArgs a) {
     int clo = a.cv0[cc], chi = cc + 1 < a.nc ? a.cv0[cc + 1] : a.n;
     uint32_t acc_n = 0;
     uint64_t acc_d = 0, hp = 0;
+    double l1[NM];
+#pragma unroll
+    for (int k = 0; k < NM; k++) l1[k] = 0.0;
 
     auto flush = [&]() {
         uint32_t tn = wave_sum_u32(acc_n);
@@ -285,6 +305,11 @@ __global__ __launch_bounds__(BLOCK) void k_snapshot(SnapArgs a) {
 #pragma unroll
                 for (int k = 0; k < S; k++) rows[u].v[k] = 0.f;
             }
+        }
+#pragma unroll
+        for (int u = 0; u < U; u++) {   // rows past n or past the wave's range are zero
+#pragma unroll
+            for (int k = 0; k < NM; k++) l1[k] += fabs((double)rows[u].v[k]);
         }
 #pragma unroll
         for (int u = 0; u < U; u++) {
@@ -335,6 +360,13 @@ __global__ __launch_bounds__(BLOCK) void k_snapshot(SnapArgs a) {
     if (a.hot) {
         uint64_t t = wave_sum_u64(hp);
         if (lane == 0 && t) atomicAdd(a.hotpend, (unsigned long long)t);
+    }
+    if (a.l1) {
+#pragma unroll
+        for (int k = 0; k < NM; k++) {
+            const double t = wave_sum_f64(l1[k]);
+            if (lane == 0) l1_flush(a.l1, gwave, k, t);
+        }
     }
 }
 
@@ -734,6 +766,7 @@ struct PullArgs {
     const int32_t* rank;
     int nsrc;
     unsigned long long* spread;   // C_SPREAD area of the counters
+    double* l1;                   // fused snapshot: FP64 residual sum slots, or null
 };
 
 // Bitmap of the p rows the fold made nonzero: one wave per 64 consecutive rows, one ballot,
@@ -922,9 +955,13 @@ __global__ __launch_bounds__(BLOCK, 8) void k_pull(PullArgs a, DevMeasures M) {
     // predicate and outputs as k_snapshot without hot set or trace -- runs on the row in
     // registers: two bitmap words per wave, per-chunk counts reduced per block.
     __shared__ unsigned long long s_cnt[NWAVES][2];
+    __shared__ double s_l1[NWAVES][NM];
     bool pend = false;
     uint64_t d = 0;
     uint32_t nmv = 0;
+    double l1[NM];
+#pragma unroll
+    for (int k = 0; k < NM; k++) l1[k] = 0.0;
     if (tid < nt) {
         const long long u = u0 + tid;
         Row<S> row = load_row<S>(M.r, u);
@@ -936,6 +973,9 @@ __global__ __launch_bounds__(BLOCK, 8) void k_pull(PullArgs a, DevMeasures M) {
                 if (acc != 0.f) { row.v[k] += acc; upd = true; }
             }
         }
+        // the snapshot state's residual: the row after this pass, before the fused fold
+#pragma unroll
+        for (int k = 0; k < NM; k++) l1[k] = fabs((double)row.v[k]);
 #pragma unroll
         for (int k = 0; k < NM; k++) pend |= fabsf(row.v[k]) > M.eps[k];
         if (pend) d = a.idx[u + 1] - a.idx[u];
@@ -976,6 +1016,20 @@ __global__ __launch_bounds__(BLOCK, 8) void k_pull(PullArgs a, DevMeasures M) {
         const long long w0 = wb >> 5, words = ((long long)a.n + 31) / 32;
         a.bitmap[w0] = (uint32_t)mask;
         if (w0 + 1 < words) a.bitmap[w0 + 1] = (uint32_t)(mask >> 32);
+    }
+    if (a.l1) {   // block sums, one FP64 atomic per measure and block
+#pragma unroll
+        for (int k = 0; k < NM; k++) {
+            const double t = wave_sum_f64(l1[k]);
+            if (lane == 0) s_l1[tid / WAVE][k] = t;
+        }
+        __syncthreads();
+        if (tid < NM) {
+            double t = 0.0;
+#pragma unroll
+            for (int w = 0; w < NWAVES; w++) t += s_l1[w][tid];
+            l1_flush(a.l1, blockIdx.x, tid, t);
+        }
     }
     const int c0 = chunk_of((int)u0, a.cv0, a.nc), c1 = chunk_of((int)(u0 + nt - 1), a.cv0, a.nc);
     if (c0 == c1) {   // the usual case: the tile lies in one chunk
@@ -1894,7 +1948,10 @@ struct Ctx {
     int* d_cv0 = nullptr;
     uint16_t* d_trace = nullptr;
     unsigned long long* d_counters = nullptr;  // C_NCOUNTERS
-    unsigned long long* h_small = nullptr;     // pinned: cnt + hotpend
+    unsigned long long* h_small = nullptr;     // pinned: cnt + hotpend + residual slots
+    double l1_last[EXC_MAXM] = {};             // FP64 residual sums of the last snapshot
+    bool l1_valid = false;
+    bool fold_ahead_ok = true;                 // exc_fold_ahead: dense passes may fold the next pass
     uint64_t max_chunk_edges = 0;
     int max_chunk_verts = 0;
     int cus = 256;
@@ -1977,6 +2034,16 @@ BatchArgs batch_args(Ctx* c, int s) {
     B.prow_count = s >= 0 ? c->pb[s].cnt + (2 * c->pb[s].phase + 1) * CNT_STRIDE : nullptr;
     B.prow_cap = s >= 0 ? c->prow_cap[s] : 0;
     return B;
+}
+
+// The counts buffer: per-chunk counts [2 nc], hot-pending, then (from a 128-byte line on)
+// the FP64 residual slots; one clear covers all of it.
+size_t l1_offset(const Ctx* c) { return ((2 * (size_t)c->nc + 1 + L1_STRIDE - 1) / L1_STRIDE) * L1_STRIDE; }
+size_t cnt_words(const Ctx* c) { return l1_offset(c) + (size_t)L1_SLOTS * L1_STRIDE; }
+double* l1_slots(const Ctx* c) { return reinterpret_cast<double*>(c->d_cnt + l1_offset(c)); }
+void clear_counts(const Ctx* c, hipStream_t st) {
+    const int w = (int)cnt_words(c);
+    hipLaunchKernelGGL(k_zero_u64, dim3((w + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, st, c->d_cnt, w);
 }
 
 int ensure_bpieces(Ctx* c, int s, uint64_t edges_bound) {
@@ -2175,13 +2242,13 @@ void launch_fold_pull(Ctx* c, hipStream_t st) {
     // fold the next pass here too while the demand is high (the next pass is then a dense pass
     // whatever its demand; below the threshold the fold stays with k_fold, so the switch to
     // push passes needs no unfolding)
-    const bool fuse_fold = fuse_snap && c->fuse_fold && c->d_p2 &&
+    const bool fuse_fold = fuse_snap && c->fuse_fold && c->fold_ahead_ok && c->d_p2 &&
                            (double)c->last_demand >= std::max(0.1, 2.0 * c->pull_threshold) * (double)c->m;
     if (fuse_snap) {
-        const int ncnt = 2 * c->nc + 1;
-        hipLaunchKernelGGL(k_zero_u64, dim3((ncnt + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, st, c->d_cnt, ncnt);
+        clear_counts(c, st);
         pa.bitmap = c->d_bitmap;
         pa.cnt = c->d_cnt;
+        pa.l1 = l1_slots(c);
         if (fuse_fold) pa.p_next = c->d_p2;
     }
     if (!c->folded) {   // else the last pull already folded this pass into p
@@ -2262,8 +2329,7 @@ void launch_bfold_pull_rows(Ctx* c, hipStream_t st, bool fuse, bool trial) {
                            dim3(BLOCK), 0, st, a);
     }
     if (fuse) {   // the next snapshot's counts are accumulated by k_bpull
-        const int ncnt = 2 * c->nc + 1;
-        hipLaunchKernelGGL(k_zero_u64, dim3((ncnt + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, st, c->d_cnt, ncnt);
+        clear_counts(c, st);
         hipLaunchKernelGGL(k_zero_u32, dim3((c->K + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, st, c->d_colpend, c->K);
     }
     if (trial) hipEventRecord(c->bq_ev[0], st);
@@ -2290,7 +2356,7 @@ void launch_bfold_pull(Ctx* c, hipStream_t st) {
     bq_collect(c);
     const bool trial = c->bq_tune && (double)c->last_demand >= 0.5 * (double)c->m;
     if (trial) c->bpull_q = BQ_CAND[c->bq_trial];
-    const bool fuse = c->fuse_fold && c->d_p2 && c->d_bitmap2 && !c->d_trace &&
+    const bool fuse = c->fuse_fold && c->fold_ahead_ok && c->d_p2 && c->d_bitmap2 && !c->d_trace &&
                       (double)c->last_demand >= std::max(0.1, 2.0 * c->pull_threshold) * (double)c->m;
     if (c->brows16) launch_bfold_pull_rows<uint16_t>(c, st, fuse, trial);
     else launch_bfold_pull_rows<float>(c, st, fuse, trial);
@@ -2434,12 +2500,13 @@ int exc_create(const exc_config* cfg, const exc_measures* M, void** out) {
     CK(hipMemset(c->d_x, 0, n * c->S * sizeof(float)));
     CK(hipMemset(c->d_r, 0, n * c->S * sizeof(float)));
     CK(hipMalloc(&c->d_bitmap, words * sizeof(uint32_t)));
-    CK(hipMalloc(&c->d_cnt, (2 * (size_t)c->nc + 1) * sizeof(unsigned long long)));
+    CK(hipMalloc(&c->d_cnt, cnt_words(c) * sizeof(unsigned long long)));
+    CK(hipMemset(c->d_cnt, 0, cnt_words(c) * sizeof(unsigned long long)));
     CK(hipMalloc(&c->d_cv0, (size_t)c->nc * sizeof(int)));
     CK(hipMemcpy(c->d_cv0, cfg->cv0, (size_t)c->nc * sizeof(int), hipMemcpyHostToDevice));
     CK(hipMalloc(&c->d_counters, C_NCOUNTERS * sizeof(unsigned long long)));
     CK(hipMemset(c->d_counters, 0, C_NCOUNTERS * sizeof(unsigned long long)));
-    CK(hipHostMalloc((void**)&c->h_small, (2 * (size_t)c->nc + 1) * sizeof(unsigned long long), hipHostMallocDefault));
+    CK(hipHostMalloc((void**)&c->h_small, cnt_words(c) * sizeof(unsigned long long), hipHostMallocDefault));
     if (cfg->trace) {
         CK(hipMalloc(&c->d_trace, n * sizeof(uint16_t)));
         CK(hipMemset(c->d_trace, 0, n * sizeof(uint16_t)));
@@ -2532,6 +2599,7 @@ int exc_init_measure(void* p, int i, float beta, float lambda) {
     if (!c || i < 0 || i >= c->nm) return fail(EXC_EINVAL, "exc_init_measure: bad slot");
     HIPCHK(hipSetDevice(c->dev));
     c->snap_ready = false;   // the state changes: a fused snapshot no longer holds
+    c->l1_valid = false;
     c->folded = false;       // the state is replaced: a folded pass no longer applies
     if (c->K) c->active = all_columns(c->K);
     if (c->K) {   // batched: every column starts at x = r = 0 (exc_batch_seed adds the teleport mass)
@@ -2554,6 +2622,7 @@ int exc_set_state(void* p, int i, const float* h_x, const float* h_r) {
     if (!c || i < 0 || i >= (c->K ? c->K : c->nm)) return fail(EXC_EINVAL, "exc_set_state: bad slot");
     HIPCHK(hipSetDevice(c->dev));
     c->snap_ready = false;   // the state changes: a fused snapshot no longer holds
+    c->l1_valid = false;
     c->folded = false;       // the state is replaced: a folded pass no longer applies
     if (c->K) c->active = all_columns(c->K);
     float* tmp;
@@ -2594,6 +2663,7 @@ int exc_state_checkpoint(void* p, int restore) {
     if (!c) return fail(EXC_EINVAL, "exc_state_checkpoint: null context");
     HIPCHK(hipSetDevice(c->dev));
     c->snap_ready = false;   // the state changes: a fused snapshot no longer holds
+    c->l1_valid = false;
     if (c->K) c->active = all_columns(c->K);
     const size_t bytes = (size_t)c->n * c->S * sizeof(float);
     if (restore) {
@@ -2621,11 +2691,17 @@ int exc_state_checkpoint(void* p, int restore) {
 // Per-chunk counts (and hot-pending) of the snapshot on the device to the host.
 int snapshot_counts(Ctx* c, uint64_t* h_cnt, uint64_t* h_hotpend) {
     hipStream_t st = c->st[0];
-    const int ncnt = 2 * c->nc + 1;
-    HIPCHK(hipMemcpyAsync(c->h_small, c->d_cnt, ncnt * sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(c->h_small, c->d_cnt, cnt_words(c) * sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
     memcpy(h_cnt, c->h_small, 2 * (size_t)c->nc * sizeof(uint64_t));
     if (h_hotpend) *h_hotpend = c->h_small[2 * c->nc];
+    const double* l1 = reinterpret_cast<const double*>(c->h_small + l1_offset(c));
+    for (int k = 0; k < EXC_MAXM; k++) {
+        double t = 0.0;
+        for (int q = 0; q < L1_SLOTS; q++) t += l1[q * L1_STRIDE + k];
+        c->l1_last[k] = k < c->nm ? t : 0.0;
+    }
+    c->l1_valid = c->K == 0;
     c->last_demand = 0;
     for (int i = 0; i < c->nc; i++) c->last_demand += c->h_small[2 * i + 1];
     return EXC_OK;
@@ -2636,13 +2712,12 @@ int exc_snapshot(void* p, int hot_skip, uint64_t* h_cnt, uint64_t* h_hotpend) {
     if (!c || !h_cnt) return fail(EXC_EINVAL, "exc_snapshot: bad arguments");
     HIPCHK(hipSetDevice(c->dev));
     hipStream_t st = c->st[0];
-    const int ncnt = 2 * c->nc + 1;
     if ((c->snap_ready || c->folded) && !hot_skip) {   // computed by the dense pass that ended the last pass
         c->snap_ready = false;
         return snapshot_counts(c, h_cnt, h_hotpend);
     }
     c->snap_ready = false;
-    hipLaunchKernelGGL(k_zero_u64, dim3((ncnt + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, st, c->d_cnt, ncnt);
+    clear_counts(c, st);
     SnapArgs a{};
     a.n = c->n;
     a.r = c->d_r;
@@ -2655,6 +2730,7 @@ int exc_snapshot(void* p, int hot_skip, uint64_t* h_cnt, uint64_t* h_hotpend) {
     a.cnt = c->d_cnt;
     a.hotpend = c->d_cnt + 2 * c->nc;
     a.trace = c->d_trace;
+    a.l1 = c->K ? nullptr : l1_slots(c);
     const long long nseg = ((long long)c->n + WAVE - 1) / WAVE;
     const long long max_waves = (long long)c->cus * 16 * NWAVES;   // 16 blocks per CU
     long long spw = (nseg + max_waves - 1) / max_waves;
@@ -2803,6 +2879,7 @@ int exc_push_resident(void* p, const exc_chunk* ch, int use_list, int group, int
     if (!c->d_arena || ch->e1 > c->arena_edges) return fail(EXC_ESTATE, "exc_push_resident: chunk not in the edge cache");
     HIPCHK(hipSetDevice(c->dev));
     c->snap_ready = false;   // the state changes: a fused snapshot no longer holds
+    c->l1_valid = false;
     const bool whole = ch->v0 == 0 && ch->v1 == c->n && !hot_skip;
     if (c->folded && !whole)
         return fail(EXC_ESTATE, "exc_push_resident: the last dense pass folded this pass; it must be a dense pass over all vertices");
@@ -2823,6 +2900,7 @@ int exc_stream_chunk(void* p, int s, const exc_chunk* ch, int to_resident, int h
     if (to_resident && (!c->d_arena || ch->e1 > c->arena_edges)) return fail(EXC_ESTATE, "exc_stream_chunk: chunk not in the edge cache");
     HIPCHK(hipSetDevice(c->dev));
     c->snap_ready = false;   // the state changes: a fused snapshot no longer holds
+    c->l1_valid = false;
     if (c->folded)
         return fail(EXC_ESTATE, "exc_stream_chunk: the last dense pass folded this pass; it must be a dense pass over all vertices");
     Slot& sl = c->slot[s];
@@ -2858,6 +2936,7 @@ int exc_gather_chunk(void* p, int s, int k, int group) {
     if (ne > c->max_chunk_edges) return fail(EXC_EINVAL, "exc_gather_chunk: gathered lists exceed the slot");
     HIPCHK(hipSetDevice(c->dev));
     c->snap_ready = false;   // the state changes: a fused snapshot no longer holds
+    c->l1_valid = false;
     if (c->folded)
         return fail(EXC_ESTATE, "exc_gather_chunk: the last dense pass folded this pass; it must be a dense pass over all vertices");
     hipStream_t st = c->st[1 + s];
@@ -2887,6 +2966,7 @@ int exc_push_hot(void* p, int lo, int hi, int group, int after_slots) {
     if (hi == lo) return EXC_OK;
     HIPCHK(hipSetDevice(c->dev));
     c->snap_ready = false;   // the state changes: a fused snapshot no longer holds
+    c->l1_valid = false;
     if (c->folded)
         return fail(EXC_ESTATE, "exc_push_hot: the last dense pass folded this pass; it must be a dense pass over all vertices");
     hipStream_t st = c->st[0];
@@ -2941,6 +3021,7 @@ int exc_batch_seed(void* p, int col, const int32_t* seeds, int count, float valu
     if (!count) return EXC_OK;
     HIPCHK(hipSetDevice(c->dev));
     c->snap_ready = false;   // the state changes: a fused snapshot no longer holds
+    c->l1_valid = false;
     c->folded = false;       // the state is replaced: a folded pass no longer applies
     if (c->K) c->active = all_columns(c->K);
     int32_t* d;
@@ -2973,11 +3054,15 @@ int exc_batch_set_active(void* p, uint64_t mask) {
 int exc_set_edge_weights(void* p, const float* h_w, uint64_t cnt) {
     Ctx* c = as_ctx(p);
     if (!c || c->K || !h_w) return fail(EXC_EINVAL, "exc_set_edge_weights: bad arguments");
+    // the weighted dense pull exists for one measure only; with more, the pull would sum the
+    // unweighted rows while the push scales them
+    if (c->nm != 1) return fail(EXC_EINVAL, "exc_set_edge_weights: weighted graphs take exactly one measure");
     if (!c->d_arena || c->arena_edges != c->m || cnt != c->m)
         return fail(EXC_ESTATE, "exc_set_edge_weights: the whole graph must be in the edge cache and cnt == m");
     if (c->d_tadj) return fail(EXC_ESTATE, "exc_set_edge_weights: set the weights before exc_enable_pull");
     HIPCHK(hipSetDevice(c->dev));
     c->snap_ready = false;   // the state changes: a fused snapshot no longer holds
+    c->l1_valid = false;
     c->folded = false;       // the state is replaced: a folded pass no longer applies
     if (!c->d_wts) HIPCHK(hipMalloc(&c->d_wts, (c->m ? c->m : 1) * sizeof(float)));
     if (c->m) HIPCHK(hipMemcpy(c->d_wts, h_w, c->m * sizeof(float), hipMemcpyHostToDevice));
@@ -3227,6 +3312,22 @@ int exc_enable_pull(void* p, double threshold) {
         HIPCHK(hipMalloc(&c->d_pull_hubs, hubs.size() * sizeof(PullPiece)));
         HIPCHK(hipMemcpy(c->d_pull_hubs, hubs.data(), hubs.size() * sizeof(PullPiece), hipMemcpyHostToDevice));
     }
+    return EXC_OK;
+}
+
+int exc_residual_l1(void* p, double* out) {
+    Ctx* c = as_ctx(p);
+    if (!c || !out) return fail(EXC_EINVAL, "exc_residual_l1: bad arguments");
+    if (c->K) return fail(EXC_ESTATE, "exc_residual_l1: not kept for batched contexts");
+    if (!c->l1_valid) return fail(EXC_ESTATE, "exc_residual_l1: no snapshot yet");
+    for (int k = 0; k < c->nm; k++) out[k] = c->l1_last[k];
+    return EXC_OK;
+}
+
+int exc_fold_ahead(void* p, int allow) {
+    Ctx* c = as_ctx(p);
+    if (!c) return fail(EXC_EINVAL, "exc_fold_ahead: null context");
+    c->fold_ahead_ok = allow != 0;
     return EXC_OK;
 }
 

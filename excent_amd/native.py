@@ -74,7 +74,8 @@ class Stats(ctypes.Structure):
 class RunStats(ctypes.Structure):
     _fields_ = [("time_passes_s", c_double), ("time_snapshot_s", c_double), ("time_io_s", c_double),
                 ("time_wait_s", c_double), ("passes", c_int), ("pushed_edges", c_u64), ("bytes_stream", c_u64),
-                ("bytes_gather", c_u64), ("launches", c_u64), ("kernel", Stats)]
+                ("bytes_gather", c_u64), ("launches", c_u64), ("kernel", Stats),
+                ("residual_l1", c_double * 3), ("stop_reason", c_int)]
 
 
 def stats_dict(s):
@@ -141,6 +142,8 @@ HIP_SIGNATURES = {
                                P(c_double), P(c_double), P(c_int)]),
     "exc_fp64_lambda": (c_int, [c_void_p, c_void_p, c_int, c_double, P(c_double), P(c_double), P(c_int)]),
     "exc_dense_required": (c_int, [c_void_p]),
+    "exc_residual_l1": (c_int, [c_void_p, P(c_double)]),
+    "exc_fold_ahead": (c_int, [c_void_p, c_int]),
     "exc_fp64_residual": (c_int, [c_void_p, c_double, c_void_p, c_double, c_int, c_void_p, P(c_double)]),
     "exc_fp64_solve_vec": (c_int, [c_void_p, c_void_p, c_double, c_int, c_int, c_double, c_void_p, P(c_double),
                                    P(c_int)]),
@@ -225,7 +228,8 @@ class Session:
         rc = F["excent_session_run"](self.h, int(reinit), ctypes.byref(st))
         if rc != 0:
             raise ExcError(rc, F["excent_session_error"]().decode())
-        out = {k: getattr(st, k) for k, _ in RunStats._fields_ if k != "kernel"}
+        out = {k: getattr(st, k) for k, _ in RunStats._fields_ if k not in ("kernel", "residual_l1")}
+        out["residual_l1"] = list(st.residual_l1)
         out["kernel"] = stats_dict(st.kernel)
         return out
 
@@ -356,6 +360,14 @@ class Context:
 
     def enable_pull(self, threshold):
         return F["exc_enable_pull"](self.ctx, threshold)
+
+    def residual_l1(self):
+        out = (c_double * 3)()
+        check(F["exc_residual_l1"](self.ctx, out), "residual_l1")
+        return list(out)[: self.nm]
+
+    def fold_ahead(self, allow):
+        check(F["exc_fold_ahead"](self.ctx, 1 if allow else 0), "fold_ahead")
 
     def dense_required(self):
         rc = F["exc_dense_required"](self.ctx)

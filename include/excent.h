@@ -30,6 +30,11 @@ typedef struct excent_run_stats {
     int passes;
     uint64_t pushed_edges, bytes_stream, bytes_gather, launches;
     exc_stats kernel;
+    /* FP64 sum of |r| per measure at the end of the loop (exc_residual_l1; 0 for batched runs) */
+    double residual_l1[EXC_MAXM];
+    /* why the loop ended: 0 nothing pending (the reference's rule), 1 every measure's
+     * sum of |r| <= n * eps (--stop global), 2 the pass limit */
+    int stop_reason;
 } excent_run_stats;
 
 /* Session: bin/excent's setup once (graph open, offsets, caches, tau), then any number of

@@ -130,6 +130,14 @@ int exc_state_checkpoint(void* ctx, int restore);
  * pending hot-cached vertices (counted into *h_hotpend).  Blocks until the 2*nc counts
  * and hotpend are on the host.  The bitmap stays on the device. */
 int exc_snapshot(void* ctx, int hot_skip, uint64_t* h_cnt, uint64_t* h_hotpend);
+/* FP64 global residual sums of the last snapshot: out[i] = sum over v of |r_i[v]| for each
+ * of the nm measures, each |r| widened to double and summed in FP64 (per wave or block, then
+ * over 64 partial-sum slots on the host), for the state the snapshot tested -- after a dense
+ * pass that also folded the next pass, the state before that fold.  No reference
+ * counterpart: the reference stops on the per-vertex test alone (src/excent.cu:680); this
+ * is the quantity a global L1 stop and the PageRank bound ||x - x*||_1 <= ||r||_1 / (1 - d)
+ * are stated in.  EXC_ESTATE before the first snapshot of a state, and on batched contexts. */
+int exc_residual_l1(void* ctx, double* out);
 /* Bitmap readback (src/excent.cu:678): ceil(n/32) words. */
 int exc_bitmap_to_host(void* ctx, uint32_t* h_bitmap);
 int exc_read_trace(void* ctx, uint16_t* h_out, size_t v0, size_t cnt);
@@ -157,7 +165,8 @@ int exc_slot_copy_probe(void* ctx, int slot, uint64_t bytes, double* seconds);
 
 /* Weighted graphs (no reference counterpart): per-edge coefficients c_e aligned with the
  * cached edges (the whole graph must be cached).  A push then adds p * c_e along edge e
- * with p = scale * d (set divdeg = 0 when the coefficients are already normalised). */
+ * with p = scale * d (set divdeg = 0 when the coefficients are already normalised).  Only
+ * for contexts with one measure (nm = 1) and no batch columns; EXC_EINVAL otherwise. */
 int exc_set_edge_weights(void* ctx, const float* h_w, uint64_t cnt);
 
 /* Dense passes by pull (no reference counterpart): builds the transpose of the cached
@@ -172,6 +181,11 @@ int exc_enable_pull(void* ctx, double threshold);
  * whole vertex range, which runs as a dense pass whatever its demand; 0 otherwise.  No
  * reference counterpart (the reference folds inside each push). */
 int exc_dense_required(void* ctx);
+/* allow = 0 keeps later dense passes from folding the pass after them (the default, 1, lets
+ * them while the demand is high).  A caller that will stop after the next pass -- a pass
+ * limit, or a stop test it will apply to the next snapshot -- turns it off, so the state it
+ * stops on holds every delta it has folded (x + r = b + W x).  No reference counterpart. */
+int exc_fold_ahead(void* ctx, int allow);
 
 /* Push a resident chunk run [ch.v0, ch.v1) from the arena on the resident stream
  * (launch_chunk with dst = resident[c], src/excent.cu:634-652, 684-688).  use_list:

@@ -52,12 +52,16 @@ def test_batched_snapshot_bit_exact(K, rmat12):
     assert np.array_equal(cnt2, ocnt2)
 
 
+@pytest.mark.parametrize("K", [16, 64])
 @pytest.mark.parametrize("gname", ["sample", "rmat12"])
-def test_batched_ppr_against_fp64(tmp_path, gname, rmat12):
+def test_batched_ppr_against_fp64(tmp_path, gname, rmat12, K):
+    """Every column of a K-column run (K = 64: the full [n][64] rows of the batched
+    configuration) within the exact FP64 bound of its own personalised solution."""
     path = SAMPLE_EGR if gname == "sample" else rmat12
     idx, adj = orc.egr_load(path)
     n = len(idx) - 1
-    K, S = 16, 4
+    S = 4
+    print(f"case batched ppr {gname} K={K}", flush=True)
     rng = np.random.default_rng(3)
     seeds = np.stack([rng.choice(n, S, replace=False) for _ in range(K)]).astype(np.int32)
     sf = str(tmp_path / "seeds.i32")
@@ -68,7 +72,7 @@ def test_batched_ppr_against_fp64(tmp_path, gname, rmat12):
                      str(tmp_path / "o"), "--stream-ratio", "0.5", "--log", log])
     text = open(log).read()
     assert rc == 0, text
-    assert "active columns" in text and re.search(r"16 columns, ppr sum", text)
+    assert "active columns" in text and re.search(rf"{K} columns, ppr sum", text)
     X = np.fromfile(str(tmp_path / "o.ppr.bin"), np.float32).reshape(K, n)
     mass = np.float32((1 - np.float64(np.float32(0.85))) * n / S)
     for j in range(K):
@@ -208,10 +212,11 @@ def test_batched_pull_rows_fixed_by_env(rmat12, monkeypatch):
         assert np.abs(X[j] - ref).sum() <= bound * (1 + 1e-6) + 1e-6, j
 
 
+@pytest.mark.parametrize("K", [5, 64])
 @pytest.mark.parametrize("rows", ["bf16", "f32"])
 @pytest.mark.parametrize("fuse", [0, 1])
 @pytest.mark.parametrize("gname", ["rmat12", "star"])
-def test_batched_dense_passes(gname, fuse, rows, rmat12, monkeypatch):
+def test_batched_dense_passes(gname, fuse, rows, K, rmat12, monkeypatch):
     """Batched fold + pull over the transpose (every pass dense): each column within the
     exact FP64 bound, per-vertex |r| <= eps at the end, the residual invariant
     r = b + W x - x per vertex within 1e-4 |x| + eps (float rounding reaches 2e-5 |x| at the
@@ -221,10 +226,11 @@ def test_batched_dense_passes(gname, fuse, rows, rmat12, monkeypatch):
     fold stays in k_bfold and the snapshot after every pass is checked bit for bit against
     the oracle's; fuse = 1: k_bpull also computes the next snapshot and fold (the state read
     back after such a pass is already folded, so only the end results are checked).
-    rows: the weight rows' storage (EXC_BATCH_ROWS)."""
+    rows: the weight rows' storage (EXC_BATCH_ROWS).  K = 64 fills every lane of a wave with a
+    column (bf16 rows: a gather covers two rows, lanes 0-31 and 32-63)."""
     monkeypatch.setenv("EXC_FUSE_FOLD", str(fuse))
     monkeypatch.setenv("EXC_BATCH_ROWS", rows)
-    print(f"case batched dense passes {gname} fuse={fuse} rows={rows}", flush=True)
+    print(f"case batched dense passes {gname} fuse={fuse} rows={rows} K={K}", flush=True)
     if gname == "star":
         L = 9000
         deg = np.concatenate([[L], np.ones(L, np.int64)])
@@ -233,7 +239,7 @@ def test_batched_dense_passes(gname, fuse, rows, rmat12, monkeypatch):
     else:
         idx, adj = orc.egr_load(rmat12)
     n, m = len(idx) - 1, len(adj)
-    K, S, eps = 5, 3, 1e-4
+    S, eps = 3, 1e-4
     rng = np.random.default_rng(21)
     seeds = [rng.choice(n, S, replace=False).astype(np.int32) for _ in range(K)]
     mass = np.float32(0.15 * n / S)

@@ -101,9 +101,100 @@ def test_out_of_core_regimes(tmp_path, rmat14, mode):
         assert float(m.group(1)) == 0.0 and float(m.group(2)) > 0
     if mode == "hot":
         assert "GPU degree cache:" in text
-        assert any("cached high-degree pending" in l and not l.rstrip().endswith("pending 0") for l in lines[1:]) or True
     if mode == "preload":
         assert " resident " in lines[0] and "stream 0" in lines[0]
+
+
+PASS_RE = re.compile(r"pass +(\d+): pending (\d+) v, demand ratio [0-9.]+ \| resident (\d+) stream (\d+) gather (\d+) \| "
+                     r"cached high-degree pending (\d+)")
+
+
+@pytest.mark.parametrize("when,order", [("end", "sweep"), ("start", "sweep"), ("both", "sweep"), ("end", "ranges"),
+                                        ("both", "ranges")])
+def test_hot_cache_partial(tmp_path, rmat14, when, order):
+    """The hot-vertex cache holding only part of the edges (--cache-mb caps its budget at
+    0.25 MB of rmat14's ~1 MB of lists): the highest-degree lists are pushed from HBM at the
+    start, the end or both ends of each pass (src/excent.cu:683, 745), as one sweep or one
+    range per chunk (612-614), while the other pending vertices are streamed or gathered."""
+    idx, adj = orc.egr_load(rmat14)
+    n, m = len(idx) - 1, len(adj)
+    args = [rmat14, "--measure", "pr", "--dump", str(tmp_path / "o"), "--chunk-medges", "0.02", "--cache", "hot",
+            "--cache-mb", "0.25", "--stream-ratio", "0.3", "--hot-when", when, "--hot-order", order]
+    print("case hot cache", when, order, flush=True)
+    rc, text = run(args, tmp_path)
+    assert rc == 0, text
+    mo = re.search(r"GPU degree cache: (\d+) vertices with degree >= (\d+) \(([0-9.]+)% of edges", text)
+    assert mo, text
+    nh, dmin, pct = int(mo.group(1)), int(mo.group(2)), float(mo.group(3))
+    deg = np.diff(idx.astype(np.int64))
+    # the cache holds the highest-degree lists: every vertex above the threshold degree
+    assert 0 < nh < n and 0 < pct < 60 and (deg > dmin).sum() <= nh <= (deg >= dmin).sum()
+    assert (4 * deg[deg > dmin] + 16).sum() <= 0.25 * (1 << 20)
+    passes = [tuple(int(g) for g in p.groups()) for p in PASS_RE.finditer(text)]
+    assert len(passes) > 2
+    # after pass 0, pending hot vertices are counted apart and pushed from the cache, and the
+    # other pending vertices are fetched from the file
+    assert any(p[0] > 0 and p[5] > 0 for p in passes), text
+    assert any(p[0] > 0 and p[3] + p[4] > 0 for p in passes), text
+    assert all(p[2] == 0 for p in passes)   # nothing is resident in hot mode
+    pr_check(idx, adj, np.fromfile(str(tmp_path / "o.pr.bin"), np.float32))
+
+
+def test_tau_calibration(tmp_path, rmat14, monkeypatch):
+    """--stream-ratio auto (the default): tau is measured on the middle chunk, written to
+    ~/.cache/excent/<graph>.<n>.<m>.tau as "%.6f %.4f" (src/excent.cu:518-579), read back by
+    the next run, and measured again with --recalibrate; tau stays within [0.005, 1] and the
+    results within the PageRank bound."""
+    monkeypatch.setenv("HOME", str(tmp_path))
+    idx, adj = orc.egr_load(rmat14)
+    n, m = len(idx) - 1, len(adj)
+    base = [rmat14, "--measure", "pr", "--dump", str(tmp_path / "o"), "--chunk-medges", "0.02", "--cache", "none"]
+    tau_file = tmp_path / ".cache" / "excent" / f"{os.path.basename(rmat14)}.{n}.{m}.tau"
+    calib = re.compile(r"fetch rule: tau = ([0-9.]+) \(stream ([0-9.]+)s; gather ([0-9.]+)s / ([0-9.]+)s / ([0-9.]+)s "
+                       r"at demand ([0-9.]+) / ([0-9.]+) / ([0-9.]+)\)  calibrate ([0-9.]+)s")
+    for step, extra in (("first", []), ("cached", []), ("recalibrate", ["--recalibrate"])):
+        print("case tau", step, flush=True)
+        rc, text = run(base + extra, tmp_path, step)
+        assert rc == 0, text
+        if step == "cached":
+            mo = re.search(r"fetch rule: tau = ([0-9.]+) \(measured earlier on this machine in ([0-9.]+)s, (.+)\)", text)
+            assert mo and mo.group(3) == str(tau_file), text
+            assert float(mo.group(1)) == pytest.approx(tau, abs=5e-4)
+        else:
+            mo = calib.search(text)
+            assert mo, text
+            tau = float(mo.group(1))
+            assert 0.005 <= tau <= 1.0
+            demands = [float(mo.group(i)) for i in (6, 7, 8)]
+            assert demands == sorted(demands) and demands[2] <= 1.0
+            body = tau_file.read_text()
+            assert re.fullmatch(r"\d+\.\d{6} \d+\.\d{4}\n", body), body
+            assert float(body.split()[0]) == pytest.approx(tau, abs=5e-4)
+        pr_check(idx, adj, np.fromfile(str(tmp_path / "o.pr.bin"), np.float32))
+
+
+def test_rmat16_pagerank_config(tmp_path, rmat16):
+    """R-MAT scale 16 PageRank (d = 0.85, eps 0.01, r0 = 0.15, whole graph cached): with the
+    dense passes (default) and with the push alone, against the FP64 solution -- the exact
+    bound ||x - x*||_1 <= ||b + W x - x||_1 / (1 - d) with the residual of the returned x in
+    FP64, the eps bound, and top-100 overlap."""
+    idx, adj = orc.egr_load(rmat16)
+    n = len(idx) - 1
+    assert n == 65536 and 900_000 < len(adj) <= 1 << 20
+    ref, _, _, _ = orc.fp64_reference(idx, adj, orc.ORC_PR, tol=1e-12, max_iter=3000)
+    for pull in ("auto", "off"):
+        print("case rmat16 pr pull", pull, flush=True)
+        rc, text = run([rmat16, "--measure", "pr", "--cache", "chunks", "--preload", "--pull", pull, "--dump",
+                        str(tmp_path / "o"), "--count-pushes"], tmp_path, pull)
+        assert rc == 0, text
+        assert "GPU edge cache: 1/1 chunks resident (100.0% of edges)" in text
+        x = np.fromfile(str(tmp_path / "o.pr.bin"), np.float32)
+        err = np.abs(x.astype(np.float64) - ref).sum()
+        res = orc.residual_l1(idx, adj, orc.ORC_PR, 0.85, 0.15, x)
+        assert err <= res / 0.15 * (1 + 1e-6) + 1e-6, (err, res)
+        assert err <= 0.01 * n / 0.15
+        c = orc.compare(ref, x)
+        assert c["top100"] >= 95, c
 
 
 @pytest.fixture(scope="module")

@@ -339,3 +339,19 @@ def test_weighted_dense_passes(gname, rmat12):
     tr = b + acc - x.astype(np.float64)
     assert np.abs(x.astype(np.float64) - ref).sum() <= np.abs(tr).sum() / (1 - sf) * (1 + 1e-6) + 1e-6
     assert np.all(np.abs(r) <= eps)
+
+
+def test_weighted_graph_refuses_several_measures(rmat12):
+    """Per-edge coefficients are accepted only by a one-measure context: with two or three
+    measures the call fails (the weighted dense pull is built for one measure)."""
+    idx, adj = orc.egr_load(rmat12)
+    n, m = len(idx) - 1, len(adj)
+    wn = np.full(m, 0.5, np.float32)
+    for kinds in ([nat.EXC_PR, nat.EXC_KATZ], [nat.EXC_PR, nat.EXC_KATZ, nat.EXC_EV]):
+        k = len(kinds)
+        with nat.Context(n, m, np.array([0], np.int32), kinds, [1e-3] * k, [0.85] * k, [1] + [0] * (k - 1)) as ctx:
+            ctx.upload_offsets(idx)
+            assert ctx.cache_chunks(m) == 0
+            ctx.cache_upload(adj, 0)
+            assert nat.F["exc_set_edge_weights"](ctx.ctx, nat.ptr(wn), m) == nat.EXC_EINVAL
+            assert "one measure" in nat.F["exc_last_error"]().decode()
