@@ -14,6 +14,7 @@
 #include <hip/hip_runtime.h>
 
 #include <chrono>
+#include <mutex>
 #include <cstdarg>
 #include <cstdio>
 #include <cstdlib>
@@ -2426,6 +2427,83 @@ unsigned grid_for(long long items, int cap_blocks) {
     return (unsigned)g;
 }
 
+// Plain (non-batched) snapshot launch: waves take consecutive 64-vertex segments, at most 16
+// blocks per CU in flight.  Shared by exc_snapshot and exc_snapshot_noalloc.
+void launch_snapshot_plain(SnapArgs a, int nm, int cus, hipStream_t st) {
+    const long long nseg = ((long long)a.n + WAVE - 1) / WAVE;
+    const long long max_waves = (long long)cus * 16 * NWAVES;
+    long long spw = (nseg + max_waves - 1) / max_waves;
+    if (spw < 1) spw = 1;
+    a.segs_per_wave = (int)spw;
+    const long long waves = (nseg + spw - 1) / spw;
+    const unsigned grid = (unsigned)((waves + NWAVES - 1) / NWAVES);
+    if (nm == 1) hipLaunchKernelGGL((k_snapshot<1, 1>), dim3(grid), dim3(BLOCK), 0, st, a);
+    else if (nm == 2) hipLaunchKernelGGL((k_snapshot<2, 2>), dim3(grid), dim3(BLOCK), 0, st, a);
+    else hipLaunchKernelGGL((k_snapshot<3, 4>), dim3(grid), dim3(BLOCK), 0, st, a);
+}
+
+// ---------------------------------------------------------- allocation-free snapshot
+//
+// exc_snapshot_noalloc's scratch: the per-chunk count accumulators and hot-pending counter
+// ([2 nc + 1], rounded up to a 128-byte line) followed by the FP64 residual slots
+// (L1_SLOTS x L1_STRIDE doubles), laid out as the context's counts buffer.  One array per
+// GPU for the life of the process; every call clears the part it uses first.
+constexpr int NOALLOC_MAX_CHUNKS = 4096;
+constexpr size_t NOALLOC_L1_OFF = ((2 * (size_t)NOALLOC_MAX_CHUNKS + 1 + L1_STRIDE - 1) / L1_STRIDE) * L1_STRIDE;
+constexpr size_t NOALLOC_WORDS = NOALLOC_L1_OFF + (size_t)L1_SLOTS * L1_STRIDE;   // 9232 words, 72 KiB
+
+}  // namespace
+
+static __device__ __align__(256) unsigned long long scratch[NOALLOC_WORDS];
+
+namespace {
+
+// Outputs of an allocation-free snapshot: the counts copied out of the scratch, and the FP64
+// residual per measure summed over the slots in slot order (the order exc_snapshot's host
+// sum uses).
+__global__ void k_snapshot_finish(const unsigned long long* __restrict__ scr, int nc, int nm,
+                                  unsigned long long* counts, double* l1) {
+    const int ncnt = 2 * nc + 1;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < ncnt; i += gridDim.x * blockDim.x) counts[i] = scr[i];
+    if (blockIdx.x == 0 && threadIdx.x < nm) {
+        const double* slots = reinterpret_cast<const double*>(scr + NOALLOC_L1_OFF);
+        double t = 0.0;
+        for (int q = 0; q < L1_SLOTS; q++) t += slots[q * L1_STRIDE + threadIdx.x];
+        l1[threadIdx.x] = t;
+    }
+}
+
+// Device address of `scratch` and the CU count, resolved once per device (with the device
+// current) by the first call on it; host code uses only these addresses, never the
+// array's name.
+constexpr int NOALLOC_MAX_DEVICES = 64;
+struct NoallocDev {
+    unsigned long long* ptr = nullptr;
+    int cus = 0;
+};
+NoallocDev g_noalloc[NOALLOC_MAX_DEVICES];
+std::mutex g_noalloc_mu;
+
+int noalloc_setup(int device, NoallocDev* out) {
+    std::lock_guard<std::mutex> lock(g_noalloc_mu);
+    NoallocDev& d = g_noalloc[device];
+    if (!d.ptr) {
+        void* ptr = nullptr;
+        hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr)
+            return fail(EXC_EHIP, "exc_snapshot_noalloc: hipGetSymbolAddress(scratch) on device %d: %s", device,
+                        hipGetErrorString(e));
+        int cus = 0;
+        e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device);
+        if (e != hipSuccess || cus < 1)
+            return fail(EXC_EHIP, "exc_snapshot_noalloc: CU count of device %d: %s", device, hipGetErrorString(e));
+        d.cus = cus;
+        d.ptr = static_cast<unsigned long long*>(ptr);
+    }
+    *out = d;
+    return EXC_OK;
+}
+
 }  // namespace
 
 void exc_internal_set_error(const char* msg) { g_err = msg; }
@@ -2732,12 +2810,6 @@ int exc_snapshot(void* p, int hot_skip, uint64_t* h_cnt, uint64_t* h_hotpend) {
     a.trace = c->d_trace;
     a.l1 = c->K ? nullptr : l1_slots(c);
     const long long nseg = ((long long)c->n + WAVE - 1) / WAVE;
-    const long long max_waves = (long long)c->cus * 16 * NWAVES;   // 16 blocks per CU
-    long long spw = (nseg + max_waves - 1) / max_waves;
-    if (spw < 1) spw = 1;
-    a.segs_per_wave = (int)spw;
-    const long long waves = (nseg + spw - 1) / spw;
-    const unsigned grid = (unsigned)((waves + NWAVES - 1) / NWAVES);
     if (c->K) {
         hipLaunchKernelGGL(k_zero_u32, dim3((c->K + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, st, c->d_colpend, c->K);
         const long long bw = (long long)c->cus * 64 * NWAVES;
@@ -2751,9 +2823,7 @@ int exc_snapshot(void* p, int hot_skip, uint64_t* h_cnt, uint64_t* h_hotpend) {
         c->snap_bytes += 4.0 * c->K * c->n + 4.0 * (((double)c->n + 31) / 32);
     } else {
         ScopedTimer t(c, st, T_SNAP);
-        if (c->nm == 1) hipLaunchKernelGGL((k_snapshot<1, 1>), dim3(grid), dim3(BLOCK), 0, st, a);
-        else if (c->nm == 2) hipLaunchKernelGGL((k_snapshot<2, 2>), dim3(grid), dim3(BLOCK), 0, st, a);
-        else hipLaunchKernelGGL((k_snapshot<3, 4>), dim3(grid), dim3(BLOCK), 0, st, a);
+        launch_snapshot_plain(a, c->nm, c->cus, st);
         c->snap_bytes += 4.0 * c->nm * c->n + 4.0 * (((double)c->n + 31) / 32);
     }
     HIPCHK(hipGetLastError());
@@ -3350,6 +3420,46 @@ int exc_reset_stats(void* p) {
     c->snap_bytes = 0;
     c->pull_passes = 0;
     c->pull_passes_live = 0;
+    return EXC_OK;
+}
+
+int exc_snapshot_noalloc(int device, void* stream, int n, int nm, const float* d_r, const float* eps,
+                         const uint64_t* d_idx, const int* d_cv0, int nc, const uint32_t* d_hot, uint32_t* d_bitmap,
+                         unsigned long long* d_counts, double* d_l1) {
+    if (n < 1 || nm < 1 || nm > EXC_MAXM || !d_r || !eps || !d_idx || !d_cv0 || !d_bitmap || !d_counts || !d_l1)
+        return fail(EXC_EINVAL, "exc_snapshot_noalloc: bad arguments (n >= 1, 1 <= nm <= %d, non-null buffers)", EXC_MAXM);
+    if (nc < 1 || nc > NOALLOC_MAX_CHUNKS)
+        return fail(EXC_EINVAL, "exc_snapshot_noalloc: %d chunks; its scratch holds the counts of 1 to %d chunks", nc,
+                    NOALLOC_MAX_CHUNKS);
+    int ndev = 0;
+    HIPCHK(hipGetDeviceCount(&ndev));
+    if (device < 0 || device >= ndev || device >= NOALLOC_MAX_DEVICES)
+        return fail(EXC_EINVAL, "exc_snapshot_noalloc: no device %d", device);
+    HIPCHK(hipSetDevice(device));
+    NoallocDev d;
+    int rc = noalloc_setup(device, &d);
+    if (rc) return rc;
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    // clear the counts and the residual slots (a fill kernel, so a captured call replays it)
+    const int words = (int)NOALLOC_WORDS;
+    hipLaunchKernelGGL(k_zero_u64, dim3((words + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, st, d.ptr, words);
+    SnapArgs a{};
+    a.n = n;
+    a.r = d_r;
+    for (int k = 0; k < EXC_MAXM; k++) a.eps[k] = k < nm ? eps[k] : 0.f;
+    a.idx = d_idx;
+    a.cv0 = d_cv0;
+    a.nc = nc;
+    a.hot = d_hot;
+    a.bitmap = d_bitmap;
+    a.cnt = d.ptr;
+    a.hotpend = d.ptr + 2 * nc;
+    a.trace = nullptr;
+    a.l1 = reinterpret_cast<double*>(d.ptr + NOALLOC_L1_OFF);
+    launch_snapshot_plain(a, nm, d.cus, st);
+    hipLaunchKernelGGL(k_snapshot_finish, dim3((2 * nc + 1 + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, st, d.ptr, nc, nm,
+                       d_counts, d_l1);
+    HIPCHK(hipGetLastError());
     return EXC_OK;
 }
 

@@ -144,6 +144,8 @@ HIP_SIGNATURES = {
     "exc_dense_required": (c_int, [c_void_p]),
     "exc_residual_l1": (c_int, [c_void_p, P(c_double)]),
     "exc_fold_ahead": (c_int, [c_void_p, c_int]),
+    "exc_snapshot_noalloc": (c_int, [c_int, c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_int,
+                                     c_void_p, c_void_p, c_void_p, c_void_p]),
     "exc_fp64_residual": (c_int, [c_void_p, c_double, c_void_p, c_double, c_int, c_void_p, P(c_double)]),
     "exc_fp64_solve_vec": (c_int, [c_void_p, c_void_p, c_double, c_int, c_int, c_double, c_void_p, P(c_double),
                                    P(c_int)]),

@@ -138,6 +138,22 @@ int exc_snapshot(void* ctx, int hot_skip, uint64_t* h_cnt, uint64_t* h_hotpend);
  * is the quantity a global L1 stop and the PageRank bound ||x - x*||_1 <= ||r||_1 / (1 - d)
  * are stated in.  EXC_ESTATE before the first snapshot of a state, and on batched contexts. */
 int exc_residual_l1(void* ctx, double* out);
+/* Allocation-free demand snapshot.  No counterpart in the reference (whose snapshot is
+ * k_snapshot, src/excent.cu:69-98, on buffers main() allocates): it is exc_snapshot -- same
+ * kernel, same outputs, bit for bit -- for callers that cannot allocate (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  No context: the caller
+ * passes device buffers -- r as [n][S] rows (S = 1, 2, 4 for nm = 1, 2, 3), the n+1 offsets,
+ * cv0[nc], an optional hot bitmask of ceil(n/32) words (NULL: none), the bitmap output of
+ * ceil(n/32) words, counts[2 nc + 1] (per-chunk pending count and out-degree sum, then
+ * hot-pending) and l1[nm] (FP64 sum of |r| per measure) -- and eps[nm] on the host.  The
+ * work is enqueued on `stream` (NULL: the null stream) and not waited for.  The count
+ * accumulators and residual partial sums live in a static device array of 72 KiB per GPU,
+ * held until the process ends: nc is limited to 1..4096 (EXC_EINVAL beyond).  Calls
+ * clear it first; calls that overlap on one GPU share it and race.  Calls no hipMalloc,
+ * hipFree or other allocating routine. */
+int exc_snapshot_noalloc(int device, void* stream, int n, int nm, const float* d_r, const float* eps,
+                         const uint64_t* d_idx, const int* d_cv0, int nc, const uint32_t* d_hot, uint32_t* d_bitmap,
+                         unsigned long long* d_counts, double* d_l1);
 /* Bitmap readback (src/excent.cu:678): ceil(n/32) words. */
 int exc_bitmap_to_host(void* ctx, uint32_t* h_bitmap);
 int exc_read_trace(void* ctx, uint16_t* h_out, size_t v0, size_t cnt);
