@@ -6,6 +6,7 @@
 #include "egr_graph.hpp"
 #include "excent.h"
 #include "planner.hpp"
+#include "staging.hpp"
 
 namespace {
 thread_local std::string g_err;
@@ -86,7 +87,20 @@ int excent_prefetch_counts(const char* path, const int32_t* vids, int k, int io_
                            uint64_t* pages, uint64_t* runs) {
     try {
         excent::EgrGraph g(path, io_threads, false);
-        g.prefetch_lists(vids, k, pages, runs);
+        g.prefetch_lists(vids, k, pages, runs, io_threads);   // the reference splits by its thread count
+        return 0;
+    } catch (const std::exception& e) { return err(e.what()); }
+}
+
+int excent_stage_lists(const char* path, const int32_t* vids, int k, int dedup, int threads, uint64_t* off_out,
+                       int32_t* edges_out, uint64_t* pages, uint64_t* runs) {
+    try {
+        excent::EgrGraph g(path, threads, false);
+        for (int i = 0; i < k; i++)
+            if (vids[i] < 0 || vids[i] >= g.n) return err("vertex id out of range");
+        const excent::GatherStats st = excent::stage_lists(g, vids, k, off_out, edges_out, dedup != 0, threads);
+        *pages = st.pages;
+        *runs = st.runs;
         return 0;
     } catch (const std::exception& e) { return err(e.what()); }
 }

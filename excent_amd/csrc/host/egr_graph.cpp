@@ -1,4 +1,4 @@
-// egr_graph.cpp -- see egr_graph.hpp.  Reference: include/StaticGraph.h:29-170.
+// egr_graph.cpp -- see egr_graph.hpp.
 #include "egr_graph.hpp"
 
 #include <fcntl.h>
@@ -8,155 +8,244 @@
 #include <unistd.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cerrno>
 #include <cstring>
+#include <mutex>
 
 namespace excent {
+namespace {
 
-EgrGraph::EgrGraph(const std::string& path, int io_threads, bool offsets_in_ram)
-    : io_threads_(io_threads), path_(path) {
-    fd_ = ::open(path.c_str(), O_RDONLY);
+constexpr size_t kBlock = 4ull << 20;   // unit of work of the I/O pool
+
+size_t page_size() {
+    static const size_t p = (size_t)sysconf(_SC_PAGESIZE);
+    return p;
+}
+
+// pread until `len` bytes are in, retrying interrupted and short reads.
+void pread_all(int fd, void* dst, size_t len, size_t at, const std::string& path) {
+    char* p = static_cast<char*>(dst);
+    while (len) {
+        const ssize_t got = pread(fd, p, len, (off_t)at);
+        if (got < 0 && errno == EINTR) continue;
+        if (got <= 0)
+            throw egr_error("pread: " + path + (got == 0 ? ": unexpected end of file" : ": " + std::string(strerror(errno))));
+        p += got;
+        at += (size_t)got;
+        len -= (size_t)got;
+    }
+}
+
+// Page runs of one group of lists: spans are offered in list order; a span that starts at
+// or before the end of the open run extends it, any other closes the run (advised with
+// MADV_WILLNEED and counted) and opens a new one.
+class RunMerger {
+public:
+    RunMerger(const char* map, size_t page) : map_(map), page_(page) {}
+    void add(size_t p0, size_t p1) {
+        if (open_ && p0 <= hi_) {
+            hi_ = std::max(hi_, p1);
+            return;
+        }
+        close();
+        lo_ = p0;
+        hi_ = p1;
+        open_ = true;
+    }
+    void close() {
+        if (!open_) return;
+        madvise(const_cast<char*>(map_) + lo_ * page_, (hi_ - lo_) * page_, MADV_WILLNEED);
+        pages += hi_ - lo_;
+        runs++;
+        open_ = false;
+    }
+    uint64_t pages = 0, runs = 0;
+
+private:
+    const char* map_;
+    size_t page_, lo_ = 0, hi_ = 0;
+    bool open_ = false;
+};
+
+}  // namespace
+
+EgrGraph::EgrGraph(const std::string& path, int io_threads, bool offsets_in_ram) : path_(path) {
+    io_threads_ = io_threads > 0 ? io_threads : std::max(1, omp_get_max_threads());
+    fd_ = ::open(path.c_str(), O_RDONLY | O_CLOEXEC);
     if (fd_ < 0) throw egr_error(path + ": " + strerror(errno));
-    int32_t hdr[2];
     try {
-        read_exact(hdr, sizeof hdr, 0);
-    } catch (...) { ::close(fd_); fd_ = -1; throw; }
-    n = hdr[0];
-    size_t off = sizeof hdr;
-    wide = hdr[1] == -1;
-    if (wide) { read_exact(&m, sizeof m, off); off += sizeof m; }
-    else m = (uint64_t)(int64_t)hdr[1];   // StaticGraph.h:38 converts the int to ull
-    if (n < 1) { ::close(fd_); fd_ = -1; throw egr_error("StaticGraph: bad header in " + path); }
-    idx_off_ = off;
-    if (!wide) offsets_in_ram = true;   // narrow offsets are widened into RAM (StaticGraph.h:42)
-    if (wide && !offsets_in_ram) {
-        off += ((size_t)n + 1) * sizeof(uint64_t);
-    } else if (wide) {
-        nindex_.resize((size_t)n + 1);
-        read_parallel(nindex_.data(), nindex_.size() * sizeof(uint64_t), off);
-        off += nindex_.size() * sizeof(uint64_t);
-    } else {
-        nindex_.resize((size_t)n + 1);
-        std::vector<int32_t> tmp((size_t)n + 1);
-        read_parallel(tmp.data(), tmp.size() * sizeof(int32_t), off);
-        off += tmp.size() * sizeof(int32_t);
-        for (size_t i = 0; i < tmp.size(); i++) nindex_[i] = (uint64_t)(uint32_t)tmp[i];
+        load(offsets_in_ram);
+    } catch (...) {
+        if (map_) munmap(const_cast<char*>(map_), file_bytes_);
+        map_ = nullptr;
+        ::close(fd_);
+        fd_ = -1;
+        throw;
     }
-    nlist_off_ = off;
-    struct stat st;
-    if (fstat(fd_, &st) != 0) throw egr_error(path + ": fstat failed");
-    map_size_ = (size_t)st.st_size;
-    const size_t need = nlist_off_ + m * sizeof(int32_t);
-    if (map_size_ < need) {
-        ::close(fd_); fd_ = -1;
-        throw egr_error("StaticGraph: " + path + " is shorter than its header says (" + std::to_string(map_size_) +
-                        " < " + std::to_string(need) + " bytes)");
-    }
-    map_ = mmap(nullptr, map_size_, PROT_READ, MAP_SHARED, fd_, 0);
-    if (map_ == MAP_FAILED) { map_ = nullptr; ::close(fd_); fd_ = -1; throw egr_error("mmap: " + std::string(strerror(errno))); }
-    madvise(map_, map_size_, MADV_RANDOM);
-    posix_fadvise(fd_, 0, 0, POSIX_FADV_SEQUENTIAL);
-    nlist_ = (const int32_t*)((const char*)map_ + nlist_off_);
-    off_ = offsets_in_ram ? nindex_.data() : (const uint64_t*)((const char*)map_ + idx_off_);
 }
 
 EgrGraph::~EgrGraph() {
-    if (map_) munmap(map_, map_size_);
+    if (map_) munmap(const_cast<char*>(map_), file_bytes_);
     if (fd_ >= 0) ::close(fd_);
 }
 
-void EgrGraph::read_exact(void* dst, size_t bytes, size_t off) const {
-    char* p = (char*)dst;
-    while (bytes) {
-        ssize_t r = pread(fd_, p, bytes, (off_t)off);
-        if (r <= 0) throw egr_error("pread: " + path_ + (r == 0 ? ": unexpected end of file" : ": " + std::string(strerror(errno))));
-        p += r; off += (size_t)r; bytes -= (size_t)r;
+void EgrGraph::load(bool offsets_in_ram) {
+    struct stat st;
+    if (fstat(fd_, &st) != 0) throw egr_error(path_ + ": fstat: " + strerror(errno));
+    file_bytes_ = (size_t)st.st_size;
+    // header: int32 n, then int32 m, or -1 and an int64 m
+    unsigned char head[16] = {0};
+    pread_all(fd_, head, std::min<size_t>(sizeof head, std::max<size_t>(file_bytes_, 8)), 0, path_);
+    int32_t h0, h1;
+    memcpy(&h0, head, 4);
+    memcpy(&h1, head + 4, 4);
+    n = h0;
+    wide = h1 == -1;
+    if (wide) {
+        int64_t mw;
+        memcpy(&mw, head + 8, 8);
+        m = (uint64_t)mw;
+    } else {
+        m = (uint64_t)(int64_t)h1;   // the reference widens the int32 to unsigned long long
     }
+    if (n < 1) throw egr_error("StaticGraph: bad header in " + path_);
+    offsets_at_ = wide ? 16 : 8;
+    const size_t off_width = wide ? sizeof(uint64_t) : sizeof(uint32_t);
+    cols_at_ = offsets_at_ + ((size_t)n + 1) * off_width;
+    const size_t need = cols_at_ + m * sizeof(int32_t);
+    if (m > (SIZE_MAX - cols_at_) / sizeof(int32_t) || file_bytes_ < need)
+        throw egr_error("StaticGraph: " + path_ + " is shorter than its header says (" + std::to_string(file_bytes_) + " < " +
+                        std::to_string(need) + " bytes)");
+
+    void* mp = mmap(nullptr, file_bytes_, PROT_READ, MAP_SHARED, fd_, 0);
+    if (mp == MAP_FAILED) throw egr_error("mmap: " + path_ + ": " + strerror(errno));
+    map_ = static_cast<const char*>(mp);
+    // gathers touch scattered lists through the mapping (no readahead beyond what they
+    // advise); streamed chunks are read in file order through the descriptor
+    madvise(mp, file_bytes_, MADV_RANDOM);
+    posix_fadvise(fd_, 0, 0, POSIX_FADV_SEQUENTIAL);
+    cols_ = reinterpret_cast<const int32_t*>(map_ + cols_at_);
+
+    if (wide && !offsets_in_ram) {
+        off_ = reinterpret_cast<const uint64_t*>(map_ + offsets_at_);
+        return;
+    }
+    ram_off_.resize((size_t)n + 1);
+    if (wide) {
+        read_range(ram_off_.data(), ram_off_.size() * sizeof(uint64_t), offsets_at_);
+    } else {   // uint32 offsets, widened
+        std::vector<uint32_t> narrow((size_t)n + 1);
+        read_range(narrow.data(), narrow.size() * sizeof(uint32_t), offsets_at_);
+        const long long cnt = (long long)narrow.size();
+        uint64_t* dst = ram_off_.data();
+        const uint32_t* src = narrow.data();
+#pragma omp parallel for schedule(static) num_threads(io_threads_)
+        for (long long i = 0; i < cnt; i++) dst[i] = src[i];
+    }
+    off_ = ram_off_.data();
 }
 
-// StaticGraph.h:160-170: 8 MB pieces, one OpenMP thread each, up to io_threads.
-void EgrGraph::read_parallel(void* dst, size_t bytes, size_t off) const {
-    const size_t piece = 8ull << 20;
-    const long long pieces = (long long)((bytes + piece - 1) / piece);
-    const int threads = (int)std::min<long long>(io_threads_, std::max<long long>(1, pieces));
-    std::string err;
-#pragma omp parallel for num_threads(threads) schedule(static, 1)
-    for (long long i = 0; i < pieces; i++) {
-        const size_t b = (size_t)i * piece, len = std::min(piece, bytes - b);
-        try {
-            read_exact((char*)dst + b, len, off + b);
-        } catch (const std::exception& e) {
-#pragma omp critical
-            err = e.what();
+// The I/O pool: blocks of kBlock bytes are claimed from a shared counter by up to
+// io_threads_ workers; the first failure stops the others and is rethrown here.
+void EgrGraph::read_range(void* dst, size_t bytes, size_t at) const {
+    if (!bytes) return;
+    const size_t nblocks = (bytes + kBlock - 1) / kBlock;
+    const int workers = (int)std::min<size_t>((size_t)io_threads_, nblocks);
+    if (workers == 1) {
+        pread_all(fd_, dst, bytes, at, path_);
+        return;
+    }
+    std::atomic<size_t> next{0};
+    std::atomic<bool> failed{false};
+    std::string first_error;
+    std::mutex mu;
+#pragma omp parallel num_threads(workers)
+    {
+        for (;;) {
+            const size_t b = next.fetch_add(1, std::memory_order_relaxed);
+            if (b >= nblocks || failed.load(std::memory_order_relaxed)) break;
+            const size_t o = b * kBlock, len = std::min(kBlock, bytes - o);
+            try {
+                pread_all(fd_, static_cast<char*>(dst) + o, len, at + o, path_);
+            } catch (const std::exception& e) {
+                std::lock_guard<std::mutex> lock(mu);
+                if (!failed.exchange(true)) first_error = e.what();
+            }
         }
     }
-    if (!err.empty()) throw egr_error(err);
+    if (failed) throw egr_error(first_error);
 }
 
 void EgrGraph::read_offsets(size_t v0, size_t cnt, uint64_t* dst) const {
-    if (!nindex_.empty()) memcpy(dst, nindex_.data() + v0, cnt * sizeof(uint64_t));
-    else read_parallel(dst, cnt * sizeof(uint64_t), idx_off_ + v0 * sizeof(uint64_t));
+    if (off_ == ram_off_.data() && !ram_off_.empty()) std::copy(off_ + v0, off_ + v0 + cnt, dst);
+    else read_range(dst, cnt * sizeof(uint64_t), offsets_at_ + v0 * sizeof(uint64_t));
 }
 
 void EgrGraph::read_edges(uint64_t e0, uint64_t e1, int32_t* dst) const {
-    read_parallel(dst, (e1 - e0) * sizeof(int32_t), nlist_off_ + e0 * sizeof(int32_t));
+    read_range(dst, (e1 - e0) * sizeof(int32_t), cols_at_ + e0 * sizeof(int32_t));
 }
 
-// StaticGraph.h:90-114: the k lists are split into up to io_threads contiguous parts
-// (at least 1024 lists each); each part merges overlapping or touching page runs and
-// advises each run once.
-void EgrGraph::prefetch_lists(const int32_t* vids, int k, uint64_t* pages, uint64_t* runs) const {
-    const size_t P = (size_t)sysconf(_SC_PAGESIZE);
-    uint64_t np = 0, nr = 0;
-    const int parts = std::max(1, std::min(io_threads_, k / 1024));
-#pragma omp parallel for num_threads(parts) reduction(+ : np, nr) schedule(static, 1)
-    for (int t = 0; t < parts; t++) {
-        const int i0 = (int)((long long)k * t / parts), i1 = (int)((long long)k * (t + 1) / parts);
-        size_t r0 = 0, r1 = 0;
-        bool open = false;
-        auto flush = [&]() {
-            if (!open) return;
-            madvise((char*)map_ + r0 * P, (r1 - r0) * P, MADV_WILLNEED);
-            np += r1 - r0;
-            nr++;
-        };
+void EgrGraph::prefetch_lists(const int32_t* vids, int k, uint64_t* pages, uint64_t* runs, int groups) const {
+    const size_t P = page_size();
+    const int G = std::max(1, std::min(groups, k / 1024));
+    std::vector<uint64_t> gp(G, 0), gr(G, 0);
+#pragma omp parallel for schedule(dynamic, 1) num_threads(std::max(1, std::min(G, io_threads_)))
+    for (int g = 0; g < G; g++) {
+        const int i0 = (int)((long long)k * g / G), i1 = (int)((long long)k * (g + 1) / G);
+        RunMerger runs_of_group(map_, P);
         for (int i = i0; i < i1; i++) {
-            const uint64_t d = degree(vids[i]);
+            const int v = vids[i];
+            const uint64_t d = off_[v + 1] - off_[v];
             if (!d) continue;
-            const size_t b0 = nlist_off_ + off_[vids[i]] * sizeof(int32_t), b1 = b0 + d * sizeof(int32_t);
-            const size_t p0 = b0 / P, p1 = (b1 + P - 1) / P;
-            if (open && p0 <= r1) r1 = std::max(r1, p1);
-            else { flush(); r0 = p0; r1 = p1; open = true; }
+            const size_t b0 = cols_at_ + off_[v] * sizeof(int32_t), b1 = b0 + d * sizeof(int32_t);
+            runs_of_group.add(b0 / P, (b1 + P - 1) / P);
         }
-        flush();
+        runs_of_group.close();
+        gp[g] = runs_of_group.pages;
+        gr[g] = runs_of_group.runs;
     }
-    *pages = np;
-    *runs = nr;
+    uint64_t tp = 0, tr = 0;
+    for (int g = 0; g < G; g++) { tp += gp[g]; tr += gr[g]; }
+    *pages = tp;
+    *runs = tr;
 }
 
-// StaticGraph.h:116-126
+// Byte range [b0, b1) widened to whole pages, dropped from this mapping (its pages would
+// otherwise keep the cache entries) and from the page cache.
+void EgrGraph::advise_out(size_t b0, size_t b1) const {
+    const size_t P = page_size();
+    const size_t a0 = b0 / P * P, a1 = std::min(file_bytes_, (b1 + P - 1) / P * P);
+    if (a1 <= a0) return;
+    madvise(const_cast<char*>(map_) + a0, a1 - a0, MADV_DONTNEED);
+    posix_fadvise(fd_, (off_t)a0, (off_t)(a1 - a0), POSIX_FADV_DONTNEED);
+}
+
 void EgrGraph::drop_cache(uint64_t e0, uint64_t e1, size_t v0, size_t v1) const {
-    const size_t P = (size_t)sysconf(_SC_PAGESIZE);
-    auto drop = [&](size_t b0, size_t b1) {
-        const size_t a0 = b0 / P * P, a1 = std::min(map_size_, (b1 + P - 1) / P * P);
-        if (a1 <= a0) return;
-        madvise((char*)map_ + a0, a1 - a0, MADV_DONTNEED);
-        posix_fadvise(fd_, (off_t)a0, (off_t)(a1 - a0), POSIX_FADV_DONTNEED);
-    };
-    drop(nlist_off_ + e0 * sizeof(int32_t), nlist_off_ + e1 * sizeof(int32_t));
-    drop(idx_off_ + v0 * sizeof(uint64_t), idx_off_ + (v1 + 1) * sizeof(uint64_t));
+    advise_out(cols_at_ + e0 * sizeof(int32_t), cols_at_ + e1 * sizeof(int32_t));
+    const size_t w = wide ? sizeof(uint64_t) : sizeof(uint32_t);
+    advise_out(offsets_at_ + v0 * w, offsets_at_ + (v1 + 1) * w);
 }
 
-// StaticGraph.h:128-139: greedy vertex-contiguous chunks of at most max_edges edges; a
-// vertex larger than the cap gets a chunk of its own.
+// Chunk end: the last vertex u in [v, n] with off[u] <= off[v] + max_edges, found by a
+// galloping search from v (chunks are short next to n, so this stays near O(log chunk)).
 std::vector<EdgeChunk> EgrGraph::plan_chunks(uint64_t max_edges) const {
     std::vector<EdgeChunk> out;
-    int v = 0;
-    while (v < n) {
-        const uint64_t limit = off_[v] + max_edges;
-        int v1 = (int)(std::upper_bound(off_ + v + 1, off_ + n + 1, limit) - off_) - 1;
-        if (v1 <= v) v1 = v + 1;
-        out.push_back({v, v1, off_[v], off_[v1]});
+    const long long last = n;   // offsets are indexed 0..n
+    for (long long v = 0; v < n;) {
+        const uint64_t limit = off_[v] > UINT64_MAX - max_edges ? UINT64_MAX : off_[v] + max_edges;
+        long long lo = v, step = 1;
+        while (lo + step <= last && off_[lo + step] <= limit) {
+            lo += step;
+            step <<= 1;
+        }
+        long long hi = std::min(lo + step, last + 1);   // off[hi] > limit, or hi past the end
+        while (hi - lo > 1) {
+            const long long mid = lo + (hi - lo) / 2;
+            if (off_[mid] <= limit) lo = mid; else hi = mid;
+        }
+        const long long v1 = lo > v ? lo : v + 1;
+        out.push_back({(int)v, (int)v1, off_[v], off_[v1]});
         v = v1;
     }
     return out;

@@ -31,6 +31,7 @@
 #include "excent.h"
 #include "excent_hip.h"
 #include "planner.hpp"
+#include "staging.hpp"
 
 namespace excent {
 namespace {
@@ -77,6 +78,7 @@ struct Options {
     std::string pull = "auto";   // dense passes by pull over a device transpose: auto | off | threshold
     double eps_ppr = -1;
     std::string stop = "vertex";  // vertex: nothing pending; global: also when sum |r_k| <= n eps_k
+    int io_threads = 0;           // host threads for reads, gathers and checks (0: all)
 };
 
 const char* USAGE =
@@ -141,21 +143,50 @@ Options parse(int argc, const char* const* argv) {
         else if (a == "--eps-ppr") o.eps_ppr = atof(next());
         else if (a == "--ppr-column") o.ppr_col = atoi(next());
         else if (a == "--stop") o.stop = next();
+        else if (a == "--io-threads") o.io_threads = atoi(next());
         else die("unknown option " + a);
     }
     return o;
 }
 
 // Every column id must name a vertex: the push kernels add to r[id] with no bounds check.
-void check_columns(const int32_t* a, size_t cnt, int n, const char* where) {
+void check_columns(const int32_t* a, size_t cnt, int n, const char* where, int threads) {
     long long bad = -1;
-#pragma omp parallel for schedule(static) num_threads(16) reduction(max : bad)
+#pragma omp parallel for schedule(static) num_threads(threads) reduction(max : bad)
     for (long long i = 0; i < (long long)cnt; i++)
         if ((uint32_t)a[i] >= (uint32_t)n) bad = std::max(bad, i);
     if (bad >= 0) {
         char b[256];
         snprintf(b, sizeof b, "graph: column id %d out of range [0, %d) in %s", a[bad], n, where);
         die(b);
+    }
+}
+
+// The tau cache of src/excent.cu:518-526, 576: ~/.cache/excent/<file name>.<n>.<m>.tau
+// holding "%.6f %.4f" (tau, calibration seconds).
+std::string tau_cache_path(const std::string& graph, int n, uint64_t m) {
+    const char* home = getenv("HOME");
+    std::string dir = std::string(home ? home : ".") + "/.cache";
+    mkdir(dir.c_str(), 0755);
+    dir += "/excent";
+    mkdir(dir.c_str(), 0755);
+    const size_t slash = graph.find_last_of('/');
+    const std::string name = slash == std::string::npos ? graph : graph.substr(slash + 1);
+    return dir + "/" + name + "." + std::to_string(n) + "." + std::to_string(m) + ".tau";
+}
+
+bool read_tau_cache(const std::string& path, double* tau, double* seconds) {
+    FILE* f = fopen(path.c_str(), "r");
+    if (!f) return false;
+    const bool ok = fscanf(f, "%lf %lf", tau, seconds) == 2;
+    fclose(f);
+    return ok;
+}
+
+void write_tau_cache(const std::string& path, double tau, double seconds) {
+    if (FILE* f = fopen(path.c_str(), "w")) {
+        fprintf(f, "%.6f %.4f\n", tau, seconds);
+        fclose(f);
     }
 }
 
@@ -194,6 +225,7 @@ private:
     Clock::time_point T0_, T_open_, T_setup_, T_iter_, T_end_;
     std::unique_ptr<EgrGraph> g_;
     int n_ = 0, nc_ = 0, push_kernel_ = EXC_KERNEL_LB, cus_ = 0;
+    int threads_ = 1;   // host threads for reads, gathers and checks
     uint64_t m_ = 0, cap_ = 0;
     char arch_[64] = {0};
     bool zc_ = false, use_cache_ = false, hot_mode_ = false;
@@ -210,6 +242,7 @@ private:
     int32_t* h_vids_[2] = {nullptr, nullptr};
     uint64_t* h_off_[2] = {nullptr, nullptr};
     double stream_ratio_ = 2.0, t_calib_ = 0;
+    TauProbe tau_probe_;   // the calibration's measurements when it ran in this process
     // per run
     double t_io_ = 0, t_wait_ = 0, t_snap_ = 0;
     uint64_t bytes_stream_ = 0, bytes_gather_ = 0;
@@ -288,11 +321,13 @@ void Engine::setup() {
     if (o.stop != "vertex" && o.stop != "global") die("--stop must be vertex or global");
     if (o.stop == "global" && o.measure == "ppr") die("--stop global needs plain measures (pr, katz, ev, all)");
 
+    if (o.io_threads < 0) die("--io-threads must be 0 (every host thread) or a thread count");
     try {
-        g_.reset(new EgrGraph(o.graph, 16, /*offsets_in_ram=*/false));
+        g_.reset(new EgrGraph(o.graph, o.io_threads, /*offsets_in_ram=*/false));
     } catch (const egr_error& e) {
         die(e.what());
     }
+    threads_ = g_->io_threads();
     EgrGraph& g = *g_;
     n_ = g.n;
     m_ = g.m;
@@ -351,7 +386,7 @@ void Engine::setup() {
     if (o.ev_init == "indeg" && std::find(kinds_.begin(), kinds_.end(), (int)EXC_EV) != kinds_.end()) {
         indeg_.assign(n, 0);
         const int32_t* nl = g.nlist();
-#pragma omp parallel for schedule(static) num_threads(16)
+#pragma omp parallel for schedule(static) num_threads(threads_)
         for (long long e = 0; e < (long long)m; e++) __atomic_fetch_add(&indeg_[nl[e]], 1u, __ATOMIC_RELAXED);
     }
     init_state();
@@ -406,7 +441,7 @@ void Engine::setup() {
             }
             fclose(f);
         }
-        hs_ = hot_select(n, g.offsets(), budget, score.empty() ? nullptr : score.data());
+        hs_ = hot_select(n, g.offsets(), budget, score.empty() ? nullptr : score.data(), threads_);
         nh_ = (int)hs_.ids.size();
         hot_edges_ = hs_.off.back();
         hot_deg_min_ = hs_.deg_min;
@@ -431,11 +466,11 @@ void Engine::setup() {
             if (hhi_[c] <= hlo_[c]) continue;
             const EdgeChunk& ch = chunks_[c];
             g.read_edges(ch.e0, ch.e1, h_edges_[0]);
-            check_columns(h_edges_[0], ch.edges(), n, "a hot chunk");
+            check_columns(h_edges_[0], ch.edges(), n, "a hot chunk", threads_);
             const uint64_t o0 = hs_.off[hlo_[c]], o1 = hs_.off[hhi_[c]];
             int32_t* dst = h_edges_[1];
             const int32_t* src = h_edges_[0];
-#pragma omp parallel for schedule(dynamic, 64) num_threads(16)
+#pragma omp parallel for schedule(dynamic, 64) num_threads(threads_)
             for (int j = hlo_[c]; j < hhi_[c]; j++)
                 memcpy(dst + (hs_.off[j] - o0), src + (g.offset(hs_.ids[j]) - ch.e0), (hs_.off[j + 1] - hs_.off[j]) * sizeof(int32_t));
             ck(exc_cache_hot_upload(ctx, dst, o0, o1 - o0), "hot fill");
@@ -445,77 +480,40 @@ void Engine::setup() {
         for (int c = 0; c < n_resident_; c++) {
             const EdgeChunk& ch = chunks_[c];
             g.read_edges(ch.e0, ch.e1, h_edges_[0]);
-            check_columns(h_edges_[0], ch.edges(), n, "a preloaded chunk");
+            check_columns(h_edges_[0], ch.edges(), n, "a preloaded chunk", threads_);
             ck(exc_cache_upload(ctx, h_edges_[0], ch.e0, ch.edges()), "preload");
             loaded_[c] = 1;
         }
     }
 
-    // tau (src/excent.cu:515-579)
+    // tau: given, cached for this graph on this machine, or measured (src/excent.cu:515-579
+    // defines the rule, the cache file and the report lines; the measurement is staging.cpp's)
     stream_ratio_ = o.stream_ratio;
-    bool stream_auto = o.stream_auto;
-    std::string tau_file;
-    if (stream_auto) {
-        const char* home = getenv("HOME");
-        std::string base = o.graph.substr(o.graph.find_last_of('/') + 1);
-        std::string dir = std::string(home ? home : ".") + "/.cache";
-        mkdir(dir.c_str(), 0755);
-        dir += "/excent";
-        mkdir(dir.c_str(), 0755);
-        tau_file = dir + "/" + base + "." + std::to_string(n) + "." + std::to_string(m) + ".tau";
-        FILE* f = o.recalibrate ? nullptr : fopen(tau_file.c_str(), "r");
-        if (f) {
-            if (fscanf(f, "%lf %lf", &stream_ratio_, &t_calib_) == 2) {
-                fprintf(out, "fetch rule: tau = %.3f (measured earlier on this machine in %.2fs, %s)\n", stream_ratio_, t_calib_, tau_file.c_str());
-                stream_auto = false;
-            }
-            fclose(f);
-        }
-    }
-    if (stream_auto) {
-        auto c0 = Clock::now();
-        const EdgeChunk& ch = chunks_[nc / 2];
-        g.drop_cache(ch.e0, ch.e1, ch.v0, ch.v1);
-        auto a = Clock::now();
-        g.read_edges(ch.e0, ch.e1, h_edges_[0]);
-        double t_copy = 0;
-        ck(exc_slot_copy_probe(ctx, 0, ch.edges() * sizeof(int32_t), &t_copy), "calibration copy");
-        const double t_stream = secs(a, Clock::now());
-        const double fr[3] = {0.05, 0.15, 0.4};
-        double f_got[3], t_g[3];
-        std::vector<std::pair<uint64_t, int>> byd;
-        for (int v = ch.v0; v < ch.v1; v++) byd.push_back({g.degree(v), v});
-        std::sort(byd.begin(), byd.end(), [](const auto& x, const auto& y) { return x.first > y.first; });
-        for (int j = 0; j < 3; j++) {
-            std::vector<int> pick;
-            for (uint64_t acc = 0; acc < fr[j] * ch.edges() && pick.size() < byd.size();) {
-                acc += byd[pick.size()].first;
-                pick.push_back(byd[pick.size()].second);
-            }
-            std::sort(pick.begin(), pick.end());
-            g.drop_cache(ch.e0, ch.e1, ch.v0, ch.v1);
-            a = Clock::now();
-            int k = 0;
-            for (int v : pick) h_vids_[0][k++] = v;
-            h_off_[0][0] = 0;
-            for (int i = 0; i < k; i++) h_off_[0][i + 1] = h_off_[0][i] + g.degree(h_vids_[0][i]);
-            uint64_t pg, rn;
-            if (o.dedup) g.prefetch_lists(h_vids_[0], k, &pg, &rn);
-            int32_t* he = h_edges_[0]; const int32_t* hv = h_vids_[0]; const uint64_t* ho = h_off_[0];
-#pragma omp parallel for schedule(dynamic, 256) num_threads(16)
-            for (int i = 0; i < k; i++) memcpy(he + ho[i], g.edges(hv[i]), g.degree(hv[i]) * sizeof(int32_t));
-            const uint64_t bytes = ho[k] * 4 + (uint64_t)k * 12;
-            ck(exc_slot_copy_probe(ctx, 0, std::min<uint64_t>(bytes, ch.edges() * sizeof(int32_t)), &t_copy), "calibration copy");
-            t_g[j] = secs(a, Clock::now());
-            f_got[j] = ch.edges() ? (double)ho[k] / ch.edges() : 0.0;
-        }
-        stream_ratio_ = tau_from_timings(t_stream, t_g, f_got);
-        t_calib_ = secs(c0, Clock::now());
-        fprintf(out, "fetch rule: tau = %.3f (stream %.3fs; gather %.3fs / %.3fs / %.3fs at demand %.3f / %.3f / %.3f)  calibrate %.2fs\n",
-                stream_ratio_, t_stream, t_g[0], t_g[1], t_g[2], f_got[0], f_got[1], f_got[2], t_calib_);
-        if (FILE* f = fopen(tau_file.c_str(), "w")) { fprintf(f, "%.6f %.4f\n", stream_ratio_, t_calib_); fclose(f); }
-    } else if (tau_file.empty()) {
+    if (!o.stream_auto) {
         fprintf(out, "fetch rule: tau = %.3f (given)\n", stream_ratio_);
+    } else {
+        const std::string tau_file = tau_cache_path(o.graph, n, m);
+        if (!o.recalibrate && read_tau_cache(tau_file, &stream_ratio_, &t_calib_)) {
+            fprintf(out, "fetch rule: tau = %.3f (measured earlier on this machine in %.2fs, %s)\n", stream_ratio_, t_calib_,
+                    tau_file.c_str());
+        } else {
+            // the largest chunk: the one whose fetch costs the most
+            const EdgeChunk* ch = &chunks_[0];
+            for (const EdgeChunk& c : chunks_) if (c.edges() > ch->edges()) ch = &c;
+            tau_probe_ = calibrate_tau(g, *ch, h_edges_[0], h_vids_[0], h_off_[0], o.dedup, threads_, [&](uint64_t bytes) {
+                double t = 0;
+                ck(exc_slot_copy_probe(ctx, 0, bytes, &t), "calibration copy");
+                return t;
+            });
+            const TauProbe& p = tau_probe_;
+            stream_ratio_ = p.tau;
+            t_calib_ = p.seconds;
+            fprintf(out, "fetch rule: tau = %.3f (stream %.3fs; gather %.3fs / %.3fs / %.3fs at demand %.3f / %.3f / %.3f)  calibrate %.2fs\n",
+                    p.tau, p.t_stream, p.t_gather[0], p.t_gather[1], p.t_gather[2], p.f_got[0], p.f_got[1], p.f_got[2], p.seconds);
+            fprintf(out, "fetch rule: storage %.2f GB/s (cold chunk read), link %.2f GB/s (pinned host to GPU), %d host threads\n",
+                    p.storage_seq_gbs, p.link_gbs, threads_);
+            write_tau_cache(tau_file, stream_ratio_, t_calib_);
+        }
     }
     maybe_enable_pull();
     ck(exc_pass_sync(ctx, nullptr), "setup sync");
@@ -628,7 +626,7 @@ void Engine::init_state() {
         if (kinds_[i] != EXC_EV) continue;
         std::vector<float> r(n_), x(n_, 1.f);
         const double lam = (double)(float)o_.lambda;
-#pragma omp parallel for schedule(static) num_threads(16)
+#pragma omp parallel for schedule(static) num_threads(threads_)
         for (int v = 0; v < n_; v++) r[v] = (float)((double)indeg_[v] / lam - 1.0);
         ck(exc_set_state(ctx_, i, x.data(), r.data()), "ev init");
     }
@@ -766,7 +764,7 @@ void Engine::run_passes() {
             if (mode[c] == F_STREAM) {
                 nstream++;
                 g.read_edges(ch.e0, ch.e1, h_edges_[s]);
-                if (!loaded_[c]) check_columns(h_edges_[s], ch.edges(), n, "a streamed chunk");
+                if (!loaded_[c]) check_columns(h_edges_[s], ch.edges(), n, "a streamed chunk", threads_);
                 bytes_stream_ += ch.edges() * sizeof(int32_t);
                 t_io_ += secs(io0, Clock::now());
                 const bool ul = use_list(h_cnt[2 * c], ch.v1 - ch.v0, o.list_frac);
@@ -778,14 +776,12 @@ void Engine::run_passes() {
             } else {
                 ngather++;
                 const int k = gather_list(h_bitmap.data(), ch.v0, ch.v1, h_vids_[s]);
-                h_off_[s][0] = 0;
-                for (int i = 0; i < k; i++) h_off_[s][i + 1] = h_off_[s][i] + g.degree(h_vids_[s][i]);
-                const uint64_t ne = h_off_[s][k];
-                if (o.dedup) { uint64_t pg, rn; g.prefetch_lists(h_vids_[s], k, &pg, &rn); pass_pages += pg; pass_runs += rn; }
-                int32_t* he = h_edges_[s]; const int32_t* hv = h_vids_[s]; const uint64_t* ho = h_off_[s];
-#pragma omp parallel for schedule(dynamic, 256) num_threads(16)
-                for (int i = 0; i < k; i++) memcpy(he + ho[i], g.edges(hv[i]), g.degree(hv[i]) * sizeof(int32_t));
-                check_columns(he, ne, n, "a gathered chunk");
+                const GatherStats gs = stage_lists(g, h_vids_[s], k, h_off_[s], h_edges_[s], o.dedup, threads_);
+                const uint64_t ne = gs.edges;
+                pass_pages += gs.pages;
+                pass_runs += gs.runs;
+                check_columns(h_edges_[s], ne, n, "a gathered chunk", threads_);
+                // the reference's byte count of a gather: columns, ids and offsets (src/excent.cu:730)
                 bytes_gather_ += ne * sizeof(int32_t) + (size_t)k * (sizeof(int32_t) + sizeof(uint64_t));
                 t_io_ += secs(io0, Clock::now());
                 const int G = o.group_opt ? o.group_opt : pick_group(k ? (double)ne / k : 1.0, gcap);
@@ -824,48 +820,54 @@ void Engine::run_passes() {
     }
 }
 
+// Reads an n-element device vector to the host in 16M-element slices, handing each slice
+// (in vertex order) to `sink`.
+template <class T, class Read, class Sink>
+void for_each_slice(size_t n, Read read, Sink sink) {
+    const size_t B = (size_t)1 << 24;
+    std::vector<T> buf(std::min(B, std::max<size_t>(n, 1)));
+    for (size_t v0 = 0; v0 < n; v0 += B) {
+        const size_t cnt = std::min(B, n - v0);
+        read(buf.data(), v0, cnt);
+        sink(buf.data(), cnt);
+    }
+}
+
+// Values (and --dump files, --trace-pending counters): src/excent.cu:761-789.  The sum of
+// a measure is a double sum over its float values in vertex order; a batched run writes its
+// K columns one after another to PREFIX.ppr.bin.
 void Engine::results() {
     const Options& o = o_;
-    const int n = n_;
+    const size_t n = (size_t)n_;
     const int ncols = batched() ? K_ : M_.nm;
     sum_.assign(ncols, 0.0);
-    {   // src/excent.cu:763-778 (batched: PREFIX.ppr.bin holds the K columns one after another)
-        const size_t B = (size_t)1 << 24;
-        std::vector<float> slice(std::min(B, (size_t)n));
-        FILE* pf = nullptr;
-        if (batched() && !o.dump.empty()) {
-            std::string p = o.dump + ".ppr.bin";
-            pf = fopen(p.c_str(), "wb");
-            if (!pf) die("cannot write " + p);
-        }
-        for (int i = 0; i < ncols; i++) {
-            FILE* df = pf;
-            if (!batched() && !o.dump.empty()) {
-                std::string p = o.dump + "." + MNAME[kinds_[i]] + ".bin";
-                df = fopen(p.c_str(), "wb");
-                if (!df) die("cannot write " + p);
-            }
-            for (size_t v0 = 0; v0 < (size_t)n; v0 += B) {
-                const size_t cnt = std::min(B, (size_t)n - v0);
-                ck(exc_read_state(ctx_, i, 0, slice.data(), v0, cnt), "result readback");
-                for (size_t j = 0; j < cnt; j++) sum_[i] += slice[j];
-                if (df) fwrite(slice.data(), sizeof(float), cnt, df);
-            }
-            if (df && !batched()) fclose(df);
-        }
-        if (pf) fclose(pf);
+    auto open_out = [&](const std::string& path) {
+        FILE* f = fopen(path.c_str(), "wb");
+        if (!f) die("cannot write " + path);
+        return f;
+    };
+    FILE* shared = batched() && !o.dump.empty() ? open_out(o.dump + ".ppr.bin") : nullptr;
+    for (int i = 0; i < ncols; i++) {
+        FILE* f = shared;
+        if (!batched() && !o.dump.empty()) f = open_out(o.dump + "." + MNAME[kinds_[i]] + ".bin");
+        double& total = sum_[i];
+        for_each_slice<float>(
+            n, [&](float* dst, size_t v0, size_t cnt) { ck(exc_read_state(ctx_, i, 0, dst, v0, cnt), "result readback"); },
+            [&](const float* xs, size_t cnt) {
+                for (size_t j = 0; j < cnt; j++) total += xs[j];
+                if (f && fwrite(xs, sizeof(float), cnt, f) != cnt) die("short write of the --dump output");
+            });
+        if (f && f != shared) fclose(f);
     }
-    if (!o.trace_file.empty()) {   // src/excent.cu:779-789
-        FILE* tf = fopen(o.trace_file.c_str(), "wb");
-        if (!tf) die("cannot write " + o.trace_file);
-        const size_t B = (size_t)1 << 24;
-        std::vector<uint16_t> sl(std::min(B, (size_t)n));
-        for (size_t v0 = 0; v0 < (size_t)n; v0 += B) {
-            const size_t cnt = std::min(B, (size_t)n - v0);
-            ck(exc_read_trace(ctx_, sl.data(), v0, cnt), "trace readback");
-            fwrite(sl.data(), sizeof(uint16_t), cnt, tf);
-        }
-        fclose(tf);
+    if (shared) fclose(shared);
+    if (!o.trace_file.empty()) {
+        FILE* f = open_out(o.trace_file);
+        for_each_slice<uint16_t>(
+            n, [&](uint16_t* dst, size_t v0, size_t cnt) { ck(exc_read_trace(ctx_, dst, v0, cnt), "trace readback"); },
+            [&](const uint16_t* t, size_t cnt) {
+                if (fwrite(t, sizeof(uint16_t), cnt, f) != cnt) die("short write of " + o.trace_file);
+            });
+        fclose(f);
     }
     T_end_ = Clock::now();
 }

@@ -37,8 +37,8 @@ struct HotSet {
     uint64_t deg_min = 0;
 };
 
-// src/excent.cu:432-491.  deg(v) through the callback-free offsets array.  score may be
-// null (degree rule) or n uint16 scores (--hot-rank rule).
+// The hot set of src/excent.cu:432-491 given the budget (the selection rules are stated in
+// planner.cpp).  score may be null (degree rule) or n uint16 scores (--hot-rank rule).
 HotSet hot_select(int n, const uint64_t* off, uint64_t budget, const uint16_t* score, int threads = 16);
 
 // src/excent.cu:566-572

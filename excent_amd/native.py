@@ -189,6 +189,8 @@ HOST_SIGNATURES = {
     "excent_tau": (c_double, [c_double, c_void_p, c_void_p]),
     "excent_prefetch_counts": (c_int, [ctypes.c_char_p, c_void_p, c_int, c_int, P(c_u64), P(c_u64)]),
     "excent_gather_list": (c_int, [c_void_p, c_int, c_int, c_void_p]),
+    "excent_stage_lists": (c_int, [ctypes.c_char_p, c_void_p, c_int, c_int, c_int, c_void_p, c_void_p, P(c_u64),
+                                   P(c_u64)]),
 }
 
 F = {}

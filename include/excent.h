@@ -102,6 +102,11 @@ double excent_tau(double t_stream, const double* t_g, const double* f_got);
 /* include/StaticGraph.h:90-114 page/run counts of a gather (issues the prefetch). */
 int excent_prefetch_counts(const char* path, const int32_t* vids, int k, int io_threads,
                            uint64_t* pages, uint64_t* runs);
+/* The gather path's host staging (src/excent.cu:709-740) on the graph file: the lists of
+ * vids[0..k) packed into edges_out with their offsets off_out[0..k], readahead counted
+ * when dedup. */
+int excent_stage_lists(const char* path, const int32_t* vids, int k, int dedup, int threads, uint64_t* off_out,
+                       int32_t* edges_out, uint64_t* pages, uint64_t* runs);
 /* src/excent.cu:711-720 */
 int excent_gather_list(const uint32_t* bitmap, int v0, int v1, int32_t* out);
 

@@ -176,3 +176,81 @@ def test_prefetch_counts_product_oracle_reference(rmat12, k, tmp_path):
         a, b = ctypes.c_uint64(0), ctypes.c_uint64(0)
         ref.ref_prefetch_counts(rmat12.encode(), nat.ptr(vids), len(vids), 16, ctypes.byref(a), ctypes.byref(b))
         assert (a.value, b.value) == (pg.value, rn.value)
+
+
+def _hot_cases():
+    rng = np.random.default_rng(17)
+    big = 1 << 20
+    yield "ties", rng.integers(0, 6, 3000)                                     # few classes, many ties
+    yield "big_ties", np.concatenate([np.full(4, big), np.full(3, big + 7), [3 * big], rng.integers(0, 40, 500)])
+    yield "isolated", np.where(rng.random(2000) < 0.7, 0, rng.integers(1, 300, 2000))
+    yield "spread", (rng.pareto(1.2, 4000) * 3).astype(np.int64)              # heavy tail, wide buckets
+    yield "only_big", np.array([big, 0, big + 1, big, 0])
+    yield "edgeless", np.zeros(100, np.int64)
+
+
+@pytest.mark.parametrize("name,deg", list(_hot_cases()))
+def test_hot_select_budget_sweep(name, deg):
+    """The product's selection (weighted radix select + parallel compaction) against the
+    oracle's restatement of the reference's rule on budgets at, just below and just above
+    every class boundary (where the split class and its quota change), with and without
+    ranked scores."""
+    deg = np.asarray(deg, np.int64)
+    n = len(deg)
+    idx = np.concatenate([[0], np.cumsum(deg)]).astype(np.uint64)
+    cost = 4 * deg + 16
+    cls = np.unique(deg[deg > 0])[::-1]
+    bounds = np.cumsum([cost[deg == d].sum() for d in cls]) if len(cls) else np.zeros(0)
+    budgets = {0, 1, 15, 16, 17, int(cost[deg > 0].sum()) if (deg > 0).any() else 0}
+    for b in bounds[:60]:
+        b = int(b)
+        budgets.update({b - 1, b, b + 1, b + 20, max(0, b - 3000)})
+    rng = np.random.default_rng(len(budgets))
+    score = rng.integers(0, 4, n).astype(np.uint16)
+    for budget in sorted(x for x in budgets if x >= 0):
+        for sc in (None, score):
+            ids = np.zeros(max(n, 1), np.int32); off = np.zeros(n + 1, np.uint64)
+            bm = np.zeros((n + 31) // 32, np.uint32); dm = ctypes.c_uint64(0)
+            nh = nat.F["excent_hot_select"](n, nat.ptr(idx), budget, nat.ptr(sc), nat.ptr(ids), nat.ptr(off), nat.ptr(bm),
+                                            ctypes.byref(dm))
+            oids, ooff, obm, odm = orc.hot_select(idx, budget, sc)
+            assert np.array_equal(ids[:nh], oids) and np.array_equal(off[: nh + 1], ooff), (name, budget, sc is None)
+            assert np.array_equal(bm, obm) and dm.value == odm, (name, budget, dm.value, odm)
+
+
+@pytest.mark.parametrize("threads", [1, 3, 16])
+def test_stage_lists_packs_the_lists(tmp_path, threads):
+    """The gather's host staging: packed columns equal the concatenated lists (numpy), the
+    offsets their prefix sums, and the readahead counts those of the oracle's and the
+    reference reader's prefetch -- on a graph with empty lists and a hub list longer than a
+    worker's share (the copy splits it)."""
+    rng = np.random.default_rng(threads)
+    n = 30000
+    deg = np.where(rng.random(n) < 0.2, 0, rng.integers(1, 40, n))
+    deg[12345] = 400000
+    idx = np.concatenate([[0], np.cumsum(deg)]).astype(np.uint64)
+    adj = rng.integers(0, n, int(idx[-1])).astype(np.int32)
+    path = str(tmp_path / "g.egr")
+    orc.egr_write(path, idx, adj, 1)
+    header = 16
+    nlist_off = header + 8 * (n + 1)
+    page = os.sysconf("SC_PAGESIZE")
+    for k in (0, 1, 700, 5000, 20000):
+        vids = np.sort(rng.choice(n, size=k, replace=False)).astype(np.int32)
+        if k >= 700:
+            vids = np.unique(np.concatenate([vids, [12345]])).astype(np.int32)
+        k = len(vids)
+        off = np.zeros(k + 1, np.uint64)
+        out = np.zeros(max(1, int(deg[vids].sum())), np.int32)
+        pg, rn = ctypes.c_uint64(0), ctypes.c_uint64(0)
+        assert nat.F["excent_stage_lists"](path.encode(), nat.ptr(vids), k, 1, threads, nat.ptr(off), nat.ptr(out),
+                                           ctypes.byref(pg), ctypes.byref(rn)) == 0
+        want = np.concatenate([adj[idx[v]:idx[v + 1]] for v in vids]) if k else np.zeros(0, np.int32)
+        assert np.array_equal(off[1:], np.cumsum(deg[vids])) and off[0] == 0
+        assert np.array_equal(out[: len(want)], want)
+        if k:
+            assert (pg.value, rn.value) == orc.prefetch_counts(vids, idx, nlist_off, page)
+            if ref is not None:
+                a, b = ctypes.c_uint64(0), ctypes.c_uint64(0)
+                ref.ref_prefetch_counts(path.encode(), nat.ptr(vids), len(vids), 16, ctypes.byref(a), ctypes.byref(b))
+                assert (a.value, b.value) == (pg.value, rn.value)
