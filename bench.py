@@ -185,7 +185,7 @@ def graph_size(nat, graph):
     return n.value, m.value
 
 
-def bench_batched(args, nat, np, work, graph):
+def bench_batched(args, nat, np, work, graph, sample=None):
     """K (default 64) personalised PageRank columns in one batched run against the same K
     problems solved one after another by the plain solver (--measure pr --ppr-column j),
     both from the resident graph; FP64 errors on sampled columns."""
@@ -194,7 +194,7 @@ def bench_batched(args, nat, np, work, graph):
     common = [graph, "--cache", "chunks", "--preload", "--stream-ratio", "1", "--timing", "--ppr-gen", gen,
               "--eps-ppr", EPS]
     n, m = graph_size(nat, graph)
-    sample = sorted({0, K // 3, (2 * K) // 3, K - 1})
+    sample = sorted(sample or {0, K // 3, (2 * K) // 3, K - 1})
     t0 = time.time()
     sess = nat.Session(common + ["--measure", "ppr", "--log", os.path.join(args.logdir or work, "batched.log")])
     log(f"batched setup {time.time() - t0:.1f}s")
@@ -302,7 +302,7 @@ def bench_ooc(args, nat, np, work, graph):
     for tag, extra, cold in modes:
         sargs = [graph, "--measure", measure, "--timing", "--log",
                  os.path.join(args.logdir or work, f"ooc_{tag.split()[0]}{'_cold' if cold else ''}.log")] + extra
-        sargs += measure_args(measure, env)
+        sargs += measure_args(measure, env) + ["--stop", args.stop]
         if args.max_passes:
             sargs += ["--max-passes", args.max_passes]
         if tag == hot:
@@ -328,7 +328,9 @@ def bench_ooc(args, nat, np, work, graph):
         sess.close()
         shipped = r["bytes_stream"] + r["bytes_gather"]
         out["runs"][tag] = {"time_to_eps_s": round(el, 4), "passes": r["passes"],
-                            "converged": not args.max_passes or r["passes"] < args.max_passes,
+                            "converged": r["stop_reason"] != 2,
+                            "stop": ["nothing pending", "global l1 <= n*eps", "pass limit"][r["stop_reason"]],
+                            "residual_l1_fp64_at_stop": [float(f"{v:.6g}") for v in r["residual_l1"]],
                             "gteps": round(r["pushed_edges"] / el / 1e9, 3),
                             "bytes_stream_gb": round(r["bytes_stream"] / 1e9, 3),
                             "bytes_gather_gb": round(r["bytes_gather"] / 1e9, 3),
@@ -356,12 +358,12 @@ def bench_ooc(args, nat, np, work, graph):
     return out
 
 
-def adjoint_fp64_checks(nat, np, idx, adj, w, s, g, x, gs, gw, gwn, seed):
+def adjoint_fp64_checks(nat, np, idx, adj, w, s, g, x, gs, gw, gwn, seed, fd_dirs=32):
     """The float32 results against the product's FP64 solves with per-edge coefficients
     (exc_fp64_solve_coef, pinned to the oracle's weighted solve in the tests): the forward
     x, the gradients against the FP64 adjoint solve lambda = (I - d W^T)^-1 g
     (dL/ds = (1 - d) lambda; dL/dw of a raw weight by the chain rule through the per-source
-    normalisation), and central differences of L = g . x in FP64 on 32 seeded directions
+    normalisation), and central differences of L = g . x in FP64 on fd_dirs seeded directions
     (even: raw edge weights, odd: teleport), step 1e-3."""
     D = 0.85
     t0 = time.time()
@@ -394,7 +396,7 @@ def adjoint_fp64_checks(nat, np, idx, adj, w, s, g, x, gs, gw, gwn, seed):
         log(f"adjoint fp64: forward {itf} / adjoint {itb} iterations, {out}")
         h = 1e-3
         worst_gpu = worst_fp64 = 0.0
-        for k in range(32):
+        for k in range(fd_dirs):
             dr = np.random.default_rng(seed + 100 + k)
             if k % 2 == 0:   # raw edge weights (stay positive: uniform(0.5, 1.5) +- 1e-3 N(0, 1))
                 dw = dr.standard_normal(m)
@@ -414,7 +416,7 @@ def adjoint_fp64_checks(nat, np, idx, adj, w, s, g, x, gs, gw, gwn, seed):
             worst_fp64 = max(worst_fp64, abs(af - fd) / max(abs(fd), 1e-300))
             log(f"adjoint fd direction {k} ({'w' if k % 2 == 0 else 's'}): fd {fd:.9g}, gpu {an:.9g}, "
                 f"fp64 {af:.9g}, {time.time() - t0:.1f}s")
-        out["fd_directions"] = 32
+        out["fd_directions"] = fd_dirs
         out["fd_max_rel_err_gpu"] = worst_gpu
         out["fd_max_rel_err_fp64_adjoint"] = worst_fp64
     out = {k: (float(f"{v:.4g}") if isinstance(v, float) else v) for k, v in out.items()}
@@ -422,7 +424,7 @@ def adjoint_fp64_checks(nat, np, idx, adj, w, s, g, x, gs, gw, gwn, seed):
     return out
 
 
-def bench_adjoint(args, nat, np, work, graph):
+def bench_adjoint(args, nat, np, work, graph, fd_dirs=32):
     """Differentiable personalised PageRank on a weighted graph: forward push, adjoint push
     over the transpose and gradient assembly, timed; the adjoint identity
     g . x = (1 - d) lambda . s checks the two solves against each other."""
@@ -444,7 +446,7 @@ def bench_adjoint(args, nat, np, work, graph):
         pf, pb = P.passes()
     lhs = float(g.astype(np.float64) @ x.astype(np.float64))
     rhs = float(gs.astype(np.float64) @ s.astype(np.float64))   # gs = (1 - d) lambda
-    fp64 = None if args.no_fp64 else adjoint_fp64_checks(nat, np, idx, adj, w, s, g, x, gs, gw, gwn, args.seed)
+    fp64 = None if args.no_fp64 else adjoint_fp64_checks(nat, np, idx, adj, w, s, g, x, gs, gw, gwn, args.seed, fd_dirs)
     return {"metric": "differentiable personalised PageRank forward + backward", "unit": "s",
             "value": round(tf + tb, 4), "higher_is_better": False, "n_gpus": 1, "dtype": "f32", "data": "synthetic",
             "config": {"workload": f"R-MAT scale {args.scale} ef {args.edge_factor}, raw weights uniform(0.5, 1.5), "
@@ -456,15 +458,226 @@ def bench_adjoint(args, nat, np, work, graph):
             "fp64_check": fp64}
 
 
+def host_info():
+    """The host the CPU baseline ran on: logical CPUs, those this process may use, the CPU
+    model and RAM."""
+    model = None
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    ram = None
+    try:
+        for line in open("/proc/meminfo"):
+            if line.startswith("MemTotal:"):
+                ram = round(int(line.split()[1]) / 2 ** 20, 1)
+    except OSError:
+        pass
+    return {"nproc": os.cpu_count(), "usable_cpus": len(os.sched_getaffinity(0)), "cpu_model": model, "ram_gib": ram}
+
+
+def cpu_threads():
+    """Every host core this process may run on (the baseline's definition: all host cores)."""
+    return len(os.sched_getaffinity(0))
+
+
+def plain_state(orc, np, idx, adj, measure, env):
+    """Initial x, r and the constants of the CPU restatement, as bin/excent sets them up
+    (the eigenvector starts from in-degrees: the graphs are directed)."""
+    n = len(idx) - 1
+    kinds = {"pr": [0], "katz": [1], "ev": [2], "all": [0, 1, 2]}[measure]
+    xs, rs, eps, scale, divdeg = [], [], [], [], []
+    indeg = np.bincount(adj, minlength=n).astype(np.float64)
+    for k in kinds:
+        if k == 0:
+            x, r = orc.init(idx, 0)
+            eps.append(0.01); scale.append(np.float32(0.85)); divdeg.append(1)
+        elif k == 1:
+            x, r = orc.init(idx, 1, beta=1.0)
+            eps.append(1e-6); scale.append(np.float32(env["ALPHA"])); divdeg.append(0)
+        else:
+            lam = float(np.float32(env["LAMBDA"]))
+            x = np.ones(n, np.float32)
+            r = (indeg / lam - 1.0).astype(np.float32)
+            eps.append(1e-6); scale.append(np.float32(1.0 / env["LAMBDA"])); divdeg.append(0)
+        xs.append(x); rs.append(r)
+    return xs, rs, eps, scale, divdeg
+
+
+def cpu_baseline_run(np, orc, idx, adj, xs, rs, eps, scale, divdeg, seconds, what):
+    """The oracle's OpenMP restatement of the pass loop (snapshot, plan, push with CAS float
+    atomics) on every usable host core, to convergence or for `seconds`."""
+    threads = cpu_threads()
+    cv0 = orc.plan_chunks(idx, orc.chunk_cap(len(adj)))
+    passes, pushed, secs = orc.run_push(idx, adj, cv0, xs, rs, eps, scale, divdeg, threads=threads, time_limit=seconds)
+    converged = all(float(np.abs(r).max(initial=0.0)) <= e for r, e in zip(rs, eps))
+    sample = (f"{what}: converged run, {passes} passes, {secs:.1f}s" if converged else
+              f"{what}: first {passes} passes of the same run, {secs:.1f}s of a {seconds:.0f}s budget")
+    log(f"cpu baseline: {sample}, {pushed / 1e9:.3f} G edges")
+    rec = {"value": round(pushed / secs / 1e9, 4) if secs > 0 else None, "unit": "GTEPS", "cores": threads,
+           "kind": "port", "sample": sample, "converged": converged, "passes": passes}
+    if converged:
+        rec["time_to_eps_s"] = round(secs, 3)
+    rec.update(host_info())
+    return rec
+
+
+def bench_plain(args, nat, np, work, graph, scale, measure, steps, warmup, env, tag):
+    """Converged runs of bin/excent's engine with the whole graph in HBM: steps timed
+    back to back after `warmup` untimed ones, each from the initial state (a device copy);
+    the dominant kernel's roofline; FP64 errors of the last run; setup times."""
+    excent_log = os.path.join(args.logdir or work, f"{tag}.log")
+    sargs = [graph, "--measure", measure, "--cache", "chunks", "--preload", "--stream-ratio", "1",
+             "--timing", "--push-kernel", args.kernel, "--pull", args.pull, "--log", excent_log,
+             "--dump", os.path.join(work, "x")] + measure_args(measure, env)
+    t0 = time.time()
+    sess = nat.Session(sargs)
+    t_session = time.time() - t0
+    log(f"{tag}: session setup {t_session:.1f}s")
+    first = True
+    for w in range(warmup):
+        r = sess.run(reinit=not first)
+        first = False
+        log(f"{tag} warmup {w}: {r['passes']} passes, {r['time_passes_s']:.3f}s, {r['pushed_edges'] / 1e9:.2f} G edges")
+    nat.synchronize()
+    runs = []
+    t_start = time.perf_counter()
+    for _ in range(steps):
+        runs.append(sess.run(reinit=not first))
+        first = False
+    nat.synchronize()
+    elapsed = time.perf_counter() - t_start
+    sess.results()
+    sess.close()
+    for i, r in enumerate(runs):
+        log(f"{tag} step {i}: {r['passes']} passes, {r['time_passes_s']:.4f}s, {r['pushed_edges'] / 1e9:.3f} G edges")
+    pushed = sum(r["pushed_edges"] for r in runs)
+    K = lambda key: sum(r["kernel"][key] for r in runs)
+    names = ["range", "bitmap", "list", "hot"]
+    ktag = "3,4" if measure == "all" else "1,1"
+    cands = [(f"k_push<{ktag},{i},0> ({names[i]} source)", sum(r["kernel"]["src_ms"][i] for r in runs),
+              sum(r["kernel"]["src_launches"][i] for r in runs), sum(r["kernel"]["src_bytes"][i] for r in runs))
+             for i in range(4)]
+    npull = K("pull_passes")
+    cands += [(f"k_fold<{ktag}>", K("fold_ms"), npull, K("fold_bytes")),
+              (f"k_pull<{ktag}>", K("pull_ms"), npull, K("pull_bytes")),
+              (f"k_pull_hub<{ktag}>", K("pull_hub_ms"), npull, K("pull_hub_bytes")),
+              (f"k_snapshot<{ktag}>", K("snapshot_ms"), K("snapshot_launches"), K("snapshot_bytes")),
+              (f"k_pieces<{ktag}>", K("piece_ms"), K("piece_launches"), K("piece_bytes"))]
+    dom = max(cands, key=lambda c: c[1])
+    achieved = dom[3] / (dom[1] / 1e3) / 1e9 if dom[1] > 0 else 0.0
+    traffic = None
+    if os.path.exists(args.traffic):
+        # per-launch HBM bytes from rocprofv3 FETCH_SIZE/WRITE_SIZE runs of the same workload
+        tr = json.load(open(args.traffic))
+        want = {"scale": scale, "edge_factor": args.edge_factor,
+                "seed": args.seed, "measure": measure, "kernel": args.kernel, "pull": args.pull}
+        if tr.get("workload") == want:
+            traffic = tr["bytes_per_launch"].get(dom[0].split("<")[0])
+    roof = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": dom[0],
+            "launches": dom[2], "avg_launch_ms": round(dom[1] / max(1, dom[2]), 5),
+            "algorithmic_bytes_per_launch": round(dom[3] / max(1, dom[2]), 1)}
+    last = runs[-1]
+    rec = {
+        "value": round(pushed / elapsed / 1e9, 4), "unit": "GTEPS", "steps": steps, "warmup": warmup,
+        "ms_per_step": round(elapsed / steps * 1e3, 3),
+        "time_to_eps_s": round(sum(r["time_passes_s"] for r in runs) / len(runs), 5),
+        "setup_s": {"session": round(t_session, 3), "open": round(last["time_open_s"], 3),
+                    "setup": round(last["time_setup_s"], 3), "of_which_transpose": round(last["time_transpose_s"], 4)},
+        "passes": [r["passes"] for r in runs],
+        "pushed_edges_per_run": last["pushed_edges"],
+        "residual_l1_fp64_at_stop": {k: float(f"{v:.6g}") for k, v in
+                                     zip(["pr", "katz", "ev"] if measure == "all" else [measure], last["residual_l1"])},
+        "kernel_ms_per_run": {c[0]: round(c[1] / len(runs), 3) for c in cands if c[2]},
+        "roofline": roof,
+    }
+    rec["fp64_err"] = None if args.no_fp64 else fp64_errors(nat, np, graph, work, measure, env)
+    return rec, excent_log
+
+
+def graph_file(args, nat, work, scale, weights=False):
+    graph = os.path.join(work, f"rmat{scale}_ef{args.edge_factor}_s{args.seed}{'_w' if weights else ''}.egr")
+    t0 = time.time()
+    if not os.path.exists(graph):
+        gargs = ["--scale", scale, "--edge-factor", args.edge_factor, "--seed", args.seed]
+        if weights:
+            gargs.append("--weights")
+        with native_stdout_to_stderr():
+            rc = nat.run_entry("excent_rmat_main", gargs + [graph])
+        assert rc == 0, "graph generation failed"
+    log(f"graph {graph} ready in {time.time() - t0:.1f}s")
+    return graph
+
+
+def sub_s16(args, nat, np, work):
+    """R-MAT scale 16 PageRank (eps 0.01, whole graph cached) with the CPU restatement run to
+    convergence beside it."""
+    from oracle import oracle as orc
+    graph = graph_file(args, nat, work, 16)
+    rec, _ = bench_plain(args, nat, np, work, graph, 16, "pr", 20, 3, {}, "rmat16_pr")
+    n, m = graph_size(nat, graph)
+    rec.update({"metric": "converged GTEPS + time-to-eps", "higher_is_better": True, "dtype": "f32",
+                "data": "synthetic", "config": {"workload": "R-MAT scale 16 ef 16 --measure pr, eps 0.01, whole "
+                                                            "graph in HBM, converged to eps", "n": n, "m": m}})
+    if not args.no_cpu:
+        idx, adj = orc.egr_load(graph)
+        st = plain_state(orc, np, idx, adj, "pr", {})
+        rec["cpu_baseline"] = cpu_baseline_run(np, orc, idx, adj, *st, args.cpu_seconds, "R-MAT scale 16 PageRank")
+    return rec
+
+
+def sub_batched(args, nat, np, work):
+    """Batched K=64 personalised PageRank at scale 26 against 64 plain runs, plus a CPU sample
+    of the plain iteration on the first column."""
+    from oracle import oracle as orc
+    a = argparse.Namespace(**vars(args))
+    a.scale, a.k, a.warmup, a.batched_only = 26, 64, 1, False
+    graph = graph_file(a, nat, work, 26)
+    rec = bench_batched(a, nat, np, work, graph, sample=[0, a.k - 1])
+    if not args.no_cpu:
+        idx, adj = orc.egr_load(graph)
+        n = len(idx) - 1
+        gen = rec["config"]["ppr_gen"]
+        with nat.Session([graph, "--measure", "pr", "--ppr-column", 0, "--ppr-gen", gen, "--eps-ppr", 1e-7,
+                          "--cache", "none", "--stream-ratio", "1", "--max-passes", 0,
+                          "--log", os.path.join(work, "cpu_column0.log")]) as S:
+            S.set_column(0)
+            r0 = S.values(0, which=1)
+        xs, rs = [np.zeros(n, np.float32)], [r0.astype(np.float32)]
+        cpu = cpu_baseline_run(np, orc, idx, adj, xs, rs, [1e-7], [np.float32(0.85)], [1], min(args.cpu_seconds, 10.0),
+                               "personalised PageRank column 0 (plain iteration, K = 1)")
+        cpu["unit"] = "G vector-pushes/s"
+        rec["cpu_baseline"] = cpu
+    return rec
+
+
+def sub_adjoint(args, nat, np, work):
+    a = argparse.Namespace(**vars(args))
+    a.scale = 22
+    graph = graph_file(a, nat, work, 22, weights=True)
+    # 8 finite-difference directions here (4 raw-weight, 4 teleport; about 10 s each at this
+    # size); --config adjoint runs the full 32
+    return bench_adjoint(a, nat, np, work, graph, fd_dirs=8)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--config", default="headline", choices=["headline", "batched", "ooc", "adjoint"],
-                    help="headline: the scale-24 all-measure in-HBM run (the bench line); batched: K=64 "
-                         "personalised PageRank vs 64 plain runs; ooc: out-of-core PageRank; adjoint: "
-                         "differentiable personalised PageRank forward + backward")
+    ap.add_argument("--config", default="headline", choices=["headline", "batched", "ooc", "adjoint", "s16"],
+                    help="headline: the scale-24 all-measure in-HBM run (the bench line), with the scale-16 "
+                         "PageRank, adjoint and batched configurations as sub-records; batched: K=64 personalised "
+                         "PageRank vs 64 plain runs; ooc: out-of-core; adjoint: differentiable personalised "
+                         "PageRank forward + backward; s16: the scale-16 PageRank record alone")
+    ap.add_argument("--sub", default="s16,adjoint,batched",
+                    help="headline: sub-records to add, in order (comma list of s16, adjoint, batched; '' for none)")
+    ap.add_argument("--sub-budget", type=float, default=420.0,
+                    help="headline: seconds since start after which remaining sub-records are skipped")
     ap.add_argument("--scale", type=int, default=None, help="R-MAT scale (default 24; batched/ooc 26; adjoint 22)")
     ap.add_argument("--cache-mb", type=int, default=1024, help="ooc: device cache cap for the hot-cache run")
     ap.add_argument("--batched-only", action="store_true", help="batched: time the batched run only (no baseline)")
@@ -473,19 +686,20 @@ def main():
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--measure", default=None, choices=["pr", "katz", "ev", "all"],
                     help="measures of the plain runs (default: all; ooc: pr)")
+    ap.add_argument("--stop", default="vertex", choices=["vertex", "global"],
+                    help="ooc: the loop's stop rule (global: also when every measure's sum |r| <= n eps)")
     ap.add_argument("--max-passes", type=int, default=0,
                     help="ooc: stop a run after this many passes (it is then reported as not converged)")
     ap.add_argument("--cold", action="store_true",
                     help="ooc: also time the hot-cache run after dropping the file from the host page cache")
     ap.add_argument("--kernel", default="lb", choices=["lb", "group"])
     ap.add_argument("--pull", default="auto", help="dense passes by pull: auto | off | demand fraction")
-    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--cpu-seconds", type=float, default=40.0,
+                    help="CPU baseline: time limit of the restatement's run (it stops earlier when it converges)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 error report of the last run")
     ap.add_argument("--workdir", default=None)
-    ap.add_argument("--log", default=None, help="bin/excent report of the runs (default: in the workdir)")
-    ap.add_argument("--logdir", default=None, help="batched / ooc: directory of the runs' bin/excent reports "
-                                                   "(default: the workdir)")
+    ap.add_argument("--logdir", default=None, help="directory of the runs' bin/excent reports (default: the workdir)")
     ap.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "traffic_rmat24_all.json"),
                     help="JSON with HBM bytes per push launch from rocprofv3 --pmc runs")
     args = ap.parse_args()
@@ -493,11 +707,21 @@ def main():
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
               "the multi-GPU design is described, not built", file=sys.stderr)
         sys.exit(2)
+    t_begin = time.time()
+    # stdout carries exactly one JSON line: everything else written to file descriptor 1 (the
+    # native tools' C stdio included) goes to stderr, and the line to the saved descriptor
+    sys.stdout.flush()
+    json_fd = os.dup(1)
+    os.dup2(2, 1)
+
+    def emit(rec):
+        sys.stdout.flush()
+        os.write(json_fd, (json.dumps(rec) + "\n").encode())
 
     if args.scale is None:
-        args.scale = {"headline": 24, "batched": 26, "ooc": 26, "adjoint": 22}[args.config]
+        args.scale = {"headline": 24, "batched": 26, "ooc": 26, "adjoint": 22, "s16": 16}[args.config]
     if args.measure is None:
-        args.measure = "pr" if args.config == "ooc" else "all"
+        args.measure = "pr" if args.config in ("ooc", "s16") else "all"
 
     import numpy as np
 
@@ -505,135 +729,58 @@ def main():
 
     work = args.workdir or tempfile.mkdtemp(prefix="excent_bench_", dir=os.environ.get("TMPDIR", "/tmp"))
     os.makedirs(work, exist_ok=True)
-    wtag = "_w" if args.config == "adjoint" else ""
-    graph = os.path.join(work, f"rmat{args.scale}_ef{args.edge_factor}_s{args.seed}{wtag}.egr")
-    t0 = time.time()
-    if not os.path.exists(graph):
-        gargs = ["--scale", args.scale, "--edge-factor", args.edge_factor, "--seed", args.seed]
-        if wtag:
-            gargs.append("--weights")
-        with native_stdout_to_stderr():
-            rc = nat.run_entry("excent_rmat_main", gargs + [graph])
-        assert rc == 0, "graph generation failed"
-    log(f"graph {graph} ready in {time.time() - t0:.1f}s")
+    if args.config == "s16":
+        emit(sub_s16(args, nat, np, work))
+        return
+    graph = graph_file(args, nat, work, args.scale, weights=args.config == "adjoint")
     if args.config != "headline":
         out = {"batched": bench_batched, "ooc": bench_ooc, "adjoint": bench_adjoint}[args.config](args, nat, np, work, graph)
-        print(json.dumps(out), flush=True)
+        emit(out)
         return
     env = constants(nat, graph)
-
-    excent_log = args.log or os.path.join(work, "excent.log")
-    sargs = [graph, "--measure", args.measure, "--cache", "chunks", "--preload", "--stream-ratio", "1",
-             "--timing", "--push-kernel", args.kernel, "--pull", args.pull, "--log", excent_log,
-             "--dump", os.path.join(work, "x")] + measure_args(args.measure, env)
-    t0 = time.time()
-    sess = nat.Session(sargs)
-    log(f"session setup {time.time() - t0:.1f}s")
-    first = True
-    for w in range(args.warmup):
-        r = sess.run(reinit=not first)
-        first = False
-        log(f"warmup {w}: {r['passes']} passes, {r['time_passes_s']:.3f}s, {r['pushed_edges'] / 1e9:.2f} G edges")
-    nat.synchronize()
-    runs = []
-    t_start = time.perf_counter()
-    for s in range(args.steps):
-        runs.append(sess.run(reinit=not first))
-        first = False
-    nat.synchronize()
-    elapsed = time.perf_counter() - t_start
-    sess.results()
-    sess.close()
-    for i, r in enumerate(runs):
-        log(f"step {i}: {r['passes']} passes, {r['time_passes_s']:.3f}s, {r['pushed_edges'] / 1e9:.3f} G edges")
-    pushed = sum(r["pushed_edges"] for r in runs)
-    gteps = pushed / elapsed / 1e9
-    # dominant kernel (by total time): each entry is (name, total ms, launches, algorithmic bytes)
-    K = lambda key: sum(r["kernel"][key] for r in runs)
-    names = ["range", "bitmap", "list", "hot"]
-    tag = "3,4" if args.measure == "all" else ("1,1" if args.measure in ("pr", "katz", "ev") else "")
-    cands = [(f"k_push<{tag},{i},0> ({names[i]} source)", sum(r["kernel"]["src_ms"][i] for r in runs),
-              sum(r["kernel"]["src_launches"][i] for r in runs), sum(r["kernel"]["src_bytes"][i] for r in runs))
-             for i in range(4)]
-    npull = K("pull_passes")
-    cands += [(f"k_fold<{tag}>", K("fold_ms"), npull, K("fold_bytes")),
-              (f"k_pull<{tag}>", K("pull_ms"), npull, K("pull_bytes")),
-              (f"k_pull_hub<{tag}>", K("pull_hub_ms"), npull, K("pull_hub_bytes")),
-              (f"k_snapshot<{tag}>", K("snapshot_ms"), K("snapshot_launches"), K("snapshot_bytes")),
-              (f"k_pieces<{tag}>", K("piece_ms"), K("piece_launches"), K("piece_bytes"))]
-    dom = max(cands, key=lambda c: c[1])
-    achieved = dom[3] / (dom[1] / 1e3) / 1e9 if dom[1] > 0 else 0.0
-    traffic = None
-    if os.path.exists(args.traffic):
-        # per-launch HBM bytes from rocprofv3 FETCH_SIZE/WRITE_SIZE runs of the same workload
-        tr = json.load(open(args.traffic))
-        want = {"scale": args.scale, "edge_factor": args.edge_factor, "seed": args.seed,
-                "measure": args.measure, "kernel": args.kernel, "pull": args.pull}
-        if tr.get("workload") == want:
-            traffic = tr["bytes_per_launch"].get(dom[0].split("<")[0])
-    kern = {c[0]: round(c[1] / len(runs), 3) for c in cands if c[2]}
-    roof = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": dom[0],
-            "launches": dom[2], "avg_launch_ms": round(dom[1] / max(1, dom[2]), 5),
-            "algorithmic_bytes_per_launch": round(dom[3] / max(1, dom[2]), 1)}
-
-    fp64 = None if args.no_fp64 else fp64_errors(nat, np, graph, work, args.measure, env)
+    rec, excent_log = bench_plain(args, nat, np, work, graph, args.scale, args.measure, args.steps, args.warmup, env,
+                                  "headline")
 
     cpu = None
     if not args.no_cpu:
         from oracle import oracle as orc
         log("cpu baseline: loading graph")
         idx, adj = orc.egr_load(graph)
-        n = len(idx) - 1
-        kinds = {"pr": [0], "katz": [1], "ev": [2], "all": [0, 1, 2]}[args.measure]
-        xs, rs, eps, scale, divdeg = [], [], [], [], []
-        indeg = np.bincount(adj, minlength=n).astype(np.float64)
-        for k in kinds:
-            if k == 0:
-                x, r = orc.init(idx, 0)
-                eps.append(0.01); scale.append(np.float32(0.85)); divdeg.append(1)
-            elif k == 1:
-                x, r = orc.init(idx, 1, beta=1.0)
-                eps.append(1e-6); scale.append(np.float32(env["ALPHA"])); divdeg.append(0)
-            else:
-                lam = float(np.float32(env["LAMBDA"]))
-                x = np.ones(n, np.float32)
-                r = (indeg / lam - 1.0).astype(np.float32)
-                eps.append(1e-6); scale.append(np.float32(1.0 / env["LAMBDA"])); divdeg.append(0)
-            xs.append(x); rs.append(r)
-        cap = orc.chunk_cap(len(adj))
-        cv0 = orc.plan_chunks(idx, cap)
-        threads = min(16, os.cpu_count() or 1)
-        passes, cpushed, csecs = orc.run_push(idx, adj, cv0, xs, rs, eps, scale, divdeg, threads=threads,
-                                              time_limit=args.cpu_seconds)
-        cpu = {"value": round(cpushed / csecs / 1e9, 4), "unit": "GTEPS", "cores": threads, "kind": "port",
-               "sample": f"first {passes} passes of the same run (same graph, constants, eps), "
-                         f"{csecs:.1f}s of a {args.cpu_seconds:.0f}s budget"}
-        log(f"cpu baseline: {passes} passes, {cpushed / 1e9:.3f} G edges in {csecs:.1f}s")
+        st = plain_state(orc, np, idx, adj, args.measure, env)
+        cpu = cpu_baseline_run(np, orc, idx, adj, *st, args.cpu_seconds,
+                               f"R-MAT scale {args.scale} --measure {args.measure}")
+        del idx, adj, st
 
-    n_v, m_e = None, None
-    for line in open(excent_log):
-        if line.startswith("graph: n="):
-            n_v = int(line.split("n=")[1].split()[0])
-            m_e = int(line.split("m=")[1].split()[0])
-            break
+    n_v, m_e = graph_size(nat, graph)
     out = {
-        "metric": METRIC, "value": round(gteps, 4), "unit": "GTEPS", "n_gpus": 1, "steps": args.steps,
-        "warmup": args.warmup, "ms_per_step": round(elapsed / args.steps * 1e3, 3), "higher_is_better": True,
+        "metric": METRIC, "value": rec["value"], "unit": "GTEPS", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": rec["ms_per_step"], "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
         "config": {"workload": f"R-MAT scale {args.scale} ef {args.edge_factor} --measure {args.measure}, "
                                f"whole graph in HBM, converged to eps", "n": n_v, "m": m_e,
                    "lambda1": env["LAMBDA"], "alpha": env["ALPHA"], "eps_pr": 0.01, "eps_katz": 1e-6,
                    "eps_ev": 1e-6, "push_kernel": args.kernel, "pull": args.pull, "parallelism": "single GPU"},
-        "time_to_eps_s": round(sum(r["time_passes_s"] for r in runs) / len(runs), 4),
-        "passes": [r["passes"] for r in runs],
-        "pushed_edges_per_run": runs[-1]["pushed_edges"],
-        "kernel_ms_per_run": kern,
-        "fp64_err": fp64,
-        "roofline": roof,
-        "cpu_baseline": cpu,
     }
-    print(json.dumps(out), flush=True)
+    out.update({k: v for k, v in rec.items() if k not in ("value", "unit", "steps", "warmup", "ms_per_step")})
+    out["cpu_baseline"] = cpu
+
+    subs = {}
+    for name in [x for x in args.sub.split(",") if x]:
+        el = time.time() - t_begin
+        if el > args.sub_budget:
+            subs[name] = {"skipped": f"time budget: {el:.0f}s of {args.sub_budget:.0f}s used before it"}
+            log(f"sub-record {name} skipped ({el:.0f}s elapsed)")
+            continue
+        t0 = time.time()
+        try:
+            subs[name] = {"s16": sub_s16, "adjoint": sub_adjoint, "batched": sub_batched}[name](args, nat, np, work)
+        except Exception as e:   # a failed sub-record must not lose the headline line
+            subs[name] = {"error": f"{type(e).__name__}: {e}"}
+        subs[name]["wall_s"] = round(time.time() - t0, 1)
+        log(f"sub-record {name}: {time.time() - t0:.1f}s")
+    if subs:
+        out["configs"] = subs
+    emit(out)
 
 
 if __name__ == "__main__":

@@ -260,6 +260,7 @@ private:
     void load_ppr_seeds(EgrGraph& g);
     void maybe_enable_pull();
     bool pull_tried_ = false, pull_on_ = false;
+    double t_open_s_ = 0, t_setup_s_ = 0, t_transpose_s_ = 0;   // setup phases (outside the pass loop)
     double pull_thr_ = 1.0;   // dense pass (fold + pull) when demand >= pull_thr_ * m
     bool batched() const { return o_.measure == "ppr"; }
     int K_ = 0;
@@ -520,6 +521,8 @@ void Engine::setup() {
     ck(exc_reset_stats(ctx), "reset stats");
     checkpoint_ = session_ && exc_state_checkpoint(ctx, 0) == EXC_OK;
     T_setup_ = Clock::now();
+    t_open_s_ = secs(T0_, T_open_);
+    t_setup_s_ = secs(T_open_, T_setup_);
     fprintf(out, "staging: %s\n", zc_ ? "zero copy (pinned, GPU-mapped buffers read in place)" : "pinned host buffers copied to the GPU");
     if (hot_mode_ && !o.hot_rank.empty())
         fprintf(out, "GPU cache ranked by passes pending (%s): %d vertices (%.1f%% of edges, %.2f GB)  setup %.2fs\n",
@@ -593,7 +596,9 @@ void Engine::maybe_enable_pull() {
     // auto: 5 % of the edges; batched rows make the scatter dearer per edge and the pull
     // reads only the pending sources' rows, so the dense pass pays from 2 %
     const double thr = o_.pull == "auto" ? (batched() ? 0.02 : 0.05) : atof(o_.pull.c_str());
+    const auto t0 = Clock::now();
     const int rc = exc_enable_pull(ctx_, thr);
+    t_transpose_s_ = secs(t0, Clock::now());
     if (rc == EXC_OK) { pull_on_ = true; pull_thr_ = thr; }
     else if (rc != EXC_ENOMEM) ck(rc, "pull setup");
 }
@@ -918,6 +923,9 @@ void Engine::fill_stats(excent_run_stats* s) {
     s->kernel = st;
     for (int i = 0; i < EXC_MAXM; i++) s->residual_l1[i] = have_l1_ && i < M_.nm ? l1_[i] : 0.0;
     s->stop_reason = stop_reason_;
+    s->time_open_s = t_open_s_;
+    s->time_setup_s = t_setup_s_;
+    s->time_transpose_s = t_transpose_s_;
 }
 
 void Engine::write_stats_json() {
@@ -931,7 +939,8 @@ void Engine::write_stats_json() {
     fprintf(j, "{\"n\": %d, \"m\": %llu, \"chunks\": %d, \"passes\": %d, \"measure\": \"%s\", \"arch\": \"%s\", \"cus\": %d,\n",
             n_, (unsigned long long)m_, nc_, pass_, o_.measure.c_str(), arch_, cus_);
     fprintf(j, " \"time_open_s\": %.6f, \"time_setup_s\": %.6f, \"time_passes_s\": %.6f, \"time_io_s\": %.6f, \"time_wait_s\": %.6f, \"time_snapshot_s\": %.6f,\n",
-            secs(T0_, T_open_), secs(T_open_, T_setup_), tp, t_io_, t_wait_, t_snap_);
+            t_open_s_, t_setup_s_, tp, t_io_, t_wait_, t_snap_);
+    fprintf(j, " \"time_transpose_s\": %.6f,\n", t_transpose_s_);
     fprintf(j, " \"pushed_edges\": %llu, \"pushed_vertices\": %llu, \"pushed_edge_measures\": %llu, \"gteps\": %.6f,\n",
             (unsigned long long)st.pushed_edges, (unsigned long long)st.pushed_vertices, (unsigned long long)st.pushed_edge_measures,
             tp > 0 ? st.pushed_edges / tp / 1e9 : 0.0);

@@ -1163,15 +1163,44 @@ __global__ void k_count_in64(uint64_t m, const int32_t* __restrict__ adj, unsign
     for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < (long long)m; e += (long long)gridDim.x * blockDim.x)
         atomicAdd(&cnt[adj[e]], 1ull);
 }
-__global__ void k_fill_in64(int n, const uint64_t* __restrict__ idx, const int32_t* __restrict__ adj,
-                            unsigned long long* cursor, const int32_t* __restrict__ rank, int32_t* tadj,
-                            const float* __restrict__ w, float* tw) {
-    for (long long v = blockIdx.x * (long long)blockDim.x + threadIdx.x; v < n; v += (long long)gridDim.x * blockDim.x)
-        for (uint64_t e = idx[v]; e < idx[v + 1]; e++) {
+// Transpose fill, one thread per edge: a block takes FILL_EDGES consecutive edges, finds the
+// sources of its first and last edge by binary search in idx, stages the offsets of the
+// sources in between in LDS (when they fit; else each edge searches idx itself) and each
+// edge finds its source there -- the last v with idx[v] <= e, so sources without edges
+// (equal offsets) are passed over.  A hub's out-list is spread over many blocks instead of
+// one lane.  In-list order follows the atomics' order.
+constexpr int FILL_EDGES = 4096, FILL_STAGE = 4096;
+__device__ __forceinline__ long long last_le(const uint64_t* __restrict__ a, long long lo, long long hi, uint64_t key) {
+    while (lo < hi) {   // the last i in [lo, hi] with a[i] <= key (a[lo] <= key)
+        const long long mid = (lo + hi + 1) >> 1;
+        if (a[mid] <= key) lo = mid; else hi = mid - 1;
+    }
+    return lo;
+}
+__global__ __launch_bounds__(BLOCK) void k_fill_in64(int n, uint64_t m, const uint64_t* __restrict__ idx,
+                                                     const int32_t* __restrict__ adj, unsigned long long* cursor,
+                                                     const int32_t* __restrict__ rank, int32_t* tadj,
+                                                     const float* __restrict__ w, float* tw) {
+    __shared__ uint64_t s_idx[FILL_STAGE + 1];
+    __shared__ long long s_v[2];
+    for (uint64_t e0 = (uint64_t)blockIdx.x * FILL_EDGES; e0 < m; e0 += (uint64_t)gridDim.x * FILL_EDGES) {
+        const uint64_t e1 = e0 + FILL_EDGES < m ? e0 + FILL_EDGES : m;
+        if (threadIdx.x < 2) s_v[threadIdx.x] = last_le(idx, 0, n, threadIdx.x == 0 ? e0 : e1 - 1);
+        __syncthreads();
+        const long long v0 = s_v[0], v1 = s_v[1];
+        const long long span = v1 - v0 + 1;   // sources of this block's edges
+        const bool staged = span <= FILL_STAGE;
+        if (staged)
+            for (long long i = threadIdx.x; i < span; i += BLOCK) s_idx[i] = idx[v0 + i];
+        __syncthreads();
+        for (uint64_t e = e0 + threadIdx.x; e < e1; e += BLOCK) {
+            const long long v = staged ? v0 + last_le(s_idx, 0, span - 1, e) : last_le(idx, v0, v1, e);
             const unsigned long long q = atomicAdd(&cursor[adj[e]], 1ull);
             tadj[q] = rank ? rank[v] : (int32_t)v;
             if (w) tw[q] = w[e];
         }
+        __syncthreads();   // s_idx and s_v are rewritten by the next range
+    }
 }
 
 // ------------------------------------------------------------------ batched PPR
@@ -3367,8 +3396,12 @@ int exc_enable_pull(void* p, double threshold) {
     e = hipMemcpy(c->d_tidx, off.data(), (n + 1) * sizeof(uint64_t), hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(cnt, cur.data(), n * sizeof(uint64_t), hipMemcpyHostToDevice);
     if (e != hipSuccess) return fail_free(e);
-    hipLaunchKernelGGL(k_fill_in64, dim3(c->cus * 16), dim3(BLOCK), 0, 0, c->n, c->d_idx, c->d_arena, cnt, c->d_rank, c->d_tadj,
-                       c->d_wts, c->d_twts);
+    {
+        const uint64_t ranges = (c->m + FILL_EDGES - 1) / FILL_EDGES;
+        const unsigned grid = (unsigned)std::max<uint64_t>(1, std::min<uint64_t>(ranges, (uint64_t)c->cus * 8));
+        hipLaunchKernelGGL(k_fill_in64, dim3(grid), dim3(BLOCK), 0, 0, c->n, c->m, c->d_idx, c->d_arena, cnt, c->d_rank,
+                           c->d_tadj, c->d_wts, c->d_twts);
+    }
     e = hipDeviceSynchronize();
     if (e != hipSuccess) return fail_free(e);
     (void)hipFree(cnt);

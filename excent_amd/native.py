@@ -75,7 +75,8 @@ class RunStats(ctypes.Structure):
     _fields_ = [("time_passes_s", c_double), ("time_snapshot_s", c_double), ("time_io_s", c_double),
                 ("time_wait_s", c_double), ("passes", c_int), ("pushed_edges", c_u64), ("bytes_stream", c_u64),
                 ("bytes_gather", c_u64), ("launches", c_u64), ("kernel", Stats),
-                ("residual_l1", c_double * 3), ("stop_reason", c_int)]
+                ("residual_l1", c_double * 3), ("stop_reason", c_int),
+                ("time_open_s", c_double), ("time_setup_s", c_double), ("time_transpose_s", c_double)]
 
 
 def stats_dict(s):

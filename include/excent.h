@@ -35,6 +35,10 @@ typedef struct excent_run_stats {
     /* why the loop ended: 0 nothing pending (the reference's rule), 1 every measure's
      * sum of |r| <= n * eps (--stop global), 2 the pass limit */
     int stop_reason;
+    /* setup of the session, outside the pass loop's window: opening the graph file, the rest of
+     * setup (offsets upload, caches, tau, the transpose for dense passes), and of that the
+     * transpose build alone (0 when it has not been built) */
+    double time_open_s, time_setup_s, time_transpose_s;
 } excent_run_stats;
 
 /* Session: bin/excent's setup once (graph open, offsets, caches, tau), then any number of
