@@ -531,8 +531,10 @@ def bench_plain(args, nat, np, work, graph, scale, measure, steps, warmup, env, 
     the dominant kernel's roofline; FP64 errors of the last run; setup times."""
     excent_log = os.path.join(args.logdir or work, f"{tag}.log")
     sargs = [graph, "--measure", measure, "--cache", "chunks", "--preload", "--stream-ratio", "1",
-             "--timing", "--push-kernel", args.kernel, "--pull", args.pull, "--log", excent_log,
-             "--dump", os.path.join(work, "x")] + measure_args(measure, env)
+             "--timing", "--push-kernel", args.kernel, "--pull", args.pull, "--hub-lds", args.hub_lds, "--log", excent_log,
+             "--dump", os.path.join(work, "x"), "--stop", args.stop,
+             # a run whose stop rule is never met ends here and is reported as not converged
+             "--max-passes", args.max_passes or 2000] + measure_args(measure, env)
     t0 = time.time()
     sess = nat.Session(sargs)
     t_session = time.time() - t0
@@ -545,9 +547,11 @@ def bench_plain(args, nat, np, work, graph, scale, measure, steps, warmup, env, 
     nat.synchronize()
     runs = []
     t_start = time.perf_counter()
-    for _ in range(steps):
+    for i in range(steps):
         runs.append(sess.run(reinit=not first))
         first = False
+        if i == 0 or i == steps - 1:
+            log(f"{tag} step {i} done: {runs[-1]['passes']} passes")
     nat.synchronize()
     elapsed = time.perf_counter() - t_start
     sess.results()
@@ -589,6 +593,8 @@ def bench_plain(args, nat, np, work, graph, scale, measure, steps, warmup, env, 
         "setup_s": {"session": round(t_session, 3), "open": round(last["time_open_s"], 3),
                     "setup": round(last["time_setup_s"], 3), "of_which_transpose": round(last["time_transpose_s"], 4)},
         "passes": [r["passes"] for r in runs],
+        "converged": all(r["stop_reason"] != 2 for r in runs),
+        "stop": sorted({["nothing pending", "global l1 <= n*eps", "pass limit"][r["stop_reason"]] for r in runs}),
         "pushed_edges_per_run": last["pushed_edges"],
         "residual_l1_fp64_at_stop": {k: float(f"{v:.6g}") for k, v in
                                      zip(["pr", "katz", "ev"] if measure == "all" else [measure], last["residual_l1"])},
@@ -687,13 +693,14 @@ def main():
     ap.add_argument("--measure", default=None, choices=["pr", "katz", "ev", "all"],
                     help="measures of the plain runs (default: all; ooc: pr)")
     ap.add_argument("--stop", default="vertex", choices=["vertex", "global"],
-                    help="ooc: the loop's stop rule (global: also when every measure's sum |r| <= n eps)")
+                    help="the loop's stop rule (global: also when every measure's sum |r| <= n eps)")
     ap.add_argument("--max-passes", type=int, default=0,
-                    help="ooc: stop a run after this many passes (it is then reported as not converged)")
+                    help="stop a run after this many passes (reported as not converged; plain runs: 2000 when 0)")
     ap.add_argument("--cold", action="store_true",
                     help="ooc: also time the hot-cache run after dropping the file from the host page cache")
     ap.add_argument("--kernel", default="lb", choices=["lb", "group"])
     ap.add_argument("--pull", default="auto", help="dense passes by pull: auto | off | demand fraction")
+    ap.add_argument("--hub-lds", type=int, default=1024, help="destinations privatised in LDS by the hub-piece push")
     ap.add_argument("--cpu-seconds", type=float, default=40.0,
                     help="CPU baseline: time limit of the restatement's run (it stops earlier when it converges)")
     ap.add_argument("--no-cpu", action="store_true")

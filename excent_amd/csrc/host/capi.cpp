@@ -1,5 +1,6 @@
 // capi.cpp -- C entry points over the host planner and reader (include/excent.h), so
 // the tests can hold the product's decisions against the oracle bit for bit.
+#include <algorithm>
 #include <cstring>
 #include <string>
 
@@ -102,6 +103,15 @@ int excent_stage_lists(const char* path, const int32_t* vids, int k, int dedup, 
         *pages = st.pages;
         *runs = st.runs;
         return 0;
+    } catch (const std::exception& e) { return err(e.what()); }
+}
+
+int excent_top_in_degree(const char* path, int count, double sample, int threads, int32_t* out) {
+    try {
+        excent::EgrGraph g(path, threads, false);
+        const std::vector<int32_t> t = excent::top_in_degree(g, count, sample, threads);
+        std::copy(t.begin(), t.end(), out);
+        return (int)t.size();
     } catch (const std::exception& e) { return err(e.what()); }
 }
 

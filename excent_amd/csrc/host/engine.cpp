@@ -79,6 +79,7 @@ struct Options {
     double eps_ppr = -1;
     std::string stop = "vertex";  // vertex: nothing pending; global: also when sum |r_k| <= n eps_k
     int io_threads = 0;           // host threads for reads, gathers and checks (0: all)
+    int hub_lds = 1024;           // destinations privatised in LDS by the hub-piece push (0: none)
 };
 
 const char* USAGE =
@@ -144,6 +145,7 @@ Options parse(int argc, const char* const* argv) {
         else if (a == "--ppr-column") o.ppr_col = atoi(next());
         else if (a == "--stop") o.stop = next();
         else if (a == "--io-threads") o.io_threads = atoi(next());
+        else if (a == "--hub-lds") o.hub_lds = atoi(next());
         else die("unknown option " + a);
     }
     return o;
@@ -320,6 +322,7 @@ void Engine::setup() {
     if (o.hot_when != "end" && o.hot_when != "start" && o.hot_when != "both") die("--hot-when must be end, start or both");
     if (o.hot_order != "sweep" && o.hot_order != "ranges") die("--hot-order must be sweep or ranges");
     if (o.stop != "vertex" && o.stop != "global") die("--stop must be vertex or global");
+    if (o.hub_lds < 0 || o.hub_lds > 1024) die("--hub-lds must be 0..1024");
     if (o.stop == "global" && o.measure == "ppr") die("--stop global needs plain measures (pr, katz, ev, all)");
 
     if (o.io_threads < 0) die("--io-threads must be 0 (every host thread) or a thread count");
@@ -515,6 +518,10 @@ void Engine::setup() {
                     p.storage_seq_gbs, p.link_gbs, threads_);
             write_tau_cache(tau_file, stream_ratio_, t_calib_);
         }
+    }
+    if (o.hub_lds > 0 && !batched()) {   // destinations with the most in-edges, from 1/16 of the columns
+        const std::vector<int32_t> hubs = top_in_degree(g, o.hub_lds, 1.0 / 16, threads_);
+        ck(exc_set_hub_destinations(ctx, hubs.data(), (int)hubs.size()), "hub destinations");
     }
     maybe_enable_pull();
     ck(exc_pass_sync(ctx, nullptr), "setup sync");

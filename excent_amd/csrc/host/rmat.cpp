@@ -108,7 +108,7 @@ extern "C" int excent_rmat_main(int argc, const char* const* argv) {
     int scale = 16, ef = 16, wide = -1;
     double a = 0.57, b = 0.19, c = 0.19;
     uint64_t seed = 1;
-    bool permute = true, dedup = true, loops = false, csc = false, weights = false;
+    bool permute = true, dedup = true, loops = false, csc = false, weights = false, symmetric = false;
     std::string out;
     for (int i = 1; i < argc; i++) {
         std::string s = argv[i];
@@ -128,12 +128,13 @@ extern "C" int excent_rmat_main(int argc, const char* const* argv) {
         else if (s == "--wide") { std::string v = next(); wide = v == "on" ? 1 : v == "off" ? 0 : -1; }
         else if (s == "--csc") csc = true;
         else if (s == "--weights") weights = true;
+        else if (s == "--symmetric") symmetric = true;   // every edge in both directions
         else if (s[0] == '-') { fprintf(stderr, "unknown option %s\n", s.c_str()); return 1; }
         else out = s;
     }
     if (out.empty() || scale < 1 || scale > 30 || ef < 1 || a < 0 || b < 0 || c < 0 || a + b + c > 1) {
         fprintf(stderr, "usage: excent_rmat [--scale 16] [--edge-factor 16] [--a .57 --b .19 --c .19] [--seed 1] "
-                        "[--no-permute] [--keep-duplicates] [--keep-self-loops] [--wide auto|on|off] [--csc] [--weights] out.egr\n");
+                        "[--no-permute] [--keep-duplicates] [--keep-self-loops] [--symmetric] [--wide auto|on|off] [--csc] [--weights] out.egr\n");
         return 1;
     }
     const uint64_t n = 1ull << scale, M = (uint64_t)ef * n;
@@ -148,6 +149,7 @@ extern "C" int excent_rmat_main(int argc, const char* const* argv) {
             R.edge((uint64_t)i, &s, &d);
             if (!loops && s == d) continue;
             __atomic_fetch_add(&deg[s], 1u, __ATOMIC_RELAXED);
+            if (symmetric && s != d) __atomic_fetch_add(&deg[d], 1u, __ATOMIC_RELAXED);
         }
         for (uint64_t v = 0; v < n; v++) off[v + 1] = off[v] + deg[v];
     }
@@ -162,6 +164,7 @@ extern "C" int excent_rmat_main(int argc, const char* const* argv) {
             if (!loops && s == d) continue;
             const uint64_t p = __atomic_fetch_add(&pos[s], 1ull, __ATOMIC_RELAXED);
             adj[p] = (int32_t)d;
+            if (symmetric && s != d) adj[__atomic_fetch_add(&pos[d], 1ull, __ATOMIC_RELAXED)] = (int32_t)s;
         }
     }
     std::vector<uint32_t> ndeg(n);

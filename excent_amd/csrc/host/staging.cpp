@@ -133,4 +133,53 @@ TauProbe calibrate_tau(const EgrGraph& g, const EdgeChunk& ch, int32_t* h_edges,
     return p;
 }
 
+std::vector<int32_t> top_in_degree(const EgrGraph& g, int count, double sample, int threads) {
+    const uint64_t m = g.m, block = (1u << 20) / sizeof(int32_t);   // columns per 1 MiB block
+    const uint64_t blocks = (m + block - 1) / block;
+    const uint64_t want = std::min<uint64_t>(m, std::max<uint64_t>(16ull << 20, (uint64_t)(sample * (double)m)));
+    const uint64_t stride = std::max<uint64_t>(1, blocks * block / std::max<uint64_t>(want, 1));
+    std::vector<uint32_t> cnt((size_t)g.n, 0);
+    threads = std::max(1, threads);
+    const long long nb = (long long)((blocks + stride - 1) / stride);
+#pragma omp parallel num_threads(threads)
+    {
+        std::vector<int32_t> buf(block);
+#pragma omp for schedule(dynamic, 1)
+        for (long long i = 0; i < nb; i++) {
+            const uint64_t e0 = (uint64_t)i * stride * block, e1 = std::min(m, e0 + block);
+            g.read_edges(e0, e1, buf.data());
+            for (uint64_t e = 0; e < e1 - e0; e++)
+                if ((uint32_t)buf[e] < (uint32_t)g.n) __atomic_fetch_add(&cnt[buf[e]], 1u, __ATOMIC_RELAXED);
+        }
+    }
+    // the largest counts: each thread keeps its best `count` in a min-heap, then they merge
+    typedef std::pair<uint32_t, int32_t> CV;
+    std::vector<CV> all;
+#pragma omp parallel num_threads(threads)
+    {
+        std::vector<CV> heap;
+        auto cmp = [](const CV& a, const CV& b) { return a.first > b.first; };
+#pragma omp for schedule(static) nowait
+        for (long long v = 0; v < (long long)g.n; v++) {
+            if (!cnt[v]) continue;
+            if ((int)heap.size() < count) {
+                heap.push_back({cnt[v], (int32_t)v});
+                std::push_heap(heap.begin(), heap.end(), cmp);
+            } else if (cnt[v] > heap.front().first) {
+                std::pop_heap(heap.begin(), heap.end(), cmp);
+                heap.back() = {cnt[v], (int32_t)v};
+                std::push_heap(heap.begin(), heap.end(), cmp);
+            }
+        }
+#pragma omp critical
+        all.insert(all.end(), heap.begin(), heap.end());
+    }
+    std::sort(all.begin(), all.end(), [](const CV& a, const CV& b) { return a.first > b.first; });
+    if ((int)all.size() > count) all.resize(count);
+    std::vector<int32_t> out;
+    out.reserve(all.size());
+    for (const CV& c : all) out.push_back(c.second);
+    return out;
+}
+
 }  // namespace excent

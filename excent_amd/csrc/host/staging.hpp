@@ -4,6 +4,7 @@
 
 #include <cstdint>
 #include <functional>
+#include <vector>
 
 #include "egr_graph.hpp"
 
@@ -47,5 +48,10 @@ struct TauProbe {
 // returns the seconds of a synchronous host-to-device copy of that many bytes of h_edges.
 TauProbe calibrate_tau(const EgrGraph& g, const EdgeChunk& ch, int32_t* h_edges, int32_t* h_vids, uint64_t* h_off,
                        bool dedup, int threads, const std::function<double(uint64_t)>& copy_probe);
+
+// The (up to) `count` vertices with the most in-edges, estimated from a sample of the column
+// array: about `sample` of the columns (at least 16M, or all), read in evenly spaced 1 MiB
+// blocks; ties and the order of the result do not matter (the caller privatises them).
+std::vector<int32_t> top_in_degree(const EgrGraph& g, int count, double sample, int threads);
 
 }  // namespace excent

@@ -592,27 +592,94 @@ __global__ __launch_bounds__(BLOCK) void k_push(PushArgs a, DevMeasures M) {
     if (tid < 6 && s_cnt[tid]) atomicAdd(&a.counters[tid], s_cnt[tid]);
 }
 
+// Destinations privatised in LDS by the push of hub pieces.  The vertices with the most
+// in-edges take a large share of every pass's scatter, and float atomics on one address
+// serialise at the memory side; a block accumulates its deltas for them in LDS and adds each
+// sum once at its end.  The set is a static open-addressing table (linear probing, at most
+// `probes` steps) built on the host, with a filter of 2^16 bits in front of it so the other
+// destinations cost one LDS read.  The sums reach r when the launch ends, like every other
+// delta of the pass.
+constexpr int HUB_SLOTS = 2048, HUB_MAX = HUB_SLOTS / 2, HUB_FILTER_WORDS = 2048;
+struct HubTable {
+    const int32_t* keys;       // [HUB_SLOTS], -1 = empty
+    const uint32_t* filter;    // [HUB_FILTER_WORDS]
+    int probes;                // longest probe sequence of a key
+    int count;                 // keys in the table (0: no privatisation)
+};
+__host__ __device__ __forceinline__ uint32_t hub_slot(uint32_t u) { return (u * 2654435761u) >> 21; }   // 11 bits
+__host__ __device__ __forceinline__ uint32_t hub_fbit(uint32_t u) {   // 16 bits, independent of hub_slot
+    uint32_t h = u * 0x85EBCA6Bu;
+    h ^= h >> 13;
+    h *= 0xC2B2AE35u;
+    return h >> 16;
+}
+
 // Hub pieces: one wave per piece, persistent over the device-side count.  Counters are
 // double-buffered per stream: launch i counts into buffer i%2 and clears buffer (i+1)%2,
 // whose previous user (launch i-1 and its pieces) finished before launch i started.
 template <int NM, int S>
 __global__ __launch_bounds__(BLOCK) void k_pieces(const Piece* __restrict__ pcs, const uint32_t* count,
                                                    uint32_t cap, const int32_t* __restrict__ edges,
-                                                   const float* __restrict__ wts, DevMeasures M) {
+                                                   const float* __restrict__ wts, DevMeasures M, HubTable ht) {
+    __shared__ int32_t s_key[HUB_SLOTS];
+    __shared__ uint32_t s_fil[HUB_FILTER_WORDS];
+    __shared__ float s_acc[NM][HUB_SLOTS];
     // the count may exceed cap after an overflow (pieces beyond cap were not written and
     // the overflow counter reports it); never read past the buffer
     uint32_t cnt = *count;
     if (cnt > cap) cnt = cap;
-    const int lane = lane_id();
     const long long nw = (long long)gridDim.x * NWAVES;
-    for (long long w = (blockIdx.x * (long long)BLOCK + threadIdx.x) / WAVE; w < cnt; w += nw) {
+    const long long w0 = (blockIdx.x * (long long)BLOCK + threadIdx.x) / WAVE;
+    if ((long long)blockIdx.x * NWAVES >= cnt) return;   // block-uniform: no piece for this block
+    const bool priv = ht.count > 0;
+    if (priv) {
+        for (int i = threadIdx.x; i < HUB_SLOTS; i += BLOCK) {
+            s_key[i] = ht.keys[i];
+#pragma unroll
+            for (int k = 0; k < NM; k++) s_acc[k][i] = 0.f;
+        }
+        for (int i = threadIdx.x; i < HUB_FILTER_WORDS; i += BLOCK) s_fil[i] = ht.filter[i];
+        __syncthreads();
+    }
+    const int lane = lane_id();
+    for (long long w = w0; w < cnt; w += nw) {
         const Piece pc = pcs[w];
         for (uint32_t t = lane; t < pc.len; t += WAVE) {
             const int u = edges[pc.start + t];
             const float we = wts ? wts[pc.start + t] : 1.f;
+            if (priv) {
+                const uint32_t fb = hub_fbit((uint32_t)u);
+                if ((s_fil[fb >> 5] >> (fb & 31)) & 1u) {
+                    int slot = -1;
+                    uint32_t h = hub_slot((uint32_t)u);
+                    for (int q = 0; q < ht.probes; q++, h = (h + 1) & (HUB_SLOTS - 1)) {
+                        const int key = s_key[h];
+                        if (key == u) { slot = (int)h; break; }
+                        if (key < 0) break;
+                    }
+                    if (slot >= 0) {
+#pragma unroll
+                        for (int k = 0; k < NM; k++)
+                            if (pc.p[k] != 0.f) atomicAdd(&s_acc[k][slot], wts ? pc.p[k] * we : pc.p[k]);
+                        continue;
+                    }
+                }
+            }
 #pragma unroll
             for (int k = 0; k < NM; k++)
                 if (pc.p[k] != 0.f) atomic_add_f32(&M.r[(long long)u * S + k], wts ? pc.p[k] * we : pc.p[k]);
+        }
+    }
+    if (priv) {   // one atomic per privatised destination and measure with a nonzero sum
+        __syncthreads();
+        for (int i = threadIdx.x; i < HUB_SLOTS; i += BLOCK) {
+            const int key = s_key[i];
+            if (key < 0) continue;
+#pragma unroll
+            for (int k = 0; k < NM; k++) {
+                const float v = s_acc[k][i];
+                if (v != 0.f) atomic_add_f32(&M.r[(long long)key * S + k], v);
+            }
         }
     }
 }
@@ -1980,6 +2047,9 @@ struct Ctx {
     unsigned long long* d_counters = nullptr;  // C_NCOUNTERS
     unsigned long long* h_small = nullptr;     // pinned: cnt + hotpend + residual slots
     double l1_last[EXC_MAXM] = {};             // FP64 residual sums of the last snapshot
+    int32_t* d_hub_keys = nullptr;             // destinations privatised by k_pieces (HubTable)
+    uint32_t* d_hub_filter = nullptr;
+    int hub_probes = 0, hub_count = 0;
     bool l1_valid = false;
     bool fold_ahead_ok = true;                 // exc_fold_ahead: dense passes may fold the next pass
     uint64_t max_chunk_edges = 0;
@@ -2190,8 +2260,9 @@ int launch_push_src(Ctx* c, hipStream_t st, unsigned grid, const PushArgs& a, in
 template <int NM, int S>
 void launch_pieces_t(Ctx* c, int s, const int32_t* edges, const float* wts) {
     const unsigned grid = (unsigned)(c->cus * 4);
+    const HubTable ht{c->d_hub_keys, c->d_hub_filter, c->hub_probes, c->hub_count};
     hipLaunchKernelGGL((k_pieces<NM, S>), dim3(grid), dim3(BLOCK), 0, c->st[s], c->pb[s].d,
-                       c->pb[s].cnt + 2 * c->pb[s].phase * CNT_STRIDE, c->pb[s].cap, edges, wts, c->DM);
+                       c->pb[s].cnt + 2 * c->pb[s].phase * CNT_STRIDE, c->pb[s].cap, edges, wts, c->DM, ht);
 }
 
 int norm_group(int g) {
@@ -2684,6 +2755,8 @@ int exc_destroy(void* p) {
                          c->d_bitmap2, c->d_hubbits, c->d_inv, c->d_tile_next};
     for (void* q : pull_ptrs) if (q) (void)hipFree(q);
     if (c->d_beps) (void)hipFree(c->d_beps);
+    if (c->d_hub_keys) (void)hipFree(c->d_hub_keys);
+    if (c->d_hub_filter) (void)hipFree(c->d_hub_filter);
     if (c->d_colpend) (void)hipFree(c->d_colpend);
     void* dev_ptrs[] = {c->d_idx, c->d_x, c->d_r, c->d_x0, c->d_r0, c->d_bitmap, c->d_cnt, c->d_cv0, c->d_trace, c->d_counters,
                         c->d_arena, c->d_hids, c->d_hoff, c->d_hedges, c->d_hot, c->d_hlist, c->d_hcount};
@@ -3415,6 +3488,37 @@ int exc_enable_pull(void* p, double threshold) {
         HIPCHK(hipMalloc(&c->d_pull_hubs, hubs.size() * sizeof(PullPiece)));
         HIPCHK(hipMemcpy(c->d_pull_hubs, hubs.data(), hubs.size() * sizeof(PullPiece), hipMemcpyHostToDevice));
     }
+    return EXC_OK;
+}
+
+int exc_set_hub_destinations(void* p, const int32_t* ids, int count) {
+    Ctx* c = as_ctx(p);
+    if (!c || count < 0 || count > HUB_MAX || (count && !ids))
+        return fail(EXC_EINVAL, "exc_set_hub_destinations: 0 to %d vertex ids", HUB_MAX);
+    std::vector<int32_t> keys(HUB_SLOTS, -1);
+    std::vector<uint32_t> filter(HUB_FILTER_WORDS, 0u);
+    int probes = 0, kept = 0;
+    for (int i = 0; i < count; i++) {
+        const int32_t u = ids[i];
+        if (u < 0 || u >= c->n) return fail(EXC_EINVAL, "exc_set_hub_destinations: vertex %d out of range", u);
+        uint32_t h = hub_slot((uint32_t)u);
+        int q = 1;
+        while (keys[h] >= 0 && keys[h] != u) { h = (h + 1) & (HUB_SLOTS - 1); q++; }
+        if (keys[h] == u) continue;   // listed twice
+        keys[h] = u;
+        kept++;
+        probes = std::max(probes, q);
+        const uint32_t fb = hub_fbit((uint32_t)u);
+        filter[fb >> 5] |= 1u << (fb & 31);
+    }
+    HIPCHK(hipSetDevice(c->dev));
+    HIPCHK(hipDeviceSynchronize());   // no push in flight reads the table being replaced
+    if (!c->d_hub_keys) HIPCHK(hipMalloc(&c->d_hub_keys, HUB_SLOTS * sizeof(int32_t)));
+    if (!c->d_hub_filter) HIPCHK(hipMalloc(&c->d_hub_filter, HUB_FILTER_WORDS * sizeof(uint32_t)));
+    HIPCHK(hipMemcpy(c->d_hub_keys, keys.data(), HUB_SLOTS * sizeof(int32_t), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(c->d_hub_filter, filter.data(), HUB_FILTER_WORDS * sizeof(uint32_t), hipMemcpyHostToDevice));
+    c->hub_probes = probes;
+    c->hub_count = kept;
     return EXC_OK;
 }
 

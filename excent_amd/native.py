@@ -145,6 +145,7 @@ HIP_SIGNATURES = {
     "exc_dense_required": (c_int, [c_void_p]),
     "exc_residual_l1": (c_int, [c_void_p, P(c_double)]),
     "exc_fold_ahead": (c_int, [c_void_p, c_int]),
+    "exc_set_hub_destinations": (c_int, [c_void_p, c_void_p, c_int]),
     "exc_snapshot_noalloc": (c_int, [c_int, c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_int,
                                      c_void_p, c_void_p, c_void_p, c_void_p]),
     "exc_fp64_residual": (c_int, [c_void_p, c_double, c_void_p, c_double, c_int, c_void_p, P(c_double)]),
@@ -190,6 +191,7 @@ HOST_SIGNATURES = {
     "excent_tau": (c_double, [c_double, c_void_p, c_void_p]),
     "excent_prefetch_counts": (c_int, [ctypes.c_char_p, c_void_p, c_int, c_int, P(c_u64), P(c_u64)]),
     "excent_gather_list": (c_int, [c_void_p, c_int, c_int, c_void_p]),
+    "excent_top_in_degree": (c_int, [ctypes.c_char_p, c_int, c_double, c_int, c_void_p]),
     "excent_stage_lists": (c_int, [ctypes.c_char_p, c_void_p, c_int, c_int, c_int, c_void_p, c_void_p, P(c_u64),
                                    P(c_u64)]),
 }
@@ -370,6 +372,10 @@ class Context:
         out = (c_double * 3)()
         check(F["exc_residual_l1"](self.ctx, out), "residual_l1")
         return list(out)[: self.nm]
+
+    def set_hub_destinations(self, ids):
+        ids = np.ascontiguousarray(ids, dtype=np.int32)
+        check(F["exc_set_hub_destinations"](self.ctx, ptr(ids), len(ids)), "set_hub_destinations")
 
     def fold_ahead(self, allow):
         check(F["exc_fold_ahead"](self.ctx, 1 if allow else 0), "fold_ahead")

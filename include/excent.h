@@ -111,6 +111,9 @@ int excent_prefetch_counts(const char* path, const int32_t* vids, int k, int io_
  * when dedup. */
 int excent_stage_lists(const char* path, const int32_t* vids, int k, int dedup, int threads, uint64_t* off_out,
                        int32_t* edges_out, uint64_t* pages, uint64_t* runs);
+/* The vertices with the most in-edges (up to count), estimated from `sample` of the graph
+ * file's columns (all of them below 16M); returns how many were written to out. */
+int excent_top_in_degree(const char* path, int count, double sample, int threads, int32_t* out);
 /* src/excent.cu:711-720 */
 int excent_gather_list(const uint32_t* bitmap, int v0, int v1, int32_t* out);
 

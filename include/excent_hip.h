@@ -179,6 +179,14 @@ int exc_slot_acquire(void* ctx, int slot, double* waited_s);
  * measures storage-only (src/excent.cu:534-576). */
 int exc_slot_copy_probe(void* ctx, int slot, uint64_t bytes, double* seconds);
 
+/* Destinations whose pushed deltas are summed per block in LDS before they reach r (no
+ * reference counterpart; the reference adds every delta with its own atomic,
+ * src/excent.cu:118-123): up to 1024 vertex ids, normally those with the most in-edges,
+ * whose float atomics would otherwise serialise at the memory side.  Applies to the
+ * pieces of long out-lists (k_pieces).  count = 0 turns it off.  Any set is correct: a
+ * destination's deltas arrive by the end of the launch either way. */
+int exc_set_hub_destinations(void* ctx, const int32_t* ids, int count);
+
 /* Weighted graphs (no reference counterpart): per-edge coefficients c_e aligned with the
  * cached edges (the whole graph must be cached).  A push then adds p * c_e along edge e
  * with p = scale * d (set divdeg = 0 when the coefficients are already normalised).  Only

@@ -304,22 +304,24 @@ def test_params_tool_reproduces_sample_env(tmp_path):
 
 
 def test_run_sh_end_to_end(tmp_path):
-    """run.sh as the reference's users call it: the sample with its shipped constants
-    (--measure=all), and a copy of the sample without a .env, for which the wrapper writes
-    the constants next to the graph with bin/excent_params (the sample's own .env digits)
-    and passes them to bin/excent (--measure=katz)."""
+    """run.sh as the reference's users call it: --sample (this build's sample graph and its
+    constants, excent_amd/data) with --measure=all, and a copy of the reference's sample
+    fixture without a .env, for which the wrapper writes the constants next to the graph
+    with bin/excent_params (the fixture's own .env digits) and passes them to bin/excent
+    (--measure=katz)."""
     import shutil
     import subprocess
     from tests.conftest import ROOT
     sh = os.path.join(ROOT, "run.sh")
+    data = os.path.join(ROOT, "excent_amd", "data")
     print("case run.sh sample all", flush=True)
     r = subprocess.run([sh, "--measure=all", "--sample", "--", "--dump", str(tmp_path / "a"), "--stream-ratio", "0.1"],
                        capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stderr
-    assert re.search(r"measure all \(n=2000\)", r.stdout), r.stdout
-    idx, adj = orc.egr_load(SAMPLE_EGR)
+    assert re.search(r"measure all \(n=2048\)", r.stdout), r.stdout
+    idx, adj = orc.egr_load(os.path.join(data, "sample.egr"))
     pr_check(idx, adj, np.fromfile(str(tmp_path / "a.pr.bin"), np.float32))
-    env = read_env(SAMPLE_ENV)
+    env = read_env(os.path.join(data, "sample.env"))
     kref = orc.fp64_reference(idx, adj, orc.ORC_KATZ, alpha=env["ALPHA"])[0]
     assert orc.compare(kref, np.fromfile(str(tmp_path / "a.katz.bin"), np.float32))["l1_rel"] < 0.05
     g = str(tmp_path / "g.egr")
@@ -329,10 +331,11 @@ def test_run_sh_end_to_end(tmp_path):
                        capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stderr
     assert os.path.exists(str(tmp_path / "g.env")) and "Computing the constants" in r.stdout
-    genv = read_env(str(tmp_path / "g.env"))
+    genv, fenv = read_env(str(tmp_path / "g.env")), read_env(SAMPLE_ENV)
     for k in ("LAMBDA", "ALPHA", "EPS_KATZ", "EPS_EV"):
-        assert abs(genv[k] - env[k]) <= 1e-6 * abs(env[k]), k
-    kref = orc.fp64_reference(idx, adj, orc.ORC_KATZ, alpha=genv["ALPHA"])[0]
+        assert abs(genv[k] - fenv[k]) <= 1e-6 * abs(fenv[k]), k
+    sidx, sadj = orc.egr_load(SAMPLE_EGR)
+    kref = orc.fp64_reference(sidx, sadj, orc.ORC_KATZ, alpha=genv["ALPHA"])[0]
     assert orc.compare(kref, np.fromfile(str(tmp_path / "k.katz.bin"), np.float32))["l1_rel"] < 0.05
 
 

@@ -355,3 +355,52 @@ def test_weighted_graph_refuses_several_measures(rmat12):
             ctx.cache_upload(adj, 0)
             assert nat.F["exc_set_edge_weights"](ctx.ctx, nat.ptr(wn), m) == nat.EXC_EINVAL
             assert "one measure" in nat.F["exc_last_error"]().decode()
+
+
+@pytest.mark.parametrize("nm", [1, 3])
+def test_hub_destinations_privatised(nm):
+    """Hub pieces whose lists converge on a few destinations (40 sources of 3000 out-edges
+    into the same 3000 vertices, every such vertex pointing back to 2 sources): with those
+    destinations (and some vertices no list reaches) privatised in LDS, the run meets the
+    FP64 bound and the per-vertex invariant like the run without privatisation, and the two
+    results agree within the push's float rounding."""
+    S_, T = 40, 3000
+    n = S_ + T + 10
+    lists = [np.arange(S_, S_ + T) for _ in range(S_)] + [np.array([(v * 7) % S_, (v * 13 + 1) % S_])
+                                                       for v in range(T)] + [np.zeros(0, np.int64)] * 10
+    deg = np.array([len(l) for l in lists])
+    idx = np.concatenate([[0], np.cumsum(deg)]).astype(np.uint64)
+    adj = np.concatenate(lists).astype(np.int32)
+    m = len(adj)
+    kinds = [nat.EXC_PR, nat.EXC_KATZ, nat.EXC_EV][:nm]
+    scale = [0.85, 1e-4, 1e-4][:nm]
+    hubs = np.concatenate([np.arange(S_, S_ + 1000), np.arange(n - 10, n)]).astype(np.int32)
+    results = []
+    for table in (None, hubs):
+        with nat.Context(n, m, np.array([0], np.int32), kinds, [1e-6] * nm, scale, [1] + [0] * (nm - 1),
+                         timing=True) as ctx:
+            ctx.upload_offsets(idx)
+            for i in range(nm):
+                ctx.set_state(i, x=np.zeros(n, np.float32), r=np.full(n, 0.15, np.float32))
+            assert ctx.cache_chunks(m) == 0
+            ctx.cache_upload(adj, 0)
+            if table is not None:
+                ctx.set_hub_destinations(table)
+            for p in range(5000):
+                cnt, _ = ctx.snapshot()
+                if cnt[0::2].sum() == 0:
+                    break
+                ctx.push_resident(0, n, 0, m, False)
+                ctx.pass_sync()
+            results.append(([ctx.read_state(i, 0) for i in range(nm)], [ctx.read_state(i, 1) for i in range(nm)]))
+            assert ctx.stats()["piece_launches"] > 0
+    ref, _, _, _ = orc.fp64_reference(idx, adj, orc.ORC_PR, tol=1e-13, max_iter=5000)
+    for xs, rs in results:
+        x = xs[0].astype(np.float64)
+        true = true_residual(idx, adj, xs[0], 0.15, 0.85, True)
+        assert np.abs(x - ref).sum() <= np.abs(true).sum() / 0.15 * (1 + 1e-6) + 1e-6
+        assert np.abs(true - rs[0]).max() < 2e-4 * max(1.0, np.abs(x).max())
+        for i in range(1, nm):   # the other measures keep their invariant r = b + W x - x too
+            tr = true_residual(idx, adj, xs[i], 0.15, scale[i], False)
+            assert np.abs(tr - rs[i]).max() < 1e-4 * max(1.0, np.abs(xs[i]).max())
+    np.testing.assert_allclose(results[0][0][0], results[1][0][0], rtol=1e-4, atol=1e-5)

@@ -254,3 +254,15 @@ def test_stage_lists_packs_the_lists(tmp_path, threads):
                 a, b = ctypes.c_uint64(0), ctypes.c_uint64(0)
                 ref.ref_prefetch_counts(path.encode(), nat.ptr(vids), len(vids), 16, ctypes.byref(a), ctypes.byref(b))
                 assert (a.value, b.value) == (pg.value, rn.value)
+
+
+def test_top_in_degree_exact_below_sample_floor(rmat12):
+    """Graphs under 16M columns are read whole: the chosen destinations are the ones with
+    the largest in-degrees (ties in any order)."""
+    idx, adj = orc.egr_load(rmat12)
+    indeg = np.bincount(adj, minlength=len(idx) - 1)
+    for count in (1, 37, 1024):
+        out = np.zeros(count, np.int32)
+        k = nat.F["excent_top_in_degree"](rmat12.encode(), count, 1.0 / 16, 4, nat.ptr(out))
+        assert k == min(count, int((indeg > 0).sum()))
+        assert np.array_equal(np.sort(indeg[out[:k]])[::-1], np.sort(indeg)[::-1][:k])

@@ -186,3 +186,27 @@ def test_bench_top_overlap_known_answers():
     b[[4999, 10]] = b[[10, 4999]]                            # the largest swapped out of the top
     assert bench.top_overlap(np, a, b, 1000) == 999
     assert bench.top_overlap(np, a[:50], a[:50], 1000) == 50   # k capped at the length
+
+
+def test_build_sample_graph_and_constants():
+    """run.sh --sample's graph is this build's own: the seeded symmetric R-MAT the generator
+    writes (same bytes when written again), with the constants bin/excent_params wrote for it
+    (lambda1 checked against a dense symmetric eigensolve, alpha = 0.85 / lambda1)."""
+    import filecmp
+    import tempfile
+    from tests.conftest import ROOT, read_env
+    data = os.path.join(ROOT, "excent_amd", "data")
+    with tempfile.TemporaryDirectory() as td:
+        p = os.path.join(td, "s.egr")
+        assert nat.run_entry("excent_rmat_main", ["--scale", 11, "--edge-factor", 8, "--seed", 2026, "--symmetric",
+                                                  "--wide", "on", p]) == 0
+        assert filecmp.cmp(p, os.path.join(data, "sample.egr"), shallow=False)
+    idx, adj = orc.egr_load(os.path.join(data, "sample.egr"))
+    n = len(idx) - 1
+    A = np.zeros((n, n))
+    A[np.repeat(np.arange(n), np.diff(idx.astype(np.int64))), adj] = 1
+    assert np.array_equal(A, A.T)
+    lam = np.linalg.eigvalsh(A)[-1]
+    env = read_env(os.path.join(data, "sample.env"))
+    assert abs(env["LAMBDA"] - lam) <= 1e-8 * lam
+    assert abs(env["ALPHA"] - 0.85 / env["LAMBDA"]) <= 1e-5 * env["ALPHA"]   # written with 6 digits
