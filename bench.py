@@ -302,7 +302,7 @@ def bench_ooc(args, nat, np, work, graph):
     for tag, extra, cold in modes:
         sargs = [graph, "--measure", measure, "--timing", "--log",
                  os.path.join(args.logdir or work, f"ooc_{tag.split()[0]}{'_cold' if cold else ''}.log")] + extra
-        sargs += measure_args(measure, env) + ["--stop", args.stop]
+        sargs += measure_args(measure, env) + ["--stop", stop_rule(args, measure)]
         if args.max_passes:
             sargs += ["--max-passes", args.max_passes]
         if tag == hot:
@@ -525,6 +525,16 @@ def cpu_baseline_run(np, orc, idx, adj, xs, rs, eps, scale, divdeg, seconds, wha
     return rec
 
 
+def stop_rule(args, measure):
+    """The loop's stop rule: --stop when given; otherwise the reference's per-vertex rule,
+    except that with --measure all Katz and the eigenvector end on their global sum
+    sum |r| <= 1e-6 |V| (the configuration's eps = 1e-6 |V| read as a global L1 bound), while
+    PageRank keeps eps 0.01 per vertex."""
+    if args.stop:
+        return args.stop
+    return "katz,ev" if measure == "all" else "vertex"
+
+
 def bench_plain(args, nat, np, work, graph, scale, measure, steps, warmup, env, tag):
     """Converged runs of bin/excent's engine with the whole graph in HBM: steps timed
     back to back after `warmup` untimed ones, each from the initial state (a device copy);
@@ -532,7 +542,7 @@ def bench_plain(args, nat, np, work, graph, scale, measure, steps, warmup, env, 
     excent_log = os.path.join(args.logdir or work, f"{tag}.log")
     sargs = [graph, "--measure", measure, "--cache", "chunks", "--preload", "--stream-ratio", "1",
              "--timing", "--push-kernel", args.kernel, "--pull", args.pull, "--hub-lds", args.hub_lds, "--log", excent_log,
-             "--dump", os.path.join(work, "x"), "--stop", args.stop,
+             "--dump", os.path.join(work, "x"), "--stop", stop_rule(args, measure),
              # a run whose stop rule is never met ends here and is reported as not converged
              "--max-passes", args.max_passes or 2000] + measure_args(measure, env)
     t0 = time.time()
@@ -692,8 +702,9 @@ def main():
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--measure", default=None, choices=["pr", "katz", "ev", "all"],
                     help="measures of the plain runs (default: all; ooc: pr)")
-    ap.add_argument("--stop", default="vertex", choices=["vertex", "global"],
-                    help="the loop's stop rule (global: also when every measure's sum |r| <= n eps)")
+    ap.add_argument("--stop", default=None,
+                    help="the loop's stop rule: vertex, global or a list of measures that end on their sum "
+                         "|r| <= n eps (default: katz,ev with --measure all, else vertex)")
     ap.add_argument("--max-passes", type=int, default=0,
                     help="stop a run after this many passes (reported as not converged; plain runs: 2000 when 0)")
     ap.add_argument("--cold", action="store_true",
@@ -766,7 +777,8 @@ def main():
         "config": {"workload": f"R-MAT scale {args.scale} ef {args.edge_factor} --measure {args.measure}, "
                                f"whole graph in HBM, converged to eps", "n": n_v, "m": m_e,
                    "lambda1": env["LAMBDA"], "alpha": env["ALPHA"], "eps_pr": 0.01, "eps_katz": 1e-6,
-                   "eps_ev": 1e-6, "push_kernel": args.kernel, "pull": args.pull, "parallelism": "single GPU"},
+                   "eps_ev": 1e-6, "stop": stop_rule(args, args.measure), "push_kernel": args.kernel, "pull": args.pull,
+                   "parallelism": "single GPU"},
     }
     out.update({k: v for k, v in rec.items() if k not in ("value", "unit", "steps", "warmup", "ms_per_step")})
     out["cpu_baseline"] = cpu

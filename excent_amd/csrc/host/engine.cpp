@@ -77,7 +77,10 @@ struct Options {
     std::string ppr_seeds, ppr_gen;
     std::string pull = "auto";   // dense passes by pull over a device transpose: auto | off | threshold
     double eps_ppr = -1;
-    std::string stop = "vertex";  // vertex: nothing pending; global: also when sum |r_k| <= n eps_k
+    // vertex: stop when nothing is pending (the reference's rule); global: when each measure
+    // either has nothing pending or sum |r_k| <= n eps_k; or a comma list of the measures
+    // (pr, katz, ev) that take the global rule while the others keep the per-vertex one
+    std::string stop = "vertex";
     int io_threads = 0;           // host threads for reads, gathers and checks (0: all)
     int hub_lds = 1024;           // destinations privatised in LDS by the hub-piece push (0: none)
 };
@@ -257,6 +260,8 @@ private:
     double l1_[EXC_MAXM] = {0, 0, 0};   // FP64 sum of |r| per measure, last snapshot
     bool have_l1_ = false;
     int stop_reason_ = 0;               // 0 nothing pending, 1 global L1 bound, 2 pass limit
+    std::vector<char> global_l1_;       // per measure slot: the global L1 rule may end it (--stop)
+    float linf_[EXC_MAXM] = {0, 0, 0};  // largest |r| per measure, last snapshot
 
     void init_state();
     void load_ppr_seeds(EgrGraph& g);
@@ -321,9 +326,23 @@ void Engine::setup() {
     if (o.group_opt < 0 || o.group_opt > 64) die("--group must be 0 (automatic) or 1..64");
     if (o.hot_when != "end" && o.hot_when != "start" && o.hot_when != "both") die("--hot-when must be end, start or both");
     if (o.hot_order != "sweep" && o.hot_order != "ranges") die("--hot-order must be sweep or ranges");
-    if (o.stop != "vertex" && o.stop != "global") die("--stop must be vertex or global");
     if (o.hub_lds < 0 || o.hub_lds > 1024) die("--hub-lds must be 0..1024");
-    if (o.stop == "global" && o.measure == "ppr") die("--stop global needs plain measures (pr, katz, ev, all)");
+    global_l1_.assign(kinds_.size(), 0);
+    if (o.stop == "global") {
+        global_l1_.assign(kinds_.size(), 1);
+    } else if (o.stop != "vertex") {
+        size_t at = 0;
+        while (at <= o.stop.size()) {
+            const size_t comma = std::min(o.stop.find(',', at), o.stop.size());
+            const std::string name = o.stop.substr(at, comma - at);
+            bool found = false;
+            for (size_t i = 0; i < kinds_.size(); i++)
+                if (name == MNAME[kinds_[i]]) { global_l1_[i] = 1; found = true; }
+            if (!found) die("--stop must be vertex, global or a list of this run's measures (pr, katz, ev)");
+            at = comma + 1;
+        }
+    }
+    if (o.stop != "vertex" && o.measure == "ppr") die("--stop global needs plain measures (pr, katz, ev, all)");
 
     if (o.io_threads < 0) die("--io-threads must be 0 (every host thread) or a thread count");
     try {
@@ -678,7 +697,8 @@ void Engine::run_passes() {
     tot_pages_ = tot_runs_ = 0;
     stop_reason_ = 2;
     have_l1_ = false;
-    const bool stop_global = o.stop == "global";
+    bool stop_global = false;
+    for (char g1 : global_l1_) stop_global = stop_global || g1;
     if (pull_on_) ck(exc_fold_ahead(ctx, 1), "fold ahead");
     bool fold_ahead_off = false;   // set once a folded state met the global bound
     const Clock::time_point T_begin = Clock::now();
@@ -717,11 +737,13 @@ void Engine::run_passes() {
         }
         const PlanOut po = fetch_plan(nc, h_cnt.data(), resident_, loaded_, stream_ratio_, chunk_edges, hot_mode_, pass_, hlo_, hhi_, mode.data());
         // FP64 global residual of the snapshot state (plain measures)
-        have_l1_ = !batched() && exc_residual_l1(ctx, l1_) == EXC_OK;
+        have_l1_ = !batched() && exc_residual_l1(ctx, l1_) == EXC_OK && exc_residual_max(ctx, linf_) == EXC_OK;
         bool global_met = false;
         if (stop_global && have_l1_) {
+            // every measure done: nothing pending for it, or (global rule) its sum within n eps
             global_met = true;
-            for (int i = 0; i < M_.nm; i++) global_met = global_met && l1_[i] <= (double)n * (double)M_.eps[i];
+            for (int i = 0; i < M_.nm; i++)
+                global_met = global_met && (linf_[i] <= M_.eps[i] || (global_l1_[i] && l1_[i] <= (double)n * (double)M_.eps[i]));
             if (global_met && exc_dense_required(ctx) == 1) {
                 // the last dense pass folded this state: its deltas are still in flight, so the
                 // state cannot be returned; deliver them with no further fold and test again
@@ -828,7 +850,7 @@ void Engine::run_passes() {
     T_setup_ = T_begin;
     if (stop_reason_ == 2 && !batched()) {   // stopped by the pass limit: the residual of the final state
         ck(exc_snapshot(ctx, 0, h_cnt.data(), &h_hotpend), "final snapshot");
-        have_l1_ = exc_residual_l1(ctx, l1_) == EXC_OK;
+        have_l1_ = exc_residual_l1(ctx, l1_) == EXC_OK && exc_residual_max(ctx, linf_) == EXC_OK;
     }
 }
 
@@ -902,6 +924,9 @@ void Engine::report() {
         fprintf(out, "residual l1 (fp64):");
         for (int i = 0; i < M_.nm; i++) fprintf(out, "  %s %.6e (n*eps %.6e)", MNAME[kinds_[i]], l1_[i], (double)n_ * (double)M_.eps[i]);
         fprintf(out, " | stop: %s\n", stop_reason_ == 0 ? "nothing pending" : stop_reason_ == 1 ? "global l1 <= n*eps" : "pass limit");
+        fprintf(out, "residual max:");
+        for (int i = 0; i < M_.nm; i++) fprintf(out, "  %s %.6e (eps %.6e)", MNAME[kinds_[i]], (double)linf_[i], (double)M_.eps[i]);
+        fprintf(out, "\n");
     }
     if (o.count_pushes) fprintf(out, "edges pushed: %.2f x |E|\n", m_ ? (double)pushed / m_ : 0.0);
     fprintf(out, "launches from GPU memory: %llu pending-only, %llu whole-range\n", (unsigned long long)n_list_, (unsigned long long)n_range_);

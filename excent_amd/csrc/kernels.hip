@@ -23,6 +23,7 @@
 #include <vector>
 
 #include "excent_hip.h"
+#include "pull_sort.hpp"
 
 namespace {
 
@@ -159,8 +160,19 @@ __device__ __forceinline__ double wave_sum_f64(double v) {
 // a few thousand FP64 atomics per pass; the host adds the slots.  The slots sit behind the
 // per-chunk counts in the counts buffer and are cleared with them.
 constexpr int L1_SLOTS = 64, L1_STRIDE = 16;   // doubles per slot line
+constexpr int LINF_AT = 8;                       // per slot: the max |r| per measure, as float bits, from double 8 on
 __device__ __forceinline__ void l1_flush(double* l1, long long who, int k, double v) {
     if (v != 0.0) __hip_atomic_fetch_add(l1 + (who % L1_SLOTS) * L1_STRIDE + k, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+// The largest |r| per measure: |r| as float bits (non-negative floats order as unsigned
+// integers), maximised per wave or block, then one atomic max per slot.
+__device__ __forceinline__ void linf_flush(double* l1, long long who, int k, uint32_t bits) {
+    if (bits) atomicMax(reinterpret_cast<unsigned int*>(l1 + (who % L1_SLOTS) * L1_STRIDE + LINF_AT) + k, bits);
+}
+__device__ __forceinline__ uint32_t wave_max_u32(uint32_t v) {
+#pragma unroll
+    for (int o = WAVE / 2; o > 0; o >>= 1) v = max(v, (uint32_t)__shfl_xor(v, o, WAVE));
+    return v;
 }
 
 // src/excent.cu:49-56: last chunk c with cv0[c] <= v.
@@ -281,8 +293,9 @@ __global__ __launch_bounds__(BLOCK) void k_snapshot(SnapArgs a) {
     uint32_t acc_n = 0;
     uint64_t acc_d = 0, hp = 0;
     double l1[NM];
+    uint32_t mx[NM];
 #pragma unroll
-    for (int k = 0; k < NM; k++) l1[k] = 0.0;
+    for (int k = 0; k < NM; k++) { l1[k] = 0.0; mx[k] = 0u; }
 
     auto flush = [&]() {
         uint32_t tn = wave_sum_u32(acc_n);
@@ -310,7 +323,10 @@ __global__ __launch_bounds__(BLOCK) void k_snapshot(SnapArgs a) {
 #pragma unroll
         for (int u = 0; u < U; u++) {   // rows past n or past the wave's range are zero
 #pragma unroll
-            for (int k = 0; k < NM; k++) l1[k] += fabs((double)rows[u].v[k]);
+            for (int k = 0; k < NM; k++) {
+                l1[k] += fabs((double)rows[u].v[k]);
+                mx[k] = max(mx[k], __float_as_uint(fabsf(rows[u].v[k])));
+            }
         }
 #pragma unroll
         for (int u = 0; u < U; u++) {
@@ -366,7 +382,11 @@ __global__ __launch_bounds__(BLOCK) void k_snapshot(SnapArgs a) {
 #pragma unroll
         for (int k = 0; k < NM; k++) {
             const double t = wave_sum_f64(l1[k]);
-            if (lane == 0) l1_flush(a.l1, gwave, k, t);
+            const uint32_t m = wave_max_u32(mx[k]);
+            if (lane == 0) {
+                l1_flush(a.l1, gwave, k, t);
+                linf_flush(a.l1, gwave, k, m);
+            }
         }
     }
 }
@@ -863,6 +883,144 @@ __device__ __forceinline__ bool row_live(const uint32_t* __restrict__ live, int 
     return !live || ((live[q >> 5] >> (q & 31)) & 1u);
 }
 
+// The end of a dense pull for BLOCK consecutive destinations u0 .. u0 + nt - 1, one per
+// thread, with `acc` the thread's sums of in-edge rows: the row update of r, and when the
+// next snapshot is fused (bitmap != nullptr) the pending test of the next pass -- the same
+// predicate and outputs as k_snapshot without hot set or trace: two bitmap words per wave,
+// per-chunk counts reduced per block, the FP64 residual of the snapshot state -- and when
+// p_next is set, the next pass's fold on the row in registers.  Every row is final here
+// (the hub rows were completed by k_pull_hub before the pull).  `who` spreads the residual
+// slots.  Block-uniform: every thread of the block calls it.
+template <int NM, int S>
+__device__ void pull_finish(const PullArgs& a, const DevMeasures& M, long long u0, int nt, const float (&acc)[NM],
+                            unsigned long long* slot, long long who) {
+    __shared__ unsigned long long s_cnt[NWAVES][2];
+    __shared__ unsigned long long s_fold[NWAVES][4];
+    __shared__ double s_l1[NWAVES][NM];
+    __shared__ uint32_t s_mx[NWAVES][NM];
+    const int tid = threadIdx.x;
+    if (!a.bitmap) {
+        if (tid < nt) {
+            const long long u = u0 + tid;
+#pragma unroll
+            for (int k = 0; k < NM; k++)
+                if (acc[k] != 0.f) M.r[u * S + k] += acc[k];
+        }
+        return;
+    }
+    bool pend = false;
+    uint64_t d = 0;
+    uint32_t nmv = 0;
+    double l1[NM];
+    uint32_t mx[NM];
+#pragma unroll
+    for (int k = 0; k < NM; k++) { l1[k] = 0.0; mx[k] = 0u; }
+    if (tid < nt) {
+        const long long u = u0 + tid;
+        Row<S> row = load_row<S>(M.r, u);
+        bool upd = false;
+#pragma unroll
+        for (int k = 0; k < NM; k++)
+            if (acc[k] != 0.f) { row.v[k] += acc[k]; upd = true; }
+        // the snapshot state's residual: the row after this pass, before the fused fold
+#pragma unroll
+        for (int k = 0; k < NM; k++) {
+            l1[k] = fabs((double)row.v[k]);
+            mx[k] = __float_as_uint(fabsf(row.v[k]));
+        }
+#pragma unroll
+        for (int k = 0; k < NM; k++) pend |= fabsf(row.v[k]) > M.eps[k];
+        if (pend) d = a.idx[u + 1] - a.idx[u];
+        if (a.p_next) {   // the next pass's fold (k_fold's rule, on the row in registers)
+            Row<S> pr, dx;
+#pragma unroll
+            for (int k = 0; k < S; k++) { pr.v[k] = 0.f; dx.v[k] = 0.f; }
+            if (pend) {
+#pragma unroll
+                for (int k = 0; k < NM; k++) {
+                    if (fabsf(row.v[k]) > M.eps[k]) {
+                        const float dv = row.v[k];
+                        row.v[k] = 0.f;
+                        dx.v[k] = dv;
+                        nmv++;
+                        if (d) pr.v[k] = M.divdeg[k] ? M.scale[k] * dv / (float)d : M.scale[k] * dv;
+                    }
+                }
+                Row<S> xr = load_row<S>(M.x, u);
+#pragma unroll
+                for (int k = 0; k < NM; k++) xr.v[k] += dx.v[k];
+                store_row<S>(M.x, u, xr);
+                upd = true;
+            }
+            const int q = a.rank[u];
+            if (q < a.nsrc) {
+                float* pq = a.p_next + (long long)q * NM;
+#pragma unroll
+                for (int k = 0; k < NM; k++) pq[k] = pr.v[k];
+            }
+        }
+        if (upd) store_row<S>(M.r, u, row);
+    }
+    const uint64_t mask = __ballot(pend);
+    const long long wb = u0 + (tid & ~(WAVE - 1));
+    const int lane = lane_id();
+    if (lane == 0 && wb < a.n) {
+        const long long w0 = wb >> 5, words = ((long long)a.n + 31) / 32;
+        a.bitmap[w0] = (uint32_t)mask;
+        if (w0 + 1 < words) a.bitmap[w0 + 1] = (uint32_t)(mask >> 32);
+    }
+    if (a.l1) {   // block sums and maxima, one atomic each per measure and block
+#pragma unroll
+        for (int k = 0; k < NM; k++) {
+            const double t = wave_sum_f64(l1[k]);
+            const uint32_t m = wave_max_u32(mx[k]);
+            if (lane == 0) { s_l1[tid / WAVE][k] = t; s_mx[tid / WAVE][k] = m; }
+        }
+        __syncthreads();
+        if (tid < NM) {
+            double t = 0.0;
+            uint32_t m = 0u;
+#pragma unroll
+            for (int w = 0; w < NWAVES; w++) { t += s_l1[w][tid]; m = max(m, s_mx[w][tid]); }
+            l1_flush(a.l1, who, tid, t);
+            linf_flush(a.l1, who, tid, m);
+        }
+    }
+    const int c0 = chunk_of((int)u0, a.cv0, a.nc), c1 = chunk_of((int)(u0 + nt - 1), a.cv0, a.nc);
+    if (c0 == c1) {   // the usual case: the destinations lie in one chunk
+        const uint32_t tn = wave_sum_u32(pend ? 1u : 0u);
+        const uint64_t td = wave_sum_u64(d);
+        if (lane == 0) { s_cnt[tid / WAVE][0] = tn; s_cnt[tid / WAVE][1] = td; }
+        __syncthreads();
+        if (tid < 2) {   // thread 0: pending vertices, thread 1: their out-degree sum
+            unsigned long long t = 0;
+#pragma unroll
+            for (int w = 0; w < NWAVES; w++) t += s_cnt[w][tid];
+            if (t) atomicAdd(&a.cnt[2 * c0 + tid], t);
+        }
+    } else if (pend) {
+        const int c = chunk_of((int)(u0 + tid), a.cv0, a.nc);
+        atomicAdd(&a.cnt[2 * c], 1ull);
+        atomicAdd(&a.cnt[2 * c + 1], (unsigned long long)d);
+    }
+    if (a.p_next) {   // fused fold statistics, one atomic per field and block (spread slots)
+        const uint64_t f[4] = {nmv ? 1ull : 0ull, nmv, nmv ? d : 0ull, d * nmv};
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            const uint64_t t = wave_sum_u64(f[i]);
+            if (lane == 0) s_fold[tid / WAVE][i] = t;
+        }
+        __syncthreads();
+        if (tid < 4) {
+            unsigned long long sum = 0;
+#pragma unroll
+            for (int w = 0; w < NWAVES; w++) sum += s_fold[w][tid];
+            if (sum) atomicAdd(&slot[S_VERT + tid], sum);
+        }
+    }
+    __syncthreads();   // a next call by the same block reuses the shared arrays
+}
+
 constexpr uint32_t PULL_HUB = 4096;   // in-lists longer than this are summed by whole blocks
 
 // Sequential-slot pull.  A block owns BLOCK consecutive destinations; with hub in-lists
@@ -993,144 +1151,120 @@ __global__ __launch_bounds__(BLOCK, 8) void k_pull(PullArgs a, DevMeasures M) {
         }
         __syncthreads();   // the next window rewrites s_src
     }
-    __shared__ unsigned long long s_red[NWAVES][5];
+    __shared__ unsigned long long s_red[NWAVES];
     unsigned long long* slot = a.spread + (blockIdx.x % SPREAD_SLOTS) * SPREAD_STRIDE;
     {
         const uint32_t t = wave_sum_u32(taken);
-        if (lane_id() == 0) s_red[tid / WAVE][0] = t;
+        if (lane_id() == 0) s_red[tid / WAVE] = t;
         __syncthreads();
         if (tid == 0) {
             unsigned long long sum = 0;
 #pragma unroll
-            for (int w = 0; w < NWAVES; w++) sum += s_red[w][0];
+            for (int w = 0; w < NWAVES; w++) sum += s_red[w];
             if (sum) atomicAdd(&slot[S_ROWS], sum);
         }
         __syncthreads();
     }
-    if (!a.bitmap) {
-        if (tid < nt && s_start[tid + 1] > s_start[tid]) {
-            const long long u = u0 + tid;
+    float accv[NM];
+#pragma unroll
+    for (int k = 0; k < NM; k++) accv[k] = tid < nt && s_start[tid + 1] > s_start[tid] ? s_acc[k][tid] : 0.f;
+    pull_finish<NM, S>(a, M, u0, nt, accv, slot, blockIdx.x);
+}
+
+// Source-sorted dense pull.  A block owns T consecutive destinations; their non-hub in-edges
+// are stored per tile sorted by source row (row id + the destination's offset in the tile),
+// and the block streams them in that order: lane t takes entries t, t + BLOCK, ... of each
+// step of BLOCK x U entries, so a wave gathers 64 consecutive entries of the sorted order --
+// rows of the same or neighbouring sources, which share cache lines, are served by one
+// request per line instead of one per lane.  Each entry's row is added to its destination's
+// accumulator in LDS (ds_add_f32).  The destinations are then finished BLOCK at a time by
+// pull_finish (row update, fused snapshot and fold), as in k_pull.
+struct SortedTiles {
+    const uint64_t* start;   // [tiles + 1]
+    const int32_t* row;      // p row of each entry, ascending within a tile
+    const uint16_t* dst;     // destination offset in the tile
+    const float* w;          // weighted graphs: the entry's coefficient
+};
+
+template <int NM, int S, int T, bool W = false>
+__global__ __launch_bounds__(BLOCK) void k_pull_sorted(PullArgs a, SortedTiles st, DevMeasures M) {
+    constexpr int U = 8;   // entries per lane in flight
+    __shared__ float s_acc[NM][T];
+    __shared__ unsigned long long s_red[NWAVES];
+    const int tid = threadIdx.x;
+    const long long u0 = (long long)blockIdx.x * T;
+    for (int i = tid; i < T; i += BLOCK) {
+#pragma unroll
+        for (int k = 0; k < NM; k++) s_acc[k][i] = 0.f;
+    }
+    __syncthreads();
+    const uint64_t e0 = st.start[blockIdx.x], e1 = st.start[blockIdx.x + 1];
+    uint32_t taken = 0;
+    for (uint64_t b = e0; b < e1; b += (uint64_t)BLOCK * U) {
+        // ids, destinations and coefficients of the lane's entries, then their live-bitmap
+        // words, then the rows: each stage issued as one batch
+        int vid[U];
+        uint32_t dl[U];
+        float wv[W ? U : 1];
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const uint64_t e = b + (uint64_t)u * BLOCK + tid;
+            const bool ok = e < e1;
+            vid[u] = ok ? st.row[e] : -1;
+            dl[u] = ok ? st.dst[e] : 0u;
+            if constexpr (W) wv[u] = ok ? st.w[e] : 0.f;
+        }
+        bool lv[U];
+        if (a.live) {
+            uint32_t wd[U];
+#pragma unroll
+            for (int u = 0; u < U; u++) wd[u] = vid[u] >= 0 ? a.live[vid[u] >> 5] : 0u;
+#pragma unroll
+            for (int u = 0; u < U; u++) lv[u] = vid[u] >= 0 && ((wd[u] >> (vid[u] & 31)) & 1u);
+        } else {
+#pragma unroll
+            for (int u = 0; u < U; u++) lv[u] = vid[u] >= 0;
+        }
+        Row<S> rows[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+#pragma unroll
+            for (int k = 0; k < S; k++) rows[u].v[k] = 0.f;
+            if (lv[u]) {
+                rows[u] = load_prow<NM, S>(a.p, vid[u]);
+                taken++;
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < U; u++) {
 #pragma unroll
             for (int k = 0; k < NM; k++) {
-                const float acc = s_acc[k][tid];
-                if (acc != 0.f) M.r[u * S + k] += acc;
+                const float v = W ? rows[u].v[k] * wv[u] : rows[u].v[k];
+                if (v != 0.f) atomicAdd(&s_acc[k][dl[u]], v);
             }
         }
-        return;
     }
-    // Fused snapshot: every destination's row is final here (the hub rows were completed by
-    // k_pull_hub before this kernel), so the pending test of the next pass -- the same
-    // predicate and outputs as k_snapshot without hot set or trace -- runs on the row in
-    // registers: two bitmap words per wave, per-chunk counts reduced per block.
-    __shared__ unsigned long long s_cnt[NWAVES][2];
-    __shared__ double s_l1[NWAVES][NM];
-    bool pend = false;
-    uint64_t d = 0;
-    uint32_t nmv = 0;
-    double l1[NM];
-#pragma unroll
-    for (int k = 0; k < NM; k++) l1[k] = 0.0;
-    if (tid < nt) {
-        const long long u = u0 + tid;
-        Row<S> row = load_row<S>(M.r, u);
-        bool upd = false;
-        if (s_start[tid + 1] > s_start[tid]) {
-#pragma unroll
-            for (int k = 0; k < NM; k++) {
-                const float acc = s_acc[k][tid];
-                if (acc != 0.f) { row.v[k] += acc; upd = true; }
-            }
-        }
-        // the snapshot state's residual: the row after this pass, before the fused fold
-#pragma unroll
-        for (int k = 0; k < NM; k++) l1[k] = fabs((double)row.v[k]);
-#pragma unroll
-        for (int k = 0; k < NM; k++) pend |= fabsf(row.v[k]) > M.eps[k];
-        if (pend) d = a.idx[u + 1] - a.idx[u];
-        if (a.p_next) {   // the next pass's fold (k_fold's rule, on the row in registers)
-            Row<S> pr, dx;
-#pragma unroll
-            for (int k = 0; k < S; k++) { pr.v[k] = 0.f; dx.v[k] = 0.f; }
-            if (pend) {
-#pragma unroll
-                for (int k = 0; k < NM; k++) {
-                    if (fabsf(row.v[k]) > M.eps[k]) {
-                        const float dv = row.v[k];
-                        row.v[k] = 0.f;
-                        dx.v[k] = dv;
-                        nmv++;
-                        if (d) pr.v[k] = M.divdeg[k] ? M.scale[k] * dv / (float)d : M.scale[k] * dv;
-                    }
-                }
-                Row<S> xr = load_row<S>(M.x, u);
-#pragma unroll
-                for (int k = 0; k < NM; k++) xr.v[k] += dx.v[k];
-                store_row<S>(M.x, u, xr);
-                upd = true;
-            }
-            const int q = a.rank[u];
-            if (q < a.nsrc) {
-                float* pq = a.p_next + (long long)q * NM;
-#pragma unroll
-                for (int k = 0; k < NM; k++) pq[k] = pr.v[k];
-            }
-        }
-        if (upd) store_row<S>(M.r, u, row);
-    }
-    const uint64_t mask = __ballot(pend);
-    const long long wb = u0 + (tid & ~(WAVE - 1));
-    const int lane = lane_id();
-    if (lane == 0 && wb < a.n) {
-        const long long w0 = wb >> 5, words = ((long long)a.n + 31) / 32;
-        a.bitmap[w0] = (uint32_t)mask;
-        if (w0 + 1 < words) a.bitmap[w0 + 1] = (uint32_t)(mask >> 32);
-    }
-    if (a.l1) {   // block sums, one FP64 atomic per measure and block
-#pragma unroll
-        for (int k = 0; k < NM; k++) {
-            const double t = wave_sum_f64(l1[k]);
-            if (lane == 0) s_l1[tid / WAVE][k] = t;
-        }
+    __syncthreads();
+    unsigned long long* slot = a.spread + (blockIdx.x % SPREAD_SLOTS) * SPREAD_STRIDE;
+    {
+        const uint32_t t = wave_sum_u32(taken);
+        if (lane_id() == 0) s_red[tid / WAVE] = t;
         __syncthreads();
-        if (tid < NM) {
-            double t = 0.0;
-#pragma unroll
-            for (int w = 0; w < NWAVES; w++) t += s_l1[w][tid];
-            l1_flush(a.l1, blockIdx.x, tid, t);
-        }
-    }
-    const int c0 = chunk_of((int)u0, a.cv0, a.nc), c1 = chunk_of((int)(u0 + nt - 1), a.cv0, a.nc);
-    if (c0 == c1) {   // the usual case: the tile lies in one chunk
-        const uint32_t tn = wave_sum_u32(pend ? 1u : 0u);
-        const uint64_t td = wave_sum_u64(d);
-        if (lane == 0) { s_cnt[tid / WAVE][0] = tn; s_cnt[tid / WAVE][1] = td; }
-        __syncthreads();
-        if (tid < 2) {   // thread 0: pending vertices, thread 1: their out-degree sum
-            unsigned long long t = 0;
-#pragma unroll
-            for (int w = 0; w < NWAVES; w++) t += s_cnt[w][tid];
-            if (t) atomicAdd(&a.cnt[2 * c0 + tid], t);
-        }
-    } else if (pend) {
-        const int c = chunk_of((int)(u0 + tid), a.cv0, a.nc);
-        atomicAdd(&a.cnt[2 * c], 1ull);
-        atomicAdd(&a.cnt[2 * c + 1], (unsigned long long)d);
-    }
-    if (a.p_next) {   // fused fold statistics, one atomic per field and block (spread slots)
-        const uint64_t f[4] = {nmv ? 1ull : 0ull, nmv, nmv ? d : 0ull, d * nmv};
-        __syncthreads();
-#pragma unroll
-        for (int i = 0; i < 4; i++) {
-            const uint64_t t = wave_sum_u64(f[i]);
-            if (lane == 0) s_red[tid / WAVE][1 + i] = t;
-        }
-        __syncthreads();
-        if (tid < 4) {
+        if (tid == 0) {
             unsigned long long sum = 0;
 #pragma unroll
-            for (int w = 0; w < NWAVES; w++) sum += s_red[w][1 + tid];
-            if (sum) atomicAdd(&slot[S_VERT + tid], sum);
+            for (int w = 0; w < NWAVES; w++) sum += s_red[w];
+            if (sum) atomicAdd(&slot[S_ROWS], sum);
         }
+    }
+    for (int j = 0; j < T / BLOCK; j++) {   // block-uniform
+        const long long ub = u0 + (long long)j * BLOCK;
+        if (ub >= a.n) break;
+        const int nt = (int)(a.n - ub < BLOCK ? a.n - ub : BLOCK);
+        float accv[NM];
+#pragma unroll
+        for (int k = 0; k < NM; k++) accv[k] = tid < nt ? s_acc[k][j * BLOCK + tid] : 0.f;
+        pull_finish<NM, S>(a, M, ub, nt, accv, slot, (long long)blockIdx.x * (T / BLOCK) + j);
     }
 }
 
@@ -2047,6 +2181,9 @@ struct Ctx {
     unsigned long long* d_counters = nullptr;  // C_NCOUNTERS
     unsigned long long* h_small = nullptr;     // pinned: cnt + hotpend + residual slots
     double l1_last[EXC_MAXM] = {};             // FP64 residual sums of the last snapshot
+    float linf_last[EXC_MAXM] = {};            // and the largest |r| per measure
+    excent_dev::SortedTileBuffers sorted;      // source-sorted dense pull (k_pull_sorted)
+    int sorted_T = 0;                          // its tile (destinations per block); 0: k_pull
     int32_t* d_hub_keys = nullptr;             // destinations privatised by k_pieces (HubTable)
     uint32_t* d_hub_filter = nullptr;
     int hub_probes = 0, hub_count = 0;
@@ -2367,7 +2504,25 @@ void launch_fold_pull(Ctx* c, hipStream_t st) {
         hipLaunchKernelGGL((k_pull_hub<NM, S>), dim3((unsigned)std::min(c->n_pull_hubs, c->cus * 8)), dim3(BLOCK), 0, st,
                            pa, c->d_pull_hubs, c->n_pull_hubs, c->DM);
     }
-    {
+    if (c->sorted_T) {
+        ScopedTimer t(c, st, T_PULL);
+        const SortedTiles tiles{c->sorted.start, c->sorted.row, c->sorted.dst, c->sorted.w};
+        const unsigned grid = (unsigned)(((long long)c->n + c->sorted_T - 1) / c->sorted_T);
+        if constexpr (NM == 1) {
+            if (c->d_twts) {
+                if (c->sorted_T == 4096) hipLaunchKernelGGL((k_pull_sorted<NM, S, 4096, true>), dim3(grid), dim3(BLOCK), 0, st, pa, tiles, c->DM);
+                else if (c->sorted_T == 512) hipLaunchKernelGGL((k_pull_sorted<NM, S, 512, true>), dim3(grid), dim3(BLOCK), 0, st, pa, tiles, c->DM);
+                else if (c->sorted_T == 1024) hipLaunchKernelGGL((k_pull_sorted<NM, S, 1024, true>), dim3(grid), dim3(BLOCK), 0, st, pa, tiles, c->DM);
+                else hipLaunchKernelGGL((k_pull_sorted<NM, S, 2048, true>), dim3(grid), dim3(BLOCK), 0, st, pa, tiles, c->DM);
+            }
+        }
+        if (!(NM == 1 && c->d_twts)) {
+            if (c->sorted_T == 4096) hipLaunchKernelGGL((k_pull_sorted<NM, S, 4096>), dim3(grid), dim3(BLOCK), 0, st, pa, tiles, c->DM);
+            else if (c->sorted_T == 512) hipLaunchKernelGGL((k_pull_sorted<NM, S, 512>), dim3(grid), dim3(BLOCK), 0, st, pa, tiles, c->DM);
+            else if (c->sorted_T == 1024) hipLaunchKernelGGL((k_pull_sorted<NM, S, 1024>), dim3(grid), dim3(BLOCK), 0, st, pa, tiles, c->DM);
+            else hipLaunchKernelGGL((k_pull_sorted<NM, S, 2048>), dim3(grid), dim3(BLOCK), 0, st, pa, tiles, c->DM);
+        }
+    } else {
         ScopedTimer t(c, st, T_PULL);
         bool weighted = false;
         if constexpr (NM == 1) {
@@ -2755,6 +2910,7 @@ int exc_destroy(void* p) {
                          c->d_bitmap2, c->d_hubbits, c->d_inv, c->d_tile_next};
     for (void* q : pull_ptrs) if (q) (void)hipFree(q);
     if (c->d_beps) (void)hipFree(c->d_beps);
+    excent_dev::free_sorted_tiles(&c->sorted);
     if (c->d_hub_keys) (void)hipFree(c->d_hub_keys);
     if (c->d_hub_filter) (void)hipFree(c->d_hub_filter);
     if (c->d_colpend) (void)hipFree(c->d_colpend);
@@ -2878,8 +3034,17 @@ int snapshot_counts(Ctx* c, uint64_t* h_cnt, uint64_t* h_hotpend) {
     const double* l1 = reinterpret_cast<const double*>(c->h_small + l1_offset(c));
     for (int k = 0; k < EXC_MAXM; k++) {
         double t = 0.0;
-        for (int q = 0; q < L1_SLOTS; q++) t += l1[q * L1_STRIDE + k];
+        uint32_t m = 0;
+        for (int q = 0; q < L1_SLOTS; q++) {
+            t += l1[q * L1_STRIDE + k];
+            uint32_t b;
+            memcpy(&b, reinterpret_cast<const char*>(l1 + q * L1_STRIDE + LINF_AT) + 4 * k, 4);
+            m = std::max(m, b);
+        }
         c->l1_last[k] = k < c->nm ? t : 0.0;
+        float f;
+        memcpy(&f, &m, 4);
+        c->linf_last[k] = k < c->nm ? f : 0.f;
     }
     c->l1_valid = c->K == 0;
     c->last_demand = 0;
@@ -3478,6 +3643,24 @@ int exc_enable_pull(void* p, double threshold) {
     e = hipDeviceSynchronize();
     if (e != hipSuccess) return fail_free(e);
     (void)hipFree(cnt);
+    // one-measure contexts: the non-hub in-edges again, per tile of destinations sorted by
+    // source row (k_pull_sorted).  With one measure it cuts the launch by a sixth at R-MAT
+    // scale 24 (its L1-to-L2 requests drop by a third); with three, its three LDS atomics per
+    // entry cost more than that saves, so k_pull stays.  EXC_PULL_TILE = 512 / 1024 / 2048 /
+    // 4096 picks the tile (any measure count), 0 keeps k_pull.
+    if (!c->K) {
+        int T = c->nm == 1 ? 2048 : 0;
+        if (const char* ev = getenv("EXC_PULL_TILE")) T = atoi(ev);
+        if ((T == 512 || T == 1024 || T == 2048 || T == 4096) && off[n] < (1ull << 31) && c->nsrc > 0) {
+            int bits = 1;
+            while (bits < 31 && (1ll << bits) < (long long)c->nsrc) bits++;
+            const hipError_t se = excent_dev::build_sorted_tiles(c->n, T, c->d_tidx, c->d_tadj, c->d_twts, off[n], bits, 0,
+                                                                 &c->sorted);
+            if (se == hipSuccess) c->sorted_T = T;
+            else if (se == hipErrorOutOfMemory) (void)hipGetLastError();   // keep k_pull
+            else return fail(EXC_EHIP, "exc_enable_pull: sorted tiles: %s", hipGetErrorString(se));
+        }
+    }
     if (c->d_hubbits) {   // batched: which destinations k_bpull_hub completes
         std::vector<uint32_t> hb(((n + 63) / 64) * 2, 0);
         for (const PullPiece& pc : hubs) hb[pc.u >> 5] |= 1u << (pc.u & 31);
@@ -3528,6 +3711,15 @@ int exc_residual_l1(void* p, double* out) {
     if (c->K) return fail(EXC_ESTATE, "exc_residual_l1: not kept for batched contexts");
     if (!c->l1_valid) return fail(EXC_ESTATE, "exc_residual_l1: no snapshot yet");
     for (int k = 0; k < c->nm; k++) out[k] = c->l1_last[k];
+    return EXC_OK;
+}
+
+int exc_residual_max(void* p, float* out) {
+    Ctx* c = as_ctx(p);
+    if (!c || !out) return fail(EXC_EINVAL, "exc_residual_max: bad arguments");
+    if (c->K) return fail(EXC_ESTATE, "exc_residual_max: not kept for batched contexts");
+    if (!c->l1_valid) return fail(EXC_ESTATE, "exc_residual_max: no snapshot yet");
+    for (int k = 0; k < c->nm; k++) out[k] = c->linf_last[k];
     return EXC_OK;
 }
 

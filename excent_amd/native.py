@@ -144,6 +144,7 @@ HIP_SIGNATURES = {
     "exc_fp64_lambda": (c_int, [c_void_p, c_void_p, c_int, c_double, P(c_double), P(c_double), P(c_int)]),
     "exc_dense_required": (c_int, [c_void_p]),
     "exc_residual_l1": (c_int, [c_void_p, P(c_double)]),
+    "exc_residual_max": (c_int, [c_void_p, P(c_float)]),
     "exc_fold_ahead": (c_int, [c_void_p, c_int]),
     "exc_set_hub_destinations": (c_int, [c_void_p, c_void_p, c_int]),
     "exc_snapshot_noalloc": (c_int, [c_int, c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_int,
@@ -376,6 +377,11 @@ class Context:
     def set_hub_destinations(self, ids):
         ids = np.ascontiguousarray(ids, dtype=np.int32)
         check(F["exc_set_hub_destinations"](self.ctx, ptr(ids), len(ids)), "set_hub_destinations")
+
+    def residual_max(self):
+        out = (c_float * 3)()
+        check(F["exc_residual_max"](self.ctx, out), "residual_max")
+        return list(out)[: self.nm]
 
     def fold_ahead(self, allow):
         check(F["exc_fold_ahead"](self.ctx, 1 if allow else 0), "fold_ahead")

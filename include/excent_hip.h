@@ -138,6 +138,9 @@ int exc_snapshot(void* ctx, int hot_skip, uint64_t* h_cnt, uint64_t* h_hotpend);
  * is the quantity a global L1 stop and the PageRank bound ||x - x*||_1 <= ||r||_1 / (1 - d)
  * are stated in.  EXC_ESTATE before the first snapshot of a state, and on batched contexts. */
 int exc_residual_l1(void* ctx, double* out);
+/* The largest |r_i[v]| per measure of the same snapshot state (the per-vertex stop holds for
+ * measure i iff out[i] <= eps_i).  No reference counterpart; EXC_ESTATE as above. */
+int exc_residual_max(void* ctx, float* out);
 /* Allocation-free demand snapshot.  No counterpart in the reference (whose snapshot is
  * k_snapshot, src/excent.cu:69-98, on buffers main() allocates): it is exc_snapshot -- same
  * kernel, same outputs, bit for bit -- for callers that cannot allocate (code under HIP graph

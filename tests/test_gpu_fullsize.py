@@ -12,12 +12,14 @@ pytestmark = pytest.mark.gpu
 
 
 def test_scale24_all_terminates_inside_its_bounds(tmpdir_factory):
-    """After a converged run: every |r| <= eps (src/excent.cu:58-64 termination); the FP64
-    residual ||c + s W x - x||_1 of the float32 x (computed exactly from x) is at most
-    n * eps, the termination guarantee, so the guarantee holds for x itself and not only for
-    the float32 r; for PageRank it also equals sum |r| to within float32 rounding of x (the
-    invariant x + r = c + s W x), and the L1 error against the FP64 fixed point stays under
-    ||residual||_1 / (1 - d).  Katz and eigenvector (unit vectors) against their FP64
+    """The headline workload's stop rule (--stop katz,ev): PageRank until every |r| <= 0.01
+    (src/excent.cu:58-64), Katz and eigenvector until their sum |r| <= 1e-6 |V| (the global
+    reading of eps = 1e-6 |V|) or nothing of theirs is pending.  After the run: PageRank's
+    every |r| <= eps; for every measure the FP64 residual ||c + s W x - x||_1 of the float32 x
+    (computed exactly from x) is at most n * eps -- the termination guarantee holds for x,
+    not only for the float32 r -- and equals sum |r| to within float32 rounding of x (the
+    invariant x + r = c + s W x); PageRank's L1 error against the FP64 fixed point stays
+    under ||residual||_1 / (1 - d).  Katz and eigenvector (unit vectors) against their FP64
     solutions, with margins over the bench's measured errors (1.3e-7 and 2.1e-5)."""
     g = rmat_graph(tmpdir_factory, 24)
     envp = g + ".env"
@@ -28,13 +30,14 @@ def test_scale24_all_terminates_inside_its_bounds(tmpdir_factory):
     env = read_env(envp)
     eps = [0.01, 1e-6, 1e-6]
     args = [g, "--measure", "all", "--cache", "chunks", "--preload", "--stream-ratio", "1", "--alpha", env["ALPHA"],
-            "--beta", 1, "--eps-katz", eps[1], "--lambda", env["LAMBDA"], "--eps-ev", eps[2], "--ev-init", "indeg"]
+            "--beta", 1, "--eps-katz", eps[1], "--lambda", env["LAMBDA"], "--eps-ev", eps[2], "--ev-init", "indeg",
+            "--stop", "katz,ev", "--max-passes", 3000]
     print("case scale 24 all: run", flush=True)
     with nat.Session(args) as sess:
         r = sess.run(reinit=False)
         X = [sess.values(i, 0) for i in range(3)]
         R = [sess.values(i, 1) for i in range(3)]
-    assert r["passes"] > 1 and r["kernel"]["pull_passes"] > 0
+    assert r["passes"] > 1 and r["kernel"]["pull_passes"] > 0 and r["stop_reason"] != 2, r["passes"]
     idx, adj = nat.load_egr(g)
     n = len(idx) - 1
     f32 = np.float32
@@ -44,12 +47,16 @@ def test_scale24_all_terminates_inside_its_bounds(tmpdir_factory):
     with nat.Fp64Solver(idx, adj) as S:
         res = []
         for i, (c, s, dd) in enumerate(consts):
-            assert np.all(np.abs(R[i]) <= f32(eps[i])), i
+            rsum = float(np.abs(R[i].astype(np.float64)).sum())
+            xsum = float(np.abs(X[i].astype(np.float64)).sum())
+            if i == 0:
+                assert np.all(np.abs(R[i]) <= f32(eps[i]))
+            else:
+                assert rsum <= n * float(f32(eps[i])) or np.all(np.abs(R[i]) <= f32(eps[i])), (i, rsum)
+            assert abs(r["residual_l1"][i] - rsum) <= 1e-12 * rsum + 1e-300, (i, r["residual_l1"][i], rsum)
             res.append(S.residual(X[i], c, s, dd))
-            assert res[i] <= n * eps[i], (i, res[i])
-        rsum = float(np.abs(R[0].astype(np.float64)).sum())
-        xsum = float(np.abs(X[0].astype(np.float64)).sum())
-        assert abs(res[0] - rsum) <= 256 * 2.0 ** -24 * xsum, (res[0], rsum, xsum)   # 0.6 % of res here
+            assert abs(res[i] - rsum) <= 256 * 2.0 ** -24 * xsum, (i, res[i], rsum, xsum)   # 0.6 % of res for pr
+            assert res[i] <= n * eps[i] + 256 * 2.0 ** -24 * xsum, (i, res[i])
         ref, _, _, _ = S.solve(nat.EXC_PR, tol=1e-12, max_iter=400)
         assert np.abs(X[0] - ref).sum() <= res[0] / (1 - 0.85)
         top = lambda v: set(np.argpartition(-v, 99)[:100].tolist())

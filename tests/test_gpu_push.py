@@ -159,10 +159,12 @@ def test_star_hub_pieces(kernel, group):
     assert pushed >= 2 * L
 
 
+@pytest.mark.parametrize("tile", ["0", "1024", "2048", "4096"])
 @pytest.mark.parametrize("threshold", [0.0, 0.2])
 @pytest.mark.parametrize("gname", ["sample", "rmat12", "star"])
-def test_fold_pull_dense_passes(threshold, gname, rmat12):
+def test_fold_pull_dense_passes(threshold, gname, tile, rmat12, monkeypatch):
     """Dense passes as fold + pull over the device transpose: same bounds and invariant."""
+    monkeypatch.setenv("EXC_PULL_TILE", tile)   # 0: k_pull; else k_pull_sorted with this tile
     if gname == "star":
         L = 9000   # in-degree of the hub above the pull hub threshold
         deg = np.concatenate([[L], np.ones(L, np.int64)])
@@ -184,7 +186,9 @@ def test_fold_pull_dense_passes(threshold, gname, rmat12):
     assert pushed >= len(adj)
 
 
-def test_fold_pull_all_measures():
+@pytest.mark.parametrize("tile", ["0", "2048"])
+def test_fold_pull_all_measures(tile, monkeypatch):
+    monkeypatch.setenv("EXC_PULL_TILE", tile)   # 0: k_pull; else k_pull_sorted with this tile
     from tests.conftest import SAMPLE_ENV, read_env
     env = read_env(SAMPLE_ENV)
     idx, adj = orc.egr_load(SAMPLE_EGR)
@@ -199,13 +203,15 @@ def test_fold_pull_all_measures():
     assert orc.compare(orc.fp64_reference(idx, adj, orc.ORC_KATZ, alpha=alpha)[0], xs[1])["l1_rel"] < 0.01
 
 
+@pytest.mark.parametrize("tile", ["0", "1024", "4096"])
 @pytest.mark.parametrize("measures", ["pr", "all"])
-def test_fused_snapshot_bit_exact(rmat12, measures, monkeypatch):
+def test_fused_snapshot_bit_exact(rmat12, measures, tile, monkeypatch):
     """The snapshot a dense pass computes for the next pass inside k_pull (bitmap words,
     per-chunk pending counts and degree sums) equals the oracle's snapshot of the state
     read back, bit for bit, on chunk boundaries that cut through the pull's tiles; and
     equals a second, separately launched snapshot of the same state.  (The next pass's
     fold is kept out of k_pull here, so the state read back is the one the snapshot saw.)"""
+    monkeypatch.setenv("EXC_PULL_TILE", tile)   # 0: k_pull; else k_pull_sorted with this tile
     monkeypatch.setenv("EXC_FUSE_FOLD", "0")
     idx, adj = orc.egr_load(rmat12)
     n, m = len(idx) - 1, len(adj)
@@ -283,10 +289,12 @@ def test_fused_fold_state_and_results(rmat12):
     assert st["pushed_edges"] >= m and st["pull_passes"] > 0
 
 
-@pytest.mark.parametrize("n", [1, 255, 257, 1000])
-def test_dense_passes_edgeless_and_ragged(n):
+@pytest.mark.parametrize("tile", ["0", "1024"])
+@pytest.mark.parametrize("n", [1, 255, 257, 1000, 1025])
+def test_dense_passes_edgeless_and_ragged(n, tile, monkeypatch):
     """Dense passes on an edgeless graph (no sources, empty transpose, no p rows): x = b in
     one pass, with tile counts that are not multiples of the 256-vertex tile."""
+    monkeypatch.setenv("EXC_PULL_TILE", tile)   # 0: k_pull; else k_pull_sorted with this tile
     idx = np.zeros(n + 1, np.uint64)
     adj = np.zeros(0, np.int32)
     x0, r0 = orc.init(idx, orc.ORC_PR)
@@ -294,12 +302,14 @@ def test_dense_passes_edgeless_and_ragged(n):
     assert np.all(xs[0] == np.float32(1) - np.float32(0.85)) and np.all(rs[0] == 0) and pushed == 0 and passes == 1
 
 
+@pytest.mark.parametrize("tile", ["0", "2048"])
 @pytest.mark.parametrize("gname", ["rmat12", "star"])
-def test_weighted_dense_passes(gname, rmat12):
+def test_weighted_dense_passes(gname, tile, rmat12, monkeypatch):
     """Dense passes on a weighted graph (the pull scales each in-edge's row by its
     coefficient, staged with the ids; hub pieces too): the result sits within the exact
     FP64 bound of the weighted solve, per-vertex |r| <= eps at the end, and weights set
     after the pull is enabled are refused."""
+    monkeypatch.setenv("EXC_PULL_TILE", tile)   # 0: k_pull; else k_pull_sorted with this tile
     if gname == "star":
         L = 9000
         deg = np.concatenate([[L], np.ones(L, np.int64)])

@@ -62,9 +62,12 @@ def test_residual_sum_matches_state(rmat12, pull, monkeypatch):
 
         def check(p):
             got = ctx.residual_l1()
-            want = [np.abs(ctx.read_state(i, 1).astype(np.float64)).sum() for i in range(3)]
+            rs = [ctx.read_state(i, 1) for i in range(3)]
+            want = [np.abs(r.astype(np.float64)).sum() for r in rs]
             for g, w in zip(got, want):
                 assert abs(g - w) <= 1e-12 * max(w, 1e-300) + 1e-300, (p, got, want)
+            # the largest |r| per measure, exactly
+            assert ctx.residual_max() == [float(np.abs(r).max()) for r in rs], p
             x = [ctx.read_state(i, 0) for i in range(3)]
             # the residual of x, recomputed from the graph in FP64 by the oracle
             true = [orc.residual_l1(idx, adj, orc.ORC_PR, float(scale[0]), 0.15, x[0]),
@@ -165,6 +168,29 @@ def test_global_stop(tmp_path, rmat14, rmat14_env, cache):
     passes_v = int(re.search(r"passes (\d+)  measure all", open(log2).read()).group(1))
     assert passes_v >= passes
     assert "stop: nothing pending" in open(log2).read()
+
+
+def test_stop_rule_per_measure(tmp_path, rmat14, rmat14_env):
+    """--stop katz,ev: PageRank keeps the reference's per-vertex rule while Katz and the
+    eigenvector may end on their global sum; at the end every PageRank |r| <= eps, each of
+    the others either has nothing pending or its sum within n * eps, and the report's
+    largest |r| per measure is the dumped state's."""
+    env = rmat14_env
+    idx, adj = orc.egr_load(rmat14)
+    n = len(idx) - 1
+    log = str(tmp_path / "s.log")
+    args = [rmat14, "--measure", "all", "--alpha", env["ALPHA"], "--beta", 1, "--lambda", env["LAMBDA"],
+            "--eps-pr", 0.001, "--eps-katz", 1e-6, "--eps-ev", 1e-6, "--ev-init", "indeg", "--cache", "chunks",
+            "--preload", "--stop", "katz,ev", "--max-passes", 3000, "--log", log]
+    with nat.Session(args) as S:
+        r = S.run(reinit=False)
+        R = [S.values(i, 1) for i in range(3)]
+    assert r["stop_reason"] in (0, 1)
+    assert np.abs(R[0]).max() <= np.float32(0.001)
+    for i in (1, 2):
+        assert np.abs(R[i]).max() <= np.float32(1e-6) or np.abs(R[i].astype(np.float64)).sum() <= n * float(np.float32(1e-6))
+        assert r["residual_l1"][i] == pytest.approx(np.abs(R[i].astype(np.float64)).sum(), rel=1e-12)
+    assert nat.excent([rmat14, "--measure", "pr", "--stop", "katz", "--log", str(tmp_path / "bad.log")]) == 1
 
 
 def test_pass_limit_keeps_invariant(rmat12):
