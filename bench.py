@@ -535,13 +535,13 @@ def stop_rule(args, measure):
     return "katz,ev" if measure == "all" else "vertex"
 
 
-def bench_plain(args, nat, np, work, graph, scale, measure, steps, warmup, env, tag):
+def bench_plain(args, nat, np, work, graph, scale, measure, steps, warmup, env, tag, timing="--timing"):
     """Converged runs of bin/excent's engine with the whole graph in HBM: steps timed
     back to back after `warmup` untimed ones, each from the initial state (a device copy);
     the dominant kernel's roofline; FP64 errors of the last run; setup times."""
     excent_log = os.path.join(args.logdir or work, f"{tag}.log")
     sargs = [graph, "--measure", measure, "--cache", "chunks", "--preload", "--stream-ratio", "1",
-             "--timing", "--push-kernel", args.kernel, "--pull", args.pull, "--hub-lds", args.hub_lds, "--log", excent_log,
+             timing, "--push-kernel", args.kernel, "--pull", args.pull, "--hub-lds", args.hub_lds, "--log", excent_log,
              "--dump", os.path.join(work, "x"), "--stop", stop_rule(args, measure),
              # a run whose stop rule is never met ends here and is reported as not converged
              "--max-passes", args.max_passes or 2000] + measure_args(measure, env)
@@ -634,7 +634,9 @@ def sub_s16(args, nat, np, work):
     convergence beside it."""
     from oracle import oracle as orc
     graph = graph_file(args, nat, work, 16)
-    rec, _ = bench_plain(args, nat, np, work, graph, 16, "pr", 20, 3, {}, "rmat16_pr")
+    # events around the dense pull only: at this size a pass is tens of microseconds and events
+    # around every kernel would cost a fifth of it
+    rec, _ = bench_plain(args, nat, np, work, graph, 16, "pr", 20, 3, {}, "rmat16_pr", timing="--timing-pull")
     n, m = graph_size(nat, graph)
     rec.update({"metric": "converged GTEPS + time-to-eps", "higher_is_better": True, "dtype": "f32",
                 "data": "synthetic", "config": {"workload": "R-MAT scale 16 ef 16 --measure pr, eps 0.01, whole "

@@ -67,7 +67,8 @@ struct Options {
     std::string dump, measure;
     double alpha = -1, beta = 1.0, lambda = -1, eps_pr = 0.01, eps_katz = -1, eps_ev = -1;
     // extensions
-    bool preload = false, timing = false;
+    bool preload = false;
+    int timing = 0;   // 1: HIP events around every kernel; 2: around the dense pull only
     double cache_mb = -1;
     std::string push_kernel = "lb", ev_init = "outdeg", stats_json, log;
     int device = 0;
@@ -133,7 +134,8 @@ Options parse(int argc, const char* const* argv) {
         else if (a == "--eps-ev") o.eps_ev = atof(next());
         // extensions (no reference counterpart)
         else if (a == "--preload") o.preload = true;
-        else if (a == "--timing") o.timing = true;
+        else if (a == "--timing") o.timing = 1;
+        else if (a == "--timing-pull") o.timing = 2;   // events around the dense pull only
         else if (a == "--cache-mb") o.cache_mb = atof(next());
         else if (a == "--push-kernel") o.push_kernel = next();
         else if (a == "--ev-init") o.ev_init = next();

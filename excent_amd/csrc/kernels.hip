@@ -2166,7 +2166,9 @@ enum { T_SNAP = 0, T_PUSH, T_PUSH_LAST = T_PUSH + 3, T_PIECE, T_OTHER, T_FOLD, T
 struct Ctx {
     int dev = 0, n = 0, nc = 0, nm = 1, S = 1, push_kernel = EXC_KERNEL_LB;
     uint64_t m = 0;
-    bool zc = false, timing = false;
+    bool zc = false, timing = false, timing_pull_only = false;
+    bool counts_enqueued = false;   // a dense pull queued its snapshot counts' copy to h_small
+    bool counts_copied = false;     // ... and a pass sync has waited for it
     exc_measures M{};
     DevMeasures DM{};
     uint64_t* d_idx = nullptr;
@@ -2320,7 +2322,7 @@ struct ScopedTimer {
     Ctx* c; hipStream_t s; int cls; hipEvent_t a = nullptr;
     ScopedTimer(Ctx* c_, hipStream_t s_, int cls_) : c(c_), s(s_), cls(cls_) {
         c->t_launch[cls]++;
-        if (c->timing) { a = get_event(c); if (a) hipEventRecord(a, s); }
+        if (c->timing && (!c->timing_pull_only || cls == T_PULL)) { a = get_event(c); if (a) hipEventRecord(a, s); }
     }
     ~ScopedTimer() {
         if (!a) return;
@@ -2538,7 +2540,11 @@ void launch_fold_pull(Ctx* c, hipStream_t st) {
         else hipLaunchKernelGGL((k_pull<NM, S, 8>), dim3(ntiles), dim3(BLOCK), 0, st, pa, c->DM);
     }
     c->snap_ready = fuse_snap;
-    if (fuse_snap) c->snap_bytes += 4.0 * (((double)c->n + 31) / 32);
+    if (fuse_snap) {   // the next snapshot's counts go to the host behind the pull
+        c->snap_bytes += 4.0 * (((double)c->n + 31) / 32);
+        c->counts_enqueued = hipMemcpyAsync(c->h_small, c->d_cnt, cnt_words(c) * sizeof(unsigned long long),
+                                            hipMemcpyDeviceToHost, st) == hipSuccess;
+    }
     c->folded = fuse_fold;
     if (fuse_fold) std::swap(c->d_p, c->d_p2);   // the rows just written serve the next pass
 }
@@ -2624,6 +2630,9 @@ void launch_bfold_pull(Ctx* c, hipStream_t st) {
     c->pull_passes_live++;
     c->folded = fuse;
     c->snap_ready = fuse;
+    if (fuse)   // the next snapshot's counts go to the host behind the pull
+        c->counts_enqueued = hipMemcpyAsync(c->h_small, c->d_cnt, cnt_words(c) * sizeof(unsigned long long),
+                                            hipMemcpyDeviceToHost, st) == hipSuccess;
     if (fuse) {   // the rows and bitmap just written serve the next pass
         std::swap(c->d_p, c->d_p2);
         std::swap(c->d_bitmap, c->d_bitmap2);
@@ -2816,6 +2825,7 @@ int exc_create(const exc_config* cfg, const exc_measures* M, void** out) {
     c->M = *M;
     c->zc = cfg->zero_copy != 0;
     c->timing = cfg->timing != 0;
+    c->timing_pull_only = cfg->timing == 2;
     c->push_kernel = cfg->push_kernel;
     c->max_chunk_edges = cfg->max_chunk_edges;
     c->max_chunk_verts = cfg->max_chunk_verts;
@@ -3025,10 +3035,19 @@ int exc_state_checkpoint(void* p, int restore) {
 }
 
 // Per-chunk counts (and hot-pending) of the snapshot on the device to the host.
+// Parses the counts buffer's host copy (h_small): per-chunk counts, hot-pending, residual sums
+// and maxima.
+void parse_counts(Ctx* c, uint64_t* h_cnt, uint64_t* h_hotpend);
+
 int snapshot_counts(Ctx* c, uint64_t* h_cnt, uint64_t* h_hotpend) {
     hipStream_t st = c->st[0];
     HIPCHK(hipMemcpyAsync(c->h_small, c->d_cnt, cnt_words(c) * sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
+    parse_counts(c, h_cnt, h_hotpend);
+    return EXC_OK;
+}
+
+void parse_counts(Ctx* c, uint64_t* h_cnt, uint64_t* h_hotpend) {
     memcpy(h_cnt, c->h_small, 2 * (size_t)c->nc * sizeof(uint64_t));
     if (h_hotpend) *h_hotpend = c->h_small[2 * c->nc];
     const double* l1 = reinterpret_cast<const double*>(c->h_small + l1_offset(c));
@@ -3049,7 +3068,6 @@ int snapshot_counts(Ctx* c, uint64_t* h_cnt, uint64_t* h_hotpend) {
     c->l1_valid = c->K == 0;
     c->last_demand = 0;
     for (int i = 0; i < c->nc; i++) c->last_demand += c->h_small[2 * i + 1];
-    return EXC_OK;
 }
 
 int exc_snapshot(void* p, int hot_skip, uint64_t* h_cnt, uint64_t* h_hotpend) {
@@ -3059,8 +3077,14 @@ int exc_snapshot(void* p, int hot_skip, uint64_t* h_cnt, uint64_t* h_hotpend) {
     hipStream_t st = c->st[0];
     if ((c->snap_ready || c->folded) && !hot_skip) {   // computed by the dense pass that ended the last pass
         c->snap_ready = false;
+        if (c->counts_copied) {   // its copy was queued behind the pull and the pass sync waited for it
+            c->counts_copied = false;
+            parse_counts(c, h_cnt, h_hotpend);
+            return EXC_OK;
+        }
         return snapshot_counts(c, h_cnt, h_hotpend);
     }
+    c->counts_copied = false;
     c->snap_ready = false;
     clear_counts(c, st);
     SnapArgs a{};
@@ -3413,6 +3437,10 @@ int exc_pass_sync(void* p, double* waited) {
     auto a = std::chrono::steady_clock::now();
     for (auto s : c->st) HIPCHK(hipStreamSynchronize(s));
     if (waited) *waited = std::chrono::duration<double>(std::chrono::steady_clock::now() - a).count();
+    if (c->counts_enqueued) {   // the copy queued behind the last dense pull is complete
+        c->counts_copied = c->snap_ready;
+        c->counts_enqueued = false;
+    }
     for (auto& sl : c->slot) sl.busy = false;
     HIPCHK(hipGetLastError());
     return collect_timers(c);
@@ -3649,7 +3677,11 @@ int exc_enable_pull(void* p, double threshold) {
     // entry cost more than that saves, so k_pull stays.  EXC_PULL_TILE = 512 / 1024 / 2048 /
     // 4096 picks the tile (any measure count), 0 keeps k_pull.
     if (!c->K) {
-        int T = c->nm == 1 ? 2048 : 0;
+        // the largest tile that still gives every CU four blocks; small graphs keep k_pull
+        int T = 0;
+        if (c->nm == 1)
+            for (int t : {2048, 1024, 512})
+                if ((long long)c->n / t >= 4ll * c->cus) { T = t; break; }
         if (const char* ev = getenv("EXC_PULL_TILE")) T = atoi(ev);
         if ((T == 512 || T == 1024 || T == 2048 || T == 4096) && off[n] < (1ull << 31) && c->nsrc > 0) {
             int bits = 1;

@@ -64,7 +64,8 @@ typedef struct exc_config {
     int trace;                  /* allocate the --trace-pending counters */
     int zero_copy;              /* staging buffers mapped into the device (read in place) */
     int push_kernel;            /* EXC_KERNEL_LB or EXC_KERNEL_GROUP */
-    int timing;                 /* record HIP events around kernels (exc_stats) */
+    int timing;                 /* record HIP events around kernels (exc_stats): 1 every kernel, 2 only the
+                                   dense pull (k_pull / k_pull_sorted, the cheapest way to time it) */
     int batch_k;                /* 0: plain measures; 1..64: batched personalised PageRank with
                                    K columns ([n][K] state, damping M.scale[0], eps M.eps[0]) */
 } exc_config;
