@@ -281,6 +281,43 @@ def bench_batched(args, nat, np, work, graph, sample=None):
     }
 
 
+def storage_probe(path, nbytes=4 << 30):
+    """Sequential read rate of the graph file from storage: the page cache is dropped first,
+    then `nbytes` from the middle of the file are read in 16 MiB requests, once through the
+    page cache and once with O_DIRECT (aligned buffer) when the file system supports it."""
+    import mmap
+    size = os.path.getsize(path)
+    nbytes = min(nbytes, size // 2) // (16 << 20) * (16 << 20)
+    start = (size // 4) // 4096 * 4096
+    out = {"bytes": nbytes}
+    buf = mmap.mmap(-1, 16 << 20)   # page-aligned
+    for mode in ("buffered", "direct"):
+        evict(path)
+        flags = os.O_RDONLY | (getattr(os, "O_DIRECT", 0) if mode == "direct" else 0)
+        try:
+            fd = os.open(path, flags)
+        except OSError as e:
+            out[mode] = f"unavailable: {e}"
+            continue
+        try:
+            t0 = time.perf_counter()
+            done = 0
+            while done < nbytes:
+                got = os.preadv(fd, [buf], start + done)
+                if got <= 0:
+                    break
+                done += got
+            el = time.perf_counter() - t0
+            out[f"{mode}_gb_per_s"] = round(done / el / 1e9, 3)
+        except OSError as e:
+            out[mode] = f"unavailable: {e}"
+        finally:
+            os.close(fd)
+    evict(path)
+    log(f"storage probe: {out}")
+    return out
+
+
 def bench_ooc(args, nat, np, work, graph):
     """Out-of-core run: the graph file streamed / gathered over PCIe each pass, with a capped
     hot cache and with no cache (and, with --cold, the hot-cache run again after the file
@@ -348,6 +385,8 @@ def bench_ooc(args, nat, np, work, graph):
         best = min(ctx.slot_copy_probe(0, chunk * 4) for _ in range(5))
     link = chunk * 4 / best / 1e9
     out["link_h2d_gb_per_s"] = round(link, 2)
+    out["storage"] = storage_probe(graph)
+    out["host"] = host_info()
     for r in out["runs"].values():
         r["link_fraction"] = round(r["shipped_gb_per_s"] / link, 3)
     log(f"ooc: pinned H2D {link:.1f} GB/s")

@@ -153,6 +153,7 @@ extern "C" int excent_rmat_main(int argc, const char* const* argv) {
         }
         for (uint64_t v = 0; v < n; v++) off[v + 1] = off[v] + deg[v];
     }
+    fprintf(stderr, "rmat: out-degrees counted\n");
     uint64_t m0 = off[n];
     std::vector<int32_t> adj(m0 ? m0 : 1);
     {   // pass 2: scatter
@@ -167,6 +168,7 @@ extern "C" int excent_rmat_main(int argc, const char* const* argv) {
             if (symmetric && s != d) adj[__atomic_fetch_add(&pos[d], 1ull, __ATOMIC_RELAXED)] = (int32_t)s;
         }
     }
+    fprintf(stderr, "rmat: edges placed\n");
     std::vector<uint32_t> ndeg(n);
 #pragma omp parallel for schedule(dynamic, 1024)
     for (long long v = 0; v < (long long)n; v++) {
@@ -188,6 +190,7 @@ extern "C" int excent_rmat_main(int argc, const char* const* argv) {
         m = w;
     }
     adj.resize(m ? m : 1);
+    fprintf(stderr, "rmat: lists sorted and deduplicated, writing %s\n", out.c_str());
     if (!write_egr(out, n, m, off, adj, wide)) { fprintf(stderr, "cannot write %s\n", out.c_str()); return 1; }
     if (weights) {
         std::vector<float> w(m);
