@@ -258,6 +258,12 @@ def bench_batched(args, nat, np, work, graph, sample=None):
             fp64[f"column {j}"] = e
             log(f"fp64 column {j}: {e}")
     vb = rb["kernel"]["pushed_edge_measures"]
+    kb = rb["kernel"]
+    bpull = kb["pull_bytes"] / (kb["pull_ms"] / 1e3) / 1e9 if kb["pull_ms"] > 0 else 0.0
+    roof = {"bound": "hbm", "achieved": round(bpull, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(bpull / HBM_PEAK_GBS, 4), "traffic": None, "kernel": "k_bpull (batched dense pull)",
+            "launches": kb["pull_passes"], "avg_launch_ms": round(kb["pull_ms"] / max(1, kb["pull_passes"]), 4),
+            "algorithmic_bytes_per_launch": round(kb["pull_bytes"] / max(1, kb["pull_passes"]), 1)}
     return {
         "metric": "batched vs plain personalised PageRank at FP64 error", "unit": "x (wall-time ratio)",
         "value": round(ts / tb, 3), "higher_is_better": True, "n_gpus": 1, "dtype": "f32", "data": "synthetic",
@@ -275,6 +281,7 @@ def bench_batched(args, nat, np, work, graph, sample=None):
         "plain": {"time_s": round(ts, 4), "time_s_per_column": round(ts / K, 4),
                   "passes_mean": round(sum(seq_passes) / K, 1), "vector_pushes": seq_pushes,
                   "vector_gteps": round(seq_pushes / ts / 1e9, 3)},
+        "roofline": roof,
         "wall_ratio": round(ts / tb, 3),
         "vector_push_rate_ratio": round((vb / tb) / (seq_pushes / ts), 3),
         "fp64_err": fp64,
@@ -688,27 +695,14 @@ def sub_s16(args, nat, np, work):
 
 
 def sub_batched(args, nat, np, work):
-    """Batched K=64 personalised PageRank at scale 26 against 64 plain runs, plus a CPU sample
-    of the plain iteration on the first column."""
-    from oracle import oracle as orc
+    """Batched K=64 personalised PageRank at scale 26 against 64 plain runs."""
     a = argparse.Namespace(**vars(args))
     a.scale, a.k, a.warmup, a.batched_only = 26, 64, 1, False
     graph = graph_file(a, nat, work, 26)
     rec = bench_batched(a, nat, np, work, graph, sample=[0, a.k - 1])
-    if not args.no_cpu:
-        idx, adj = orc.egr_load(graph)
-        n = len(idx) - 1
-        gen = rec["config"]["ppr_gen"]
-        with nat.Session([graph, "--measure", "pr", "--ppr-column", 0, "--ppr-gen", gen, "--eps-ppr", 1e-7,
-                          "--cache", "none", "--stream-ratio", "1", "--max-passes", 0,
-                          "--log", os.path.join(work, "cpu_column0.log")]) as S:
-            S.set_column(0)
-            r0 = S.values(0, which=1)
-        xs, rs = [np.zeros(n, np.float32)], [r0.astype(np.float32)]
-        cpu = cpu_baseline_run(np, orc, idx, adj, xs, rs, [1e-7], [np.float32(0.85)], [1], min(args.cpu_seconds, 10.0),
-                               "personalised PageRank column 0 (plain iteration, K = 1)")
-        cpu["unit"] = "G vector-pushes/s"
-        rec["cpu_baseline"] = cpu
+    # the batched run's baseline is the 64 plain GPU runs above; a CPU sample of one column at
+    # this size (--config batched --cpu) costs half a minute of the default run
+    rec["cpu_baseline"] = None
     return rec
 
 
@@ -716,9 +710,9 @@ def sub_adjoint(args, nat, np, work):
     a = argparse.Namespace(**vars(args))
     a.scale = 22
     graph = graph_file(a, nat, work, 22, weights=True)
-    # 8 finite-difference directions here (4 raw-weight, 4 teleport; about 10 s each at this
+    # 6 finite-difference directions here (3 raw-weight, 3 teleport; about 10 s each at this
     # size); --config adjoint runs the full 32
-    return bench_adjoint(a, nat, np, work, graph, fd_dirs=8)
+    return bench_adjoint(a, nat, np, work, graph, fd_dirs=6)
 
 
 def main():
@@ -753,7 +747,7 @@ def main():
     ap.add_argument("--kernel", default="lb", choices=["lb", "group"])
     ap.add_argument("--pull", default="auto", help="dense passes by pull: auto | off | demand fraction")
     ap.add_argument("--hub-lds", type=int, default=1024, help="destinations privatised in LDS by the hub-piece push")
-    ap.add_argument("--cpu-seconds", type=float, default=40.0,
+    ap.add_argument("--cpu-seconds", type=float, default=30.0,
                     help="CPU baseline: time limit of the restatement's run (it stops earlier when it converges)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 error report of the last run")
