@@ -17,6 +17,7 @@ graph and constants, on this host's cores, for a bounded number of seconds.
 Prints one JSON line on stdout; progress goes to stderr.
 """
 import argparse
+import glob
 import json
 import os
 import subprocess
@@ -209,15 +210,21 @@ def bench_batched(args, nat, np, work, graph, sample=None):
     xb = {j: sess.values(j) for j in sample}
     sess.close()
     log(f"batched: {rb['passes']} passes, {tb:.3f}s, {rb['kernel']['pushed_edge_measures'] / 1e9:.2f} G vector pushes")
+    kb = rb["kernel"]
+    bpull = kb["pull_bytes"] / (kb["pull_ms"] / 1e3) / 1e9 if kb["pull_ms"] > 0 else 0.0
+    roof = {"bound": "hbm", "achieved": round(bpull, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(bpull / HBM_PEAK_GBS, 4), "traffic": None, "kernel": "k_bpull (batched dense pull)",
+            "launches": kb["pull_passes"], "avg_launch_ms": round(kb["pull_ms"] / max(1, kb["pull_passes"]), 4),
+            "algorithmic_bytes_per_launch": round(kb["pull_bytes"] / max(1, kb["pull_passes"]), 1)}
     if args.batched_only:
-        k = rb["kernel"]
         return {"metric": "batched personalised PageRank (no baseline)", "unit": "s", "value": round(tb, 4),
-                "k": K, "passes": rb["passes"], "dense_passes": k["pull_passes"],
-                "pull_rows_in_flight": k["pull_rows_in_flight"],
-                "pull_tune_ns_per_edge": [round(v, 4) for v in k["pull_tune_ns_per_edge"]],
-                "vector_pushes": k["pushed_edge_measures"], "vector_gteps": round(k["pushed_edge_measures"] / tb / 1e9, 3),
-                "kernel_ms": {x: round(k[x], 2) for x in ("snapshot_ms", "push_ms", "piece_ms", "fold_ms", "pull_ms",
-                                                           "pull_hub_ms")}}
+                "k": K, "passes": rb["passes"], "dense_passes": kb["pull_passes"],
+                "pull_rows_in_flight": kb["pull_rows_in_flight"],
+                "pull_tune_ns_per_edge": [round(v, 4) for v in kb["pull_tune_ns_per_edge"]],
+                "vector_pushes": kb["pushed_edge_measures"], "vector_gteps": round(kb["pushed_edge_measures"] / tb / 1e9, 3),
+                "kernel_ms": {x: round(kb[x], 2) for x in ("snapshot_ms", "push_ms", "piece_ms", "fold_ms", "pull_ms",
+                                                            "pull_hub_ms")},
+                "roofline": roof}
 
     sess = nat.Session(common + ["--measure", "pr", "--ppr-column", 0, "--pull", args.pull,
                                  "--log", os.path.join(args.logdir or work, "plain.log")])
@@ -258,12 +265,6 @@ def bench_batched(args, nat, np, work, graph, sample=None):
             fp64[f"column {j}"] = e
             log(f"fp64 column {j}: {e}")
     vb = rb["kernel"]["pushed_edge_measures"]
-    kb = rb["kernel"]
-    bpull = kb["pull_bytes"] / (kb["pull_ms"] / 1e3) / 1e9 if kb["pull_ms"] > 0 else 0.0
-    roof = {"bound": "hbm", "achieved": round(bpull, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(bpull / HBM_PEAK_GBS, 4), "traffic": None, "kernel": "k_bpull (batched dense pull)",
-            "launches": kb["pull_passes"], "avg_launch_ms": round(kb["pull_ms"] / max(1, kb["pull_passes"]), 4),
-            "algorithmic_bytes_per_launch": round(kb["pull_bytes"] / max(1, kb["pull_passes"]), 1)}
     return {
         "metric": "batched vs plain personalised PageRank at FP64 error", "unit": "x (wall-time ratio)",
         "value": round(ts / tb, 3), "higher_is_better": True, "n_gpus": 1, "dtype": "f32", "data": "synthetic",
@@ -622,20 +623,22 @@ def bench_plain(args, nat, np, work, graph, scale, measure, steps, warmup, env, 
               sum(r["kernel"]["src_launches"][i] for r in runs), sum(r["kernel"]["src_bytes"][i] for r in runs))
              for i in range(4)]
     npull = K("pull_passes")
+    tile = runs[-1]["kernel"]["pull_tile"]
+    pull_name = f"k_pull_sorted<{ktag},{tile}>" if tile else f"k_pull<{ktag}>"
     cands += [(f"k_fold<{ktag}>", K("fold_ms"), npull, K("fold_bytes")),
-              (f"k_pull<{ktag}>", K("pull_ms"), npull, K("pull_bytes")),
+              (pull_name, K("pull_ms"), npull, K("pull_bytes")),
               (f"k_pull_hub<{ktag}>", K("pull_hub_ms"), npull, K("pull_hub_bytes")),
               (f"k_snapshot<{ktag}>", K("snapshot_ms"), K("snapshot_launches"), K("snapshot_bytes")),
               (f"k_pieces<{ktag}>", K("piece_ms"), K("piece_launches"), K("piece_bytes"))]
     dom = max(cands, key=lambda c: c[1])
     achieved = dom[3] / (dom[1] / 1e3) / 1e9 if dom[1] > 0 else 0.0
     traffic = None
-    if os.path.exists(args.traffic):
+    want = {"scale": scale, "edge_factor": args.edge_factor,
+            "seed": args.seed, "measure": measure, "kernel": args.kernel, "pull": args.pull}
+    for path in sorted(glob.glob(args.traffic)):
         # per-launch HBM bytes from rocprofv3 FETCH_SIZE/WRITE_SIZE runs of the same workload
-        tr = json.load(open(args.traffic))
-        want = {"scale": scale, "edge_factor": args.edge_factor,
-                "seed": args.seed, "measure": measure, "kernel": args.kernel, "pull": args.pull}
-        if tr.get("workload") == want:
+        tr = json.load(open(path))
+        if tr.get("workload") == want and traffic is None:
             traffic = tr["bytes_per_launch"].get(dom[0].split("<")[0])
     roof = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": dom[0],
@@ -753,8 +756,8 @@ def main():
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 error report of the last run")
     ap.add_argument("--workdir", default=None)
     ap.add_argument("--logdir", default=None, help="directory of the runs' bin/excent reports (default: the workdir)")
-    ap.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "traffic_rmat24_all.json"),
-                    help="JSON with HBM bytes per push launch from rocprofv3 --pmc runs")
+    ap.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "traffic_*.json"),
+                    help="JSON files (a glob) with HBM bytes per launch from rocprofv3 --pmc runs, one workload each")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "

@@ -2789,6 +2789,50 @@ int exc_mem_info(int device, size_t* free_bytes, size_t* total_bytes) {
     return EXC_OK;
 }
 
+// Device memory held for bin/gpu_ballast: blocks are taken while more than keep_free bytes
+// are free, starting at 1 GiB and quartering the request after a refusal (down to 2 MiB),
+// so fragmentation or other processes' allocations do not stop it short of the target.
+struct MemoryHold {
+    int device;
+    std::vector<void*> blocks;
+};
+
+int exc_hold_memory(int device, size_t keep_free, void** handle, size_t* held_bytes) {
+    if (!handle) return fail(EXC_EINVAL, "exc_hold_memory: null handle");
+    *handle = nullptr;
+    HIPCHK(hipSetDevice(device));
+    auto* h = new MemoryHold{device, {}};
+    size_t piece = 1ull << 30, held = 0;
+    for (;;) {
+        size_t freeB = 0, totalB = 0;
+        if (hipMemGetInfo(&freeB, &totalB) != hipSuccess || freeB <= keep_free) break;
+        const size_t want = std::min(piece, freeB - keep_free);
+        void* b = nullptr;
+        if (hipMalloc(&b, want) == hipSuccess) {
+            h->blocks.push_back(b);
+            held += want;
+            continue;
+        }
+        (void)hipGetLastError();
+        if (piece <= (2ull << 20)) break;
+        piece /= 4;
+    }
+    if (held_bytes) *held_bytes = held;
+    *handle = h;
+    return EXC_OK;
+}
+
+int exc_release_memory(void* handle) {
+    auto* h = static_cast<MemoryHold*>(handle);
+    if (!h) return EXC_OK;
+    int rc = EXC_OK;
+    if (hipSetDevice(h->device) != hipSuccess) rc = fail(EXC_EHIP, "exc_release_memory: device %d", h->device);
+    for (void* b : h->blocks)
+        if (hipFree(b) != hipSuccess && rc == EXC_OK) rc = fail(EXC_EHIP, "exc_release_memory: hipFree");
+    delete h;
+    return rc;
+}
+
 int exc_device_info(int device, int* cus, int* integrated, char* name, int name_len) {
     hipDeviceProp_t p;
     HIPCHK(hipGetDeviceProperties(&p, device));
@@ -3498,6 +3542,7 @@ int exc_get_stats(void* p, exc_stats* o) {
         o->pull_ms = c->t_ms[T_PULL];
         o->pull_hub_ms = c->t_ms[T_PULL_HUB];
         o->pull_passes = c->pull_passes;
+        o->pull_tile = c->sorted_T;
         // fold: read r for every vertex, write p for every source; offsets, x read + write for
         // folded ones
         o->fold_bytes = P * c->n * 4.0 * nm + P * c->nsrc * 4.0 * nm + 16.0 * (r4[C_VERT] + r5[C_VERT]) +
@@ -3507,7 +3552,9 @@ int exc_get_stats(void* p, exc_stats* o) {
                     // live-row bitmap from the row->vertex map
             o->fold_bytes = P * 4.0 * (((double)c->n + 31) / 32) + (double)r5[C_VERT] * (20.0 + (4.0 + pb) * nm) +
                             12.0 * (double)r5[C_VERT_NM] + P * (4.0 * c->nsrc + c->nsrc / 8.0);
-        // pull: every in-edge's id, every vertex's offsets and r (read + write), and the p rows
+        // pull: every in-edge's id (the sorted pull: row id and 16-bit destination offset) and
+        // weight on weighted graphs,
+        // every vertex's offsets and r (read + write), and the p rows
         // read: every in-edge's in a plain pass; in a pass with the live-row bitmap, the
         // bitmap plus one row per out-edge of a pushing source (the fold's edge count -- a
         // pushed weight is zero only on underflow), split between the kernels by
@@ -3521,8 +3568,10 @@ int exc_get_stats(void* p, exc_stats* o) {
         // k_pull counts the rows it reads (batched passes count them through the fold)
         const double rows = c->K ? std::max(0.0, (double)(r5[C_EDGES] + sp[S_EDGES]) - (double)r5[C_PULL_HUB_LIVE])
                                  : (double)sp[S_ROWS];
-        o->pull_bytes = P * (me * 4.0 + nv * (8.0 + 8.0 * nm)) + PL * bm + pb * nm * rows;
-        o->pull_hub_bytes = P * ((double)c->pull_hub_edges * 4.0 + (double)c->pull_hub_vertices * (8.0 + 8.0 * nm)) +
+        const double entry_bytes = (c->sorted_T ? 6.0 : 4.0) + (c->d_twts ? 4.0 : 0.0);   // + a weight
+        o->pull_bytes = P * (me * entry_bytes + nv * (8.0 + 8.0 * nm)) + PL * bm + pb * nm * rows;
+        o->pull_hub_bytes = P * ((double)c->pull_hub_edges * (c->d_twts ? 8.0 : 4.0) +
+                                 (double)c->pull_hub_vertices * (8.0 + 8.0 * nm)) +
                             PL * bm + pb * nm * hub_rows;
         for (const unsigned long long* r : {r4, r5}) {
             o->pushed_vertices += r[C_VERT];

@@ -68,7 +68,8 @@ class Stats(ctypes.Structure):
                 ("src_ms", c_double * 4), ("src_launches", c_u64 * 4), ("src_bytes", c_double * 4),
                 ("fold_ms", c_double), ("pull_ms", c_double), ("pull_hub_ms", c_double), ("pull_passes", c_u64),
                 ("fold_bytes", c_double), ("pull_bytes", c_double), ("pull_hub_bytes", c_double),
-                ("pull_rows_in_flight", c_int), ("pull_tune_ns_per_edge", c_double * 2)]
+                ("pull_rows_in_flight", c_int), ("pull_tune_ns_per_edge", c_double * 2),
+                ("pull_tile", c_int)]
 
 
 class RunStats(ctypes.Structure):
@@ -104,6 +105,8 @@ HIP_SIGNATURES = {
     "exc_device_count": (c_int, [P(c_int)]),
     "exc_mem_info": (c_int, [c_int, P(ctypes.c_size_t), P(ctypes.c_size_t)]),
     "exc_device_info": (c_int, [c_int, P(c_int), P(c_int), ctypes.c_char_p, c_int]),
+    "exc_hold_memory": (c_int, [c_int, ctypes.c_size_t, P(c_void_p), P(ctypes.c_size_t)]),
+    "exc_release_memory": (c_int, [c_void_p]),
     "exc_device_synchronize": (c_int, [c_int]),
     "exc_create": (c_int, [P(Config), P(Measures), P(c_void_p)]),
     "exc_destroy": (c_int, [c_void_p]),
@@ -165,6 +168,9 @@ HOST_SIGNATURES = {
     "excent_convert_main": (c_int, [c_int, P(ctypes.c_char_p)]),
     "excent_reference_main": (c_int, [c_int, P(ctypes.c_char_p)]),
     "excent_params_main": (c_int, [c_int, P(ctypes.c_char_p)]),
+    "excent_egr_info_main": (c_int, [c_int, P(ctypes.c_char_p)]),
+    "excent_evict_main": (c_int, [c_int, P(ctypes.c_char_p)]),
+    "excent_ballast_main": (c_int, [c_int, P(ctypes.c_char_p)]),
     "excent_session_open": (c_int, [c_int, P(ctypes.c_char_p), P(c_void_p)]),
     "excent_session_run": (c_int, [c_void_p, c_int, P(RunStats)]),
     "excent_session_results": (c_int, [c_void_p]),

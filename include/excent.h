@@ -66,6 +66,15 @@ int excent_convert_main(int argc, const char* const* argv);
 int excent_reference_main(int argc, const char* const* argv);
 /* bin/excent_params <graph.egr> <out.env> (tools/excent_params.cu:73-174). */
 int excent_params_main(int argc, const char* const* argv);
+/* bin/egr_info <graph.egr> (tools/egr_info.cpp:10-29): n, m, largest degree, a sampled
+ * reverse-edge and self-loop count. */
+int excent_egr_info_main(int argc, const char* const* argv);
+/* bin/evict_file [--report] <file>... (tools/evict_file.c:36-65): drop files from the page
+ * cache, or report their resident share. */
+int excent_evict_main(int argc, const char* const* argv);
+/* bin/gpu_ballast <budget_GB> [--device D] (tools/gpu_ballast.cu:13-41): hold device memory
+ * until only the budget stays free, until SIGINT / SIGTERM. */
+int excent_ballast_main(int argc, const char* const* argv);
 
 /* Differentiable personalised PageRank (no reference counterpart).  x = (1-d) s + d A x with
  * A[u][v] = w_vu / S_v (raw positive weights w in CSR order, S_v = v's out-weight sum).

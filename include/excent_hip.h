@@ -95,12 +95,20 @@ typedef struct exc_stats {
      * (ns) with 8 and with 16 (0 until that trial pass ran) */
     int pull_rows_in_flight;
     double pull_tune_ns_per_edge[2];
+    /* plain dense pull: destinations per tile of the source-sorted pull (k_pull_sorted), or 0
+     * when the pull runs over the in-lists in vertex order (k_pull) */
+    int pull_tile;
 } exc_stats;
 
 const char* exc_last_error(void);
 int exc_device_count(int* count);
 int exc_mem_info(int device, size_t* free_bytes, size_t* total_bytes);
 int exc_device_info(int device, int* cus, int* integrated, char* name, int name_len);
+/* bin/gpu_ballast (tools/gpu_ballast.cu:20-40): takes device memory on `device` until at most
+ * keep_free bytes stay free (or the device refuses more), and holds it until
+ * exc_release_memory(handle).  held_bytes (may be NULL) receives the amount taken. */
+int exc_hold_memory(int device, size_t keep_free, void** handle, size_t* held_bytes);
+int exc_release_memory(void* handle);
 /* Waits for all work on the device (every stream of every context). */
 int exc_device_synchronize(int device);
 

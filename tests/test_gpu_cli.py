@@ -140,6 +140,43 @@ def test_hot_cache_partial(tmp_path, rmat14, when, order):
     pr_check(idx, adj, np.fromfile(str(tmp_path / "o.pr.bin"), np.float32))
 
 
+def test_hot_rank(tmp_path, rmat14, capfd):
+    """--hot-rank (src/excent.cu:459-483, 583-585): a --trace-pending run writes the passes each
+    vertex pended (uint16 per vertex), and a second run caches the lists ranked by that count
+    within a 0.25 MB budget.  The cached set is the oracle's ranked selection for the same
+    scores and budget; the run pushes cached vertices from HBM, fetches the rest, and
+    converges to the FP64 bound.  A file shorter than n entries is refused."""
+    idx, adj = orc.egr_load(rmat14)
+    n = len(idx) - 1
+    tf = str(tmp_path / "t.bin")
+    print("case hot rank: trace run", flush=True)
+    rc, text = run([rmat14, "--measure", "pr", "--trace-pending", tf, "--stream-ratio", "0.3",
+                    "--chunk-medges", "0.02"], tmp_path, "trace")
+    assert rc == 0, text
+    score = np.fromfile(tf, np.uint16)
+    assert score.shape == (n,) and score.max() > 1 and len(np.unique(score)) > 2
+    print("case hot rank: ranked run", flush=True)
+    args = [rmat14, "--measure", "pr", "--dump", str(tmp_path / "o"), "--chunk-medges", "0.02", "--cache", "hot",
+            "--cache-mb", "0.25", "--stream-ratio", "0.3", "--hot-rank", tf]
+    rc, text = run(args, tmp_path)
+    assert rc == 0, text
+    mo = re.search(r"GPU cache ranked by passes pending \((.+)\): (\d+) vertices \(([0-9.]+)% of edges", text)
+    assert mo and mo.group(1) == tf, text
+    ids, off, bm, _ = orc.hot_select(idx, 262144, score)
+    assert int(mo.group(2)) == len(ids) > 0
+    assert abs(float(mo.group(3)) - 100.0 * int(off[-1]) / len(adj)) < 0.06
+    passes = [tuple(int(g) for g in p.groups()) for p in PASS_RE.finditer(text)]
+    assert any(p[0] > 0 and p[5] > 0 for p in passes), text
+    assert any(p[0] > 0 and p[3] + p[4] > 0 for p in passes), text
+    pr_check(idx, adj, np.fromfile(str(tmp_path / "o.pr.bin"), np.float32))
+    short = str(tmp_path / "short.bin")
+    score[: n // 2].tofile(short)
+    print("case hot rank: short file", flush=True)
+    rc, text = run(args[:-1] + [short], tmp_path, "short")
+    assert rc != 0
+    assert f"--hot-rank: cannot read {n} entries from {short}" in capfd.readouterr().err + text
+
+
 def test_tau_calibration(tmp_path, rmat14, monkeypatch):
     """--stream-ratio auto (the default): tau is measured on the middle chunk, written to
     ~/.cache/excent/<graph>.<n>.<m>.tau as "%.6f %.4f" (src/excent.cu:518-579), read back by
@@ -357,3 +394,54 @@ def test_fp64_solver_strided_grid(tmpdir_factory):
         x = np.random.default_rng(4).random(n).astype(np.float32)
         want = orc.residual_l1(idx, adj, orc.ORC_PR, 0.85, 0.15, x)
         assert abs(S.residual(x, 0.15, 0.85, 1) - want) <= 1e-10 * want
+
+
+def test_gpu_ballast_caps_the_budget(tmp_path, rmat14):
+    """bin/gpu_ballast (tools/gpu_ballast.cu:13-41) in a second process takes 8 GiB of device
+    memory and holds it until SIGTERM: the free memory this process sees drops by that much,
+    and a --cache auto run whose --reserve-mb exceeds the remaining free memory falls back to
+    the high-degree cache, where the same run without the ballast caches the whole graph."""
+    import queue
+    import signal
+    import subprocess
+    import threading
+    from tests.conftest import ROOT
+
+    def free_bytes():
+        free, total = nat.ctypes.c_size_t(0), nat.ctypes.c_size_t(0)
+        assert nat.F["exc_mem_info"](0, nat.ctypes.byref(free), nat.ctypes.byref(total)) == 0
+        return free.value
+
+    gib, mib = 1 << 30, 1 << 20
+    idx, adj = orc.egr_load(rmat14)
+    free0 = free_bytes()
+    budget = free0 - 8 * gib
+    reserve_mb = (budget + 64 * mib) // mib   # 64 MiB more than stays free under the ballast
+    args = [rmat14, "--measure", "pr", "--dump", str(tmp_path / "o"), "--chunk-medges", "0.02", "--stream-ratio", "0.3",
+            "--reserve-mb", reserve_mb]
+    print("case ballast: control run", flush=True)
+    rc, text = run(args, tmp_path, "control")
+    assert rc == 0 and "cache: auto -> chunks (the whole graph fits)" in text, text
+    print("case ballast: start", flush=True)
+    p = subprocess.Popen([os.path.join(ROOT, "bin", "gpu_ballast"), f"{budget / gib:.6f}"], stdout=subprocess.PIPE,
+                         stderr=subprocess.PIPE, text=True)
+    try:
+        q = queue.Queue()
+        threading.Thread(target=lambda: q.put(p.stdout.readline()), daemon=True).start()
+        line = q.get(timeout=120)
+        mo = re.match(r"READY free=([0-9.]+) GB \(budget ([0-9.]+) GB\) of ([0-9.]+) GB", line)
+        assert mo, (line, p.poll())
+        assert abs(float(mo.group(2)) - budget / gib) < 0.01
+        held_free = free_bytes()
+        assert budget - gib <= held_free <= budget + 32 * mib, (held_free, budget)
+        print("case ballast: capped run", flush=True)
+        rc, text = run(args, tmp_path, "capped")
+        assert rc == 0 and "cache: auto -> high-degree lists" in text, text
+        pr_check(idx, adj, np.fromfile(str(tmp_path / "o.pr.bin"), np.float32))
+        p.send_signal(signal.SIGTERM)
+        assert p.wait(timeout=60) == 0, p.stderr.read()
+    finally:
+        if p.poll() is None:
+            p.kill()
+            p.wait()
+    assert free_bytes() >= free0 - gib
