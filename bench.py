@@ -590,6 +590,7 @@ def bench_plain(args, nat, np, work, graph, scale, measure, steps, warmup, env, 
     sargs = [graph, "--measure", measure, "--cache", "chunks", "--preload", "--stream-ratio", "1",
              timing, "--push-kernel", args.kernel, "--pull", args.pull, "--hub-lds", args.hub_lds, "--log", excent_log,
              "--dump", os.path.join(work, "x"), "--stop", stop_rule(args, measure),
+             "--drop-measures", args.drop_measures,
              # a run whose stop rule is never met ends here and is reported as not converged
              "--max-passes", args.max_passes or 2000] + measure_args(measure, env)
     t0 = time.time()
@@ -623,10 +624,18 @@ def bench_plain(args, nat, np, work, graph, scale, measure, steps, warmup, env, 
               sum(r["kernel"]["src_launches"][i] for r in runs), sum(r["kernel"]["src_bytes"][i] for r in runs))
              for i in range(4)]
     npull = K("pull_passes")
-    tile = runs[-1]["kernel"]["pull_tile"]
-    pull_name = f"k_pull_sorted<{ktag},{tile}>" if tile else f"k_pull<{ktag}>"
+    # the dense pull by width: passes over every measure, and (--measure all) those over the
+    # first two / first measure once the others have nothing pending
+    S = ktag.split(",")[1]
+    for w in range(3):
+        launches = sum(r["kernel"]["pull_w_launches"][w] for r in runs)
+        if not launches:
+            continue
+        tile = runs[-1]["kernel"]["pull_w_tile"][w]
+        name = f"k_pull_sorted<{w + 1},{S},{tile}>" if tile else f"k_pull<{w + 1},{S}>"
+        cands.append((name, sum(r["kernel"]["pull_w_ms"][w] for r in runs), launches,
+                      sum(r["kernel"]["pull_w_bytes"][w] for r in runs)))
     cands += [(f"k_fold<{ktag}>", K("fold_ms"), npull, K("fold_bytes")),
-              (pull_name, K("pull_ms"), npull, K("pull_bytes")),
               (f"k_pull_hub<{ktag}>", K("pull_hub_ms"), npull, K("pull_hub_bytes")),
               (f"k_snapshot<{ktag}>", K("snapshot_ms"), K("snapshot_launches"), K("snapshot_bytes")),
               (f"k_pieces<{ktag}>", K("piece_ms"), K("piece_launches"), K("piece_bytes"))]
@@ -638,8 +647,9 @@ def bench_plain(args, nat, np, work, graph, scale, measure, steps, warmup, env, 
     for path in sorted(glob.glob(args.traffic)):
         # per-launch HBM bytes from rocprofv3 FETCH_SIZE/WRITE_SIZE runs of the same workload
         tr = json.load(open(path))
-        if tr.get("workload") == want and traffic is None:
-            traffic = tr["bytes_per_launch"].get(dom[0].split("<")[0])
+        if tr.get("workload") == want and traffic is None:   # by full name, else the kernel's base name
+            bpl = tr["bytes_per_launch"]
+            traffic = bpl.get(dom[0], bpl.get(dom[0].split("<")[0]))
     roof = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": dom[0],
             "launches": dom[2], "avg_launch_ms": round(dom[1] / max(1, dom[2]), 5),
@@ -750,6 +760,8 @@ def main():
     ap.add_argument("--kernel", default="lb", choices=["lb", "group"])
     ap.add_argument("--pull", default="auto", help="dense passes by pull: auto | off | demand fraction")
     ap.add_argument("--hub-lds", type=int, default=1024, help="destinations privatised in LDS by the hub-piece push")
+    ap.add_argument("--drop-measures", default="on", choices=["on", "off"],
+                    help="dense passes leave out the measures with nothing pending")
     ap.add_argument("--cpu-seconds", type=float, default=30.0,
                     help="CPU baseline: time limit of the restatement's run (it stops earlier when it converges)")
     ap.add_argument("--no-cpu", action="store_true")
@@ -816,6 +828,7 @@ def main():
                                f"whole graph in HBM, converged to eps", "n": n_v, "m": m_e,
                    "lambda1": env["LAMBDA"], "alpha": env["ALPHA"], "eps_pr": 0.01, "eps_katz": 1e-6,
                    "eps_ev": 1e-6, "stop": stop_rule(args, args.measure), "push_kernel": args.kernel, "pull": args.pull,
+                   "drop_measures": args.drop_measures,
                    "parallelism": "single GPU"},
     }
     out.update({k: v for k, v in rec.items() if k not in ("value", "unit", "steps", "warmup", "ms_per_step")})

@@ -68,6 +68,7 @@ struct Options {
     double alpha = -1, beta = 1.0, lambda = -1, eps_pr = 0.01, eps_katz = -1, eps_ev = -1;
     // extensions
     bool preload = false;
+    std::string drop_measures = "on";   // dense passes leave out measures with nothing pending
     int timing = 0;   // 1: HIP events around every kernel; 2: around the dense pull only
     double cache_mb = -1;
     std::string push_kernel = "lb", ev_init = "outdeg", stats_json, log;
@@ -151,6 +152,7 @@ Options parse(int argc, const char* const* argv) {
         else if (a == "--stop") o.stop = next();
         else if (a == "--io-threads") o.io_threads = atoi(next());
         else if (a == "--hub-lds") o.hub_lds = atoi(next());
+        else if (a == "--drop-measures") o.drop_measures = next();
         else die("unknown option " + a);
     }
     return o;
@@ -263,6 +265,7 @@ private:
     bool have_l1_ = false;
     int stop_reason_ = 0;               // 0 nothing pending, 1 global L1 bound, 2 pass limit
     std::vector<char> global_l1_;       // per measure slot: the global L1 rule may end it (--stop)
+    int act_ = 1;                       // measure slots the dense passes update (exc_set_active_measures)
     float linf_[EXC_MAXM] = {0, 0, 0};  // largest |r| per measure, last snapshot
 
     void init_state();
@@ -329,6 +332,7 @@ void Engine::setup() {
     if (o.hot_when != "end" && o.hot_when != "start" && o.hot_when != "both") die("--hot-when must be end, start or both");
     if (o.hot_order != "sweep" && o.hot_order != "ranges") die("--hot-order must be sweep or ranges");
     if (o.hub_lds < 0 || o.hub_lds > 1024) die("--hub-lds must be 0..1024");
+    if (o.drop_measures != "on" && o.drop_measures != "off") die("--drop-measures must be on or off");
     global_l1_.assign(kinds_.size(), 0);
     if (o.stop == "global") {
         global_l1_.assign(kinds_.size(), 1);
@@ -702,6 +706,7 @@ void Engine::run_passes() {
     bool stop_global = false;
     for (char g1 : global_l1_) stop_global = stop_global || g1;
     if (pull_on_) ck(exc_fold_ahead(ctx, 1), "fold ahead");
+    act_ = M_.nm;   // a run starts from a replaced state: every measure active
     bool fold_ahead_off = false;   // set once a folded state met the global bound
     const Clock::time_point T_begin = Clock::now();
     const size_t words = ((size_t)n + 31) / 32;
@@ -740,6 +745,16 @@ void Engine::run_passes() {
         const PlanOut po = fetch_plan(nc, h_cnt.data(), resident_, loaded_, stream_ratio_, chunk_edges, hot_mode_, pass_, hlo_, hhi_, mode.data());
         // FP64 global residual of the snapshot state (plain measures)
         have_l1_ = !batched() && exc_residual_l1(ctx, l1_) == EXC_OK && exc_residual_max(ctx, linf_) == EXC_OK;
+        if (pull_on_ && have_l1_ && M_.nm > 1 && o.drop_measures == "on") {
+            // dense passes need only the measures up to the last one still pending; the others
+            // have converged for good (a measure's residual moves only by its own pushes)
+            int act = 0;
+            for (int i = 0; i < M_.nm; i++) if (linf_[i] > M_.eps[i]) act = i + 1;
+            if (act >= 1 && act < act_) {
+                ck(exc_set_active_measures(ctx, act), "active measures");
+                act_ = act;
+            }
+        }
         bool global_met = false;
         if (stop_global && have_l1_) {
             // every measure done: nothing pending for it, or (global rule) its sum within n eps
@@ -844,6 +859,8 @@ void Engine::run_passes() {
         if (have_l1_) {
             fprintf(out, " | residual l1");
             for (int i = 0; i < M_.nm; i++) fprintf(out, " %s %.4e", MNAME[kinds_[i]], l1_[i]);
+            fprintf(out, " | max |r|");
+            for (int i = 0; i < M_.nm; i++) fprintf(out, " %s %.4e", MNAME[kinds_[i]], (double)linf_[i]);
         }
         fprintf(out, "\n");
         fflush(out);   // a long run shows its progress pass by pass

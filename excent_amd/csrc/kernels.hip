@@ -82,11 +82,13 @@ enum { C_DENSE = 4, C_DENSE_LIVE = 5, C_OVERFLOW = 6 * C_FIELDS };
 // Counters updated once per block by kernels with one block per tile (k_pull): spread over
 // SPREAD_SLOTS slots on separate 128-byte lines (slot = block mod SPREAD_SLOTS), summed on
 // the host.  Fields: rows k_pull read; the fused fold's vertices, vertex-measures, edges and
-// edge-measures.
-enum { SPREAD_SLOTS = 64, SPREAD_STRIDE = 16, S_ROWS = 0, S_VERT, S_VERT_NM, S_EDGES, S_EDGES_NM };
+// edge-measures; rows read by dense passes over the first one and the first two measures
+// only (S_ROWS counts the passes over every measure).
+enum { SPREAD_SLOTS = 64, SPREAD_STRIDE = 16, S_ROWS = 0, S_VERT, S_VERT_NM, S_EDGES, S_EDGES_NM, S_ROWS_A1, S_ROWS_A2 };
 enum { C_SPREAD = C_OVERFLOW + 16, C_NCOUNTERS = C_SPREAD + SPREAD_SLOTS * SPREAD_STRIDE };
-// the dense-pass rows have no piece edges; a spare field counts k_pull_hub's row reads
-enum { C_PULL_HUB_LIVE = C_PEDGES };
+// the dense-pass rows have no piece edges; a spare field counts k_pull_hub's row reads, two
+// more those of hub passes over the first one / two measures
+enum { C_PULL_HUB_LIVE = C_PEDGES, C_PULL_HUB_LIVE_A1 = 6, C_PULL_HUB_LIVE_A2 = 7 };
 
 __device__ __forceinline__ void atomic_add_f32(float* p, float v) {
     // No-return hardware float atomic on coarse-grained HBM (global_atomic_add_f32).
@@ -855,6 +857,7 @@ struct PullArgs {
     int nsrc;
     unsigned long long* spread;   // C_SPREAD area of the counters
     double* l1;                   // fused snapshot: FP64 residual sum slots, or null
+    int rows_field;               // spread field that counts the rows read (S_ROWS, S_ROWS_A1/A2)
 };
 
 // Bitmap of the p rows the fold made nonzero: one wave per 64 consecutive rows, one ballot,
@@ -876,6 +879,21 @@ __global__ __launch_bounds__(BLOCK) void k_pbits(const float* __restrict__ p, in
         }
         const uint64_t b = __ballot(nz);
         if (lane < 2) bits[2 * w + lane] = (uint32_t)(b >> (32 * lane));
+    }
+}
+
+// Rows of p written with IN columns (a fold over IN measures) repacked to their first OUT
+// columns, into another buffer.  A dense pass that drops measures right after a pull that
+// folded it finds p in the wider layout; the dropped columns are zero there (their measures
+// had nothing pending when the fold ran), so keeping the first OUT columns is exact.
+template <int IN, int OUT>
+__global__ __launch_bounds__(BLOCK) void k_prepack(const float* __restrict__ src, float* __restrict__ dst, int nsrc) {
+    for (long long q = blockIdx.x * (long long)BLOCK + threadIdx.x; q < nsrc; q += (long long)gridDim.x * BLOCK) {
+        float v[OUT];
+#pragma unroll
+        for (int k = 0; k < OUT; k++) v[k] = src[q * IN + k];
+#pragma unroll
+        for (int k = 0; k < OUT; k++) dst[q * OUT + k] = v[k];
     }
 }
 
@@ -1161,7 +1179,7 @@ __global__ __launch_bounds__(BLOCK, 8) void k_pull(PullArgs a, DevMeasures M) {
             unsigned long long sum = 0;
 #pragma unroll
             for (int w = 0; w < NWAVES; w++) sum += s_red[w];
-            if (sum) atomicAdd(&slot[S_ROWS], sum);
+            if (sum) atomicAdd(&slot[a.rows_field], sum);
         }
         __syncthreads();
     }
@@ -1254,7 +1272,7 @@ __global__ __launch_bounds__(BLOCK) void k_pull_sorted(PullArgs a, SortedTiles s
             unsigned long long sum = 0;
 #pragma unroll
             for (int w = 0; w < NWAVES; w++) sum += s_red[w];
-            if (sum) atomicAdd(&slot[S_ROWS], sum);
+            if (sum) atomicAdd(&slot[a.rows_field], sum);
         }
     }
     for (int j = 0; j < T / BLOCK; j++) {   // block-uniform
@@ -2161,7 +2179,10 @@ struct Timer {
     int cls;
 };
 
-enum { T_SNAP = 0, T_PUSH, T_PUSH_LAST = T_PUSH + 3, T_PIECE, T_OTHER, T_FOLD, T_PULL, T_PULL_HUB, T_NCLS };
+// T_PULL: the dense pull (k_pull / k_pull_sorted / k_bpull) over every measure of the context;
+// T_PULL_A1, T_PULL_A2: dense pulls over its first one / two measures (exc_set_active_measures)
+enum { T_SNAP = 0, T_PUSH, T_PUSH_LAST = T_PUSH + 3, T_PIECE, T_OTHER, T_FOLD, T_PULL, T_PULL_HUB, T_PULL_A1, T_PULL_A2,
+       T_NCLS };
 
 struct Ctx {
     int dev = 0, n = 0, nc = 0, nm = 1, S = 1, push_kernel = EXC_KERNEL_LB;
@@ -2184,6 +2205,21 @@ struct Ctx {
     unsigned long long* h_small = nullptr;     // pinned: cnt + hotpend + residual slots
     double l1_last[EXC_MAXM] = {};             // FP64 residual sums of the last snapshot
     float linf_last[EXC_MAXM] = {};            // and the largest |r| per measure
+    int snap_nm = 0;                           // measures the last snapshot computed (the others
+                                               // were dropped: their sums above stay as they were)
+    // Dense passes update measures [0, nm_act) only: the others had nothing pending when they
+    // were dropped (exc_set_active_measures), and a measure's residual changes only by its own
+    // pushes, so they stay converged.  p_cols: columns of the rows in d_p (the width of the
+    // fold that wrote them).
+    int nm_act = 1;
+    int p_cols = 0;
+    // per-launch algorithmic bytes of the plain dense passes, by active measures - 1: the
+    // parts known at launch (in-list entries, destinations, live bitmap, hub rows of passes
+    // without the bitmap); the rows the kernels count are added in exc_get_stats
+    double pull_fixed_bytes[3] = {}, hub_fixed_bytes[3] = {}, fold_fixed_bytes = 0;
+    uint64_t pull_launches_w[3] = {};
+    int pull_tile_w[3] = {};                   // the sorted tile each width ran with (0: k_pull)
+    bool sorted_all = false;                   // EXC_PULL_TILE: the sorted pull at every width
     excent_dev::SortedTileBuffers sorted;      // source-sorted dense pull (k_pull_sorted)
     int sorted_T = 0;                          // its tile (destinations per block); 0: k_pull
     int32_t* d_hub_keys = nullptr;             // destinations privatised by k_pieces (HubTable)
@@ -2322,7 +2358,7 @@ struct ScopedTimer {
     Ctx* c; hipStream_t s; int cls; hipEvent_t a = nullptr;
     ScopedTimer(Ctx* c_, hipStream_t s_, int cls_) : c(c_), s(s_), cls(cls_) {
         c->t_launch[cls]++;
-        if (c->timing && (!c->timing_pull_only || cls == T_PULL)) { a = get_event(c); if (a) hipEventRecord(a, s); }
+        if (c->timing && (!c->timing_pull_only || cls == T_PULL || cls == T_PULL_A1 || cls == T_PULL_A2)) { a = get_event(c); if (a) hipEventRecord(a, s); }
     }
     ~ScopedTimer() {
         if (!a) return;
@@ -2466,17 +2502,35 @@ int push_launch(Ctx* c, int s, int src, unsigned grid, PushArgs a, int group, ui
     return EXC_OK;
 }
 
+// Sorted-pull launch for tile T (one of 512 / 1024 / 2048 / 4096).
+template <int NM, int S, bool W>
+void launch_sorted(Ctx* c, hipStream_t st, const PullArgs& pa) {
+    const SortedTiles tiles{c->sorted.start, c->sorted.row, c->sorted.dst, c->sorted.w};
+    const unsigned grid = (unsigned)(((long long)c->n + c->sorted_T - 1) / c->sorted_T);
+    if (c->sorted_T == 4096) hipLaunchKernelGGL((k_pull_sorted<NM, S, 4096, W>), dim3(grid), dim3(BLOCK), 0, st, pa, tiles, c->DM);
+    else if (c->sorted_T == 512) hipLaunchKernelGGL((k_pull_sorted<NM, S, 512, W>), dim3(grid), dim3(BLOCK), 0, st, pa, tiles, c->DM);
+    else if (c->sorted_T == 1024) hipLaunchKernelGGL((k_pull_sorted<NM, S, 1024, W>), dim3(grid), dim3(BLOCK), 0, st, pa, tiles, c->DM);
+    else hipLaunchKernelGGL((k_pull_sorted<NM, S, 2048, W>), dim3(grid), dim3(BLOCK), 0, st, pa, tiles, c->DM);
+}
+
+// A plain dense pass over measures [0, NM) of a context with state stride S (NM = nm_act:
+// every measure of the context, or the prefix still pending).
 template <int NM, int S>
 void launch_fold_pull(Ctx* c, hipStream_t st) {
+    static_assert(NM <= S, "measures fit the state row");
     // the bitmap costs a probe per in-edge and saves the row gathers of the sources that did
     // not push: it pays once the pass's demand (out-edges of pending sources) is well below m
     const bool live = c->nsrc && (c->pull_live == 1 ||
                                   (c->pull_live == 2 && (double)c->last_demand <= c->pull_live_frac * (double)c->m));
     unsigned long long* row = c->d_counters + (live ? C_DENSE_LIVE : C_DENSE) * C_FIELDS;
+    // every measure of the context (state stride S holds 1, 2 or 3 measures)
+    constexpr bool full = S == 1 || (S == 2 && NM == 2) || (S == 4 && NM == 3);
+    const int hub_field = full ? C_PULL_HUB_LIVE : NM == 1 ? C_PULL_HUB_LIVE_A1 : C_PULL_HUB_LIVE_A2;
     FoldArgs f{c->n, c->d_idx, c->d_p, c->d_rank, row};
     const int ntiles = (int)(((long long)c->n + BLOCK - 1) / BLOCK);
-    PullArgs pa{c->n, c->d_tidx, c->d_tadj, c->d_p, live ? c->d_pbits : nullptr, row + C_PULL_HUB_LIVE, c->d_twts,
+    PullArgs pa{c->n, c->d_tidx, c->d_tadj, c->d_p, live ? c->d_pbits : nullptr, row + hub_field, c->d_twts,
                 c->d_idx, c->d_cv0, c->nc, nullptr, nullptr, nullptr, c->d_rank, c->nsrc, c->d_counters + C_SPREAD};
+    pa.rows_field = full ? S_ROWS : NM == 1 ? S_ROWS_A1 : S_ROWS_A2;
     c->pull_passes_live += live;
     const bool fuse_snap = !c->d_trace;   // the trace counter stays with k_snapshot
     // fold the next pass here too while the demand is high (the next pass is then a dense pass
@@ -2484,16 +2538,29 @@ void launch_fold_pull(Ctx* c, hipStream_t st) {
     // push passes needs no unfolding)
     const bool fuse_fold = fuse_snap && c->fuse_fold && c->fold_ahead_ok && c->d_p2 &&
                            (double)c->last_demand >= std::max(0.1, 2.0 * c->pull_threshold) * (double)c->m;
+    if (c->folded && c->p_cols != NM) {   // the last pull folded this pass over more measures
+        ScopedTimer t(c, st, T_OTHER);
+        const unsigned g = (unsigned)std::max(1, std::min((c->nsrc + BLOCK - 1) / BLOCK, c->cus * 8));
+        if (c->p_cols == 3 && NM == 2) hipLaunchKernelGGL((k_prepack<3, 2>), dim3(g), dim3(BLOCK), 0, st, c->d_p, c->d_p2, c->nsrc);
+        else if (c->p_cols == 3) hipLaunchKernelGGL((k_prepack<3, 1>), dim3(g), dim3(BLOCK), 0, st, c->d_p, c->d_p2, c->nsrc);
+        else hipLaunchKernelGGL((k_prepack<2, 1>), dim3(g), dim3(BLOCK), 0, st, c->d_p, c->d_p2, c->nsrc);
+        std::swap(c->d_p, c->d_p2);
+        pa.p = c->d_p;
+        f.p = c->d_p;
+        c->p_cols = NM;
+    }
     if (fuse_snap) {
         clear_counts(c, st);
         pa.bitmap = c->d_bitmap;
         pa.cnt = c->d_cnt;
         pa.l1 = l1_slots(c);
         if (fuse_fold) pa.p_next = c->d_p2;
+        c->snap_nm = NM;
     }
     if (!c->folded) {   // else the last pull already folded this pass into p
         ScopedTimer t(c, st, T_FOLD);
         hipLaunchKernelGGL((k_fold<NM, S>), dim3(std::min(ntiles, c->cus * 8)), dim3(BLOCK), 0, st, f, c->DM);
+        c->fold_fixed_bytes += 4.0 * NM * ((double)c->n + (double)c->nsrc);
     }
     if (live) {
         ScopedTimer t(c, st, T_OTHER);
@@ -2506,38 +2573,38 @@ void launch_fold_pull(Ctx* c, hipStream_t st) {
         hipLaunchKernelGGL((k_pull_hub<NM, S>), dim3((unsigned)std::min(c->n_pull_hubs, c->cus * 8)), dim3(BLOCK), 0, st,
                            pa, c->d_pull_hubs, c->n_pull_hubs, c->DM);
     }
-    if (c->sorted_T) {
-        ScopedTimer t(c, st, T_PULL);
-        const SortedTiles tiles{c->sorted.start, c->sorted.row, c->sorted.dst, c->sorted.w};
-        const unsigned grid = (unsigned)(((long long)c->n + c->sorted_T - 1) / c->sorted_T);
-        if constexpr (NM == 1) {
-            if (c->d_twts) {
-                if (c->sorted_T == 4096) hipLaunchKernelGGL((k_pull_sorted<NM, S, 4096, true>), dim3(grid), dim3(BLOCK), 0, st, pa, tiles, c->DM);
-                else if (c->sorted_T == 512) hipLaunchKernelGGL((k_pull_sorted<NM, S, 512, true>), dim3(grid), dim3(BLOCK), 0, st, pa, tiles, c->DM);
-                else if (c->sorted_T == 1024) hipLaunchKernelGGL((k_pull_sorted<NM, S, 1024, true>), dim3(grid), dim3(BLOCK), 0, st, pa, tiles, c->DM);
-                else hipLaunchKernelGGL((k_pull_sorted<NM, S, 2048, true>), dim3(grid), dim3(BLOCK), 0, st, pa, tiles, c->DM);
-            }
-        }
-        if (!(NM == 1 && c->d_twts)) {
-            if (c->sorted_T == 4096) hipLaunchKernelGGL((k_pull_sorted<NM, S, 4096>), dim3(grid), dim3(BLOCK), 0, st, pa, tiles, c->DM);
-            else if (c->sorted_T == 512) hipLaunchKernelGGL((k_pull_sorted<NM, S, 512>), dim3(grid), dim3(BLOCK), 0, st, pa, tiles, c->DM);
-            else if (c->sorted_T == 1024) hipLaunchKernelGGL((k_pull_sorted<NM, S, 1024>), dim3(grid), dim3(BLOCK), 0, st, pa, tiles, c->DM);
-            else hipLaunchKernelGGL((k_pull_sorted<NM, S, 2048>), dim3(grid), dim3(BLOCK), 0, st, pa, tiles, c->DM);
-        }
-    } else {
-        ScopedTimer t(c, st, T_PULL);
+    const bool sorted = c->sorted_T && (NM == 1 || c->sorted_all);
+    {
+        ScopedTimer t(c, st, full ? T_PULL : NM == 1 ? T_PULL_A1 : T_PULL_A2);
         bool weighted = false;
         if constexpr (NM == 1) {
             if (c->d_twts) {   // weighted graphs (one measure)
-                hipLaunchKernelGGL((k_pull<NM, S, 8, true>), dim3(ntiles), dim3(BLOCK), 0, st, pa, c->DM);
+                if (sorted) launch_sorted<NM, S, true>(c, st, pa);
+                else hipLaunchKernelGGL((k_pull<NM, S, 8, true>), dim3(ntiles), dim3(BLOCK), 0, st, pa, c->DM);
                 weighted = true;
             }
         }
         if (weighted) {
-        } else if (c->pull_seq == 4) hipLaunchKernelGGL((k_pull<NM, S, 4>), dim3(ntiles), dim3(BLOCK), 0, st, pa, c->DM);
+        } else if (sorted) launch_sorted<NM, S, false>(c, st, pa);
+        else if constexpr (!full) hipLaunchKernelGGL((k_pull<NM, S, 8>), dim3(ntiles), dim3(BLOCK), 0, st, pa, c->DM);
+        else if (c->pull_seq == 4) hipLaunchKernelGGL((k_pull<NM, S, 4>), dim3(ntiles), dim3(BLOCK), 0, st, pa, c->DM);
         else if (c->pull_seq == 32) hipLaunchKernelGGL((k_pull<NM, S, 32>), dim3(ntiles), dim3(BLOCK), 0, st, pa, c->DM);
         else if (c->pull_seq == 16) hipLaunchKernelGGL((k_pull<NM, S, 16>), dim3(ntiles), dim3(BLOCK), 0, st, pa, c->DM);
         else hipLaunchKernelGGL((k_pull<NM, S, 8>), dim3(ntiles), dim3(BLOCK), 0, st, pa, c->DM);
+    }
+    {   // algorithmic bytes known at launch: every non-hub in-edge's entry (the sorted pull: row
+        // id and 16-bit destination offset) and coefficient, every destination's offsets and r
+        // row (read + write), the live bitmap; k_pull_hub: its entries, destinations, bitmap,
+        // and without the bitmap every entry's row
+        const double me = (double)(c->m - c->pull_hub_edges), nv = (double)(c->n - c->pull_hub_vertices);
+        const double bm = live ? 8.0 * ((c->nsrc + 63) / 64) : 0.0;
+        const double entry = (sorted ? 6.0 : 4.0) + (c->d_twts ? 4.0 : 0.0);
+        c->pull_fixed_bytes[NM - 1] += me * entry + nv * (8.0 + 8.0 * NM) + bm;
+        c->hub_fixed_bytes[NM - 1] += (double)c->pull_hub_edges * (c->d_twts ? 8.0 : 4.0) +
+                                      (double)c->pull_hub_vertices * (8.0 + 8.0 * NM) + bm +
+                                      (live ? 0.0 : 4.0 * NM * (double)c->pull_hub_edges);
+        c->pull_launches_w[NM - 1]++;
+        c->pull_tile_w[NM - 1] = sorted ? c->sorted_T : 0;
     }
     c->snap_ready = fuse_snap;
     if (fuse_snap) {   // the next snapshot's counts go to the host behind the pull
@@ -2546,7 +2613,10 @@ void launch_fold_pull(Ctx* c, hipStream_t st) {
                                             hipMemcpyDeviceToHost, st) == hipSuccess;
     }
     c->folded = fuse_fold;
-    if (fuse_fold) std::swap(c->d_p, c->d_p2);   // the rows just written serve the next pass
+    if (fuse_fold) {   // the rows just written serve the next pass
+        std::swap(c->d_p, c->d_p2);
+        c->p_cols = NM;
+    }
 }
 
 constexpr int BQ_CAND[2] = {8, 16};
@@ -2644,8 +2714,11 @@ int fold_pull(Ctx* c) {
     hipStream_t st = c->st[0];
     if (c->K) launch_bfold_pull(c, st);
     else if (c->nm == 1) launch_fold_pull<1, 1>(c, st);
-    else if (c->nm == 2) launch_fold_pull<2, 2>(c, st);
-    else launch_fold_pull<3, 4>(c, st);
+    else if (c->nm == 2 && c->nm_act == 2) launch_fold_pull<2, 2>(c, st);
+    else if (c->nm == 2) launch_fold_pull<1, 2>(c, st);
+    else if (c->nm_act == 3) launch_fold_pull<3, 4>(c, st);
+    else if (c->nm_act == 2) launch_fold_pull<2, 4>(c, st);
+    else launch_fold_pull<1, 4>(c, st);
     c->pull_passes++;
     HIPCHK(hipGetLastError());
     return EXC_OK;
@@ -2867,6 +2940,7 @@ int exc_create(const exc_config* cfg, const exc_measures* M, void** out) {
         c->active = all_columns(c->K);
     }
     c->M = *M;
+    c->nm_act = c->nm;
     c->zc = cfg->zero_copy != 0;
     c->timing = cfg->timing != 0;
     c->timing_pull_only = cfg->timing == 2;
@@ -2991,6 +3065,7 @@ int exc_init_measure(void* p, int i, float beta, float lambda) {
     c->snap_ready = false;   // the state changes: a fused snapshot no longer holds
     c->l1_valid = false;
     c->folded = false;       // the state is replaced: a folded pass no longer applies
+    c->nm_act = c->nm;       // ... and every measure may be pending again
     if (c->K) c->active = all_columns(c->K);
     if (c->K) {   // batched: every column starts at x = r = 0 (exc_batch_seed adds the teleport mass)
         const size_t bytes = (size_t)c->n * c->S * sizeof(float);
@@ -3014,6 +3089,7 @@ int exc_set_state(void* p, int i, const float* h_x, const float* h_r) {
     c->snap_ready = false;   // the state changes: a fused snapshot no longer holds
     c->l1_valid = false;
     c->folded = false;       // the state is replaced: a folded pass no longer applies
+    c->nm_act = c->nm;       // ... and every measure may be pending again
     if (c->K) c->active = all_columns(c->K);
     float* tmp;
     const size_t bytes = (size_t)c->n * sizeof(float);
@@ -3058,6 +3134,7 @@ int exc_state_checkpoint(void* p, int restore) {
     const size_t bytes = (size_t)c->n * c->S * sizeof(float);
     if (restore) {
         c->folded = false;
+        c->nm_act = c->nm;
         if (!c->d_x0) return fail(EXC_ESTATE, "exc_state_checkpoint: nothing saved");
         HIPCHK(hipMemcpyAsync(c->d_x, c->d_x0, bytes, hipMemcpyDeviceToDevice, c->st[0]));
         HIPCHK(hipMemcpyAsync(c->d_r, c->d_r0, bytes, hipMemcpyDeviceToDevice, c->st[0]));
@@ -3104,10 +3181,13 @@ void parse_counts(Ctx* c, uint64_t* h_cnt, uint64_t* h_hotpend) {
             memcpy(&b, reinterpret_cast<const char*>(l1 + q * L1_STRIDE + LINF_AT) + 4 * k, 4);
             m = std::max(m, b);
         }
-        c->l1_last[k] = k < c->nm ? t : 0.0;
         float f;
         memcpy(&f, &m, 4);
-        c->linf_last[k] = k < c->nm ? f : 0.f;
+        if (k >= c->nm) { c->l1_last[k] = 0.0; c->linf_last[k] = 0.f; }
+        else if (k < c->snap_nm) { c->l1_last[k] = t; c->linf_last[k] = f; }
+        // else: a dropped measure, not summed by the pull that took this snapshot; its state
+        // has not changed since the snapshot that found it converged
+
     }
     c->l1_valid = c->K == 0;
     c->last_demand = 0;
@@ -3158,6 +3238,7 @@ int exc_snapshot(void* p, int hot_skip, uint64_t* h_cnt, uint64_t* h_hotpend) {
         c->snap_bytes += 4.0 * c->K * c->n + 4.0 * (((double)c->n + 31) / 32);
     } else {
         ScopedTimer t(c, st, T_SNAP);
+        c->snap_nm = c->nm;
         launch_snapshot_plain(a, c->nm, c->cus, st);
         c->snap_bytes += 4.0 * c->nm * c->n + 4.0 * (((double)c->n + 31) / 32);
     }
@@ -3428,6 +3509,7 @@ int exc_batch_seed(void* p, int col, const int32_t* seeds, int count, float valu
     c->snap_ready = false;   // the state changes: a fused snapshot no longer holds
     c->l1_valid = false;
     c->folded = false;       // the state is replaced: a folded pass no longer applies
+    c->nm_act = c->nm;       // ... and every measure may be pending again
     if (c->K) c->active = all_columns(c->K);
     int32_t* d;
     HIPCHK(hipMalloc(&d, (size_t)count * sizeof(int32_t)));
@@ -3469,6 +3551,7 @@ int exc_set_edge_weights(void* p, const float* h_w, uint64_t cnt) {
     c->snap_ready = false;   // the state changes: a fused snapshot no longer holds
     c->l1_valid = false;
     c->folded = false;       // the state is replaced: a folded pass no longer applies
+    c->nm_act = c->nm;       // ... and every measure may be pending again
     if (!c->d_wts) HIPCHK(hipMalloc(&c->d_wts, (c->m ? c->m : 1) * sizeof(float)));
     if (c->m) HIPCHK(hipMemcpy(c->d_wts, h_w, c->m * sizeof(float), hipMemcpyHostToDevice));
     return EXC_OK;
@@ -3539,14 +3622,13 @@ int exc_get_stats(void* p, exc_stats* o) {
         const double pb = c->K && c->brows16 ? 2.0 : 4.0;   // bytes per column of a p row
         const double P = (double)c->pull_passes, PL = (double)c->pull_passes_live;
         o->fold_ms = c->t_ms[T_FOLD];
-        o->pull_ms = c->t_ms[T_PULL];
+        o->pull_ms = c->t_ms[T_PULL] + c->t_ms[T_PULL_A1] + c->t_ms[T_PULL_A2];
         o->pull_hub_ms = c->t_ms[T_PULL_HUB];
         o->pull_passes = c->pull_passes;
-        o->pull_tile = c->sorted_T;
+        o->pull_tile = c->pull_tile_w[c->nm - 1];
         // fold: read r for every vertex, write p for every source; offsets, x read + write for
         // folded ones
-        o->fold_bytes = P * c->n * 4.0 * nm + P * c->nsrc * 4.0 * nm + 16.0 * (r4[C_VERT] + r5[C_VERT]) +
-                        8.0 * (r4[C_VERT_NM] + r5[C_VERT_NM]);
+        o->fold_bytes = c->fold_fixed_bytes + 16.0 * (r4[C_VERT] + r5[C_VERT]) + 8.0 * (r4[C_VERT_NM] + r5[C_VERT_NM]);
         if (c->K)   // batched: the bitmap, then per pending vertex its offsets, p-row number, r row
                     // and p row, per folded column r written and x read + written; per pass the
                     // live-row bitmap from the row->vertex map
@@ -3561,18 +3643,34 @@ int exc_get_stats(void* p, exc_stats* o) {
         // k_pull_hub's own count.  In-lists above the hub threshold go to the hub kernel.
         const double me = (double)(c->m - c->pull_hub_edges), nv = (double)(c->n - c->pull_hub_vertices);
         const double bm = 8.0 * ((c->nsrc + 63) / 64);
-        unsigned long long sp[5] = {0, 0, 0, 0, 0};
+        unsigned long long sp[S_ROWS_A2 + 1] = {};
         for (int q = 0; q < SPREAD_SLOTS; q++)
-            for (int f = 0; f < 5; f++) sp[f] += h[C_SPREAD + q * SPREAD_STRIDE + f];
+            for (int f = 0; f <= S_ROWS_A2; f++) sp[f] += h[C_SPREAD + q * SPREAD_STRIDE + f];
         const double hub_rows = (P - PL) * c->pull_hub_edges + (double)r5[C_PULL_HUB_LIVE];
         // k_pull counts the rows it reads (batched passes count them through the fold)
         const double rows = c->K ? std::max(0.0, (double)(r5[C_EDGES] + sp[S_EDGES]) - (double)r5[C_PULL_HUB_LIVE])
                                  : (double)sp[S_ROWS];
-        const double entry_bytes = (c->sorted_T ? 6.0 : 4.0) + (c->d_twts ? 4.0 : 0.0);   // + a weight
-        o->pull_bytes = P * (me * entry_bytes + nv * (8.0 + 8.0 * nm)) + PL * bm + pb * nm * rows;
-        o->pull_hub_bytes = P * ((double)c->pull_hub_edges * (c->d_twts ? 8.0 : 4.0) +
-                                 (double)c->pull_hub_vertices * (8.0 + 8.0 * nm)) +
-                            PL * bm + pb * nm * hub_rows;
+        if (c->K) {
+            o->pull_bytes = P * (me * 4.0 + nv * (8.0 + 8.0 * nm)) + PL * bm + pb * nm * rows;
+            o->pull_hub_bytes = P * ((double)c->pull_hub_edges * 4.0 + (double)c->pull_hub_vertices * (8.0 + 8.0 * nm)) +
+                                PL * bm + pb * nm * hub_rows;
+            o->pull_w_ms[0] = o->pull_ms;
+            o->pull_w_launches[0] = c->pull_passes;
+            o->pull_w_bytes[0] = o->pull_bytes;
+        } else {   // plain: per width (measures the pass updated), the rows counted by the kernels
+            const int srow[3] = {S_ROWS_A1, S_ROWS_A2, S_ROWS};
+            const int hrow[3] = {C_PULL_HUB_LIVE_A1, C_PULL_HUB_LIVE_A2, C_PULL_HUB_LIVE};
+            const int tcls[3] = {T_PULL_A1, T_PULL_A2, T_PULL};
+            for (int w = 1; w <= c->nm; w++) {
+                const int i = w == c->nm ? 2 : w - 1;   // the context's full width counts in S_ROWS
+                o->pull_w_ms[w - 1] = c->t_ms[tcls[i]];
+                o->pull_w_launches[w - 1] = c->pull_launches_w[w - 1];
+                o->pull_w_tile[w - 1] = c->pull_tile_w[w - 1];
+                o->pull_w_bytes[w - 1] = c->pull_fixed_bytes[w - 1] + 4.0 * w * (double)sp[srow[i]];
+                o->pull_bytes += o->pull_w_bytes[w - 1];
+                o->pull_hub_bytes += c->hub_fixed_bytes[w - 1] + 4.0 * w * (double)r5[hrow[i]];
+            }
+        }
         for (const unsigned long long* r : {r4, r5}) {
             o->pushed_vertices += r[C_VERT];
             o->pushed_vertex_measures += r[C_VERT_NM];
@@ -3720,18 +3818,22 @@ int exc_enable_pull(void* p, double threshold) {
     e = hipDeviceSynchronize();
     if (e != hipSuccess) return fail_free(e);
     (void)hipFree(cnt);
-    // one-measure contexts: the non-hub in-edges again, per tile of destinations sorted by
-    // source row (k_pull_sorted).  With one measure it cuts the launch by a sixth at R-MAT
-    // scale 24 (its L1-to-L2 requests drop by a third); with three, its three LDS atomics per
-    // entry cost more than that saves, so k_pull stays.  EXC_PULL_TILE = 512 / 1024 / 2048 /
-    // 4096 picks the tile (any measure count), 0 keeps k_pull.
+    // the non-hub in-edges again, per tile of destinations sorted by source row
+    // (k_pull_sorted), for the dense passes over one measure (a one-measure context, or the
+    // first measure once the others are dropped).  With one measure it cuts the launch by a
+    // sixth at R-MAT scale 24 (its L1-to-L2 requests drop by a third); over two or three
+    // measures its LDS atomics per entry cost more than that saves, so k_pull runs those.
+    // EXC_PULL_TILE = 512 / 1024 / 2048 / 4096 picks the tile and the sorted pull at every
+    // width, 0 keeps k_pull.
     if (!c->K) {
         // the largest tile that still gives every CU four blocks; small graphs keep k_pull
         int T = 0;
-        if (c->nm == 1)
-            for (int t : {2048, 1024, 512})
-                if ((long long)c->n / t >= 4ll * c->cus) { T = t; break; }
-        if (const char* ev = getenv("EXC_PULL_TILE")) T = atoi(ev);
+        for (int t : {2048, 1024, 512})
+            if ((long long)c->n / t >= 4ll * c->cus) { T = t; break; }
+        if (const char* ev = getenv("EXC_PULL_TILE")) {
+            T = atoi(ev);
+            c->sorted_all = T != 0;
+        }
         if ((T == 512 || T == 1024 || T == 2048 || T == 4096) && off[n] < (1ull << 31) && c->nsrc > 0) {
             int bits = 1;
             while (bits < 31 && (1ll << bits) < (long long)c->nsrc) bits++;
@@ -3804,6 +3906,25 @@ int exc_residual_max(void* p, float* out) {
     return EXC_OK;
 }
 
+int exc_set_active_measures(void* p, int count) {
+    Ctx* c = as_ctx(p);
+    if (!c) return fail(EXC_EINVAL, "exc_set_active_measures: null context");
+    if (c->K) return fail(EXC_ESTATE, "exc_set_active_measures: batched contexts drop columns (exc_batch_set_active)");
+    if (count < 1 || count > c->nm) return fail(EXC_EINVAL, "exc_set_active_measures: count must be 1..%d", c->nm);
+    if (count > c->nm_act && c->folded)
+        return fail(EXC_ESTATE, "exc_set_active_measures: the last dense pass folded the next one over %d measures",
+                    c->nm_act);
+    if (count < c->nm_act) {
+        if (!c->l1_valid) return fail(EXC_ESTATE, "exc_set_active_measures: no snapshot yet");
+        for (int k = count; k < c->nm; k++)
+            if (c->linf_last[k] > c->M.eps[k])
+                return fail(EXC_ESTATE, "exc_set_active_measures: measure slot %d still has |r| %g > eps %g", k,
+                            (double)c->linf_last[k], (double)c->M.eps[k]);
+    }
+    c->nm_act = count;
+    return EXC_OK;
+}
+
 int exc_fold_ahead(void* p, int allow) {
     Ctx* c = as_ctx(p);
     if (!c) return fail(EXC_EINVAL, "exc_fold_ahead: null context");
@@ -3830,6 +3951,11 @@ int exc_reset_stats(void* p) {
     c->snap_bytes = 0;
     c->pull_passes = 0;
     c->pull_passes_live = 0;
+    c->fold_fixed_bytes = 0;
+    for (int w = 0; w < 3; w++) {
+        c->pull_fixed_bytes[w] = c->hub_fixed_bytes[w] = 0;
+        c->pull_launches_w[w] = 0;
+    }
     return EXC_OK;
 }
 

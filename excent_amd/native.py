@@ -69,7 +69,8 @@ class Stats(ctypes.Structure):
                 ("fold_ms", c_double), ("pull_ms", c_double), ("pull_hub_ms", c_double), ("pull_passes", c_u64),
                 ("fold_bytes", c_double), ("pull_bytes", c_double), ("pull_hub_bytes", c_double),
                 ("pull_rows_in_flight", c_int), ("pull_tune_ns_per_edge", c_double * 2),
-                ("pull_tile", c_int)]
+                ("pull_tile", c_int), ("pull_w_ms", c_double * 3), ("pull_w_launches", c_u64 * 3),
+                ("pull_w_bytes", c_double * 3), ("pull_w_tile", c_int * 3)]
 
 
 class RunStats(ctypes.Structure):
@@ -149,6 +150,7 @@ HIP_SIGNATURES = {
     "exc_residual_l1": (c_int, [c_void_p, P(c_double)]),
     "exc_residual_max": (c_int, [c_void_p, P(c_float)]),
     "exc_fold_ahead": (c_int, [c_void_p, c_int]),
+    "exc_set_active_measures": (c_int, [c_void_p, c_int]),
     "exc_set_hub_destinations": (c_int, [c_void_p, c_void_p, c_int]),
     "exc_snapshot_noalloc": (c_int, [c_int, c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_int,
                                      c_void_p, c_void_p, c_void_p, c_void_p]),
@@ -391,6 +393,10 @@ class Context:
 
     def fold_ahead(self, allow):
         check(F["exc_fold_ahead"](self.ctx, 1 if allow else 0), "fold_ahead")
+
+    def set_active_measures(self, count):
+        """Dense passes over the first `count` measure slots only (the others converged)."""
+        check(F["exc_set_active_measures"](self.ctx, count), "set_active_measures")
 
     def dense_required(self):
         rc = F["exc_dense_required"](self.ctx)

@@ -98,6 +98,14 @@ typedef struct exc_stats {
     /* plain dense pull: destinations per tile of the source-sorted pull (k_pull_sorted), or 0
      * when the pull runs over the in-lists in vertex order (k_pull) */
     int pull_tile;
+    /* the dense pull split by width: entry w-1 for passes over the first w measure slots
+     * (exc_set_active_measures; entry nm-1 holds the passes over every slot, and a batched
+     * context counts all its passes in entry 0): time, launches, algorithmic bytes, and the
+     * sorted tile it ran with (0: k_pull).  pull_ms and pull_bytes are their sums. */
+    double pull_w_ms[3];
+    uint64_t pull_w_launches[3];
+    double pull_w_bytes[3];
+    int pull_w_tile[3];
 } exc_stats;
 
 const char* exc_last_error(void);
@@ -222,6 +230,16 @@ int exc_dense_required(void* ctx);
  * limit, or a stop test it will apply to the next snapshot -- turns it off, so the state it
  * stops on holds every delta it has folded (x + r = b + W x).  No reference counterpart. */
 int exc_fold_ahead(void* ctx, int allow);
+/* Dense passes over the first `count` measure slots only (1 <= count <= nm; nm, every slot,
+ * after exc_create and after any call that replaces the state).  Lowering the count needs a
+ * snapshot in which every slot from `count` on has no pending vertex (largest |r| <= eps;
+ * EXC_ESTATE otherwise): a slot's residual changes only through its own pushes, so such a
+ * slot stays converged and its values and snapshot sums stay as they were.  Passes then
+ * gather rows of `count` weights instead of nm (one measure: the source-sorted pull).
+ * Raising it is refused while the last dense pass has folded the next one.  Push passes and
+ * the standalone snapshot always cover every slot.  No reference counterpart (the
+ * reference pushes every measure of a pending vertex; a converged measure has none). */
+int exc_set_active_measures(void* ctx, int count);
 
 /* Push a resident chunk run [ch.v0, ch.v1) from the arena on the resident stream
  * (launch_chunk with dst = resident[c], src/excent.cu:634-652, 684-688).  use_list:
