@@ -590,7 +590,7 @@ def bench_plain(args, nat, np, work, graph, scale, measure, steps, warmup, env, 
     sargs = [graph, "--measure", measure, "--cache", "chunks", "--preload", "--stream-ratio", "1",
              timing, "--push-kernel", args.kernel, "--pull", args.pull, "--hub-lds", args.hub_lds, "--log", excent_log,
              "--dump", os.path.join(work, "x"), "--stop", stop_rule(args, measure),
-             "--drop-measures", args.drop_measures,
+             "--drop-measures", args.drop_measures, "--burst", args.burst,
              # a run whose stop rule is never met ends here and is reported as not converged
              "--max-passes", args.max_passes or 2000] + measure_args(measure, env)
     t0 = time.time()
@@ -699,7 +699,8 @@ def sub_s16(args, nat, np, work):
     n, m = graph_size(nat, graph)
     rec.update({"metric": "converged GTEPS + time-to-eps", "higher_is_better": True, "dtype": "f32",
                 "data": "synthetic", "config": {"workload": "R-MAT scale 16 ef 16 --measure pr, eps 0.01, whole "
-                                                            "graph in HBM, converged to eps", "n": n, "m": m}})
+                                                            "graph in HBM, converged to eps", "n": n, "m": m,
+                                                "burst": args.burst or "auto"}})
     if not args.no_cpu:
         idx, adj = orc.egr_load(graph)
         st = plain_state(orc, np, idx, adj, "pr", {})
@@ -762,6 +763,8 @@ def main():
     ap.add_argument("--hub-lds", type=int, default=1024, help="destinations privatised in LDS by the hub-piece push")
     ap.add_argument("--drop-measures", default="on", choices=["on", "off"],
                     help="dense passes leave out the measures with nothing pending")
+    ap.add_argument("--burst", type=int, default=0,
+                    help="passes per host synchronisation on a cached graph (0: automatic, 1: every pass)")
     ap.add_argument("--cpu-seconds", type=float, default=30.0,
                     help="CPU baseline: time limit of the restatement's run (it stops earlier when it converges)")
     ap.add_argument("--no-cpu", action="store_true")
@@ -828,7 +831,7 @@ def main():
                                f"whole graph in HBM, converged to eps", "n": n_v, "m": m_e,
                    "lambda1": env["LAMBDA"], "alpha": env["ALPHA"], "eps_pr": 0.01, "eps_katz": 1e-6,
                    "eps_ev": 1e-6, "stop": stop_rule(args, args.measure), "push_kernel": args.kernel, "pull": args.pull,
-                   "drop_measures": args.drop_measures,
+                   "drop_measures": args.drop_measures, "burst": args.burst or "auto",
                    "parallelism": "single GPU"},
     }
     out.update({k: v for k, v in rec.items() if k not in ("value", "unit", "steps", "warmup", "ms_per_step")})

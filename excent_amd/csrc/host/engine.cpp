@@ -85,6 +85,7 @@ struct Options {
     std::string stop = "vertex";
     int io_threads = 0;           // host threads for reads, gathers and checks (0: all)
     int hub_lds = 1024;           // destinations privatised in LDS by the hub-piece push (0: none)
+    int burst = 0;                // passes per host synchronisation over a cached graph (0: automatic)
 };
 
 const char* USAGE =
@@ -153,6 +154,7 @@ Options parse(int argc, const char* const* argv) {
         else if (a == "--io-threads") o.io_threads = atoi(next());
         else if (a == "--hub-lds") o.hub_lds = atoi(next());
         else if (a == "--drop-measures") o.drop_measures = next();
+        else if (a == "--burst") o.burst = atoi(next());
         else die("unknown option " + a);
     }
     return o;
@@ -266,6 +268,7 @@ private:
     int stop_reason_ = 0;               // 0 nothing pending, 1 global L1 bound, 2 pass limit
     std::vector<char> global_l1_;       // per measure slot: the global L1 rule may end it (--stop)
     int act_ = 1;                       // measure slots the dense passes update (exc_set_active_measures)
+    int burst_ = 1;                     // passes per burst (exc_run_passes); 1: every pass synchronises
     float linf_[EXC_MAXM] = {0, 0, 0};  // largest |r| per measure, last snapshot
 
     void init_state();
@@ -333,6 +336,7 @@ void Engine::setup() {
     if (o.hot_order != "sweep" && o.hot_order != "ranges") die("--hot-order must be sweep or ranges");
     if (o.hub_lds < 0 || o.hub_lds > 1024) die("--hub-lds must be 0..1024");
     if (o.drop_measures != "on" && o.drop_measures != "off") die("--drop-measures must be on or off");
+    if (o.burst < 0 || o.burst > EXC_BURST_MAX) die("--burst must be 0 (automatic) or 1.." + std::to_string(EXC_BURST_MAX));
     global_l1_.assign(kinds_.size(), 0);
     if (o.stop == "global") {
         global_l1_.assign(kinds_.size(), 1);
@@ -549,6 +553,8 @@ void Engine::setup() {
         ck(exc_set_hub_destinations(ctx, hubs.data(), (int)hubs.size()), "hub destinations");
     }
     maybe_enable_pull();
+    // bursts pay where a pass is short next to a host round trip: graphs of up to 2^25 edges
+    burst_ = o.burst ? o.burst : (m_ <= (1ull << 25) ? 16 : 1);
     ck(exc_pass_sync(ctx, nullptr), "setup sync");
     ck(exc_reset_stats(ctx), "reset stats");
     checkpoint_ = session_ && exc_state_checkpoint(ctx, 0) == EXC_OK;
@@ -728,6 +734,27 @@ void Engine::run_passes() {
         return pick_group(k ? (double)e / k : 1.0, gcap);
     };
     std::vector<int8_t> mode(nc);
+    // the report line of one pass, from the snapshot it started from (src/excent.cu:754-759)
+    auto pass_line = [&](int pass, uint64_t pend_v, uint64_t demand, int nres, int nstream, int ngather, uint64_t hotpend,
+                         long maj, long mn, uint64_t pages, uint64_t runs, bool have, const double* l1, const float* linf) {
+        fprintf(out, "pass %3d: pending %llu v, demand ratio %.4f | resident %d stream %d gather %d | cached high-degree pending %llu"
+                     " | faults major %ld minor %ld | gather pages %llu runs %llu",
+                pass, (unsigned long long)pend_v, m ? (double)demand / m : 0.0, nres, nstream, ngather,
+                (unsigned long long)hotpend, maj, mn, (unsigned long long)pages, (unsigned long long)runs);
+        if (batched()) fprintf(out, " | active columns %d", active_cols_);
+        if (have) {
+            fprintf(out, " | residual l1");
+            for (int i = 0; i < M_.nm; i++) fprintf(out, " %s %.4e", MNAME[kinds_[i]], l1[i]);
+            fprintf(out, " | max |r|");
+            for (int i = 0; i < M_.nm; i++) fprintf(out, " %s %.4e", MNAME[kinds_[i]], (double)linf[i]);
+        }
+        fprintf(out, "\n");
+        fflush(out);   // a long run shows its progress pass by pass
+    };
+    bool whole_cached = !hot_mode_ && n_resident_ == nc;
+    std::vector<uint64_t> b_cnt;
+    std::vector<double> b_l1;
+    std::vector<float> b_linf;
     for (; pass_ < o.max_passes; pass_++) {
         auto a = Clock::now();
         const Faults f0 = faults();
@@ -774,6 +801,44 @@ void Engine::run_passes() {
         if (global_met) { stop_reason_ = 1; break; }
         // a dense pass folds the pass after it only when the loop will go on from its state
         if (pull_on_) ck(exc_fold_ahead(ctx, !fold_ahead_off && pass_ + 1 < o.max_passes), "fold ahead");
+
+        // A burst: the next burst_ passes over the whole cached graph enqueued together with one
+        // host synchronisation (exc_run_passes).  The per-vertex rule ends the run at the first
+        // snapshot with nothing pending; the passes after it changed nothing.
+        if (whole_cached) for (int c = 0; c < nc && whole_cached; c++) whole_cached = loaded_[c] != 0;
+        if (burst_ > 1 && whole_cached && o.trace_file.empty() && !stop_global && !batched() &&
+            pass_ + burst_ < o.max_passes) {
+            uint64_t demand = 0;
+            for (int c = 0; c < nc; c++) demand += h_cnt[2 * c + 1];
+            const bool dense = pull_on_ && ((double)demand >= pull_thr_ * (double)m || exc_dense_required(ctx) == 1);
+            const int B = burst_, nm = M_.nm;
+            b_cnt.resize((size_t)B * 2 * nc);
+            b_l1.resize((size_t)B * nm);
+            b_linf.resize((size_t)B * nm);
+            ck(exc_run_passes(ctx, B, dense ? 1 : 0, b_cnt.data(), b_l1.data(), b_linf.data()), "burst");
+            maybe_enable_pull();
+            const Faults f1 = faults();
+            tot_maj_ += f1.maj - f0.maj;
+            tot_min_ += f1.min - f0.min;
+            const int first = pass_;
+            bool converged = false;
+            for (int i = 0; i < B && !converged; i++) {
+                const uint64_t* cn = i == 0 ? h_cnt.data() : &b_cnt[(size_t)(i - 1) * 2 * nc];
+                uint64_t pv = 0, dm = 0;
+                int nr = 0;
+                for (int c = 0; c < nc; c++) { pv += cn[2 * c]; dm += cn[2 * c + 1]; nr += cn[2 * c] ? 1 : 0; }
+                pass_line(first + i, pv, dm, nr, 0, 0, 0, i == 0 ? f1.maj - f0.maj : 0, i == 0 ? f1.min - f0.min : 0, 0, 0,
+                          true, i == 0 ? l1_ : &b_l1[(size_t)(i - 1) * nm], i == 0 ? linf_ : &b_linf[(size_t)(i - 1) * nm]);
+                n_launch_++;
+                if (dense) n_range_ += nr; else n_list_ += nr;
+                uint64_t after = 0;
+                for (int c = 0; c < nc; c++) after += b_cnt[(size_t)i * 2 * nc + 2 * c];
+                if (after == 0) { converged = true; pass_ = first + i + 1; }
+            }
+            if (converged) { stop_reason_ = 0; break; }
+            pass_ = first + B - 1;   // the loop statement counts the last one
+            continue;
+        }
 
         int nres = 0, nstream = 0, ngather = 0;
         if (hot_skip && h_hotpend && o.hot_when != "end") launch_hot(0);
@@ -850,20 +915,8 @@ void Engine::run_passes() {
         tot_min_ += f1.min - f0.min;
         tot_pages_ += pass_pages;
         tot_runs_ += pass_runs;
-        fprintf(out, "pass %3d: pending %llu v, demand ratio %.4f | resident %d stream %d gather %d | cached high-degree pending %llu"
-                     " | faults major %ld minor %ld | gather pages %llu runs %llu",
-                pass_, (unsigned long long)po.pend_v, m ? (double)po.demand / m : 0.0, nres, nstream, ngather,
-                (unsigned long long)h_hotpend, f1.maj - f0.maj, f1.min - f0.min, (unsigned long long)pass_pages,
-                (unsigned long long)pass_runs);
-        if (batched()) fprintf(out, " | active columns %d", active_cols_);
-        if (have_l1_) {
-            fprintf(out, " | residual l1");
-            for (int i = 0; i < M_.nm; i++) fprintf(out, " %s %.4e", MNAME[kinds_[i]], l1_[i]);
-            fprintf(out, " | max |r|");
-            for (int i = 0; i < M_.nm; i++) fprintf(out, " %s %.4e", MNAME[kinds_[i]], (double)linf_[i]);
-        }
-        fprintf(out, "\n");
-        fflush(out);   // a long run shows its progress pass by pass
+        pass_line(pass_, po.pend_v, po.demand, nres, nstream, ngather, h_hotpend, f1.maj - f0.maj, f1.min - f0.min,
+                  pass_pages, pass_runs, have_l1_, l1_, linf_);
     }
     T_iter_ = Clock::now();
     T_setup_ = T_begin;

@@ -2203,6 +2203,12 @@ struct Ctx {
     uint16_t* d_trace = nullptr;
     unsigned long long* d_counters = nullptr;  // C_NCOUNTERS
     unsigned long long* h_small = nullptr;     // pinned: cnt + hotpend + residual slots
+    // exc_run_passes: one count region (counts + residual slots, cnt_words) per pass of a
+    // burst, on the device and pinned on the host
+    unsigned long long* d_burst = nullptr;
+    unsigned long long* h_burst = nullptr;
+    bool in_burst = false;                     // its passes are being enqueued: the regions are
+                                               // cleared and copied once for the whole burst
     double l1_last[EXC_MAXM] = {};             // FP64 residual sums of the last snapshot
     float linf_last[EXC_MAXM] = {};            // and the largest |r| per measure
     int snap_nm = 0;                           // measures the last snapshot computed (the others
@@ -2317,6 +2323,7 @@ size_t l1_offset(const Ctx* c) { return ((2 * (size_t)c->nc + 1 + L1_STRIDE - 1)
 size_t cnt_words(const Ctx* c) { return l1_offset(c) + (size_t)L1_SLOTS * L1_STRIDE; }
 double* l1_slots(const Ctx* c) { return reinterpret_cast<double*>(c->d_cnt + l1_offset(c)); }
 void clear_counts(const Ctx* c, hipStream_t st) {
+    if (c->in_burst) return;   // exc_run_passes cleared every pass's region at its start
     const int w = (int)cnt_words(c);
     hipLaunchKernelGGL(k_zero_u64, dim3((w + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, st, c->d_cnt, w);
 }
@@ -2609,8 +2616,9 @@ void launch_fold_pull(Ctx* c, hipStream_t st) {
     c->snap_ready = fuse_snap;
     if (fuse_snap) {   // the next snapshot's counts go to the host behind the pull
         c->snap_bytes += 4.0 * (((double)c->n + 31) / 32);
-        c->counts_enqueued = hipMemcpyAsync(c->h_small, c->d_cnt, cnt_words(c) * sizeof(unsigned long long),
-                                            hipMemcpyDeviceToHost, st) == hipSuccess;
+        if (!c->in_burst)
+            c->counts_enqueued = hipMemcpyAsync(c->h_small, c->d_cnt, cnt_words(c) * sizeof(unsigned long long),
+                                                hipMemcpyDeviceToHost, st) == hipSuccess;
     }
     c->folded = fuse_fold;
     if (fuse_fold) {   // the rows just written serve the next pass
@@ -3046,6 +3054,8 @@ int exc_destroy(void* p) {
                         c->d_arena, c->d_hids, c->d_hoff, c->d_hedges, c->d_hot, c->d_hlist, c->d_hcount};
     for (void* q : dev_ptrs) if (q) hipFree(q);
     if (c->h_small) hipHostFree(c->h_small);
+    if (c->d_burst) hipFree(c->d_burst);
+    if (c->h_burst) hipHostFree(c->h_burst);
     delete c;
     return EXC_OK;
 }
@@ -3194,22 +3204,9 @@ void parse_counts(Ctx* c, uint64_t* h_cnt, uint64_t* h_hotpend) {
     for (int i = 0; i < c->nc; i++) c->last_demand += c->h_small[2 * i + 1];
 }
 
-int exc_snapshot(void* p, int hot_skip, uint64_t* h_cnt, uint64_t* h_hotpend) {
-    Ctx* c = as_ctx(p);
-    if (!c || !h_cnt) return fail(EXC_EINVAL, "exc_snapshot: bad arguments");
-    HIPCHK(hipSetDevice(c->dev));
-    hipStream_t st = c->st[0];
-    if ((c->snap_ready || c->folded) && !hot_skip) {   // computed by the dense pass that ended the last pass
-        c->snap_ready = false;
-        if (c->counts_copied) {   // its copy was queued behind the pull and the pass sync waited for it
-            c->counts_copied = false;
-            parse_counts(c, h_cnt, h_hotpend);
-            return EXC_OK;
-        }
-        return snapshot_counts(c, h_cnt, h_hotpend);
-    }
-    c->counts_copied = false;
-    c->snap_ready = false;
+// Enqueues a snapshot of the current state on `st`: counts and residual slots cleared, then
+// k_snapshot (k_bsnapshot for batched contexts) into c->d_cnt and the bitmap.
+void enqueue_snapshot(Ctx* c, hipStream_t st, int hot_skip) {
     clear_counts(c, st);
     SnapArgs a{};
     a.n = c->n;
@@ -3242,6 +3239,25 @@ int exc_snapshot(void* p, int hot_skip, uint64_t* h_cnt, uint64_t* h_hotpend) {
         launch_snapshot_plain(a, c->nm, c->cus, st);
         c->snap_bytes += 4.0 * c->nm * c->n + 4.0 * (((double)c->n + 31) / 32);
     }
+}
+
+int exc_snapshot(void* p, int hot_skip, uint64_t* h_cnt, uint64_t* h_hotpend) {
+    Ctx* c = as_ctx(p);
+    if (!c || !h_cnt) return fail(EXC_EINVAL, "exc_snapshot: bad arguments");
+    HIPCHK(hipSetDevice(c->dev));
+    hipStream_t st = c->st[0];
+    if ((c->snap_ready || c->folded) && !hot_skip) {   // computed by the dense pass that ended the last pass
+        c->snap_ready = false;
+        if (c->counts_copied) {   // its copy was queued behind the pull and the pass sync waited for it
+            c->counts_copied = false;
+            parse_counts(c, h_cnt, h_hotpend);
+            return EXC_OK;
+        }
+        return snapshot_counts(c, h_cnt, h_hotpend);
+    }
+    c->counts_copied = false;
+    c->snap_ready = false;
+    enqueue_snapshot(c, st, hot_skip);
     HIPCHK(hipGetLastError());
     return snapshot_counts(c, h_cnt, h_hotpend);
 }
@@ -3554,6 +3570,71 @@ int exc_set_edge_weights(void* p, const float* h_w, uint64_t cnt) {
     c->nm_act = c->nm;       // ... and every measure may be pending again
     if (!c->d_wts) HIPCHK(hipMalloc(&c->d_wts, (c->m ? c->m : 1) * sizeof(float)));
     if (c->m) HIPCHK(hipMemcpy(c->d_wts, h_w, c->m * sizeof(float), hipMemcpyHostToDevice));
+    return EXC_OK;
+}
+
+int exc_run_passes(void* p, int passes, int dense, uint64_t* h_cnt, double* h_l1, float* h_linf) {
+    Ctx* c = as_ctx(p);
+    if (!c || passes < 1 || passes > EXC_BURST_MAX || !h_cnt)
+        return fail(EXC_EINVAL, "exc_run_passes: 1 to %d passes and a count array", EXC_BURST_MAX);
+    if (c->K) return fail(EXC_ESTATE, "exc_run_passes: plain contexts only");
+    if (!c->d_arena || c->arena_edges != c->m) return fail(EXC_ESTATE, "exc_run_passes: the whole graph must be in the edge cache");
+    if (c->d_trace) return fail(EXC_ESTATE, "exc_run_passes: not with the pending-pass trace");
+    if (dense && !(c->d_tadj && (!c->d_wts || c->d_twts))) return fail(EXC_ESTATE, "exc_run_passes: dense passes need exc_enable_pull");
+    if (!dense && c->folded)
+        return fail(EXC_ESTATE, "exc_run_passes: the last dense pass folded the next one; it must be a dense pass");
+    HIPCHK(hipSetDevice(c->dev));
+    const size_t W = cnt_words(c);
+    if (!c->d_burst) {
+        HIPCHK(hipMalloc(&c->d_burst, EXC_BURST_MAX * W * sizeof(unsigned long long)));
+        HIPCHK(hipHostMalloc((void**)&c->h_burst, EXC_BURST_MAX * W * sizeof(unsigned long long), hipHostMallocDefault));
+    }
+    hipStream_t st = c->st[0];
+    unsigned long long* const d_cnt = c->d_cnt;
+    unsigned long long* const h_small = c->h_small;
+    // every pass's region cleared at once, and copied to the host at once after the last pass
+    const long long words = (long long)passes * (long long)W;
+    hipLaunchKernelGGL(k_zero_u64, dim3((unsigned)((words + BLOCK - 1) / BLOCK)), dim3(BLOCK), 0, st, c->d_burst, (int)words);
+    c->in_burst = true;
+    int rc = EXC_OK;
+    for (int i = 0; i < passes && rc == EXC_OK; i++) {
+        // pass i writes the snapshot of the state it leaves into region i
+        c->d_cnt = c->d_burst + i * W;
+        c->h_small = c->h_burst + i * W;
+        c->snap_ready = false;
+        c->l1_valid = false;
+        if (dense) {
+            rc = fold_pull(c);   // the snapshot is fused into the pull
+        } else {
+            rc = push_chunk(c, 0, 0, c->n, c->d_arena, 0, 1, 0, 0, c->arena_edges, c->d_wts);
+            if (rc == EXC_OK) enqueue_snapshot(c, st, 0);
+        }
+    }
+    c->in_burst = false;
+    c->d_cnt = d_cnt;
+    c->h_small = h_small;
+    if (rc != EXC_OK) return rc;
+    HIPCHK(hipMemcpyAsync(c->h_burst, c->d_burst, (size_t)words * sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
+    // the last snapshot also in the context's own count buffer (the bitmap is already the last
+    // snapshot's), then one wait for the whole burst
+    HIPCHK(hipMemcpyAsync(c->d_cnt, c->d_burst + (passes - 1) * W, W * sizeof(unsigned long long), hipMemcpyDeviceToDevice, st));
+    for (auto s2 : c->st) HIPCHK(hipStreamSynchronize(s2));
+    c->counts_enqueued = false;
+    for (auto& sl : c->slot) sl.busy = false;
+    HIPCHK(hipGetLastError());
+    rc = collect_timers(c);
+    if (rc) return rc;
+    for (int i = 0; i < passes; i++) {
+        memcpy(c->h_small, c->h_burst + i * W, W * sizeof(unsigned long long));
+        parse_counts(c, h_cnt + (size_t)i * 2 * c->nc, nullptr);
+        for (int k = 0; k < c->nm; k++) {
+            if (h_l1) h_l1[(size_t)i * c->nm + k] = c->l1_last[k];
+            if (h_linf) h_linf[(size_t)i * c->nm + k] = c->linf_last[k];
+        }
+    }
+    // the next exc_snapshot returns the last pass's snapshot without a launch
+    c->snap_ready = true;
+    c->counts_copied = true;
     return EXC_OK;
 }
 

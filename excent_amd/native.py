@@ -151,6 +151,7 @@ HIP_SIGNATURES = {
     "exc_residual_max": (c_int, [c_void_p, P(c_float)]),
     "exc_fold_ahead": (c_int, [c_void_p, c_int]),
     "exc_set_active_measures": (c_int, [c_void_p, c_int]),
+    "exc_run_passes": (c_int, [c_void_p, c_int, c_int, P(c_u64), P(c_double), P(ctypes.c_float)]),
     "exc_set_hub_destinations": (c_int, [c_void_p, c_void_p, c_int]),
     "exc_snapshot_noalloc": (c_int, [c_int, c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_int,
                                      c_void_p, c_void_p, c_void_p, c_void_p]),
@@ -393,6 +394,16 @@ class Context:
 
     def fold_ahead(self, allow):
         check(F["exc_fold_ahead"](self.ctx, 1 if allow else 0), "fold_ahead")
+
+    def run_passes(self, passes, dense):
+        """A burst of passes with one synchronisation: per pass the counts [2 nc] of the snapshot
+        it leaves, its FP64 residual sums and largest |r| per measure."""
+        cnt = np.zeros((passes, 2 * self.nc), np.uint64)
+        l1 = np.zeros((passes, self.nm), np.float64)
+        mx = np.zeros((passes, self.nm), np.float32)
+        check(F["exc_run_passes"](self.ctx, passes, 1 if dense else 0, cnt.ctypes.data_as(P(c_u64)),
+                                  l1.ctypes.data_as(P(c_double)), mx.ctypes.data_as(P(ctypes.c_float))), "run_passes")
+        return cnt, l1, mx
 
     def set_active_measures(self, count):
         """Dense passes over the first `count` measure slots only (the others converged)."""

@@ -27,6 +27,7 @@ extern "C" {
 #endif
 
 #define EXC_MAXM 3
+#define EXC_BURST_MAX 16     /* passes of one exc_run_passes call */
 enum { EXC_PR = 0, EXC_KATZ = 1, EXC_EV = 2 };
 
 enum {
@@ -240,6 +241,19 @@ int exc_fold_ahead(void* ctx, int allow);
  * the standalone snapshot always cover every slot.  No reference counterpart (the
  * reference pushes every measure of a pending vertex; a converged measure has none). */
 int exc_set_active_measures(void* ctx, int count);
+/* A burst: `passes` (1..EXC_BURST_MAX) passes over the whole cached graph enqueued back to back
+ * with one host synchronisation at the end -- dense passes (dense = 1; exc_enable_pull) or
+ * bitmap-driven pushes of every pending vertex (dense = 0; refused while the last dense pass
+ * has folded the next one).  Each pass starts from the snapshot before it (the first from the
+ * last exc_snapshot) and takes the snapshot of the state it leaves: its per-chunk counts go to
+ * h_cnt[i * 2 nc ...] and, when the arrays are given, its FP64 residual sums and largest |r|
+ * per measure to h_l1[i * nm + k], h_linf[i * nm + k]; the next exc_snapshot returns the last
+ * pass's without a launch.  A pass that starts from a snapshot with nothing pending changes
+ * nothing (no vertex folds or pushes), so the caller ends its run at the first snapshot of the
+ * burst with nothing pending, and the state is the one that snapshot saw.  Plain contexts
+ * with the whole graph in the edge cache, no hot set and no trace.  No reference counterpart
+ * (the reference synchronises with the host at every pass, src/excent.cu:664-665, 750). */
+int exc_run_passes(void* ctx, int passes, int dense, uint64_t* h_cnt, double* h_l1, float* h_linf);
 
 /* Push a resident chunk run [ch.v0, ch.v1) from the arena on the resident stream
  * (launch_chunk with dst = resident[c], src/excent.cu:634-652, 684-688).  use_list:
