@@ -2159,6 +2159,13 @@ __global__ __launch_bounds__(BLOCK) void k_bpull_hub(BDenseArgs<PT> a, BatchArgs
 
 // ------------------------------------------------------------------ host-side context
 
+struct BurstGraph {
+    int passes = 0, phase = 0;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    double snap_bytes = 0;   // the snapshot bytes its launches count, added per replay
+};
+
 struct PieceBuf {
     Piece* d = nullptr;          // pieces (Piece, or BPiece for batched contexts)
     uint32_t* cnt = nullptr;     // phase p: piece count at cnt[2p*CNT_STRIDE], p-row count one line later
@@ -2209,6 +2216,9 @@ struct Ctx {
     unsigned long long* h_burst = nullptr;
     bool in_burst = false;                     // its passes are being enqueued: the regions are
                                                // cleared and copied once for the whole burst
+    std::vector<BurstGraph> burst_graphs;      // recorded push bursts (launch_push_burst_graph)
+    bool use_burst_graphs = true;              // EXC_BURST_GRAPH=0: push bursts launched directly
+    uint64_t burst_graph_launches = 0;
     double l1_last[EXC_MAXM] = {};             // FP64 residual sums of the last snapshot
     float linf_last[EXC_MAXM] = {};            // and the largest |r| per measure
     int snap_nm = 0;                           // measures the last snapshot computed (the others
@@ -2388,6 +2398,8 @@ int collect_timers(Ctx* c) {
     return EXC_OK;
 }
 
+void burst_graphs_clear(Ctx* c);
+
 int ensure_pieces(Ctx* c, int s, uint64_t edges_bound) {
     if (c->K > 0) return ensure_bpieces(c, s, edges_bound);
     uint64_t need64 = edges_bound / HUB + edges_bound / PIECE + 64;
@@ -2395,6 +2407,7 @@ int ensure_pieces(Ctx* c, int s, uint64_t edges_bound) {
     uint32_t need = (uint32_t)need64;
     PieceBuf& p = c->pb[s];
     if (p.d && p.cap >= need) return EXC_OK;
+    if (s == 0) burst_graphs_clear(c);   // recorded pushes name the old buffer
     if (p.d) { HIPCHK(hipStreamSynchronize(c->st[s])); hipFree(p.d); p.d = nullptr; }
     if (hipMalloc(&p.d, (size_t)need * sizeof(Piece)) != hipSuccess) {
         p.d = nullptr;
@@ -2797,6 +2810,132 @@ constexpr int NOALLOC_MAX_CHUNKS = 4096;
 constexpr size_t NOALLOC_L1_OFF = ((2 * (size_t)NOALLOC_MAX_CHUNKS + 1 + L1_STRIDE - 1) / L1_STRIDE) * L1_STRIDE;
 constexpr size_t NOALLOC_WORDS = NOALLOC_L1_OFF + (size_t)L1_SLOTS * L1_STRIDE;   // 9232 words, 72 KiB
 
+// Enqueues a snapshot of the current state on `st`: counts and residual slots cleared, then
+// k_snapshot (k_bsnapshot for batched contexts) into c->d_cnt and the bitmap.
+void enqueue_snapshot(Ctx* c, hipStream_t st, int hot_skip) {
+    clear_counts(c, st);
+    SnapArgs a{};
+    a.n = c->n;
+    a.r = c->d_r;
+    for (int k = 0; k < EXC_MAXM; k++) a.eps[k] = c->DM.eps[k];
+    a.idx = c->d_idx;
+    a.cv0 = c->d_cv0;
+    a.nc = c->nc;
+    a.hot = hot_skip ? c->d_hot : nullptr;
+    a.bitmap = c->d_bitmap;
+    a.cnt = c->d_cnt;
+    a.hotpend = c->d_cnt + 2 * c->nc;
+    a.trace = c->d_trace;
+    a.l1 = c->K ? nullptr : l1_slots(c);
+    const long long nseg = ((long long)c->n + WAVE - 1) / WAVE;
+    if (c->K) {
+        hipLaunchKernelGGL(k_zero_u32, dim3((c->K + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, st, c->d_colpend, c->K);
+        const long long bw = (long long)c->cus * 64 * NWAVES;
+        long long bspw = (nseg + bw - 1) / bw;
+        if (bspw < 1) bspw = 1;
+        a.segs_per_wave = (int)bspw;
+        const long long bwaves = (nseg + bspw - 1) / bspw;
+        const unsigned bgrid = (unsigned)((bwaves + NWAVES - 1) / NWAVES);
+        ScopedTimer t(c, st, T_SNAP);
+        hipLaunchKernelGGL(k_bsnapshot, dim3(bgrid), dim3(BLOCK), 0, st, a, batch_args(c, -1));
+        c->snap_bytes += 4.0 * c->K * c->n + 4.0 * (((double)c->n + 31) / 32);
+    } else {
+        ScopedTimer t(c, st, T_SNAP);
+        c->snap_nm = c->nm;
+        launch_snapshot_plain(a, c->nm, c->cus, st);
+        c->snap_bytes += 4.0 * c->nm * c->n + 4.0 * (((double)c->n + 31) / 32);
+    }
+}
+
+// The launches of a burst of `passes` passes on stream 0 (see exc_run_passes): every pass's
+// count region cleared at once, the passes, then the regions copied to the host and the last
+// one into the context's own count buffer.  Nothing here waits or allocates, so a push burst
+// can be recorded into a graph.
+int enqueue_burst(Ctx* c, int passes, int dense) {
+    hipStream_t st = c->st[0];
+    const size_t W = cnt_words(c);
+    unsigned long long* const d_cnt = c->d_cnt;
+    unsigned long long* const h_small = c->h_small;
+    const long long words = (long long)passes * (long long)W;
+    hipLaunchKernelGGL(k_zero_u64, dim3((unsigned)((words + BLOCK - 1) / BLOCK)), dim3(BLOCK), 0, st, c->d_burst, (int)words);
+    c->in_burst = true;
+    int rc = EXC_OK;
+    for (int i = 0; i < passes && rc == EXC_OK; i++) {
+        // pass i writes the snapshot of the state it leaves into region i
+        c->d_cnt = c->d_burst + i * W;
+        c->h_small = c->h_burst + i * W;
+        c->snap_ready = false;
+        c->l1_valid = false;
+        if (dense) {
+            rc = fold_pull(c);   // the snapshot is fused into the pull
+        } else {
+            rc = push_chunk(c, 0, 0, c->n, c->d_arena, 0, 1, 0, 0, c->arena_edges, c->d_wts);
+            if (rc == EXC_OK) enqueue_snapshot(c, st, 0);
+        }
+    }
+    c->in_burst = false;
+    c->d_cnt = d_cnt;
+    c->h_small = h_small;
+    if (rc != EXC_OK) return rc;
+    HIPCHK(hipMemcpyAsync(c->h_burst, c->d_burst, (size_t)words * sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
+    // the bitmap is already the last snapshot's
+    HIPCHK(hipMemcpyAsync(c->d_cnt, c->d_burst + (passes - 1) * W, W * sizeof(unsigned long long), hipMemcpyDeviceToDevice, st));
+    return EXC_OK;
+}
+
+// Push bursts replayed from graphs: a push burst's launches are fixed by its length and the
+// phase of stream 0's piece counters (it flips at every push), so one recorded graph per
+// (length, phase) serves every later burst.  Anything that changes a buffer or a by-value
+// argument of those launches drops the graphs (burst_graphs_clear).
+void burst_graphs_clear(Ctx* c) {
+    for (auto& g : c->burst_graphs) {
+        if (g.exec) (void)hipGraphExecDestroy(g.exec);
+        if (g.graph) (void)hipGraphDestroy(g.graph);
+    }
+    c->burst_graphs.clear();
+}
+
+int launch_push_burst_graph(Ctx* c, int passes) {
+    hipStream_t st = c->st[0];
+    // the launches reserve the piece buffer for the whole cache: done before recording, so
+    // recording never allocates or waits
+    int rc = ensure_pieces(c, 0, c->arena_edges);
+    if (rc) return rc;
+    const int phase = c->pb[0].phase;
+    BurstGraph* hit = nullptr;
+    for (auto& g : c->burst_graphs)
+        if (g.passes == passes && g.phase == phase) hit = &g;
+    if (!hit) {
+        BurstGraph g;
+        g.passes = passes;
+        g.phase = phase;
+        const bool timing = c->timing;
+        const double sb = c->snap_bytes;
+        c->timing = false;   // no event nodes: the burst is timed as a whole
+        HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+        rc = enqueue_burst(c, passes, 0);
+        const hipError_t ce = hipStreamEndCapture(st, &g.graph);
+        c->timing = timing;
+        g.snap_bytes = c->snap_bytes - sb;
+        c->snap_bytes = sb;
+        c->pb[0].phase = phase;   // recorded, not run
+        if (rc) { if (g.graph) (void)hipGraphDestroy(g.graph); return rc; }
+        if (ce != hipSuccess) return fail(EXC_EHIP, "exc_run_passes: graph capture: %s", hipGetErrorString(ce));
+        HIPCHK(hipGraphInstantiate(&g.exec, g.graph, nullptr, nullptr, 0));
+        c->burst_graphs.push_back(g);
+        hit = &c->burst_graphs.back();
+    }
+    {
+        ScopedTimer t(c, st, T_PUSH + SRC_BITMAP);
+        HIPCHK(hipGraphLaunch(hit->exec, st));
+    }
+    c->pb[0].phase ^= passes & 1;
+    c->snap_bytes += hit->snap_bytes;
+    c->snap_nm = c->nm;
+    c->burst_graph_launches++;
+    return EXC_OK;
+}
+
 }  // namespace
 
 static __device__ __align__(256) unsigned long long scratch[NOALLOC_WORDS];
@@ -2953,6 +3092,7 @@ int exc_create(const exc_config* cfg, const exc_measures* M, void** out) {
     c->timing = cfg->timing != 0;
     c->timing_pull_only = cfg->timing == 2;
     c->push_kernel = cfg->push_kernel;
+    if (const char* bg = getenv("EXC_BURST_GRAPH")) c->use_burst_graphs = atoi(bg) != 0;   // measurements
     c->max_chunk_edges = cfg->max_chunk_edges;
     c->max_chunk_verts = cfg->max_chunk_verts;
     auto bail = [&](int rc) { exc_destroy(c); return rc; };
@@ -3032,6 +3172,7 @@ int exc_destroy(void* p) {
     for (auto& t : c->pending_timers) { hipEventDestroy(t.a); hipEventDestroy(t.b); }
     for (auto e : c->free_events) hipEventDestroy(e);
     for (auto e : c->bq_ev) if (e) hipEventDestroy(e);
+    burst_graphs_clear(c);
     for (auto& sl : c->slot) {
         if (sl.h_edges) hipHostFree(sl.h_edges);
         if (sl.h_vids) hipHostFree(sl.h_vids);
@@ -3204,43 +3345,6 @@ void parse_counts(Ctx* c, uint64_t* h_cnt, uint64_t* h_hotpend) {
     for (int i = 0; i < c->nc; i++) c->last_demand += c->h_small[2 * i + 1];
 }
 
-// Enqueues a snapshot of the current state on `st`: counts and residual slots cleared, then
-// k_snapshot (k_bsnapshot for batched contexts) into c->d_cnt and the bitmap.
-void enqueue_snapshot(Ctx* c, hipStream_t st, int hot_skip) {
-    clear_counts(c, st);
-    SnapArgs a{};
-    a.n = c->n;
-    a.r = c->d_r;
-    for (int k = 0; k < EXC_MAXM; k++) a.eps[k] = c->DM.eps[k];
-    a.idx = c->d_idx;
-    a.cv0 = c->d_cv0;
-    a.nc = c->nc;
-    a.hot = hot_skip ? c->d_hot : nullptr;
-    a.bitmap = c->d_bitmap;
-    a.cnt = c->d_cnt;
-    a.hotpend = c->d_cnt + 2 * c->nc;
-    a.trace = c->d_trace;
-    a.l1 = c->K ? nullptr : l1_slots(c);
-    const long long nseg = ((long long)c->n + WAVE - 1) / WAVE;
-    if (c->K) {
-        hipLaunchKernelGGL(k_zero_u32, dim3((c->K + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, st, c->d_colpend, c->K);
-        const long long bw = (long long)c->cus * 64 * NWAVES;
-        long long bspw = (nseg + bw - 1) / bw;
-        if (bspw < 1) bspw = 1;
-        a.segs_per_wave = (int)bspw;
-        const long long bwaves = (nseg + bspw - 1) / bspw;
-        const unsigned bgrid = (unsigned)((bwaves + NWAVES - 1) / NWAVES);
-        ScopedTimer t(c, st, T_SNAP);
-        hipLaunchKernelGGL(k_bsnapshot, dim3(bgrid), dim3(BLOCK), 0, st, a, batch_args(c, -1));
-        c->snap_bytes += 4.0 * c->K * c->n + 4.0 * (((double)c->n + 31) / 32);
-    } else {
-        ScopedTimer t(c, st, T_SNAP);
-        c->snap_nm = c->nm;
-        launch_snapshot_plain(a, c->nm, c->cus, st);
-        c->snap_bytes += 4.0 * c->nm * c->n + 4.0 * (((double)c->n + 31) / 32);
-    }
-}
-
 int exc_snapshot(void* p, int hot_skip, uint64_t* h_cnt, uint64_t* h_hotpend) {
     Ctx* c = as_ctx(p);
     if (!c || !h_cnt) return fail(EXC_EINVAL, "exc_snapshot: bad arguments");
@@ -3284,6 +3388,7 @@ int exc_cache_chunks(void* p, uint64_t resident_edges) {
     Ctx* c = as_ctx(p);
     if (!c || resident_edges > c->m) return fail(EXC_EINVAL, "exc_cache_chunks: bad size");
     HIPCHK(hipSetDevice(c->dev));
+    burst_graphs_clear(c);   // recorded pushes name the old arena
     if (c->d_arena) { (void)hipFree(c->d_arena); c->d_arena = nullptr; c->arena_edges = 0; }
     // a prefix of zero-edge chunks is still a cache: keep one element so the pointer is valid
     if (hipMalloc(&c->d_arena, (resident_edges ? resident_edges : 1) * sizeof(int32_t)) != hipSuccess) {
@@ -3568,6 +3673,7 @@ int exc_set_edge_weights(void* p, const float* h_w, uint64_t cnt) {
     c->l1_valid = false;
     c->folded = false;       // the state is replaced: a folded pass no longer applies
     c->nm_act = c->nm;       // ... and every measure may be pending again
+    burst_graphs_clear(c);   // recorded pushes ran unweighted
     if (!c->d_wts) HIPCHK(hipMalloc(&c->d_wts, (c->m ? c->m : 1) * sizeof(float)));
     if (c->m) HIPCHK(hipMemcpy(c->d_wts, h_w, c->m * sizeof(float), hipMemcpyHostToDevice));
     return EXC_OK;
@@ -3589,35 +3695,8 @@ int exc_run_passes(void* p, int passes, int dense, uint64_t* h_cnt, double* h_l1
         HIPCHK(hipMalloc(&c->d_burst, EXC_BURST_MAX * W * sizeof(unsigned long long)));
         HIPCHK(hipHostMalloc((void**)&c->h_burst, EXC_BURST_MAX * W * sizeof(unsigned long long), hipHostMallocDefault));
     }
-    hipStream_t st = c->st[0];
-    unsigned long long* const d_cnt = c->d_cnt;
-    unsigned long long* const h_small = c->h_small;
-    // every pass's region cleared at once, and copied to the host at once after the last pass
-    const long long words = (long long)passes * (long long)W;
-    hipLaunchKernelGGL(k_zero_u64, dim3((unsigned)((words + BLOCK - 1) / BLOCK)), dim3(BLOCK), 0, st, c->d_burst, (int)words);
-    c->in_burst = true;
-    int rc = EXC_OK;
-    for (int i = 0; i < passes && rc == EXC_OK; i++) {
-        // pass i writes the snapshot of the state it leaves into region i
-        c->d_cnt = c->d_burst + i * W;
-        c->h_small = c->h_burst + i * W;
-        c->snap_ready = false;
-        c->l1_valid = false;
-        if (dense) {
-            rc = fold_pull(c);   // the snapshot is fused into the pull
-        } else {
-            rc = push_chunk(c, 0, 0, c->n, c->d_arena, 0, 1, 0, 0, c->arena_edges, c->d_wts);
-            if (rc == EXC_OK) enqueue_snapshot(c, st, 0);
-        }
-    }
-    c->in_burst = false;
-    c->d_cnt = d_cnt;
-    c->h_small = h_small;
-    if (rc != EXC_OK) return rc;
-    HIPCHK(hipMemcpyAsync(c->h_burst, c->d_burst, (size_t)words * sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
-    // the last snapshot also in the context's own count buffer (the bitmap is already the last
-    // snapshot's), then one wait for the whole burst
-    HIPCHK(hipMemcpyAsync(c->d_cnt, c->d_burst + (passes - 1) * W, W * sizeof(unsigned long long), hipMemcpyDeviceToDevice, st));
+    int rc = !dense && c->use_burst_graphs ? launch_push_burst_graph(c, passes) : enqueue_burst(c, passes, dense);
+    if (rc) return rc;
     for (auto s2 : c->st) HIPCHK(hipStreamSynchronize(s2));
     c->counts_enqueued = false;
     for (auto& sl : c->slot) sl.busy = false;
@@ -3765,6 +3844,7 @@ int exc_get_stats(void* p, exc_stats* o) {
         o->pushed_edge_measures += sp[S_EDGES_NM];
     }
     o->pull_rows_in_flight = c->K ? c->bpull_q : 0;
+    o->burst_graph_launches = c->burst_graph_launches;
     o->pull_tune_ns_per_edge[0] = c->bq_ns_edge[0];
     o->pull_tune_ns_per_edge[1] = c->bq_ns_edge[1];
     if (h[C_OVERFLOW]) return fail(EXC_ESTATE, "hub piece buffer overflowed %llu times", h[C_OVERFLOW]);
@@ -3960,6 +4040,7 @@ int exc_set_hub_destinations(void* p, const int32_t* ids, int count) {
     }
     HIPCHK(hipSetDevice(c->dev));
     HIPCHK(hipDeviceSynchronize());   // no push in flight reads the table being replaced
+    burst_graphs_clear(c);            // recorded pushes carry the table's size and probe length
     if (!c->d_hub_keys) HIPCHK(hipMalloc(&c->d_hub_keys, HUB_SLOTS * sizeof(int32_t)));
     if (!c->d_hub_filter) HIPCHK(hipMalloc(&c->d_hub_filter, HUB_FILTER_WORDS * sizeof(uint32_t)));
     HIPCHK(hipMemcpy(c->d_hub_keys, keys.data(), HUB_SLOTS * sizeof(int32_t), hipMemcpyHostToDevice));

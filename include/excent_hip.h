@@ -107,6 +107,8 @@ typedef struct exc_stats {
     uint64_t pull_w_launches[3];
     double pull_w_bytes[3];
     int pull_w_tile[3];
+    /* push bursts (exc_run_passes) replayed from recorded HIP graphs */
+    uint64_t burst_graph_launches;
 } exc_stats;
 
 const char* exc_last_error(void);

@@ -67,6 +67,7 @@ def burst_passes(idx, adj, eps, pull, B, dense, max_bursts=100, init=None):
         # the next snapshot is the burst's last one, without a launch
         cnt, _ = ctx.snapshot()
         assert np.array_equal(cnt, cnts[B - 1])
+        burst_passes.graph_launches = ctx.stats()["burst_graph_launches"]
         return snaps, ctx.read_state(0, 0), ctx.read_state(0, 1)
 
 
@@ -86,13 +87,17 @@ def test_burst_dense_cycle_bit_exact(B):
     assert np.array_equal(a[1], b[1]) and np.array_equal(a[2], b[2])
 
 
+@pytest.mark.parametrize("graph", ["1", "0"])
 @pytest.mark.parametrize("B", [2, 8])
-def test_burst_push_paths_bit_exact(B):
+def test_burst_push_paths_bit_exact(B, graph, monkeypatch):
     """Bitmap-driven pushes along 300 disjoint paths of 20 edges with the residual mass at the
     heads (r = 1 there, 0 elsewhere): each pass one vertex per path pushes into a vertex that
     was not pending, so every destination receives one delta and the arithmetic does not
-    depend on atomic order; the bursts equal pass-by-pass runs bit for bit (21 passes)."""
-    print("case burst push paths", B, flush=True)
+    depend on atomic order; the bursts equal pass-by-pass runs bit for bit (21 passes).
+    graph 1: push bursts replayed from recorded HIP graphs (every replay checked against the
+    pass-by-pass run), 0: launched directly."""
+    print("case burst push paths", B, graph, flush=True)
+    monkeypatch.setenv("EXC_BURST_GRAPH", graph)
     paths, L = 300, 20
     n = paths * (L + 1)
     v = np.arange(n)
@@ -103,6 +108,7 @@ def test_burst_push_paths_bit_exact(B):
     init = (np.zeros(n, np.float32), r0)
     a = single_passes(idx, adj, 1e-3, None, init=init)
     b = burst_passes(idx, adj, 1e-3, None, B, False, init=init)
+    assert (burst_passes.graph_launches > 0) == (graph == "1")
     assert len(a[0]) == len(b[0]) == L + 2
     for p, (sa, sb) in enumerate(zip(a[0], b[0])):
         assert np.array_equal(sa[0], sb[0]) and sa[1] == sb[1] and sa[2] == sb[2], p
@@ -215,3 +221,37 @@ def test_cli_burst(tmp_path, rmat14):
         assert np.abs(x.astype(np.float64) - ref).sum() <= np.abs(tr).sum() / 0.15 * (1 + 1e-6) + 1e-6
         assert np.abs(tr).max() <= 1e-4 * (1 + 1e-3) + 2e-4 * max(1.0, np.abs(x).max())
     assert nat.excent([rmat14, "--burst", "17", "--log", str(tmp_path / "bad.log")]) == 1
+
+
+def test_burst_graphs_follow_the_hub_table(rmat12):
+    """Push-only bursts replayed from recorded graphs on rmat12, with the hub-destination table
+    replaced between bursts (the recorded launches carry its size, so the graphs must be
+    recorded again): the run meets the FP64 bound, the invariant and |r| <= eps, and graphs
+    were replayed before and after the change."""
+    idx, adj = orc.egr_load(rmat12)
+    n, m = len(idx) - 1, len(adj)
+    indeg = np.bincount(adj, minlength=n)
+    eps = 1e-4
+    with context(idx, adj, eps, None) as ctx:
+        ctx.set_hub_destinations(np.argsort(-indeg)[:16].astype(np.int32))
+        cnt, _ = ctx.snapshot()
+        for k in range(200):
+            if cnt[0::2].sum() == 0:
+                break
+            if k == 2:
+                g0 = ctx.stats()["burst_graph_launches"]
+                assert g0 > 0
+                ctx.set_hub_destinations(np.argsort(-indeg)[:300].astype(np.int32))
+            cnts, _, _ = ctx.run_passes(4, False)
+            stop = [i for i in range(4) if cnts[i][0::2].sum() == 0]
+            cnt = cnts[stop[0]] if stop else cnts[3]
+        else:
+            raise AssertionError("did not converge")
+        assert k > 3 and ctx.stats()["burst_graph_launches"] > g0
+        x, r = ctx.read_state(0, 0), ctx.read_state(0, 1)
+    b = np.full(n, np.float64(np.float32(1) - np.float32(0.85)))
+    ref, _, _ = orc.fp64_solve_b(idx, adj, b)
+    tr = true_residual(idx, adj, x, b, 0.85, True)
+    assert np.abs(x.astype(np.float64) - ref).sum() <= np.abs(tr).sum() / 0.15 * (1 + 1e-6) + 1e-6
+    assert np.abs(tr - r).max() < 2e-4 * max(1.0, np.abs(x).max())
+    assert np.all(np.abs(r) <= np.float32(eps))
