@@ -758,7 +758,7 @@ def main():
                     help="stop a run after this many passes (reported as not converged; plain runs: 2000 when 0)")
     ap.add_argument("--cold", action="store_true",
                     help="ooc: also time the hot-cache run after dropping the file from the host page cache")
-    ap.add_argument("--kernel", default="lb", choices=["lb", "group"])
+    ap.add_argument("--kernel", default="lb", choices=["lb", "group", "tune"])
     ap.add_argument("--pull", default="auto", help="dense passes by pull: auto | off | demand fraction")
     ap.add_argument("--hub-lds", type=int, default=1024, help="destinations privatised in LDS by the hub-piece push")
     ap.add_argument("--drop-measures", default="on", choices=["on", "off"],

@@ -327,7 +327,8 @@ void Engine::setup() {
     }
     if (o.push_kernel == "lb") push_kernel_ = EXC_KERNEL_LB;
     else if (o.push_kernel == "group") push_kernel_ = EXC_KERNEL_GROUP;
-    else die("--push-kernel must be lb or group");
+    else if (o.push_kernel == "tune") push_kernel_ = EXC_KERNEL_TUNE;
+    else die("--push-kernel must be lb, group or tune");
     if (o.ev_init != "outdeg" && o.ev_init != "indeg") die("--ev-init must be outdeg or indeg");
     if (o.pull != "auto" && o.pull != "off" && atof(o.pull.c_str()) <= 0) die("--pull must be auto, off or a positive demand fraction");
     if (o.zero_copy != "auto" && o.zero_copy != "on" && o.zero_copy != "off") die("--zero-copy must be auto, on or off");
@@ -858,7 +859,7 @@ void Engine::run_passes() {
             const bool ul = use_list(h_cnt[2 * c], chunks_[c].v1 - chunks_[c].v0, o.list_frac);
             uint64_t k = h_cnt[2 * c], e = h_cnt[2 * c + 1];
             int c1 = c + 1;
-            if (push_kernel_ == EXC_KERNEL_LB)
+            if (push_kernel_ != EXC_KERNEL_GROUP)   // one launch for a run of chunks
                 while (c1 < nc && mode[c1] == F_RESIDENT && use_list(h_cnt[2 * c1], chunks_[c1].v1 - chunks_[c1].v0, o.list_frac) == ul) {
                     k += h_cnt[2 * c1]; e += h_cnt[2 * c1 + 1]; c1++;
                 }

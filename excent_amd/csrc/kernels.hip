@@ -2159,6 +2159,18 @@ __global__ __launch_bounds__(BLOCK) void k_bpull_hub(BDenseArgs<PT> a, BatchArgs
 
 // ------------------------------------------------------------------ host-side context
 
+struct PushTune {
+    int best = -1;               // candidate kept, -1 while undecided
+    int next = 0;                // next candidate to time
+    int misses = 0;              // trials that pushed too few edges to time
+    double ns_edge[3] = {0, 0, 0};
+};
+struct PushTrial {
+    bool on = false;
+    int cls = 0, cand = 0;
+    unsigned long long before = 0;
+};
+
 struct BurstGraph {
     int passes = 0, phase = 0;
     hipGraph_t graph = nullptr;
@@ -2217,6 +2229,9 @@ struct Ctx {
     bool in_burst = false;                     // its passes are being enqueued: the regions are
                                                // cleared and copied once for the whole burst
     std::vector<BurstGraph> burst_graphs;      // recorded push bursts (launch_push_burst_graph)
+    PushTune ptune[7];                         // EXC_KERNEL_TUNE, by degree class
+    hipEvent_t ptune_ev[2] = {nullptr, nullptr};
+    uint64_t ptune_bound = 0;                  // edges the launch being made may push
     bool use_burst_graphs = true;              // EXC_BURST_GRAPH=0: push bursts launched directly
     uint64_t burst_graph_launches = 0;
     double l1_last[EXC_MAXM] = {};             // FP64 residual sums of the last snapshot
@@ -2467,6 +2482,68 @@ int norm_group(int g) {
     return p;
 }
 
+// Push autotuner (EXC_KERNEL_TUNE).  A launch's degree class is the log2 of its group hint
+// (the host's pick_group of the pushed vertices' average degree).  Candidates of class c: the
+// load-balanced kernel (G = 0), G = 2^c, G = max(1, 2^c / 4).  Until a class has timed all
+// three, its launches that may push at least PTUNE_MIN_EDGES edges run the next candidate
+// alone (the stream is drained first, the kernel is bracketed by events, and the non-hub
+// edges it pushed are the change of its source's counter); then the class keeps the fastest
+// per pushed edge.  The others, and the launches of a burst (which must not wait), use the
+// class's choice once made, else the load-balanced kernel.
+constexpr uint64_t PTUNE_MIN_EDGES = 1u << 16;
+int ptune_candidate(int cls, int i) {
+    const int g = 1 << cls;
+    return i == 0 ? 0 : i == 1 ? g : std::max(1, g / 4);
+}
+int ptune_class(int group) {
+    int c = 0;
+    while (c < 6 && (1 << c) < group) c++;
+    return c;
+}
+
+int push_tune_pick(Ctx* c, int s, int src, int group, int* G, PushTrial* trial) {
+    const int cls = ptune_class(group > 0 ? group : 1);
+    PushTune& t = c->ptune[cls];
+    if (t.best >= 0) { *G = ptune_candidate(cls, t.best); return EXC_OK; }
+    *G = 0;
+    if (c->in_burst || group <= 0 || c->ptune_bound < PTUNE_MIN_EDGES) return EXC_OK;
+    if (!c->ptune_ev[0]) {
+        HIPCHK(hipEventCreate(&c->ptune_ev[0]));
+        HIPCHK(hipEventCreate(&c->ptune_ev[1]));
+    }
+    HIPCHK(hipStreamSynchronize(c->st[s]));   // nothing else on this stream during the trial
+    HIPCHK(hipMemcpy(&trial->before, c->d_counters + src * C_FIELDS + C_EDGES, sizeof(unsigned long long),
+                     hipMemcpyDeviceToHost));
+    trial->on = true;
+    trial->cls = cls;
+    trial->cand = t.next;
+    *G = ptune_candidate(cls, t.next);
+    return EXC_OK;
+}
+
+int push_tune_collect(Ctx* c, int src, const PushTrial& trial) {
+    HIPCHK(hipEventSynchronize(c->ptune_ev[1]));
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, c->ptune_ev[0], c->ptune_ev[1]));
+    unsigned long long after = 0;
+    HIPCHK(hipMemcpy(&after, c->d_counters + src * C_FIELDS + C_EDGES, sizeof(unsigned long long), hipMemcpyDeviceToHost));
+    const unsigned long long edges = after - trial.before;
+    PushTune& t = c->ptune[trial.cls];
+    if (edges >= PTUNE_MIN_EDGES) {
+        t.ns_edge[trial.cand] = (double)ms * 1e6 / (double)edges;
+        t.next++;
+    } else if (++t.misses == 8) {
+        t.next = 3;   // this class's launches push too little to time: decide on what was timed
+    }
+    if (t.next == 3) {   // the fastest timed candidate, the load-balanced kernel if none was
+        int b = 0;
+        for (int i = 1; i < 3; i++)
+            if (t.ns_edge[i] > 0 && (t.ns_edge[b] == 0 || t.ns_edge[i] < t.ns_edge[b])) b = i;
+        t.best = b;
+    }
+    return EXC_OK;
+}
+
 // One push launch + its hub-piece launch on stream index s.
 int push_launch(Ctx* c, int s, int src, unsigned grid, PushArgs a, int group, uint64_t edges_bound) {
     if (grid == 0) return EXC_OK;
@@ -2504,12 +2581,24 @@ int push_launch(Ctx* c, int s, int src, unsigned grid, PushArgs a, int group, ui
         HIPCHK(hipGetLastError());
         return EXC_OK;
     }
-    const int G = c->push_kernel == EXC_KERNEL_GROUP ? norm_group(group > 0 ? group : 1) : 0;
+    int G = c->push_kernel == EXC_KERNEL_GROUP ? norm_group(group > 0 ? group : 1) : 0;
+    PushTrial trial;
+    if (c->push_kernel == EXC_KERNEL_TUNE) {
+        c->ptune_bound = edges_bound;
+        int rc2 = push_tune_pick(c, s, src, group, &G, &trial);
+        if (rc2) return rc2;
+    }
     {
         ScopedTimer t(c, c->st[s], T_PUSH + src);
+        if (trial.on) HIPCHK(hipEventRecord(c->ptune_ev[0], c->st[s]));
         if (c->nm == 1) launch_push_src<1, 1>(c, c->st[s], grid, a, src, G);
         else if (c->nm == 2) launch_push_src<2, 2>(c, c->st[s], grid, a, src, G);
         else launch_push_src<3, 4>(c, c->st[s], grid, a, src, G);
+        if (trial.on) HIPCHK(hipEventRecord(c->ptune_ev[1], c->st[s]));
+    }
+    if (trial.on) {
+        int rc2 = push_tune_collect(c, src, trial);
+        if (rc2) return rc2;
     }
     {
         ScopedTimer t(c, c->st[s], T_PIECE);
@@ -3072,7 +3161,7 @@ int exc_create(const exc_config* cfg, const exc_measures* M, void** out) {
     if (!cfg || !M || !out) return fail(EXC_EINVAL, "exc_create: null argument");
     if (cfg->n < 1 || cfg->nc < 1 || !cfg->cv0) return fail(EXC_EINVAL, "exc_create: bad n/nc/cv0");
     if (M->nm < 1 || M->nm > EXC_MAXM) return fail(EXC_EINVAL, "exc_create: nm must be 1..3");
-    if (cfg->push_kernel != EXC_KERNEL_LB && cfg->push_kernel != EXC_KERNEL_GROUP)
+    if (cfg->push_kernel != EXC_KERNEL_LB && cfg->push_kernel != EXC_KERNEL_GROUP && cfg->push_kernel != EXC_KERNEL_TUNE)
         return fail(EXC_EINVAL, "exc_create: unknown push kernel %d", cfg->push_kernel);
     Ctx* c = new Ctx();
     *out = nullptr;
@@ -3172,6 +3261,7 @@ int exc_destroy(void* p) {
     for (auto& t : c->pending_timers) { hipEventDestroy(t.a); hipEventDestroy(t.b); }
     for (auto e : c->free_events) hipEventDestroy(e);
     for (auto e : c->bq_ev) if (e) hipEventDestroy(e);
+    for (auto e : c->ptune_ev) if (e) hipEventDestroy(e);
     burst_graphs_clear(c);
     for (auto& sl : c->slot) {
         if (sl.h_edges) hipHostFree(sl.h_edges);
@@ -3845,6 +3935,10 @@ int exc_get_stats(void* p, exc_stats* o) {
     }
     o->pull_rows_in_flight = c->K ? c->bpull_q : 0;
     o->burst_graph_launches = c->burst_graph_launches;
+    for (int k = 0; k < 7; k++) {
+        o->push_tune_best[k] = c->ptune[k].best;
+        for (int i = 0; i < 3; i++) o->push_tune_ns_edge[k][i] = c->ptune[k].ns_edge[i];
+    }
     o->pull_tune_ns_per_edge[0] = c->bq_ns_edge[0];
     o->pull_tune_ns_per_edge[1] = c->bq_ns_edge[1];
     if (h[C_OVERFLOW]) return fail(EXC_ESTATE, "hub piece buffer overflowed %llu times", h[C_OVERFLOW]);

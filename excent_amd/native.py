@@ -16,7 +16,7 @@ LIBDIR = os.path.join(HERE, "lib")
 BINDIR = os.path.join(ROOT, "bin")
 
 EXC_PR, EXC_KATZ, EXC_EV = 0, 1, 2
-KERNEL_LB, KERNEL_GROUP = 0, 1
+KERNEL_LB, KERNEL_GROUP, KERNEL_TUNE = 0, 1, 2
 EXC_OK, EXC_EINVAL, EXC_ENOMEM, EXC_EHIP, EXC_ESTATE = 0, -1, -2, -3, -4
 
 
@@ -70,7 +70,8 @@ class Stats(ctypes.Structure):
                 ("fold_bytes", c_double), ("pull_bytes", c_double), ("pull_hub_bytes", c_double),
                 ("pull_rows_in_flight", c_int), ("pull_tune_ns_per_edge", c_double * 2),
                 ("pull_tile", c_int), ("pull_w_ms", c_double * 3), ("pull_w_launches", c_u64 * 3),
-                ("pull_w_bytes", c_double * 3), ("pull_w_tile", c_int * 3), ("burst_graph_launches", c_u64)]
+                ("pull_w_bytes", c_double * 3), ("pull_w_tile", c_int * 3), ("burst_graph_launches", c_u64),
+                ("push_tune_ns_edge", (c_double * 3) * 7), ("push_tune_best", c_int * 7)]
 
 
 class RunStats(ctypes.Structure):
@@ -85,7 +86,10 @@ def stats_dict(s):
     out = {}
     for k, _ in Stats._fields_:
         v = getattr(s, k)
-        out[k] = list(v) if not isinstance(v, (int, float)) else v
+        if isinstance(v, (int, float)):
+            out[k] = v
+        else:   # arrays, and arrays of arrays
+            out[k] = [list(e) if not isinstance(e, (int, float)) else e for e in v]
     return out
 
 

@@ -42,8 +42,10 @@ enum {
  * takes one adjacency slot of the block's pushed vertices).  EXC_KERNEL_GROUP: G lanes
  * per vertex, G in {1,2,4,8,16,32,64}, the width chosen from the average degree
  * (src/excent.cu:238, 629-633).  Both split vertices of degree above the hub
- * threshold into pieces pushed by whole waves. */
-enum { EXC_KERNEL_LB = 0, EXC_KERNEL_GROUP = 1 };
+ * threshold into pieces pushed by whole waves.  EXC_KERNEL_TUNE: an autotuner picks per
+ * degree class (the launch's group hint) among the load-balanced kernel and two group
+ * widths by timing each alone on the class's first launches that push enough edges. */
+enum { EXC_KERNEL_LB = 0, EXC_KERNEL_GROUP = 1, EXC_KERNEL_TUNE = 2 };
 
 /* Per-run measure table (src/excent.cu:28-37, 340-351). */
 typedef struct exc_measures {
@@ -64,7 +66,7 @@ typedef struct exc_config {
     int max_chunk_verts;        /* staging slot capacity (vertices) */
     int trace;                  /* allocate the --trace-pending counters */
     int zero_copy;              /* staging buffers mapped into the device (read in place) */
-    int push_kernel;            /* EXC_KERNEL_LB or EXC_KERNEL_GROUP */
+    int push_kernel;            /* EXC_KERNEL_LB, EXC_KERNEL_GROUP or EXC_KERNEL_TUNE */
     int timing;                 /* record HIP events around kernels (exc_stats): 1 every kernel, 2 only the
                                    dense pull (k_pull / k_pull_sorted, the cheapest way to time it) */
     int batch_k;                /* 0: plain measures; 1..64: batched personalised PageRank with
@@ -109,6 +111,11 @@ typedef struct exc_stats {
     int pull_w_tile[3];
     /* push bursts (exc_run_passes) replayed from recorded HIP graphs */
     uint64_t burst_graph_launches;
+    /* EXC_KERNEL_TUNE: per degree class c (the launch's group hint 2^c, c = 0..6), the three
+     * candidates' measured time per pushed edge (ns; 0 = not timed yet) -- the load-balanced
+     * kernel, G = 2^c and G = max(1, 2^c / 4) -- and the one kept (-1 while undecided) */
+    double push_tune_ns_edge[7][3];
+    int push_tune_best[7];
 } exc_stats;
 
 const char* exc_last_error(void);

@@ -121,7 +121,7 @@ def test_converter_edge_cases(tmp_path):
     (["--measure", "pr", "--dedup", "maybe"], "--dedup must be on or off"),
     (["--measure", "pr", "--frobnicate"], "unknown option --frobnicate"),
     (["--measure", "pr", "--max-passes"], "missing value for --max-passes"),
-    (["--measure", "pr", "--push-kernel", "x"], "--push-kernel must be lb or group"),
+    (["--measure", "pr", "--push-kernel", "x"], "--push-kernel must be lb, group or tune"),
     (["--measure", "katz", "--ppr-column", "0"], "--ppr-column needs --measure pr"),
     (["--measure", "pr", "--ppr-column", "0"], "--ppr-column needs one of --ppr-seeds FILE or --ppr-gen"),
     (["--measure", "ppr", "--ppr-gen", "4:2:1:3"], "ppr needs --eps-ppr"),
