@@ -2185,11 +2185,18 @@ struct PieceBuf {
     int phase = 0;
 };
 
+// A staging slot: pinned host buffers, their device copies, and two events on the slot's
+// stream -- `copied` after the host-to-device copies of its last chunk (the host buffers are
+// free again: the next chunk can be read into them while this one is still being pushed; the
+// next copy into the device buffers is ordered behind that push on the same stream), `done`
+// after its push.  Zero-copy slots are read in place by the push, so they free on `done`.
 struct Slot {
     int32_t* h_edges = nullptr; int32_t* d_edges = nullptr;
     int32_t* h_vids = nullptr;  int32_t* d_vids = nullptr;
     uint64_t* h_off = nullptr;  uint64_t* d_off = nullptr;
     hipEvent_t done = nullptr;
+    hipEvent_t copied = nullptr;
+    hipEvent_t frees = nullptr;   // the event exc_slot_acquire waits for
     bool busy = false;
 };
 
@@ -3238,6 +3245,7 @@ int exc_create(const exc_config* cfg, const exc_measures* M, void** out) {
             CK(hipMalloc(&sl.d_off, ob));
         }
         CK(hipEventCreateWithFlags(&sl.done, hipEventDisableTiming));
+        CK(hipEventCreateWithFlags(&sl.copied, hipEventDisableTiming));
     }
     // the clears above ran on the null stream; the context's streams are non-blocking
     CK(hipDeviceSynchronize());
@@ -3269,6 +3277,7 @@ int exc_destroy(void* p) {
         if (sl.h_off) hipHostFree(sl.h_off);
         if (!c->zc) { if (sl.d_edges) hipFree(sl.d_edges); if (sl.d_vids) hipFree(sl.d_vids); if (sl.d_off) hipFree(sl.d_off); }
         if (sl.done) hipEventDestroy(sl.done);
+        if (sl.copied) hipEventDestroy(sl.copied);
     }
     for (auto& pb : c->pb) { if (pb.d) hipFree(pb.d); if (pb.cnt) hipFree(pb.cnt); }
     for (auto s : c->st) if (s) hipStreamDestroy(s);
@@ -3549,7 +3558,7 @@ int exc_slot_acquire(void* p, int s, double* waited) {
     Slot& sl = c->slot[s];
     if (!sl.busy) return EXC_OK;
     auto a = std::chrono::steady_clock::now();
-    HIPCHK(hipEventSynchronize(sl.done));
+    HIPCHK(hipEventSynchronize(sl.frees));
     if (waited) *waited = std::chrono::duration<double>(std::chrono::steady_clock::now() - a).count();
     sl.busy = false;
     return EXC_OK;
@@ -3610,6 +3619,10 @@ int exc_stream_chunk(void* p, int s, const exc_chunk* ch, int to_resident, int h
         if (!c->zc) HIPCHK(hipMemcpyAsync(to_resident ? dst + ch->e0 : dst, sl.h_edges, ne * sizeof(int32_t), hipMemcpyHostToDevice, st));
         else if (to_resident) HIPCHK(hipMemcpyAsync(dst + ch->e0, sl.d_edges, ne * sizeof(int32_t), hipMemcpyDeviceToDevice, st));
     }
+    // the host buffer is free once this copy has run (a zero-copy chunk copied to the cache is
+    // read from the host buffer, so it frees after that copy too)
+    const bool free_after_copy = !c->zc || to_resident;
+    if (free_after_copy) HIPCHK(hipEventRecord(sl.copied, st));
     if (hot_hi > hot_lo && c->d_hedges) {
         ScopedTimer t(c, st, T_OTHER);
         const int32_t* src = to_resident ? dst + ch->e0 : dst;
@@ -3621,6 +3634,7 @@ int exc_stream_chunk(void* p, int s, const exc_chunk* ch, int to_resident, int h
                         to_resident ? c->arena_edges : c->max_chunk_edges);
     if (rc) return rc;
     HIPCHK(hipEventRecord(sl.done, st));
+    sl.frees = free_after_copy ? sl.copied : sl.done;
     sl.busy = true;
     return EXC_OK;
 }
@@ -3641,6 +3655,7 @@ int exc_gather_chunk(void* p, int s, int k, int group) {
         HIPCHK(hipMemcpyAsync(sl.d_vids, sl.h_vids, (size_t)k * sizeof(int32_t), hipMemcpyHostToDevice, st));
         HIPCHK(hipMemcpyAsync(sl.d_off, sl.h_off, ((size_t)k + 1) * sizeof(uint64_t), hipMemcpyHostToDevice, st));
         if (ne) HIPCHK(hipMemcpyAsync(sl.d_edges, sl.h_edges, ne * sizeof(int32_t), hipMemcpyHostToDevice, st));
+        HIPCHK(hipEventRecord(sl.copied, st));   // the host buffers are free again
     }
     if (k) {
         PushArgs a{};
@@ -3653,6 +3668,7 @@ int exc_gather_chunk(void* p, int s, int k, int group) {
         if (rc) return rc;
     }
     HIPCHK(hipEventRecord(sl.done, st));
+    sl.frees = c->zc ? sl.done : sl.copied;
     sl.busy = true;
     return EXC_OK;
 }
