@@ -10,6 +10,7 @@
 #include <algorithm>
 #include <atomic>
 #include <cerrno>
+#include <cstdlib>
 #include <cstring>
 #include <mutex>
 
@@ -91,6 +92,7 @@ EgrGraph::~EgrGraph() {
 }
 
 void EgrGraph::load(bool offsets_in_ram) {
+    if (const char* r = getenv("EXC_READ")) copy_from_map_ = strcmp(r, "pread") != 0;
     struct stat st;
     if (fstat(fd_, &st) != 0) throw egr_error(path_ + ": fstat: " + strerror(errno));
     file_bytes_ = (size_t)st.st_size;
@@ -146,14 +148,33 @@ void EgrGraph::load(bool offsets_in_ram) {
     off_ = ram_off_.data();
 }
 
+// True when every page of [at, at + bytes) of the file is in the page cache (mincore over
+// the mapping).
+bool EgrGraph::resident(size_t at, size_t bytes) const {
+    const size_t P = (size_t)sysconf(_SC_PAGESIZE);
+    const size_t a0 = at / P * P, a1 = std::min(file_bytes_, at + bytes);
+    if (a1 <= a0) return true;
+    std::vector<unsigned char> in((a1 - a0 + P - 1) / P);   // mincore fills one byte per page begun
+    if (mincore(const_cast<char*>(map_) + a0, a1 - a0, in.data()) != 0) return false;
+    for (unsigned char b : in)
+        if (!(b & 1)) return false;
+    return true;
+}
+
 // The I/O pool: blocks of kBlock bytes are claimed from a shared counter by up to
 // io_threads_ workers; the first failure stops the others and is rethrown here.
 void EgrGraph::read_range(void* dst, size_t bytes, size_t at) const {
     if (!bytes) return;
     const size_t nblocks = (bytes + kBlock - 1) / kBlock;
     const int workers = (int)std::min<size_t>((size_t)io_threads_, nblocks);
+    // A range already in the page cache is copied out of the mapping (one copy, no system
+    // call per block: about twice the rate of pread into pinned memory on the MI355X hosts);
+    // a range with pages on storage is read with pread, whose readahead serves it far better
+    // than page faults on a MADV_RANDOM mapping.
+    const bool from_map = copy_from_map_ && resident(at, bytes);
     if (workers == 1) {
-        pread_all(fd_, dst, bytes, at, path_);
+        if (from_map) memcpy(dst, map_ + at, bytes);
+        else pread_all(fd_, dst, bytes, at, path_);
         return;
     }
     std::atomic<size_t> next{0};
@@ -166,6 +187,10 @@ void EgrGraph::read_range(void* dst, size_t bytes, size_t at) const {
             const size_t b = next.fetch_add(1, std::memory_order_relaxed);
             if (b >= nblocks || failed.load(std::memory_order_relaxed)) break;
             const size_t o = b * kBlock, len = std::min(kBlock, bytes - o);
+            if (from_map) {
+                memcpy(static_cast<char*>(dst) + o, map_ + at + o, len);
+                continue;
+            }
             try {
                 pread_all(fd_, static_cast<char*>(dst) + o, len, at + o, path_);
             } catch (const std::exception& e) {

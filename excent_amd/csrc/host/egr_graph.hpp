@@ -8,9 +8,10 @@
 //
 // Design: the whole file is mapped read-only (gathers copy lists straight out of the
 // mapping; the offsets of a wide file are read in place when they are not copied to RAM);
-// streamed chunks are read with pread in 4 MiB blocks that a pool of I/O threads claims
-// from a shared counter, so a slow block does not hold up a fixed share of the range; the
-// number of I/O threads is the caller's (0: every host thread OpenMP offers).  Errors throw
+// streamed chunks are copied in 4 MiB blocks that a pool of I/O threads claims from a shared
+// counter, so a slow block does not hold up a fixed share of the range -- out of the mapping
+// when the whole range is in the page cache, else with pread; the number of I/O threads is
+// the caller's (0: every host thread OpenMP offers).  Errors throw
 // egr_error; the command line turns them into its exit path.
 #pragma once
 
@@ -83,7 +84,9 @@ private:
     const uint64_t* off_ = nullptr;
     const int32_t* cols_ = nullptr;
 
+    bool copy_from_map_ = true;   // EXC_READ=pread: always pread (measurements)
     void load(bool offsets_in_ram);
+    bool resident(size_t at, size_t bytes) const;
     void read_range(void* dst, size_t bytes, size_t at) const;
     void advise_out(size_t b0, size_t b1) const;
 };
