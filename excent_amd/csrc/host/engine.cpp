@@ -267,7 +267,8 @@ private:
     bool have_l1_ = false;
     int stop_reason_ = 0;               // 0 nothing pending, 1 global L1 bound, 2 pass limit
     std::vector<char> global_l1_;       // per measure slot: the global L1 rule may end it (--stop)
-    int act_ = 1;                       // measure slots the dense passes update (exc_set_active_measures)
+    int act_ = 1;                       // measure slots the passes update (exc_set_active_measures)
+    bool dropped_global_ = false;       // a measure was frozen by the global rule with vertices above eps
     int burst_ = 1;                     // passes per burst (exc_run_passes); 1: every pass synchronises
     float linf_[EXC_MAXM] = {0, 0, 0};  // largest |r| per measure, last snapshot
 
@@ -714,6 +715,8 @@ void Engine::run_passes() {
     for (char g1 : global_l1_) stop_global = stop_global || g1;
     if (pull_on_) ck(exc_fold_ahead(ctx, 1), "fold ahead");
     act_ = M_.nm;   // a run starts from a replaced state: every measure active
+    dropped_global_ = false;
+    bool drop_wait = false;        // a measure waits to be dropped until a pass has not folded
     bool fold_ahead_off = false;   // set once a folded state met the global bound
     const Clock::time_point T_begin = Clock::now();
     const size_t words = ((size_t)n + 31) / 32;
@@ -773,14 +776,26 @@ void Engine::run_passes() {
         const PlanOut po = fetch_plan(nc, h_cnt.data(), resident_, loaded_, stream_ratio_, chunk_edges, hot_mode_, pass_, hlo_, hhi_, mode.data());
         // FP64 global residual of the snapshot state (plain measures)
         have_l1_ = !batched() && exc_residual_l1(ctx, l1_) == EXC_OK && exc_residual_max(ctx, linf_) == EXC_OK;
-        if (pull_on_ && have_l1_ && M_.nm > 1 && o.drop_measures == "on") {
-            // dense passes need only the measures up to the last one still pending; the others
-            // have converged for good (a measure's residual moves only by its own pushes)
+        drop_wait = false;
+        if (have_l1_ && M_.nm > 1 && o.drop_measures == "on") {
+            // A measure is done once nothing is pending in it, or, where the global rule may end
+            // it (--stop), once its FP64 sum |r| <= n eps; the passes then leave out the done
+            // measures after the last one still going (they are frozen: never pushed again, so
+            // their state and sums stay as the rule found them).
             int act = 0;
-            for (int i = 0; i < M_.nm; i++) if (linf_[i] > M_.eps[i]) act = i + 1;
+            for (int i = 0; i < M_.nm; i++) {
+                const bool done = linf_[i] <= M_.eps[i] || (global_l1_[i] && l1_[i] <= (double)n * (double)M_.eps[i]);
+                if (!done) act = i + 1;
+            }
             if (act >= 1 && act < act_) {
-                ck(exc_set_active_measures(ctx, act), "active measures");
-                act_ = act;
+                if (exc_set_active_measures(ctx, act) == EXC_OK) {
+                    for (int i = act; i < act_; i++) dropped_global_ = dropped_global_ || linf_[i] > M_.eps[i];
+                    act_ = act;
+                } else if (exc_dense_required(ctx) == 1) {
+                    drop_wait = true;   // the last pull folded a pass that holds their deltas: no fold this pass
+                } else {
+                    die(std::string("HIP layer error in active measures: ") + exc_last_error());
+                }
             }
         }
         bool global_met = false;
@@ -798,10 +813,10 @@ void Engine::run_passes() {
         }
         if (po.need_bitmap && !global_met) ck(exc_bitmap_to_host(ctx, h_bitmap.data()), "bitmap readback");
         t_snap_ += secs(a, Clock::now());
-        if (po.pend_v == 0 && h_hotpend == 0) { stop_reason_ = 0; break; }
+        if (po.pend_v == 0 && h_hotpend == 0) { stop_reason_ = dropped_global_ ? 1 : 0; break; }
         if (global_met) { stop_reason_ = 1; break; }
         // a dense pass folds the pass after it only when the loop will go on from its state
-        if (pull_on_) ck(exc_fold_ahead(ctx, !fold_ahead_off && pass_ + 1 < o.max_passes), "fold ahead");
+        if (pull_on_) ck(exc_fold_ahead(ctx, !fold_ahead_off && !drop_wait && pass_ + 1 < o.max_passes), "fold ahead");
 
         // A burst: the next burst_ passes over the whole cached graph enqueued together with one
         // host synchronisation (exc_run_passes).  The per-vertex rule ends the run at the first
@@ -836,7 +851,7 @@ void Engine::run_passes() {
                 for (int c = 0; c < nc; c++) after += b_cnt[(size_t)i * 2 * nc + 2 * c];
                 if (after == 0) { converged = true; pass_ = first + i + 1; }
             }
-            if (converged) { stop_reason_ = 0; break; }
+            if (converged) { stop_reason_ = dropped_global_ ? 1 : 0; break; }
             pass_ = first + B - 1;   // the loop statement counts the last one
             continue;
         }

@@ -2979,6 +2979,16 @@ int enqueue_burst(Ctx* c, int passes, int dense) {
     return EXC_OK;
 }
 
+// Measure slots from `count` on are frozen: their eps becomes infinite in the device measure
+// table, so no kernel finds a vertex pending in them (they are neither pushed nor folded,
+// their x and r stay as they are, and snapshots still sum their |r|); dense passes gather
+// rows of the first `count` measures only.  count = nm restores every slot.
+void set_active(Ctx* c, int count) {
+    c->nm_act = count;
+    for (int k = 0; k < c->nm; k++) c->DM.eps[k] = k < count ? c->M.eps[k] : INFINITY;
+    burst_graphs_clear(c);   // recorded launches carry the measure table by value
+}
+
 // Push bursts replayed from graphs: a push burst's launches are fixed by its length and the
 // phase of stream 0's piece counters (it flips at every push), so one recorded graph per
 // (length, phase) serves every later burst.  Anything that changes a buffer or a by-value
@@ -3315,7 +3325,7 @@ int exc_init_measure(void* p, int i, float beta, float lambda) {
     c->snap_ready = false;   // the state changes: a fused snapshot no longer holds
     c->l1_valid = false;
     c->folded = false;       // the state is replaced: a folded pass no longer applies
-    c->nm_act = c->nm;       // ... and every measure may be pending again
+    set_active(c, c->nm);    // ... and every measure may be pending again
     if (c->K) c->active = all_columns(c->K);
     if (c->K) {   // batched: every column starts at x = r = 0 (exc_batch_seed adds the teleport mass)
         const size_t bytes = (size_t)c->n * c->S * sizeof(float);
@@ -3339,7 +3349,7 @@ int exc_set_state(void* p, int i, const float* h_x, const float* h_r) {
     c->snap_ready = false;   // the state changes: a fused snapshot no longer holds
     c->l1_valid = false;
     c->folded = false;       // the state is replaced: a folded pass no longer applies
-    c->nm_act = c->nm;       // ... and every measure may be pending again
+    set_active(c, c->nm);    // ... and every measure may be pending again
     if (c->K) c->active = all_columns(c->K);
     float* tmp;
     const size_t bytes = (size_t)c->n * sizeof(float);
@@ -3384,7 +3394,7 @@ int exc_state_checkpoint(void* p, int restore) {
     const size_t bytes = (size_t)c->n * c->S * sizeof(float);
     if (restore) {
         c->folded = false;
-        c->nm_act = c->nm;
+        set_active(c, c->nm);
         if (!c->d_x0) return fail(EXC_ESTATE, "exc_state_checkpoint: nothing saved");
         HIPCHK(hipMemcpyAsync(c->d_x, c->d_x0, bytes, hipMemcpyDeviceToDevice, c->st[0]));
         HIPCHK(hipMemcpyAsync(c->d_r, c->d_r0, bytes, hipMemcpyDeviceToDevice, c->st[0]));
@@ -3736,7 +3746,7 @@ int exc_batch_seed(void* p, int col, const int32_t* seeds, int count, float valu
     c->snap_ready = false;   // the state changes: a fused snapshot no longer holds
     c->l1_valid = false;
     c->folded = false;       // the state is replaced: a folded pass no longer applies
-    c->nm_act = c->nm;       // ... and every measure may be pending again
+    set_active(c, c->nm);    // ... and every measure may be pending again
     if (c->K) c->active = all_columns(c->K);
     int32_t* d;
     HIPCHK(hipMalloc(&d, (size_t)count * sizeof(int32_t)));
@@ -3778,7 +3788,7 @@ int exc_set_edge_weights(void* p, const float* h_w, uint64_t cnt) {
     c->snap_ready = false;   // the state changes: a fused snapshot no longer holds
     c->l1_valid = false;
     c->folded = false;       // the state is replaced: a folded pass no longer applies
-    c->nm_act = c->nm;       // ... and every measure may be pending again
+    set_active(c, c->nm);    // ... and every measure may be pending again
     burst_graphs_clear(c);   // recorded pushes ran unweighted
     if (!c->d_wts) HIPCHK(hipMalloc(&c->d_wts, (c->m ? c->m : 1) * sizeof(float)));
     if (c->m) HIPCHK(hipMemcpy(c->d_wts, h_w, c->m * sizeof(float), hipMemcpyHostToDevice));
@@ -4188,12 +4198,15 @@ int exc_set_active_measures(void* p, int count) {
                     c->nm_act);
     if (count < c->nm_act) {
         if (!c->l1_valid) return fail(EXC_ESTATE, "exc_set_active_measures: no snapshot yet");
-        for (int k = count; k < c->nm; k++)
-            if (c->linf_last[k] > c->M.eps[k])
-                return fail(EXC_ESTATE, "exc_set_active_measures: measure slot %d still has |r| %g > eps %g", k,
-                            (double)c->linf_last[k], (double)c->M.eps[k]);
+        // a folded pass holds the deltas of every slot that had a vertex above eps: narrowing
+        // then would lose them
+        if (c->folded)
+            for (int k = count; k < c->nm_act; k++)
+                if (c->linf_last[k] > c->M.eps[k])
+                    return fail(EXC_ESTATE, "exc_set_active_measures: the last dense pass folded slot %d (|r| %g > eps %g)",
+                                k, (double)c->linf_last[k], (double)c->M.eps[k]);
     }
-    c->nm_act = count;
+    set_active(c, count);
     return EXC_OK;
 }
 

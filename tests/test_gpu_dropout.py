@@ -6,9 +6,9 @@ Checks: a measure dropped from the dense passes keeps its x and r bit for bit fr
 it was dropped on (its residual moves only by its own pushes, and it had none left), and its
 snapshot sums stay those of that pass; the measures still active converge as without
 dropping, to the exact FP64 bounds and the invariant x + r = b + W x; the passes run at the
-narrower widths (per-width launch counts); the call refuses to drop a measure with a vertex
-above eps, to widen while a folded pass is pending, counts outside 1..nm, and batched
-contexts; and bin/excent's --drop-measures on / off end within the same bounds."""
+narrower widths (per-width launch counts); slots frozen while still pending (the global
+rule) stay unchanged through dense and push passes; the refusals; and bin/excent's
+--drop-measures on / off end within the same bounds."""
 import numpy as np
 import pytest
 
@@ -126,7 +126,11 @@ def test_dropping_matches_full_width(rmat12):
     assert np.abs(a[0][0].astype(np.float64) - b[0][0]).sum() <= 2 * 1e-4 * n / 0.15
 
 
-def test_set_active_measures_refusals(rmat12):
+def test_freeze_pending_slots_and_refusals(rmat12):
+    """Slots frozen while they still have vertices above eps (what the global rule does) stay
+    bit for bit unchanged through dense and push passes, and resume when widened again; the
+    call refuses before any snapshot, counts outside 1..nm, narrowing over a folded pass that
+    holds the frozen slots' deltas, widening while a pass is folded, and batched contexts."""
     idx, adj = orc.egr_load(rmat12)
     n, m = len(idx) - 1, len(adj)
     kinds, eps, scale, divdeg, inits = katz_slots(idx, adj)
@@ -136,35 +140,46 @@ def test_set_active_measures_refusals(rmat12):
             ctx.set_state(i, x=x0, r=r0)
         assert ctx.cache_chunks(m) == 0
         ctx.cache_upload(adj, 0)
-        assert ctx.enable_pull(0.0) == 0
+        assert ctx.enable_pull(0.3) == 0
         with pytest.raises(nat.ExcError, match="no snapshot"):
             ctx.set_active_measures(2)
         for bad in (0, 4):
             with pytest.raises(nat.ExcError, match="1..3"):
                 ctx.set_active_measures(bad)
         ctx.snapshot()
-        with pytest.raises(nat.ExcError, match="still has"):   # every slot is pending at the start
-            ctx.set_active_measures(1)
-        ctx.set_active_measures(3)   # no change is always allowed
-        # run until slot 2 has converged, drop it, and widen again while a pass is folded
-        for _ in range(200):
+        ctx.set_active_measures(1)   # every slot is pending: slots 1 and 2 freeze as they are
+        frozen = [(ctx.read_state(i, 0), ctx.read_state(i, 1)) for i in (1, 2)]
+        for p in range(12):   # dense passes first (demand >= 0.3 m), then pushes
+            ctx.push_resident(0, n, 0, m, p % 2 == 0)
+            ctx.pass_sync()
+            ctx.snapshot()
+        for i, (x, r) in zip((1, 2), frozen):
+            assert np.array_equal(ctx.read_state(i, 0), x) and np.array_equal(ctx.read_state(i, 1), r), i
+        assert ctx.stats()["pull_w_launches"][0] > 0
+        # widen again (a folded state must be continued by a dense pass first)
+        while ctx.dense_required():
             ctx.push_resident(0, n, 0, m, False)
             ctx.pass_sync()
             ctx.snapshot()
-            if np.float32(ctx.residual_max()[2]) <= np.float32(eps[2]):
-                break
-        ctx.set_active_measures(2)
-        ctx.push_resident(0, n, 0, m, False)   # a dense pass at high demand folds the next one
+        ctx.set_active_measures(3)
+        # a dense pass at full demand folds the next pass over all three slots; narrowing now,
+        # with slot 2 still above eps, would lose its folded deltas
+        ctx.push_resident(0, n, 0, m, False)
         ctx.pass_sync()
         ctx.snapshot()
-        if ctx.dense_required():
+        if ctx.dense_required() and np.float32(ctx.residual_max()[2]) > np.float32(eps[2]):
             with pytest.raises(nat.ExcError, match="folded"):
-                ctx.set_active_measures(3)
-        # replacing the state makes every slot active again
-        ctx.set_state(2, x=inits[2][0], r=inits[2][1])
-        ctx.snapshot()
-        with pytest.raises(nat.ExcError, match="still has"):
-            ctx.set_active_measures(2)
+                ctx.set_active_measures(2)
+        # run to convergence at full width from here: the bounds hold for every slot
+        for _ in range(3000):
+            cnt, _ = ctx.snapshot()
+            if cnt[0::2].sum() == 0:
+                break
+            ctx.push_resident(0, n, 0, m, False)
+            ctx.pass_sync()
+        xs = [ctx.read_state(i, 0) for i in range(3)]
+        rs = [ctx.read_state(i, 1) for i in range(3)]
+    check_results(idx, adj, xs, rs, eps, scale)
     with nat.Context(n, m, np.array([0], np.int32), [nat.EXC_PR], [1e-4], [0.85], [1], batch_k=4) as ctx:
         with pytest.raises(nat.ExcError, match="batched"):
             ctx.set_active_measures(1)
@@ -198,4 +213,18 @@ def test_cli_drop_measures(tmp_path, rmat14):
         x = out[mode][1][0]
         tr = true_residual(idx, adj, x, b, 0.85, True)
         assert np.abs(x.astype(np.float64) - ref).sum() <= np.abs(tr).sum() / 0.15 * (1 + 1e-6) + 1e-6, mode
+    # --stop katz,ev: Katz and the eigenvector freeze once their FP64 sums are within n eps
+    args = [rmat14, "--measure", "all", "--alpha", 0.5 / lam, "--beta", 1, "--lambda", lam, "--eps-pr", 1e-3,
+            "--eps-katz", 1e-6, "--eps-ev", 1e-6, "--ev-init", "indeg", "--cache", "chunks", "--preload",
+            "--max-passes", 3000, "--stop", "katz,ev", "--log", str(tmp_path / "g.log")]
+    with nat.Session(args) as S:
+        r = S.run(reinit=False)
+        R = [S.values(i, 1) for i in range(3)]
+    assert r["stop_reason"] in (0, 1)
+    assert np.abs(R[0]).max() <= np.float32(1e-3)
+    for i in (1, 2):
+        l1 = np.abs(R[i].astype(np.float64)).sum()
+        assert np.abs(R[i]).max() <= np.float32(1e-6) or l1 <= n * float(np.float32(1e-6)), i
+        assert r["residual_l1"][i] == pytest.approx(l1, rel=1e-12)
+    assert r["kernel"]["pull_w_launches"][0] > 0
     assert nat.excent([rmat14, "--drop-measures", "maybe", "--log", str(tmp_path / "bad.log")]) == 1
