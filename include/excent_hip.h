@@ -240,15 +240,16 @@ int exc_dense_required(void* ctx);
  * limit, or a stop test it will apply to the next snapshot -- turns it off, so the state it
  * stops on holds every delta it has folded (x + r = b + W x).  No reference counterpart. */
 int exc_fold_ahead(void* ctx, int allow);
-/* Dense passes over the first `count` measure slots only (1 <= count <= nm; nm, every slot,
- * after exc_create and after any call that replaces the state).  Lowering the count needs a
- * snapshot in which every slot from `count` on has no pending vertex (largest |r| <= eps;
- * EXC_ESTATE otherwise): a slot's residual changes only through its own pushes, so such a
- * slot stays converged and its values and snapshot sums stay as they were.  Passes then
- * gather rows of `count` weights instead of nm (one measure: the source-sorted pull).
- * Raising it is refused while the last dense pass has folded the next one.  Push passes and
- * the standalone snapshot always cover every slot.  No reference counterpart (the
- * reference pushes every measure of a pending vertex; a converged measure has none). */
+/* Measure slots from `count` on frozen (1 <= count <= nm; nm, every slot, after exc_create
+ * and after any call that replaces the state): their eps becomes infinite for every kernel,
+ * so no vertex is pending in them -- they are neither pushed nor folded, their x and r stay
+ * as they are, and snapshots still sum their |r| -- and dense passes gather rows of the first
+ * `count` measures only (one measure: the source-sorted pull).  The caller decides when a
+ * slot is done (bin/excent: nothing pending in it, or its FP64 residual sum within n eps
+ * under the global rule).  Needs a snapshot; narrowing is refused (EXC_ESTATE) while the
+ * last dense pass has folded the next one and a slot being frozen had a vertex above eps
+ * (its folded deltas would be lost), and widening while any pass is folded.  No reference
+ * counterpart (the reference pushes every measure of a pending vertex). */
 int exc_set_active_measures(void* ctx, int count);
 /* A burst: `passes` (1..EXC_BURST_MAX) passes over the whole cached graph enqueued back to back
  * with one host synchronisation at the end -- dense passes (dense = 1; exc_enable_pull) or
