@@ -238,6 +238,8 @@ private:
     int n_ = 0, nc_ = 0, push_kernel_ = EXC_KERNEL_LB, cus_ = 0;
     int threads_ = 1;   // host threads for reads, gathers and checks
     uint64_t m_ = 0, cap_ = 0;
+    uint64_t slot_edges_ = 0;           // staging slot capacity: edges, vertices
+    int slot_verts_ = 0;
     char arch_[64] = {0};
     bool zc_ = false, use_cache_ = false, hot_mode_ = false;
     std::vector<EdgeChunk> chunks_;
@@ -398,6 +400,8 @@ void Engine::setup() {
     exc_config cfg{};
     cfg.device = o.device; cfg.n = n; cfg.m = m; cfg.nc = nc; cfg.cv0 = cv0.data();
     cfg.max_chunk_edges = max_chunk_edges; cfg.max_chunk_verts = max_chunk_verts;
+    slot_edges_ = max_chunk_edges;
+    slot_verts_ = max_chunk_verts;
     cfg.trace = !o.trace_file.empty(); cfg.zero_copy = zc_; cfg.push_kernel = push_kernel_; cfg.timing = o.timing;
     if (batched()) {
         load_ppr_seeds(g);
@@ -906,8 +910,21 @@ void Engine::run_passes() {
                 n_launch_++;
                 if (resident_[c]) loaded_[c] = 1;
             } else {
-                ngather++;
-                const int k = gather_list(h_bitmap.data(), ch.v0, ch.v1, h_vids_[s]);
+                // consecutive gathered chunks are staged together while their lists fit one
+                // slot (the counts are exact: the snapshot's count and degree sum of a chunk
+                // are its bitmap's pending vertices and their edges), so a pass with a few
+                // pending vertices over many chunks makes one copy and one push, not one each
+                int c1 = c + 1;
+                uint64_t kk = h_cnt[2 * c], ee = h_cnt[2 * c + 1];
+                while (c1 < nc && mode[c1] == F_GATHER && kk + h_cnt[2 * c1] <= (uint64_t)slot_verts_ &&
+                       ee + h_cnt[2 * c1 + 1] <= slot_edges_) {
+                    kk += h_cnt[2 * c1];
+                    ee += h_cnt[2 * c1 + 1];
+                    c1++;
+                }
+                ngather += c1 - c;
+                const int k = gather_list(h_bitmap.data(), ch.v0, chunks_[c1 - 1].v1, h_vids_[s]);
+                c = c1 - 1;
                 const GatherStats gs = stage_lists(g, h_vids_[s], k, h_off_[s], h_edges_[s], o.dedup, threads_);
                 const uint64_t ne = gs.edges;
                 pass_pages += gs.pages;
