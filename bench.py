@@ -83,6 +83,21 @@ def top_overlap(np, a, b, k):
     return len(top(a) & top(b))
 
 
+def kendall_top(np, x, ref, t=1000):
+    """Kendall tau over the t largest entries of ref, ranked by (value desc, index asc):
+    (concordant - discordant) pairs / all pairs, ties in either vector counting neither
+    (tools/centrality_reference.cu:111-136)."""
+    t = min(t, len(ref))
+    if t < 2:
+        return 1.0
+    part = np.argpartition(-ref, t - 1)[:t]
+    top = part[np.lexsort((part, -ref[part]))]
+    r, v = ref[top], np.asarray(x, dtype=np.float64)[top]
+    s = np.sign(r[:, None] - r[None, :]) * np.sign(v[:, None] - v[None, :])
+    iu = np.triu_indices(t, 1)
+    return float(s[iu].sum() / (t * (t - 1) / 2))
+
+
 def fp64_errors(nat, np, graph, work, measure, env):
     """Errors of the last timed run's values (the --dump files) against the product's FP64
     solver, and the FP64 global residual of the float32 state (tools/centrality_reference.cu:
@@ -110,7 +125,7 @@ def fp64_errors(nat, np, graph, work, measure, env):
             d = np.abs(xv - ref)
             e = {"l1_rel": float(d.sum() / np.abs(ref).sum()), "l2": float(np.sqrt((d * d).sum())),
                  "max_abs": float(d.max()), "top100_overlap": top_overlap(np, xv, ref, 100),
-                 "top1000_overlap": top_overlap(np, xv, ref, 1000),
+                 "top1000_overlap": top_overlap(np, xv, ref, 1000), "tau1000": kendall_top(np, xv, ref),
                  "residual_l1_fp64": res, "ref_iters": it, "ref_rel_change": rel}
             if name == "pr":
                 e["l1_bound"] = res / (1 - 0.85)
@@ -261,7 +276,8 @@ def bench_batched(args, nat, np, work, graph, sample=None):
                 e[tag] = {"l1_rel": round(float(dx.sum() / np.abs(ref).sum()), 9),
                           "max_abs": round(float(dx.max()), 9),
                           "l1_bound_rel": round(S.residual(x, cvec[j], d, 1) / (1 - 0.85) / np.abs(ref).sum(), 9),
-                          "top1000_overlap": top_overlap(np, x.astype(np.float64), ref, 1000)}
+                          "top1000_overlap": top_overlap(np, x.astype(np.float64), ref, 1000),
+                          "tau1000": round(kendall_top(np, x, ref), 6)}
             fp64[f"column {j}"] = e
             log(f"fp64 column {j}: {e}")
     vb = rb["kernel"]["pushed_edge_measures"]

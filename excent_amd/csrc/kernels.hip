@@ -1204,9 +1204,13 @@ struct SortedTiles {
     const float* w;          // weighted graphs: the entry's coefficient
 };
 
+// A tile's accumulators (NM x T floats) must fit the block's LDS budget of 64 KB.
+constexpr bool sorted_fits(int nm, int T) { return (long long)nm * T * 4 <= 65536; }
+
 template <int NM, int S, int T, bool W = false>
 __global__ __launch_bounds__(BLOCK) void k_pull_sorted(PullArgs a, SortedTiles st, DevMeasures M) {
-    constexpr int U = 8;   // entries per lane in flight
+    static_assert(sorted_fits(NM, T), "tile accumulators exceed the LDS budget");
+    constexpr int U = T >= 16384 ? 16 : 8;   // entries per lane in flight (fewer blocks per CU at large tiles)
     __shared__ float s_acc[NM][T];
     __shared__ unsigned long long s_red[NWAVES];
     const int tid = threadIdx.x;
@@ -2257,7 +2261,7 @@ struct Ctx {
     double pull_fixed_bytes[3] = {}, hub_fixed_bytes[3] = {}, fold_fixed_bytes = 0;
     uint64_t pull_launches_w[3] = {};
     int pull_tile_w[3] = {};                   // the sorted tile each width ran with (0: k_pull)
-    bool sorted_all = false;                   // EXC_PULL_TILE: the sorted pull at every width
+    int sorted_w = 1;                          // widths up to this run the sorted pull (where its LDS fits)
     excent_dev::SortedTileBuffers sorted;      // source-sorted dense pull (k_pull_sorted)
     int sorted_T = 0;                          // its tile (destinations per block); 0: k_pull
     int32_t* d_hub_keys = nullptr;             // destinations privatised by k_pieces (HubTable)
@@ -2618,11 +2622,21 @@ int push_launch(Ctx* c, int s, int src, unsigned grid, PushArgs a, int group, ui
     return EXC_OK;
 }
 
-// Sorted-pull launch for tile T (one of 512 / 1024 / 2048 / 4096).
+// Sorted-pull launch for tile T (one of 512 ... 16384; sorted_fits(NM, T) holds).
 template <int NM, int S, bool W>
 void launch_sorted(Ctx* c, hipStream_t st, const PullArgs& pa) {
     const SortedTiles tiles{c->sorted.start, c->sorted.row, c->sorted.dst, c->sorted.w};
     const unsigned grid = (unsigned)(((long long)c->n + c->sorted_T - 1) / c->sorted_T);
+    if constexpr (sorted_fits(NM, 16384))
+        if (c->sorted_T == 16384) {
+            hipLaunchKernelGGL((k_pull_sorted<NM, S, 16384, W>), dim3(grid), dim3(BLOCK), 0, st, pa, tiles, c->DM);
+            return;
+        }
+    if constexpr (sorted_fits(NM, 8192))
+        if (c->sorted_T == 8192) {
+            hipLaunchKernelGGL((k_pull_sorted<NM, S, 8192, W>), dim3(grid), dim3(BLOCK), 0, st, pa, tiles, c->DM);
+            return;
+        }
     if (c->sorted_T == 4096) hipLaunchKernelGGL((k_pull_sorted<NM, S, 4096, W>), dim3(grid), dim3(BLOCK), 0, st, pa, tiles, c->DM);
     else if (c->sorted_T == 512) hipLaunchKernelGGL((k_pull_sorted<NM, S, 512, W>), dim3(grid), dim3(BLOCK), 0, st, pa, tiles, c->DM);
     else if (c->sorted_T == 1024) hipLaunchKernelGGL((k_pull_sorted<NM, S, 1024, W>), dim3(grid), dim3(BLOCK), 0, st, pa, tiles, c->DM);
@@ -2689,7 +2703,7 @@ void launch_fold_pull(Ctx* c, hipStream_t st) {
         hipLaunchKernelGGL((k_pull_hub<NM, S>), dim3((unsigned)std::min(c->n_pull_hubs, c->cus * 8)), dim3(BLOCK), 0, st,
                            pa, c->d_pull_hubs, c->n_pull_hubs, c->DM);
     }
-    const bool sorted = c->sorted_T && (NM == 1 || c->sorted_all);
+    const bool sorted = c->sorted_T && NM <= c->sorted_w && sorted_fits(NM, c->sorted_T);
     {
         ScopedTimer t(c, st, full ? T_PULL : NM == 1 ? T_PULL_A1 : T_PULL_A2);
         bool weighted = false;
@@ -4104,8 +4118,9 @@ int exc_enable_pull(void* p, double threshold) {
     // first measure once the others are dropped).  With one measure it cuts the launch by a
     // sixth at R-MAT scale 24 (its L1-to-L2 requests drop by a third); over two or three
     // measures its LDS atomics per entry cost more than that saves, so k_pull runs those.
-    // EXC_PULL_TILE = 512 / 1024 / 2048 / 4096 picks the tile and the sorted pull at every
-    // width, 0 keeps k_pull.
+    // EXC_PULL_TILE = T (512 / 1024 / 2048 / 4096 / 8192 / 16384) picks the tile and the
+    // sorted pull at every width whose accumulators fit (NM x T floats <= 64 KB), "T/w" at
+    // widths up to w only; 0 keeps k_pull.
     if (!c->K) {
         // the largest tile that still gives every CU four blocks; small graphs keep k_pull
         int T = 0;
@@ -4113,9 +4128,11 @@ int exc_enable_pull(void* p, double threshold) {
             if ((long long)c->n / t >= 4ll * c->cus) { T = t; break; }
         if (const char* ev = getenv("EXC_PULL_TILE")) {
             T = atoi(ev);
-            c->sorted_all = T != 0;
+            const char* sl = strchr(ev, '/');
+            c->sorted_w = sl ? std::max(1, atoi(sl + 1)) : 3;
         }
-        if ((T == 512 || T == 1024 || T == 2048 || T == 4096) && off[n] < (1ull << 31) && c->nsrc > 0) {
+        if ((T == 512 || T == 1024 || T == 2048 || T == 4096 || T == 8192 || T == 16384) && off[n] < (1ull << 31) &&
+            c->nsrc > 0) {
             int bits = 1;
             while (bits < 31 && (1ll << bits) < (long long)c->nsrc) bits++;
             const hipError_t se = excent_dev::build_sorted_tiles(c->n, T, c->d_tidx, c->d_tadj, c->d_twts, off[n], bits, 0,

@@ -87,11 +87,12 @@ def check_results(idx, adj, xs, rs, eps, scale):
         assert np.abs(rs[i]).max() <= np.float32(eps[i]), i
 
 
-@pytest.mark.parametrize("tile", ["0", "512"])
+@pytest.mark.parametrize("tile", ["0", "512", "8192/1"])
 def test_dropped_measures_stay_and_active_converge(rmat12, tile, monkeypatch):
     """Slots drop in the order 2, 1 (widths 3 -> 2 -> 1); each dropped slot's x and r are
     unchanged to the end; the results meet the FP64 bounds; launches ran at every width.
-    tile 512: the source-sorted pull at every width (EXC_PULL_TILE), else k_pull."""
+    tile 512: the source-sorted pull at every width (EXC_PULL_TILE); 8192/1: at width 1
+    only; else k_pull."""
     print("case dropout tile", tile, flush=True)
     monkeypatch.setenv("EXC_PULL_TILE", tile)
     idx, adj = orc.egr_load(rmat12)
@@ -108,7 +109,9 @@ def test_dropped_measures_stay_and_active_converge(rmat12, tile, monkeypatch):
     per = [st["pull_w_bytes"][k] / w[k] for k in range(3) if w[k]]
     assert all(b > 0 for b in per)
     if tile != "0":
-        assert st["pull_w_tile"][0] == 512
+        assert st["pull_w_tile"][0] == int(tile.split("/")[0])
+    if tile.endswith("/1"):
+        assert st["pull_w_tile"][1] == 0 and st["pull_w_tile"][2] == 0
 
 
 def test_dropping_matches_full_width(rmat12):

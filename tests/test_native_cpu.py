@@ -188,6 +188,29 @@ def test_bench_top_overlap_known_answers():
     assert bench.top_overlap(np, a[:50], a[:50], 1000) == 50   # k capped at the length
 
 
+def test_bench_kendall_top():
+    """bench.py's tau@1000 against the pairwise count over the reference's top t, on tiny
+    inputs, and scipy's tau (the same as tau-a when nothing ties)."""
+    import bench
+    from scipy.stats import kendalltau
+    rng = np.random.default_rng(7)
+    ref = rng.random(3000)
+    assert bench.kendall_top(np, ref, ref) == 1.0
+    assert bench.kendall_top(np, -ref, ref) == -1.0
+    x = ref + rng.normal(0, 0.02, ref.size)
+    top = np.argsort(-ref, kind="stable")[:1000]
+    assert abs(bench.kendall_top(np, x, ref) - kendalltau(ref[top], x[top])[0]) < 1e-12
+    ref6, x6 = np.array([5.0, 4, 3, 2, 1, 0]), np.array([5.0, 3, 4, 2, 2, 9])
+    t, c, d = 4, 0, 0   # top 4 of ref6 = indices 0..3; pairs (1,2) discordant, the rest concordant
+    for i in range(t):
+        for j in range(i + 1, t):
+            p = (ref6[i] - ref6[j]) * (x6[i] - x6[j])
+            c, d = c + (p > 0), d + (p < 0)
+    assert (c, d) == (5, 1)
+    assert bench.kendall_top(np, x6, ref6, t) == (c - d) / 6
+    assert bench.kendall_top(np, x6[:1], ref6[:1]) == 1.0
+
+
 def test_build_sample_graph_and_constants():
     """run.sh --sample's graph is this build's own: the seeded symmetric R-MAT the generator
     writes (same bytes when written again), with the constants bin/excent_params wrote for it
