@@ -1210,7 +1210,7 @@ constexpr bool sorted_fits(int nm, int T) { return (long long)nm * T * 4 <= 6553
 template <int NM, int S, int T, bool W = false>
 __global__ __launch_bounds__(BLOCK) void k_pull_sorted(PullArgs a, SortedTiles st, DevMeasures M) {
     static_assert(sorted_fits(NM, T), "tile accumulators exceed the LDS budget");
-    constexpr int U = T >= 16384 ? 16 : 8;   // entries per lane in flight (fewer blocks per CU at large tiles)
+    constexpr int U = 8;   // entries per lane in flight
     __shared__ float s_acc[NM][T];
     __shared__ unsigned long long s_red[NWAVES];
     const int tid = threadIdx.x;
@@ -2622,21 +2622,11 @@ int push_launch(Ctx* c, int s, int src, unsigned grid, PushArgs a, int group, ui
     return EXC_OK;
 }
 
-// Sorted-pull launch for tile T (one of 512 ... 16384; sorted_fits(NM, T) holds).
+// Sorted-pull launch for tile T (one of 512 / 1024 / 2048 / 4096).
 template <int NM, int S, bool W>
 void launch_sorted(Ctx* c, hipStream_t st, const PullArgs& pa) {
     const SortedTiles tiles{c->sorted.start, c->sorted.row, c->sorted.dst, c->sorted.w};
     const unsigned grid = (unsigned)(((long long)c->n + c->sorted_T - 1) / c->sorted_T);
-    if constexpr (sorted_fits(NM, 16384))
-        if (c->sorted_T == 16384) {
-            hipLaunchKernelGGL((k_pull_sorted<NM, S, 16384, W>), dim3(grid), dim3(BLOCK), 0, st, pa, tiles, c->DM);
-            return;
-        }
-    if constexpr (sorted_fits(NM, 8192))
-        if (c->sorted_T == 8192) {
-            hipLaunchKernelGGL((k_pull_sorted<NM, S, 8192, W>), dim3(grid), dim3(BLOCK), 0, st, pa, tiles, c->DM);
-            return;
-        }
     if (c->sorted_T == 4096) hipLaunchKernelGGL((k_pull_sorted<NM, S, 4096, W>), dim3(grid), dim3(BLOCK), 0, st, pa, tiles, c->DM);
     else if (c->sorted_T == 512) hipLaunchKernelGGL((k_pull_sorted<NM, S, 512, W>), dim3(grid), dim3(BLOCK), 0, st, pa, tiles, c->DM);
     else if (c->sorted_T == 1024) hipLaunchKernelGGL((k_pull_sorted<NM, S, 1024, W>), dim3(grid), dim3(BLOCK), 0, st, pa, tiles, c->DM);
@@ -4118,9 +4108,10 @@ int exc_enable_pull(void* p, double threshold) {
     // first measure once the others are dropped).  With one measure it cuts the launch by a
     // sixth at R-MAT scale 24 (its L1-to-L2 requests drop by a third); over two or three
     // measures its LDS atomics per entry cost more than that saves, so k_pull runs those.
-    // EXC_PULL_TILE = T (512 / 1024 / 2048 / 4096 / 8192 / 16384) picks the tile and the
-    // sorted pull at every width whose accumulators fit (NM x T floats <= 64 KB), "T/w" at
-    // widths up to w only; 0 keeps k_pull.
+    // EXC_PULL_TILE = T (512 / 1024 / 2048 / 4096) picks the tile and the sorted pull at
+    // every width, "T/w" at widths up to w only; 0 keeps k_pull.  Larger tiles share more
+    // row lines per wave but fit fewer blocks per CU; 8192 and 16384 ran slower at R-MAT
+    // scale 24 (PageRank alone 1.39 and 1.77 ms against 1.21 ms per launch).
     if (!c->K) {
         // the largest tile that still gives every CU four blocks; small graphs keep k_pull
         int T = 0;
@@ -4131,8 +4122,7 @@ int exc_enable_pull(void* p, double threshold) {
             const char* sl = strchr(ev, '/');
             c->sorted_w = sl ? std::max(1, atoi(sl + 1)) : 3;
         }
-        if ((T == 512 || T == 1024 || T == 2048 || T == 4096 || T == 8192 || T == 16384) && off[n] < (1ull << 31) &&
-            c->nsrc > 0) {
+        if ((T == 512 || T == 1024 || T == 2048 || T == 4096) && off[n] < (1ull << 31) && c->nsrc > 0) {
             int bits = 1;
             while (bits < 31 && (1ll << bits) < (long long)c->nsrc) bits++;
             const hipError_t se = excent_dev::build_sorted_tiles(c->n, T, c->d_tidx, c->d_tadj, c->d_twts, off[n], bits, 0,

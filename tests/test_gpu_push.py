@@ -159,7 +159,7 @@ def test_star_hub_pieces(kernel, group):
     assert pushed >= 2 * L
 
 
-@pytest.mark.parametrize("tile", ["0", "1024", "2048", "4096", "16384"])
+@pytest.mark.parametrize("tile", ["0", "1024", "2048", "4096"])
 @pytest.mark.parametrize("threshold", [0.0, 0.2])
 @pytest.mark.parametrize("gname", ["sample", "rmat12", "star"])
 def test_fold_pull_dense_passes(threshold, gname, tile, rmat12, monkeypatch):
@@ -203,7 +203,7 @@ def test_fold_pull_all_measures(tile, monkeypatch):
     assert orc.compare(orc.fp64_reference(idx, adj, orc.ORC_KATZ, alpha=alpha)[0], xs[1])["l1_rel"] < 0.01
 
 
-@pytest.mark.parametrize("tile", ["0", "1024", "4096", "8192"])
+@pytest.mark.parametrize("tile", ["0", "1024", "4096", "4096/1"])
 @pytest.mark.parametrize("measures", ["pr", "all"])
 def test_fused_snapshot_bit_exact(rmat12, measures, tile, monkeypatch):
     """The snapshot a dense pass computes for the next pass inside k_pull (bitmap words,
@@ -289,7 +289,7 @@ def test_fused_fold_state_and_results(rmat12):
     assert st["pushed_edges"] >= m and st["pull_passes"] > 0
 
 
-@pytest.mark.parametrize("tile", ["0", "1024", "16384"])
+@pytest.mark.parametrize("tile", ["0", "1024"])
 @pytest.mark.parametrize("n", [1, 255, 257, 1000, 1025])
 def test_dense_passes_edgeless_and_ragged(n, tile, monkeypatch):
     """Dense passes on an edgeless graph (no sources, empty transpose, no p rows): x = b in
@@ -302,7 +302,7 @@ def test_dense_passes_edgeless_and_ragged(n, tile, monkeypatch):
     assert np.all(xs[0] == np.float32(1) - np.float32(0.85)) and np.all(rs[0] == 0) and pushed == 0 and passes == 1
 
 
-@pytest.mark.parametrize("tile", ["0", "2048", "16384"])
+@pytest.mark.parametrize("tile", ["0", "2048"])
 @pytest.mark.parametrize("gname", ["rmat12", "star"])
 def test_weighted_dense_passes(gname, tile, rmat12, monkeypatch):
     """Dense passes on a weighted graph (the pull scales each in-edge's row by its
