@@ -224,7 +224,8 @@ def bench_batched(args, nat, np, work, graph, sample=None):
     tb = time.perf_counter() - t0
     xb = {j: sess.values(j) for j in sample}
     sess.close()
-    log(f"batched: {rb['passes']} passes, {tb:.3f}s, {rb['kernel']['pushed_edge_measures'] / 1e9:.2f} G vector pushes")
+    log(f"batched: {rb['passes']} passes, {tb:.3f}s, {rb['kernel']['pushed_edge_measures'] / 1e9:.2f} G vector pushes, "
+        f"{rb['kernel']['front_snapshots']} snapshots over the touched vertices only")
     kb = rb["kernel"]
     bpull = kb["pull_bytes"] / (kb["pull_ms"] / 1e3) / 1e9 if kb["pull_ms"] > 0 else 0.0
     roof = {"bound": "hbm", "achieved": round(bpull, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -234,7 +235,7 @@ def bench_batched(args, nat, np, work, graph, sample=None):
     if args.batched_only:
         return {"metric": "batched personalised PageRank (no baseline)", "unit": "s", "value": round(tb, 4),
                 "k": K, "passes": rb["passes"], "dense_passes": kb["pull_passes"],
-                "pull_rows_in_flight": kb["pull_rows_in_flight"],
+                "pull_rows_in_flight": kb["pull_rows_in_flight"], "front_snapshots": kb["front_snapshots"],
                 "pull_tune_ns_per_edge": [round(v, 4) for v in kb["pull_tune_ns_per_edge"]],
                 "vector_pushes": kb["pushed_edge_measures"], "vector_gteps": round(kb["pushed_edge_measures"] / tb / 1e9, 3),
                 "kernel_ms": {x: round(kb[x], 2) for x in ("snapshot_ms", "push_ms", "piece_ms", "fold_ms", "pull_ms",

@@ -1443,6 +1443,7 @@ struct BatchArgs {
     float* prows;         // hub p rows [cap][K]
     uint32_t* prow_count;
     uint32_t prow_cap;
+    uint8_t* touch;       // [n] or null: pushes set touch[u] = 1 for every destination they add to
 };
 
 struct BPiece {          // 16 bytes: a slice of one hub vertex's list
@@ -1457,6 +1458,13 @@ constexpr uint32_t BPIECE = 2048;
 // Snapshot of a [n][K] state: vertex v is pending iff any active column j has
 // |r[v][j]| > eps[j].  Same bitmap / per-chunk counts / hot / trace outputs as
 // k_snapshot, plus per-column pending counts.  One wave per 64-vertex segment.
+// FRONT: the snapshot after a push pass that pushed every vertex the last snapshot found
+// pending.  A pushed vertex's r ends at zero in the columns it pushed and at most eps in the
+// others, and a vertex that was not pending is unchanged, unless a push added to it; so only
+// the vertices the pass's pushes touched can be pending.  The kernel reads (and clears) the
+// touch bytes and reads only those vertices' rows; every other bit is zero.  (Hot and trace
+// outputs are off in this mode.)
+template <bool FRONT>
 __global__ __launch_bounds__(BLOCK) void k_bsnapshot(SnapArgs a, BatchArgs B) {
     __shared__ uint32_t s_col[NWAVES][WAVE];
     const int lane = lane_id(), wid = threadIdx.x / WAVE;
@@ -1486,18 +1494,43 @@ __global__ __launch_bounds__(BLOCK) void k_bsnapshot(SnapArgs a, BatchArgs B) {
         const long long base = seg * WAVE;
         uint64_t mask = 0;
         constexpr int U = 8;
-        for (int i0 = 0; i0 < WAVE; i0 += U) {
-            float row[U];
+        if constexpr (FRONT) {
+            const long long vt = base + lane;
+            const bool t = vt < a.n && B.touch[vt] != 0;
+            uint64_t tm = __ballot(t);
+            if (t) B.touch[vt] = 0;   // clean for the next push pass
+            while (tm) {   // the touched vertices, U rows at a time
+                int off[U];
+                float row[U];
 #pragma unroll
-            for (int u = 0; u < U; u++) {
-                const long long v = base + i0 + u;
-                row[u] = (v < a.n && col_on) ? B.r[v * B.K + lane] : 0.f;   // inactive columns: no load
+                for (int u = 0; u < U; u++) {
+                    off[u] = tm ? __ffsll((long long)tm) - 1 : -1;
+                    tm &= tm - 1;
+                }
+#pragma unroll
+                for (int u = 0; u < U; u++)
+                    row[u] = (off[u] >= 0 && col_on) ? B.r[(base + off[u]) * B.K + lane] : 0.f;
+#pragma unroll
+                for (int u = 0; u < U; u++) {
+                    const bool pj = col_on && off[u] >= 0 && fabsf(row[u]) > eps;
+                    colcnt += pj ? 1u : 0u;
+                    if (__ballot(pj)) mask |= 1ull << off[u];
+                }
             }
+        } else {
+            for (int i0 = 0; i0 < WAVE; i0 += U) {
+                float row[U];
 #pragma unroll
-            for (int u = 0; u < U; u++) {
-                const bool pj = col_on && fabsf(row[u]) > eps && base + i0 + u < a.n;
-                colcnt += pj ? 1u : 0u;
-                if (__ballot(pj)) mask |= 1ull << (i0 + u);
+                for (int u = 0; u < U; u++) {
+                    const long long v = base + i0 + u;
+                    row[u] = (v < a.n && col_on) ? B.r[v * B.K + lane] : 0.f;   // inactive columns: no load
+                }
+#pragma unroll
+                for (int u = 0; u < U; u++) {
+                    const bool pj = col_on && fabsf(row[u]) > eps && base + i0 + u < a.n;
+                    colcnt += pj ? 1u : 0u;
+                    if (__ballot(pj)) mask |= 1ull << (i0 + u);
+                }
             }
         }
         const long long v = base + lane;
@@ -1598,6 +1631,7 @@ __device__ __forceinline__ void bpush_vertex(int v, uint64_t ptr, uint64_t deg, 
     for (uint64_t e0 = 0; e0 < deg; e0 += WAVE) {
         const int id = e0 + lane < deg ? a.edges[ptr + e0 + lane] : 0;
         const int cnt = (int)(deg - e0 < (uint64_t)WAVE ? deg - e0 : WAVE);
+        if (B.touch && e0 + lane < deg) B.touch[id] = 1;
         for (int t = 0; t < cnt; t++) {
             const int u = __shfl(id, t, WAVE);
             if (p != 0.f) atomic_add_f32(&B.r[(long long)u * B.K + lane], p);
@@ -1670,6 +1704,7 @@ __global__ __launch_bounds__(BLOCK) void k_bpieces(const BPiece* __restrict__ pc
         for (uint32_t e0 = 0; e0 < pc.len; e0 += WAVE) {
             const int id = e0 + lane < pc.len ? edges[pc.start + e0 + lane] : 0;
             const int c = (int)(pc.len - e0 < (uint32_t)WAVE ? pc.len - e0 : WAVE);
+            if (B.touch && e0 + lane < pc.len) B.touch[id] = 1;
             for (int t = 0; t < c; t++) {
                 const int u = __shfl(id, t, WAVE);
                 if (p != 0.f) atomic_add_f32(&B.r[(long long)u * B.K + lane], p);
@@ -2336,6 +2371,18 @@ struct Ctx {
     uint64_t active = ~0ull;
     float* prows[3] = {nullptr, nullptr, nullptr};
     uint32_t prow_cap[3] = {0, 0, 0};
+    // frontier snapshots (k_bsnapshot<true>): after a snapshot the touch bytes are clean and
+    // front_ok holds until anything but a resident push changes the state; the next snapshot
+    // may read the touched vertices only if the pushes since covered every chunk that had
+    // pending vertices (front_pend, from the last counts read)
+    uint8_t* d_touch = nullptr;     // [n]
+    bool use_front = true;          // EXC_BATCH_FRONT=0: every snapshot reads every row
+    bool front_ok = false;
+    bool front_pend_known = false;
+    std::vector<int> h_cv0;
+    std::vector<uint64_t> front_pend;
+    std::vector<char> front_cov;
+    uint64_t front_snaps = 0;
 };
 
 BatchArgs batch_args(Ctx* c, int s) {
@@ -2350,6 +2397,7 @@ BatchArgs batch_args(Ctx* c, int s) {
     B.prows = s >= 0 ? c->prows[s] : nullptr;
     B.prow_count = s >= 0 ? c->pb[s].cnt + (2 * c->pb[s].phase + 1) * CNT_STRIDE : nullptr;
     B.prow_cap = s >= 0 ? c->prow_cap[s] : 0;
+    B.touch = c->d_touch;
     return B;
 }
 
@@ -2833,6 +2881,7 @@ void launch_bfold_pull(Ctx* c, hipStream_t st) {
 
 int fold_pull(Ctx* c) {
     hipStream_t st = c->st[0];
+    c->front_ok = false;   // a dense pass changes every row
     if (c->K) launch_bfold_pull(c, st);
     else if (c->nm == 1) launch_fold_pull<1, 1>(c, st);
     else if (c->nm == 2 && c->nm_act == 2) launch_fold_pull<2, 2>(c, st);
@@ -2910,6 +2959,16 @@ constexpr int NOALLOC_MAX_CHUNKS = 4096;
 constexpr size_t NOALLOC_L1_OFF = ((2 * (size_t)NOALLOC_MAX_CHUNKS + 1 + L1_STRIDE - 1) / L1_STRIDE) * L1_STRIDE;
 constexpr size_t NOALLOC_WORDS = NOALLOC_L1_OFF + (size_t)L1_SLOTS * L1_STRIDE;   // 9232 words, 72 KiB
 
+// Whether a batched snapshot may read only the touched vertices (k_bsnapshot<true>): the
+// state changed since the last snapshot by resident pushes alone, and they covered every
+// chunk that snapshot found pending vertices in.
+bool front_usable(const Ctx* c, int hot_skip) {
+    if (!c->use_front || !c->front_ok || !c->front_pend_known || hot_skip || c->d_hot || c->d_trace) return false;
+    for (int i = 0; i < c->nc; i++)
+        if (c->front_pend[i] && !c->front_cov[i]) return false;
+    return true;
+}
+
 // Enqueues a snapshot of the current state on `st`: counts and residual slots cleared, then
 // k_snapshot (k_bsnapshot for batched contexts) into c->d_cnt and the bitmap.
 void enqueue_snapshot(Ctx* c, hipStream_t st, int hot_skip) {
@@ -2936,9 +2995,24 @@ void enqueue_snapshot(Ctx* c, hipStream_t st, int hot_skip) {
         a.segs_per_wave = (int)bspw;
         const long long bwaves = (nseg + bspw - 1) / bspw;
         const unsigned bgrid = (unsigned)((bwaves + NWAVES - 1) / NWAVES);
-        ScopedTimer t(c, st, T_SNAP);
-        hipLaunchKernelGGL(k_bsnapshot, dim3(bgrid), dim3(BLOCK), 0, st, a, batch_args(c, -1));
-        c->snap_bytes += 4.0 * c->K * c->n + 4.0 * (((double)c->n + 31) / 32);
+        const bool front = front_usable(c, hot_skip);
+        {
+            ScopedTimer t(c, st, T_SNAP);
+            if (front) hipLaunchKernelGGL(k_bsnapshot<true>, dim3(bgrid), dim3(BLOCK), 0, st, a, batch_args(c, -1));
+            else hipLaunchKernelGGL(k_bsnapshot<false>, dim3(bgrid), dim3(BLOCK), 0, st, a, batch_args(c, -1));
+        }
+        if (front) {   // the touch bytes and the bitmap (the touched rows are not counted)
+            c->front_snaps++;
+            c->snap_bytes += 2.0 * c->n + 4.0 * (((double)c->n + 31) / 32);
+        } else {
+            c->snap_bytes += 4.0 * c->K * c->n + 4.0 * (((double)c->n + 31) / 32);
+            if (c->d_touch)   // a full snapshot leaves the touch bytes clean too
+                hipLaunchKernelGGL(k_zero_u32, dim3((unsigned)(((long long)(c->n + 3) / 4 + BLOCK - 1) / BLOCK)), dim3(BLOCK), 0,
+                                   st, reinterpret_cast<uint32_t*>(c->d_touch), (c->n + 3) / 4);
+        }
+        c->front_ok = c->d_touch && !c->d_hot && !c->d_trace;
+        c->front_pend_known = false;
+        c->front_cov.assign(c->nc, 0);
     } else {
         ScopedTimer t(c, st, T_SNAP);
         c->snap_nm = c->nm;
@@ -3231,10 +3305,16 @@ int exc_create(const exc_config* cfg, const exc_measures* M, void** out) {
         CK(hipMemset(c->d_trace, 0, n * sizeof(uint16_t)));
     }
     CK(hipMalloc(&c->d_hcount, sizeof(uint32_t)));
+    c->h_cv0.assign(cfg->cv0, cfg->cv0 + c->nc);
     if (c->K) {
         CK(hipMalloc(&c->d_beps, (size_t)c->K * sizeof(float)));
         CK(hipMalloc(&c->d_colpend, (size_t)c->K * sizeof(uint32_t)));
         CK(hipMemset(c->d_colpend, 0, (size_t)c->K * sizeof(uint32_t)));
+        if (const char* ev = getenv("EXC_BATCH_FRONT")) c->use_front = atoi(ev) != 0;
+        if (!cfg->trace) {   // touch bytes, whole 32-bit words (cleared as such)
+            CK(hipMalloc(&c->d_touch, ((size_t)n + 3) / 4 * 4));
+            CK(hipMemset(c->d_touch, 0, ((size_t)n + 3) / 4 * 4));
+        }
         std::vector<float> e(c->K, M->eps[0]);
         CK(hipMemcpy(c->d_beps, e.data(), e.size() * sizeof(float), hipMemcpyHostToDevice));
     }
@@ -3304,6 +3384,7 @@ int exc_destroy(void* p) {
     if (c->d_hub_keys) (void)hipFree(c->d_hub_keys);
     if (c->d_hub_filter) (void)hipFree(c->d_hub_filter);
     if (c->d_colpend) (void)hipFree(c->d_colpend);
+    if (c->d_touch) (void)hipFree(c->d_touch);
     void* dev_ptrs[] = {c->d_idx, c->d_x, c->d_r, c->d_x0, c->d_r0, c->d_bitmap, c->d_cnt, c->d_cv0, c->d_trace, c->d_counters,
                         c->d_arena, c->d_hids, c->d_hoff, c->d_hedges, c->d_hot, c->d_hlist, c->d_hcount};
     for (void* q : dev_ptrs) if (q) hipFree(q);
@@ -3327,6 +3408,7 @@ int exc_init_measure(void* p, int i, float beta, float lambda) {
     if (!c || i < 0 || i >= c->nm) return fail(EXC_EINVAL, "exc_init_measure: bad slot");
     HIPCHK(hipSetDevice(c->dev));
     c->snap_ready = false;   // the state changes: a fused snapshot no longer holds
+    c->front_ok = false;     // ... nor may the next batched snapshot read the touched rows only
     c->l1_valid = false;
     c->folded = false;       // the state is replaced: a folded pass no longer applies
     set_active(c, c->nm);    // ... and every measure may be pending again
@@ -3351,6 +3433,7 @@ int exc_set_state(void* p, int i, const float* h_x, const float* h_r) {
     if (!c || i < 0 || i >= (c->K ? c->K : c->nm)) return fail(EXC_EINVAL, "exc_set_state: bad slot");
     HIPCHK(hipSetDevice(c->dev));
     c->snap_ready = false;   // the state changes: a fused snapshot no longer holds
+    c->front_ok = false;     // ... nor may the next batched snapshot read the touched rows only
     c->l1_valid = false;
     c->folded = false;       // the state is replaced: a folded pass no longer applies
     set_active(c, c->nm);    // ... and every measure may be pending again
@@ -3393,6 +3476,7 @@ int exc_state_checkpoint(void* p, int restore) {
     if (!c) return fail(EXC_EINVAL, "exc_state_checkpoint: null context");
     HIPCHK(hipSetDevice(c->dev));
     c->snap_ready = false;   // the state changes: a fused snapshot no longer holds
+    c->front_ok = false;     // ... nor may the next batched snapshot read the touched rows only
     c->l1_valid = false;
     if (c->K) c->active = all_columns(c->K);
     const size_t bytes = (size_t)c->n * c->S * sizeof(float);
@@ -3456,6 +3540,11 @@ void parse_counts(Ctx* c, uint64_t* h_cnt, uint64_t* h_hotpend) {
     c->l1_valid = c->K == 0;
     c->last_demand = 0;
     for (int i = 0; i < c->nc; i++) c->last_demand += c->h_small[2 * i + 1];
+    if (c->K) {
+        c->front_pend.resize(c->nc);
+        for (int i = 0; i < c->nc; i++) c->front_pend[i] = c->h_small[2 * i];
+        c->front_pend_known = true;
+    }
 }
 
 int exc_snapshot(void* p, int hot_skip, uint64_t* h_cnt, uint64_t* h_hotpend) {
@@ -3606,6 +3695,13 @@ int exc_push_resident(void* p, const exc_chunk* ch, int use_list, int group, int
     if (c->d_tadj && (!c->d_wts || c->d_twts) && whole &&
         (c->folded || (double)c->last_demand >= c->pull_threshold * (double)c->m))
         return fold_pull(c);
+    if (c->K && c->front_ok) {   // the chunks whose every vertex this push covers
+        if ((int)c->front_cov.size() != c->nc) c->front_cov.assign(c->nc, 0);
+        for (int i = 0; i < c->nc; i++) {
+            const int lo = c->h_cv0[i], hi = i + 1 < c->nc ? c->h_cv0[i + 1] : c->n;
+            if (ch->v0 <= lo && hi <= ch->v1) c->front_cov[i] = 1;
+        }
+    }
     return push_chunk(c, 0, ch->v0, ch->v1, c->d_arena, 0, use_list, group, hot_skip, c->arena_edges, c->d_wts);
 }
 
@@ -3620,6 +3716,7 @@ int exc_stream_chunk(void* p, int s, const exc_chunk* ch, int to_resident, int h
     if (to_resident && (!c->d_arena || ch->e1 > c->arena_edges)) return fail(EXC_ESTATE, "exc_stream_chunk: chunk not in the edge cache");
     HIPCHK(hipSetDevice(c->dev));
     c->snap_ready = false;   // the state changes: a fused snapshot no longer holds
+    c->front_ok = false;     // ... nor may the next batched snapshot read the touched rows only
     c->l1_valid = false;
     if (c->folded)
         return fail(EXC_ESTATE, "exc_stream_chunk: the last dense pass folded this pass; it must be a dense pass over all vertices");
@@ -3661,6 +3758,7 @@ int exc_gather_chunk(void* p, int s, int k, int group) {
     if (ne > c->max_chunk_edges) return fail(EXC_EINVAL, "exc_gather_chunk: gathered lists exceed the slot");
     HIPCHK(hipSetDevice(c->dev));
     c->snap_ready = false;   // the state changes: a fused snapshot no longer holds
+    c->front_ok = false;     // ... nor may the next batched snapshot read the touched rows only
     c->l1_valid = false;
     if (c->folded)
         return fail(EXC_ESTATE, "exc_gather_chunk: the last dense pass folded this pass; it must be a dense pass over all vertices");
@@ -3693,6 +3791,7 @@ int exc_push_hot(void* p, int lo, int hi, int group, int after_slots) {
     if (hi == lo) return EXC_OK;
     HIPCHK(hipSetDevice(c->dev));
     c->snap_ready = false;   // the state changes: a fused snapshot no longer holds
+    c->front_ok = false;     // ... nor may the next batched snapshot read the touched rows only
     c->l1_valid = false;
     if (c->folded)
         return fail(EXC_ESTATE, "exc_push_hot: the last dense pass folded this pass; it must be a dense pass over all vertices");
@@ -3736,6 +3835,7 @@ int exc_batch_eps(void* p, const float* eps) {
     if (!c || !c->K || !eps) return fail(EXC_EINVAL, "exc_batch_eps: not a batched context");
     HIPCHK(hipSetDevice(c->dev));
     HIPCHK(hipMemcpy(c->d_beps, eps, (size_t)c->K * sizeof(float), hipMemcpyHostToDevice));
+    c->front_ok = false;   // the pending test changes for every vertex
     return EXC_OK;
 }
 
@@ -3748,6 +3848,7 @@ int exc_batch_seed(void* p, int col, const int32_t* seeds, int count, float valu
     if (!count) return EXC_OK;
     HIPCHK(hipSetDevice(c->dev));
     c->snap_ready = false;   // the state changes: a fused snapshot no longer holds
+    c->front_ok = false;     // ... nor may the next batched snapshot read the touched rows only
     c->l1_valid = false;
     c->folded = false;       // the state is replaced: a folded pass no longer applies
     set_active(c, c->nm);    // ... and every measure may be pending again
@@ -3775,7 +3876,9 @@ int exc_batch_colpend(void* p, uint32_t* out) {
 int exc_batch_set_active(void* p, uint64_t mask) {
     Ctx* c = as_ctx(p);
     if (!c || !c->K) return fail(EXC_EINVAL, "exc_batch_set_active: not a batched context");
-    c->active = c->K == 64 ? mask : (mask & ((1ull << c->K) - 1));
+    const uint64_t m = c->K == 64 ? mask : (mask & ((1ull << c->K) - 1));
+    if (m & ~c->active) c->front_ok = false;   // a column back on may hold pending vertices nobody pushed
+    c->active = m;
     return EXC_OK;
 }
 
@@ -3790,6 +3893,7 @@ int exc_set_edge_weights(void* p, const float* h_w, uint64_t cnt) {
     if (c->d_tadj) return fail(EXC_ESTATE, "exc_set_edge_weights: set the weights before exc_enable_pull");
     HIPCHK(hipSetDevice(c->dev));
     c->snap_ready = false;   // the state changes: a fused snapshot no longer holds
+    c->front_ok = false;     // ... nor may the next batched snapshot read the touched rows only
     c->l1_valid = false;
     c->folded = false;       // the state is replaced: a folded pass no longer applies
     set_active(c, c->nm);    // ... and every measure may be pending again
@@ -3965,6 +4069,7 @@ int exc_get_stats(void* p, exc_stats* o) {
     }
     o->pull_rows_in_flight = c->K ? c->bpull_q : 0;
     o->burst_graph_launches = c->burst_graph_launches;
+    o->front_snapshots = c->front_snaps;
     for (int k = 0; k < 7; k++) {
         o->push_tune_best[k] = c->ptune[k].best;
         for (int i = 0; i < 3; i++) o->push_tune_ns_edge[k][i] = c->ptune[k].ns_edge[i];
@@ -4248,6 +4353,7 @@ int exc_reset_stats(void* p) {
         c->pull_fixed_bytes[w] = c->hub_fixed_bytes[w] = 0;
         c->pull_launches_w[w] = 0;
     }
+    c->front_snaps = 0;
     return EXC_OK;
 }
 

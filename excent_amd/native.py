@@ -71,7 +71,8 @@ class Stats(ctypes.Structure):
                 ("pull_rows_in_flight", c_int), ("pull_tune_ns_per_edge", c_double * 2),
                 ("pull_tile", c_int), ("pull_w_ms", c_double * 3), ("pull_w_launches", c_u64 * 3),
                 ("pull_w_bytes", c_double * 3), ("pull_w_tile", c_int * 3), ("burst_graph_launches", c_u64),
-                ("push_tune_ns_edge", (c_double * 3) * 7), ("push_tune_best", c_int * 7)]
+                ("push_tune_ns_edge", (c_double * 3) * 7), ("push_tune_best", c_int * 7),
+                ("front_snapshots", c_u64)]
 
 
 class RunStats(ctypes.Structure):

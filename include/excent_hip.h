@@ -116,6 +116,9 @@ typedef struct exc_stats {
      * kernel, G = 2^c and G = max(1, 2^c / 4) -- and the one kept (-1 while undecided) */
     double push_tune_ns_edge[7][3];
     int push_tune_best[7];
+    /* batched: snapshots that read only the vertices the pushes since the last snapshot
+     * added to (every chunk with pending vertices pushed in between; EXC_BATCH_FRONT=0 off) */
+    uint64_t front_snapshots;
 } exc_stats;
 
 const char* exc_last_error(void);

@@ -52,6 +52,72 @@ def test_batched_snapshot_bit_exact(K, rmat12):
     assert np.array_equal(cnt2, ocnt2)
 
 
+@pytest.mark.parametrize("K", [7, 64])
+@pytest.mark.parametrize("use_list", [True, False])
+def test_batched_front_snapshot_bit_exact(K, use_list, rmat12):
+    """After push passes that pushed every chunk with pending vertices, the snapshot reads
+    only the vertices the pushes added to: bitmap, per-chunk counts and per-column counts
+    equal the oracle's snapshot of the state read back, bit for bit, on every pass until
+    nothing pends (columns dropping out as they finish).  A pass that leaves a pending chunk
+    unpushed, a change of eps and a column switched back on each fall back to a full
+    snapshot, which is exact as well."""
+    print(f"case front snapshot K={K} list={use_list}", flush=True)
+    idx, adj = orc.egr_load(rmat12)
+    n, m = len(idx) - 1, len(adj)
+    rng = np.random.default_rng(K + 5)
+    seeds = [rng.choice(n, 4, replace=False).astype(np.int32) for _ in range(K)]
+    eps = (rng.random(K) * 2e-3 + 1e-3).astype(np.float32)
+    cv0 = orc.plan_chunks(idx, m // 6)
+    nc = len(cv0)
+    bounds = list(cv0) + [n]
+
+    def check(ctx, cnt, active, tag):
+        rs = [ctx.read_state(j, 1) for j in range(K)]
+        on = [j for j in range(K) if (active >> j) & 1]
+        obm, ocnt, _ = orc.snapshot([rs[j] for j in on], eps[on], idx, cv0)
+        assert np.array_equal(ctx.bitmap(), obm) and np.array_equal(cnt, ocnt), tag
+        cp = ctx.batch_colpend()
+        assert all(cp[j] == ((np.abs(rs[j]) > eps[j]).sum() if j in on else 0) for j in range(K)), tag
+
+    with nat.Context(n, m, cv0, [nat.EXC_PR], [1e-3], [0.85], [1], batch_k=K) as ctx:
+        ctx.upload_offsets(idx)
+        assert ctx.cache_chunks(m) == 0
+        ctx.cache_upload(adj)
+        ctx.batch_eps(eps)
+        for j in range(K):
+            ctx.batch_seed(j, seeds[j], np.float32(0.15 * n / 4))
+        active = (1 << K) - 1 if K < 64 else ~0 & ((1 << 64) - 1)
+        cnt, _ = ctx.snapshot()
+        check(ctx, cnt, active, "first")
+        fronts, passes, full_next, reenabled = 0, 0, False, False
+        while cnt[0::2].sum() and passes < 300:
+            full = full_next or passes in (3, 5)
+            last = max(c for c in range(nc) if cnt[2 * c])
+            for c in range(nc):   # pass 3 leaves the last pending chunk out
+                if cnt[2 * c] and not (passes == 3 and c == last):
+                    ctx.push_resident(bounds[c], bounds[c + 1], int(idx[bounds[c]]), int(idx[bounds[c + 1]]), use_list)
+            if passes == 5:
+                ctx.batch_eps(eps)   # the same values: the pending test is reset all the same
+            before = ctx.stats()["front_snapshots"]
+            cnt, _ = ctx.snapshot()
+            took_front = ctx.stats()["front_snapshots"] - before
+            assert took_front == (0 if full else 1), (passes, took_front)
+            fronts += took_front
+            check(ctx, cnt, active, passes)
+            cp = ctx.batch_colpend()
+            new = sum(1 << j for j in range(K) if cp[j])
+            full_next = False
+            if new != active and not reenabled and passes >= 6:
+                ctx.batch_set_active(new)
+                ctx.batch_set_active(active)   # the finished columns back on: a full snapshot next
+                reenabled = full_next = True
+            else:
+                active = new
+                ctx.batch_set_active(active)
+            passes += 1
+        assert cnt[0::2].sum() == 0 and fronts >= 3, (passes, fronts)
+
+
 @pytest.mark.parametrize("K", [16, 64])
 @pytest.mark.parametrize("gname", ["sample", "rmat12"])
 def test_batched_ppr_against_fp64(tmp_path, gname, rmat12, K):
