@@ -236,6 +236,7 @@ def bench_batched(args, nat, np, work, graph, sample=None):
         return {"metric": "batched personalised PageRank (no baseline)", "unit": "s", "value": round(tb, 4),
                 "k": K, "passes": rb["passes"], "dense_passes": kb["pull_passes"],
                 "pull_rows_in_flight": kb["pull_rows_in_flight"], "front_snapshots": kb["front_snapshots"],
+                "pull_hub_gbs": round(kb["pull_hub_bytes"] / (kb["pull_hub_ms"] / 1e3) / 1e9, 2) if kb["pull_hub_ms"] > 0 else None,
                 "pull_tune_ns_per_edge": [round(v, 4) for v in kb["pull_tune_ns_per_edge"]],
                 "vector_pushes": kb["pushed_edge_measures"], "vector_gteps": round(kb["pushed_edge_measures"] / tb / 1e9, 3),
                 "kernel_ms": {x: round(kb[x], 2) for x in ("snapshot_ms", "push_ms", "piece_ms", "fold_ms", "pull_ms",

@@ -2344,6 +2344,7 @@ struct Ctx {
     uint32_t* d_hubbits = nullptr;      // batched: destinations summed by k_bpull_hub
     int pull_seq = 8;                   // in-list entries per lane in k_pull (4, 8, 16 or 32)
     int bpull_q = 8;                    // batched pull: rows in flight per wave (8 or 16)
+    int bhub_q = 0;                     // EXC_BHUB_Q: the hub pieces' rows in flight (0: by bpull_q)
     // batched dense passes: p rows in bf16 (2 bytes per column; EXC_BATCH_ROWS=f32: float)
     bool brows16 = true;
     // Batched pull tuner: the first two dense passes whose demand is at least half the edges
@@ -2837,9 +2838,10 @@ void launch_bfold_pull_rows(Ctx* c, hipStream_t st, bool fuse, bool trial) {
     if (c->n_pull_hubs) {
         ScopedTimer t(c, st, T_PULL_HUB);
         const dim3 g((unsigned)std::min<long long>((c->n_pull_hubs + NWAVES - 1) / NWAVES, cap));
-        // bf16 hub pieces: 8 rows in flight (16 measured 2.4x slower there, while the pull
-        // itself gains from 16)
-        if (c->bpull_q == 16 && sizeof(PT) == 4)
+        // bf16 hub pieces: 8 rows in flight (16 measured 2.5x slower there, 1.69 against
+        // 4.21 TB/s at R-MAT scale 26, K = 64, while the pull itself gains from 16);
+        // EXC_BHUB_Q=16 for measurements
+        if ((c->bpull_q == 16 && sizeof(PT) == 4) || c->bhub_q == 16)
             hipLaunchKernelGGL((k_bpull_hub<16, PT>), g, dim3(BLOCK), 0, st, a, B, c->d_pull_hubs, c->n_pull_hubs);
         else
             hipLaunchKernelGGL((k_bpull_hub<8, PT>), g, dim3(BLOCK), 0, st, a, B, c->d_pull_hubs, c->n_pull_hubs);
@@ -4092,6 +4094,7 @@ int exc_enable_pull(void* p, double threshold) {
         c->pull_live = ev && (ev[0] == '0' || ev[0] == '1') ? ev[0] - '0' : 2;
         if (const char* lf = getenv("EXC_PULL_LIVE_FRAC")) c->pull_live_frac = atof(lf);
         if (const char* ff = getenv("EXC_FUSE_FOLD")) c->fuse_fold = atoi(ff);   // 0: fold with k_fold always
+        if (const char* hq = getenv("EXC_BHUB_Q")) c->bhub_q = atoi(hq) == 16 ? 16 : 8;
         if (const char* bq = getenv("EXC_BPULL_Q")) {   // 8 / 16 rows per wave, no tuning
             c->bpull_q = atoi(bq) == 16 ? 16 : 8;
         } else if (c->K && !c->bq_ev[0] && c->bq_trial == 0) {
