@@ -500,20 +500,40 @@ __device__ __forceinline__ void push_entries(bool valid, int v, uint64_t ptr, ui
 #pragma unroll
         for (int k = 0; k < NM; k++) s_p[k][tid] = p[k];
         __syncthreads();
+        // PU slots per lane per round: their owners (LDS searches), then their ids loaded
+        // together, then the atomics -- a block's rounds are a chain of id loads
+        constexpr int PU = 4;
         int lo = 0;
-        for (uint32_t j = tid; j < total; j += BLOCK) {
-            int hi = BLOCK - 1;
-            while (lo < hi) {
-                int mid = (lo + hi + 1) >> 1;
-                if (s_start[mid] <= j) lo = mid; else hi = mid - 1;
-            }
-            const uint64_t e = s_ptr[lo] + (j - s_start[lo]);
-            const int u = a.edges[e];
-            const float we = a.wts ? a.wts[e] : 1.f;
+        for (uint32_t j0 = tid; j0 < total; j0 += BLOCK * PU) {
+            int own[PU], uq[PU];
+            float wq[PU];
 #pragma unroll
-            for (int k = 0; k < NM; k++) {
-                const float pk = s_p[k][lo];
-                if (pk != 0.f) atomic_add_f32(&M.r[(long long)u * S + k], a.wts ? pk * we : pk);
+            for (int q = 0; q < PU; q++) {
+                const uint32_t j = j0 + (uint32_t)q * BLOCK;
+                own[q] = -1;
+                if (j < total) {   // j grows with q, so the owner does too
+                    int hi = BLOCK - 1;
+                    while (lo < hi) {
+                        int mid = (lo + hi + 1) >> 1;
+                        if (s_start[mid] <= j) lo = mid; else hi = mid - 1;
+                    }
+                    own[q] = lo;
+                }
+            }
+#pragma unroll
+            for (int q = 0; q < PU; q++) {
+                const uint64_t e = own[q] >= 0 ? s_ptr[own[q]] + (j0 + (uint32_t)q * BLOCK - s_start[own[q]]) : 0;
+                uq[q] = own[q] >= 0 ? a.edges[e] : 0;
+                wq[q] = own[q] >= 0 && a.wts ? a.wts[e] : 1.f;
+            }
+#pragma unroll
+            for (int q = 0; q < PU; q++) {
+                if (own[q] < 0) continue;
+#pragma unroll
+                for (int k = 0; k < NM; k++) {
+                    const float pk = s_p[k][own[q]];
+                    if (pk != 0.f) atomic_add_f32(&M.r[(long long)uq[q] * S + k], a.wts ? pk * wq[q] : pk);
+                }
             }
         }
         __syncthreads();
@@ -664,32 +684,45 @@ __global__ __launch_bounds__(BLOCK) void k_pieces(const Piece* __restrict__ pcs,
         __syncthreads();
     }
     const int lane = lane_id();
+    constexpr int PU = 8;   // edge ids per lane loaded together (a short pass is the chain of its longest piece)
     for (long long w = w0; w < cnt; w += nw) {
         const Piece pc = pcs[w];
-        for (uint32_t t = lane; t < pc.len; t += WAVE) {
-            const int u = edges[pc.start + t];
-            const float we = wts ? wts[pc.start + t] : 1.f;
-            if (priv) {
-                const uint32_t fb = hub_fbit((uint32_t)u);
-                if ((s_fil[fb >> 5] >> (fb & 31)) & 1u) {
-                    int slot = -1;
-                    uint32_t h = hub_slot((uint32_t)u);
-                    for (int q = 0; q < ht.probes; q++, h = (h + 1) & (HUB_SLOTS - 1)) {
-                        const int key = s_key[h];
-                        if (key == u) { slot = (int)h; break; }
-                        if (key < 0) break;
-                    }
-                    if (slot >= 0) {
+        for (uint32_t t0 = 0; t0 < pc.len; t0 += WAVE * PU) {
+            int uq[PU];
+            float wq[PU];
 #pragma unroll
-                        for (int k = 0; k < NM; k++)
-                            if (pc.p[k] != 0.f) atomicAdd(&s_acc[k][slot], wts ? pc.p[k] * we : pc.p[k]);
-                        continue;
-                    }
-                }
+            for (int q = 0; q < PU; q++) {
+                const uint32_t t = t0 + (uint32_t)q * WAVE + lane;
+                uq[q] = t < pc.len ? edges[pc.start + t] : -1;
+                wq[q] = wts && t < pc.len ? wts[pc.start + t] : 1.f;
             }
 #pragma unroll
-            for (int k = 0; k < NM; k++)
-                if (pc.p[k] != 0.f) atomic_add_f32(&M.r[(long long)u * S + k], wts ? pc.p[k] * we : pc.p[k]);
+            for (int q = 0; q < PU; q++) {
+                const int u = uq[q];
+                if (u < 0) continue;
+                const float we = wq[q];
+                int slot = -1;
+                if (priv) {
+                    const uint32_t fb = hub_fbit((uint32_t)u);
+                    if ((s_fil[fb >> 5] >> (fb & 31)) & 1u) {
+                        uint32_t h = hub_slot((uint32_t)u);
+                        for (int r = 0; r < ht.probes; r++, h = (h + 1) & (HUB_SLOTS - 1)) {
+                            const int key = s_key[h];
+                            if (key == u) { slot = (int)h; break; }
+                            if (key < 0) break;
+                        }
+                    }
+                }
+                if (slot >= 0) {
+#pragma unroll
+                    for (int k = 0; k < NM; k++)
+                        if (pc.p[k] != 0.f) atomicAdd(&s_acc[k][slot], wts ? pc.p[k] * we : pc.p[k]);
+                } else {
+#pragma unroll
+                    for (int k = 0; k < NM; k++)
+                        if (pc.p[k] != 0.f) atomic_add_f32(&M.r[(long long)u * S + k], wts ? pc.p[k] * we : pc.p[k]);
+                }
+            }
         }
     }
     if (priv) {   // one atomic per privatised destination and measure with a nonzero sum
