@@ -1015,7 +1015,7 @@ __device__ void pull_finish(const PullArgs& a, const DevMeasures& M, long long u
     const uint64_t mask = __ballot(pend);
     const long long wb = u0 + (tid & ~(WAVE - 1));
     const int lane = lane_id();
-    if (lane == 0 && wb < a.n) {
+    if (lane == 0 && wb < u0 + nt) {   // the waves holding this call's destinations
         const long long w0 = wb >> 5, words = ((long long)a.n + 31) / 32;
         a.bitmap[w0] = (uint32_t)mask;
         if (w0 + 1 < words) a.bitmap[w0 + 1] = (uint32_t)(mask >> 32);
@@ -1087,16 +1087,20 @@ constexpr uint32_t PULL_HUB = 4096;   // in-lists longer than this are summed by
 constexpr int PULL_LD = BLOCK + 1;           // LDS row pitch: slot j at (j % SEQ) * LD + j / SEQ
 
 // W: weighted graph -- each row is scaled by its in-edge's coefficient (staged with the ids).
-template <int NM, int S, int PULL_SEQ, bool W = false>
+// TILE: destinations per block (BLOCK, or 64 on graphs too small to give every CU four
+// blocks of BLOCK destinations: a quarter of the lanes then finish the tile, but four times
+// the blocks keep gathers in flight).
+template <int NM, int S, int PULL_SEQ, bool W = false, int TILE = BLOCK>
 __global__ __launch_bounds__(BLOCK, 8) void k_pull(PullArgs a, DevMeasures M) {
+    static_assert(TILE % WAVE == 0 && TILE <= BLOCK, "a tile is whole waves of destinations");
     constexpr int PULL_WIN = BLOCK * PULL_SEQ;   // 2048 slots (8 KB of LDS) at PULL_SEQ 8
     __shared__ uint32_t s_start[BLOCK + 1];
     __shared__ float s_w[W ? PULL_SEQ * PULL_LD : 1];
     __shared__ int32_t s_src[PULL_SEQ * PULL_LD];
     __shared__ float s_acc[NM][BLOCK];
     const int tid = threadIdx.x;
-    const long long u0 = (long long)blockIdx.x * BLOCK;
-    const int nt = (int)((long long)a.n - u0 < BLOCK ? (long long)a.n - u0 : BLOCK);
+    const long long u0 = (long long)blockIdx.x * TILE;
+    const int nt = (int)((long long)a.n - u0 < TILE ? (long long)a.n - u0 : TILE);
     const uint64_t base = a.tidx[u0];
     for (int i = tid; i <= nt; i += BLOCK) s_start[i] = (uint32_t)(a.tidx[u0 + i] - base);
 #pragma unroll
@@ -2330,6 +2334,7 @@ struct Ctx {
     uint64_t pull_launches_w[3] = {};
     int pull_tile_w[3] = {};                   // the sorted tile each width ran with (0: k_pull)
     int sorted_w = 1;                          // widths up to this run the sorted pull (where its LDS fits)
+    int pull_small = 2;                        // k_pull tiles of 64 destinations: 0 never, 1 always, 2 small graphs
     excent_dev::SortedTileBuffers sorted;      // source-sorted dense pull (k_pull_sorted)
     int sorted_T = 0;                          // its tile (destinations per block); 0: k_pull
     int32_t* d_hub_keys = nullptr;             // destinations privatised by k_pieces (HubTable)
@@ -2730,6 +2735,9 @@ void launch_fold_pull(Ctx* c, hipStream_t st) {
     const int hub_field = full ? C_PULL_HUB_LIVE : NM == 1 ? C_PULL_HUB_LIVE_A1 : C_PULL_HUB_LIVE_A2;
     FoldArgs f{c->n, c->d_idx, c->d_p, c->d_rank, row};
     const int ntiles = (int)(((long long)c->n + BLOCK - 1) / BLOCK);
+    // small graphs: tiles of 64 destinations (see k_pull)
+    const bool small = c->pull_small == 1 || (c->pull_small == 2 && ntiles < 4 * c->cus);
+    const int ntiles64 = (int)(((long long)c->n + 63) / 64);
     PullArgs pa{c->n, c->d_tidx, c->d_tadj, c->d_p, live ? c->d_pbits : nullptr, row + hub_field, c->d_twts,
                 c->d_idx, c->d_cv0, c->nc, nullptr, nullptr, nullptr, c->d_rank, c->nsrc, c->d_counters + C_SPREAD};
     pa.rows_field = full ? S_ROWS : NM == 1 ? S_ROWS_A1 : S_ROWS_A2;
@@ -2788,6 +2796,7 @@ void launch_fold_pull(Ctx* c, hipStream_t st) {
         }
         if (weighted) {
         } else if (sorted) launch_sorted<NM, S, false>(c, st, pa);
+        else if (small) hipLaunchKernelGGL((k_pull<NM, S, 8, false, 64>), dim3(ntiles64), dim3(BLOCK), 0, st, pa, c->DM);
         else if constexpr (!full) hipLaunchKernelGGL((k_pull<NM, S, 8>), dim3(ntiles), dim3(BLOCK), 0, st, pa, c->DM);
         else if (c->pull_seq == 4) hipLaunchKernelGGL((k_pull<NM, S, 4>), dim3(ntiles), dim3(BLOCK), 0, st, pa, c->DM);
         else if (c->pull_seq == 32) hipLaunchKernelGGL((k_pull<NM, S, 32>), dim3(ntiles), dim3(BLOCK), 0, st, pa, c->DM);
@@ -4128,6 +4137,7 @@ int exc_enable_pull(void* p, double threshold) {
         if (const char* lf = getenv("EXC_PULL_LIVE_FRAC")) c->pull_live_frac = atof(lf);
         if (const char* ff = getenv("EXC_FUSE_FOLD")) c->fuse_fold = atoi(ff);   // 0: fold with k_fold always
         if (const char* hq = getenv("EXC_BHUB_Q")) c->bhub_q = atoi(hq) == 16 ? 16 : 8;
+        if (const char* ps = getenv("EXC_PULL_SMALL")) c->pull_small = std::min(2, std::max(0, atoi(ps)));
         if (const char* bq = getenv("EXC_BPULL_Q")) {   // 8 / 16 rows per wave, no tuning
             c->bpull_q = atoi(bq) == 16 ? 16 : 8;
         } else if (c->K && !c->bq_ev[0] && c->bq_trial == 0) {

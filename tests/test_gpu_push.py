@@ -203,15 +203,18 @@ def test_fold_pull_all_measures(tile, monkeypatch):
     assert orc.compare(orc.fp64_reference(idx, adj, orc.ORC_KATZ, alpha=alpha)[0], xs[1])["l1_rel"] < 0.01
 
 
-@pytest.mark.parametrize("tile", ["0", "1024", "4096", "4096/1"])
+@pytest.mark.parametrize("tile", ["0", "0:256", "1024", "4096", "4096/1"])
 @pytest.mark.parametrize("measures", ["pr", "all"])
 def test_fused_snapshot_bit_exact(rmat12, measures, tile, monkeypatch):
     """The snapshot a dense pass computes for the next pass inside k_pull (bitmap words,
     per-chunk pending counts and degree sums) equals the oracle's snapshot of the state
     read back, bit for bit, on chunk boundaries that cut through the pull's tiles; and
     equals a second, separately launched snapshot of the same state.  (The next pass's
-    fold is kept out of k_pull here, so the state read back is the one the snapshot saw.)"""
+    fold is kept out of k_pull here, so the state read back is the one the snapshot saw.)
+    k_pull runs 64-destination tiles on a graph this small; "0:256" forces 256."""
+    tile, _, small = tile.partition(":")
     monkeypatch.setenv("EXC_PULL_TILE", tile)   # 0: k_pull; else k_pull_sorted with this tile
+    monkeypatch.setenv("EXC_PULL_SMALL", "0" if small == "256" else "2")
     monkeypatch.setenv("EXC_FUSE_FOLD", "0")
     idx, adj = orc.egr_load(rmat12)
     n, m = len(idx) - 1, len(adj)
@@ -289,12 +292,14 @@ def test_fused_fold_state_and_results(rmat12):
     assert st["pushed_edges"] >= m and st["pull_passes"] > 0
 
 
-@pytest.mark.parametrize("tile", ["0", "1024"])
+@pytest.mark.parametrize("tile", ["0", "0:256", "1024"])
 @pytest.mark.parametrize("n", [1, 255, 257, 1000, 1025])
 def test_dense_passes_edgeless_and_ragged(n, tile, monkeypatch):
     """Dense passes on an edgeless graph (no sources, empty transpose, no p rows): x = b in
-    one pass, with tile counts that are not multiples of the 256-vertex tile."""
+    one pass, with tile counts that are not multiples of the 64- or 256-vertex tile."""
+    tile, _, small = tile.partition(":")
     monkeypatch.setenv("EXC_PULL_TILE", tile)   # 0: k_pull; else k_pull_sorted with this tile
+    monkeypatch.setenv("EXC_PULL_SMALL", "0" if small == "256" else "2")
     idx = np.zeros(n + 1, np.uint64)
     adj = np.zeros(0, np.int32)
     x0, r0 = orc.init(idx, orc.ORC_PR)
