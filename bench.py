@@ -605,12 +605,13 @@ def bench_plain(args, nat, np, work, graph, scale, measure, steps, warmup, env, 
     back to back after `warmup` untimed ones, each from the initial state (a device copy);
     the dominant kernel's roofline; FP64 errors of the last run; setup times."""
     excent_log = os.path.join(args.logdir or work, f"{tag}.log")
-    sargs = [graph, "--measure", measure, "--cache", "chunks", "--preload", "--stream-ratio", "1",
-             timing, "--push-kernel", args.kernel, "--pull", args.pull, "--hub-lds", args.hub_lds, "--log", excent_log,
-             "--dump", os.path.join(work, "x"), "--stop", stop_rule(args, measure),
-             "--drop-measures", args.drop_measures, "--burst", args.burst,
-             # a run whose stop rule is never met ends here and is reported as not converged
-             "--max-passes", args.max_passes or 2000] + measure_args(measure, env)
+    sargs = ([graph, "--measure", measure, "--cache", "chunks", "--preload", "--stream-ratio", "1"] +
+             ([timing] if timing else []) +
+             ["--push-kernel", args.kernel, "--pull", args.pull, "--hub-lds", args.hub_lds, "--log", excent_log,
+              "--dump", os.path.join(work, "x"), "--stop", stop_rule(args, measure),
+              "--drop-measures", args.drop_measures, "--burst", args.burst,
+              # a run whose stop rule is never met ends here and is reported as not converged
+              "--max-passes", args.max_passes or 2000] + measure_args(measure, env))
     t0 = time.time()
     sess = nat.Session(sargs)
     t_session = time.time() - t0
@@ -685,7 +686,7 @@ def bench_plain(args, nat, np, work, graph, scale, measure, steps, warmup, env, 
         "pushed_edges_per_run": last["pushed_edges"],
         "residual_l1_fp64_at_stop": {k: float(f"{v:.6g}") for k, v in
                                      zip(["pr", "katz", "ev"] if measure == "all" else [measure], last["residual_l1"])},
-        "kernel_ms_per_run": {c[0]: round(c[1] / len(runs), 3) for c in cands if c[2]},
+        "kernel_ms_per_run": {c[0]: round(c[1] / len(runs), 3) for c in cands if c[2] and c[1] > 0},
         "roofline": roof,
     }
     rec["fp64_err"] = None if args.no_fp64 else fp64_errors(nat, np, graph, work, measure, env)
@@ -781,6 +782,9 @@ def main():
     ap.add_argument("--hub-lds", type=int, default=1024, help="destinations privatised in LDS by the hub-piece push")
     ap.add_argument("--drop-measures", default="on", choices=["on", "off"],
                     help="dense passes leave out the measures with nothing pending")
+    ap.add_argument("--timing", choices=["full", "pull", "off"], default="pull",
+                    help="headline: HIP events around the dense pulls (the roofline's kernel; the default, "
+                         "~0.5%% of the run), around every kernel (~1%%), or none (off: no roofline)")
     ap.add_argument("--burst", type=int, default=0,
                     help="passes per host synchronisation on a cached graph (0: automatic, 1: every pass)")
     ap.add_argument("--cpu-seconds", type=float, default=30.0,
@@ -828,7 +832,7 @@ def main():
         return
     env = constants(nat, graph)
     rec, excent_log = bench_plain(args, nat, np, work, graph, args.scale, args.measure, args.steps, args.warmup, env,
-                                  "headline")
+                                  "headline", timing={"full": "--timing", "pull": "--timing-pull", "off": None}[args.timing])
 
     cpu = None
     if not args.no_cpu:
