@@ -609,7 +609,7 @@ def bench_plain(args, nat, np, work, graph, scale, measure, steps, warmup, env, 
              ([timing] if timing else []) +
              ["--push-kernel", args.kernel, "--pull", args.pull, "--hub-lds", args.hub_lds, "--log", excent_log,
               "--dump", os.path.join(work, "x"), "--stop", stop_rule(args, measure),
-              "--drop-measures", args.drop_measures, "--burst", args.burst,
+              "--drop-measures", args.drop_measures, "--burst", args.burst, "--burst-ahead", args.burst_ahead,
               # a run whose stop rule is never met ends here and is reported as not converged
               "--max-passes", args.max_passes or 2000] + measure_args(measure, env))
     t0 = time.time()
@@ -719,7 +719,7 @@ def sub_s16(args, nat, np, work):
     rec.update({"metric": "converged GTEPS + time-to-eps", "higher_is_better": True, "dtype": "f32",
                 "data": "synthetic", "config": {"workload": "R-MAT scale 16 ef 16 --measure pr, eps 0.01, whole "
                                                             "graph in HBM, converged to eps", "n": n, "m": m,
-                                                "burst": args.burst or "auto"}})
+                                                "burst": args.burst or "auto", "burst_ahead": args.burst_ahead}})
     if not args.no_cpu:
         idx, adj = orc.egr_load(graph)
         st = plain_state(orc, np, idx, adj, "pr", {})
@@ -785,6 +785,8 @@ def main():
     ap.add_argument("--timing", choices=["full", "pull", "off"], default="pull",
                     help="headline: HIP events around the dense pulls (the roofline's kernel; the default, "
                          "~0.5%% of the run), around every kernel (~1%%), or none (off: no roofline)")
+    ap.add_argument("--burst-ahead", choices=["on", "off"], default="on",
+                    help="submit the next burst before reading the current one")
     ap.add_argument("--burst", type=int, default=0,
                     help="passes per host synchronisation on a cached graph (0: automatic, 1: every pass)")
     ap.add_argument("--cpu-seconds", type=float, default=30.0,
@@ -854,7 +856,7 @@ def main():
                    "lambda1": env["LAMBDA"], "alpha": env["ALPHA"], "eps_pr": 0.01, "eps_katz": 1e-6,
                    "eps_ev": 1e-6, "stop": stop_rule(args, args.measure), "push_kernel": args.kernel, "pull": args.pull,
                    "drop_measures": args.drop_measures, "burst": args.burst or "auto",
-                   "parallelism": "single GPU"},
+                   "burst_ahead": args.burst_ahead, "timing": args.timing, "parallelism": "single GPU"},
     }
     out.update({k: v for k, v in rec.items() if k not in ("value", "unit", "steps", "warmup", "ms_per_step")})
     out["cpu_baseline"] = cpu

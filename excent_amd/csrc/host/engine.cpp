@@ -86,6 +86,7 @@ struct Options {
     int io_threads = 0;           // host threads for reads, gathers and checks (0: all)
     int hub_lds = 1024;           // destinations privatised in LDS by the hub-piece push (0: none)
     int burst = 0;                // passes per host synchronisation over a cached graph (0: automatic)
+    std::string burst_ahead = "on";   // submit the next burst before reading the current one
 };
 
 const char* USAGE =
@@ -155,6 +156,7 @@ Options parse(int argc, const char* const* argv) {
         else if (a == "--hub-lds") o.hub_lds = atoi(next());
         else if (a == "--drop-measures") o.drop_measures = next();
         else if (a == "--burst") o.burst = atoi(next());
+        else if (a == "--burst-ahead") o.burst_ahead = next();
         else die("unknown option " + a);
     }
     return o;
@@ -272,6 +274,7 @@ private:
     int act_ = 1;                       // measure slots the passes update (exc_set_active_measures)
     bool dropped_global_ = false;       // a measure was frozen by the global rule with vertices above eps
     int burst_ = 1;                     // passes per burst (exc_run_passes); 1: every pass synchronises
+    bool burst_ahead_ = true;           // --burst-ahead
     float linf_[EXC_MAXM] = {0, 0, 0};  // largest |r| per measure, last snapshot
 
     void init_state();
@@ -340,6 +343,7 @@ void Engine::setup() {
     if (o.hot_order != "sweep" && o.hot_order != "ranges") die("--hot-order must be sweep or ranges");
     if (o.hub_lds < 0 || o.hub_lds > 1024) die("--hub-lds must be 0..1024");
     if (o.drop_measures != "on" && o.drop_measures != "off") die("--drop-measures must be on or off");
+    if (o.burst_ahead != "on" && o.burst_ahead != "off") die("--burst-ahead must be on or off");
     if (o.burst < 0 || o.burst > EXC_BURST_MAX) die("--burst must be 0 (automatic) or 1.." + std::to_string(EXC_BURST_MAX));
     global_l1_.assign(kinds_.size(), 0);
     if (o.stop == "global") {
@@ -561,6 +565,7 @@ void Engine::setup() {
     maybe_enable_pull();
     // bursts pay where a pass is short next to a host round trip: graphs of up to 2^25 edges
     burst_ = o.burst ? o.burst : (m_ <= (1ull << 25) ? 16 : 1);
+    burst_ahead_ = o.burst_ahead == "on";
     ck(exc_pass_sync(ctx, nullptr), "setup sync");
     ck(exc_reset_stats(ctx), "reset stats");
     checkpoint_ = session_ && exc_state_checkpoint(ctx, 0) == EXC_OK;
@@ -823,40 +828,88 @@ void Engine::run_passes() {
         if (pull_on_) ck(exc_fold_ahead(ctx, !fold_ahead_off && !drop_wait && pass_ + 1 < o.max_passes), "fold ahead");
 
         // A burst: the next burst_ passes over the whole cached graph enqueued together with one
-        // host synchronisation (exc_run_passes).  The per-vertex rule ends the run at the first
-        // snapshot with nothing pending; the passes after it changed nothing.
+        // host synchronisation (exc_burst_submit / exc_burst_collect).  The per-vertex rule ends
+        // the run at the first snapshot with nothing pending; the passes after it changed
+        // nothing.  With --burst-ahead on the next burst (same kind) is submitted before the
+        // current one is read, so the GPU runs it while the host reads, logs and decides; it
+        // goes on so while the kind stays the one the demand picks and no measure finishes.
         if (whole_cached) for (int c = 0; c < nc && whole_cached; c++) whole_cached = loaded_[c] != 0;
         if (burst_ > 1 && whole_cached && o.trace_file.empty() && !stop_global && !batched() &&
             pass_ + burst_ < o.max_passes) {
-            uint64_t demand = 0;
-            for (int c = 0; c < nc; c++) demand += h_cnt[2 * c + 1];
-            const bool dense = pull_on_ && ((double)demand >= pull_thr_ * (double)m || exc_dense_required(ctx) == 1);
+            auto want_dense = [&](const uint64_t* cn) {
+                uint64_t demand = 0;
+                for (int c = 0; c < nc; c++) demand += cn[2 * c + 1];
+                return pull_on_ && ((double)demand >= pull_thr_ * (double)m || exc_dense_required(ctx) == 1);
+            };
+            const bool dense = want_dense(h_cnt.data());
             const int B = burst_, nm = M_.nm;
             b_cnt.resize((size_t)B * 2 * nc);
             b_l1.resize((size_t)B * nm);
             b_linf.resize((size_t)B * nm);
-            ck(exc_run_passes(ctx, B, dense ? 1 : 0, b_cnt.data(), b_l1.data(), b_linf.data()), "burst");
-            maybe_enable_pull();
-            const Faults f1 = faults();
-            tot_maj_ += f1.maj - f0.maj;
-            tot_min_ += f1.min - f0.min;
-            const int first = pass_;
+            // the snapshot each reported pass started from: before the burst, the loop's own
+            std::vector<uint64_t> p_cnt(h_cnt);
+            std::vector<double> p_l1(l1_, l1_ + nm);
+            std::vector<float> p_linf(linf_, linf_ + nm);
+            // the report lines of the burst just collected (passes from pass_ on); false once a
+            // pass ended with nothing pending (pass_ is then the run's pass count)
+            auto report = [&](long maj, long mn) {
+                const int first = pass_;
+                for (int i = 0; i < B; i++) {
+                    const uint64_t* cn = i == 0 ? p_cnt.data() : &b_cnt[(size_t)(i - 1) * 2 * nc];
+                    uint64_t pv = 0, dm = 0;
+                    int nr = 0;
+                    for (int c = 0; c < nc; c++) { pv += cn[2 * c]; dm += cn[2 * c + 1]; nr += cn[2 * c] ? 1 : 0; }
+                    pass_line(first + i, pv, dm, nr, 0, 0, 0, i == 0 ? maj : 0, i == 0 ? mn : 0, 0, 0, true,
+                              i == 0 ? p_l1.data() : &b_l1[(size_t)(i - 1) * nm],
+                              i == 0 ? p_linf.data() : &b_linf[(size_t)(i - 1) * nm]);
+                    n_launch_++;
+                    if (dense) n_range_ += nr; else n_list_ += nr;
+                    uint64_t after = 0;
+                    for (int c = 0; c < nc; c++) after += b_cnt[(size_t)i * 2 * nc + 2 * c];
+                    if (after == 0) { pass_ = first + i + 1; return false; }
+                }
+                pass_ = first + B;
+                std::copy(&b_cnt[(size_t)(B - 1) * 2 * nc], &b_cnt[(size_t)B * 2 * nc], p_cnt.begin());
+                std::copy(&b_l1[(size_t)(B - 1) * nm], &b_l1[(size_t)B * nm], p_l1.begin());
+                std::copy(&b_linf[(size_t)(B - 1) * nm], &b_linf[(size_t)B * nm], p_linf.begin());
+                return true;
+            };
+            ck(exc_burst_submit(ctx, B, dense ? 1 : 0), "burst");
+            int inflight = 1;
             bool converged = false;
-            for (int i = 0; i < B && !converged; i++) {
-                const uint64_t* cn = i == 0 ? h_cnt.data() : &b_cnt[(size_t)(i - 1) * 2 * nc];
-                uint64_t pv = 0, dm = 0;
-                int nr = 0;
-                for (int c = 0; c < nc; c++) { pv += cn[2 * c]; dm += cn[2 * c + 1]; nr += cn[2 * c] ? 1 : 0; }
-                pass_line(first + i, pv, dm, nr, 0, 0, 0, i == 0 ? f1.maj - f0.maj : 0, i == 0 ? f1.min - f0.min : 0, 0, 0,
-                          true, i == 0 ? l1_ : &b_l1[(size_t)(i - 1) * nm], i == 0 ? linf_ : &b_linf[(size_t)(i - 1) * nm]);
-                n_launch_++;
-                if (dense) n_range_ += nr; else n_list_ += nr;
-                uint64_t after = 0;
-                for (int c = 0; c < nc; c++) after += b_cnt[(size_t)i * 2 * nc + 2 * c];
-                if (after == 0) { converged = true; pass_ = first + i + 1; }
+            Faults fb = f0;
+            for (;;) {
+                if (burst_ahead_ && inflight == 1 && pass_ + 2 * B < o.max_passes) {
+                    ck(exc_burst_submit(ctx, B, dense ? 1 : 0), "burst ahead");
+                    inflight = 2;
+                }
+                ck(exc_burst_collect(ctx, b_cnt.data(), b_l1.data(), b_linf.data(), nullptr), "burst");
+                inflight--;
+                const Faults f1 = faults();
+                tot_maj_ += f1.maj - fb.maj;
+                tot_min_ += f1.min - fb.min;
+                converged = !report(f1.maj - fb.maj, f1.min - fb.min);
+                fb = f1;
+                if (converged || !inflight) break;
+                // the burst ahead runs as this kind: read it and go back to the loop once the
+                // demand picks the other kind or a measure has finished (the loop drops it)
+                bool same = want_dense(p_cnt.data()) == dense;
+                if (nm > 1 && o.drop_measures == "on")
+                    for (int k = 0; k < nm; k++) same = same && p_linf[k] > M_.eps[k];
+                if (!same) {
+                    ck(exc_burst_collect(ctx, b_cnt.data(), b_l1.data(), b_linf.data(), nullptr), "burst ahead");
+                    inflight--;
+                    converged = !report(0, 0);
+                    break;
+                }
             }
+            if (converged) {   // a burst still in flight starts from nothing pending: it changed nothing
+                std::vector<uint64_t> scratch(b_cnt.size());
+                while (inflight) { ck(exc_burst_collect(ctx, scratch.data(), nullptr, nullptr, nullptr), "burst ahead"); inflight--; }
+            }
+            maybe_enable_pull();
             if (converged) { stop_reason_ = dropped_global_ ? 1 : 0; break; }
-            pass_ = first + B - 1;   // the loop statement counts the last one
+            pass_--;   // the loop statement counts the last one
             continue;
         }
 

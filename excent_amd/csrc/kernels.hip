@@ -2248,7 +2248,7 @@ struct PushTrial {
 };
 
 struct BurstGraph {
-    int passes = 0, phase = 0;
+    int passes = 0, phase = 0, slot = 0;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     double snap_bytes = 0;   // the snapshot bytes its launches count, added per replay
@@ -2305,10 +2305,14 @@ struct Ctx {
     uint16_t* d_trace = nullptr;
     unsigned long long* d_counters = nullptr;  // C_NCOUNTERS
     unsigned long long* h_small = nullptr;     // pinned: cnt + hotpend + residual slots
-    // exc_run_passes: one count region (counts + residual slots, cnt_words) per pass of a
-    // burst, on the device and pinned on the host
+    // bursts: one count region (counts + residual slots, cnt_words) per pass, on the device
+    // and pinned on the host, in two slots of EXC_BURST_MAX regions so that a second burst can
+    // be submitted while the first is collected (slot = submission number & 1)
     unsigned long long* d_burst = nullptr;
     unsigned long long* h_burst = nullptr;
+    int burst_subs = 0, burst_cols = 0;        // bursts submitted / collected
+    int burst_len[2] = {0, 0};
+    hipEvent_t burst_ev[2] = {nullptr, nullptr};   // after a slot's copy to the host
     bool in_burst = false;                     // its passes are being enqueued: the regions are
                                                // cleared and copied once for the whole burst
     std::vector<BurstGraph> burst_graphs;      // recorded push bursts (launch_push_burst_graph)
@@ -2971,6 +2975,11 @@ uint64_t all_columns(int K) { return K >= 64 ? ~0ull : ((1ull << K) - 1); }
 
 Ctx* as_ctx(void* p) { return static_cast<Ctx*>(p); }
 
+// Entry points that read or change the state refuse while a submitted burst is not collected.
+int burst_busy(const char* fn) {
+    return fail(EXC_ESTATE, "%s: a submitted burst is not collected (exc_burst_collect)", fn);
+}
+
 unsigned grid_for(long long items, int cap_blocks) {
     long long g = (items + BLOCK - 1) / BLOCK;
     if (g < 1) g = 1;
@@ -3069,19 +3078,21 @@ void enqueue_snapshot(Ctx* c, hipStream_t st, int hot_skip) {
 // count region cleared at once, the passes, then the regions copied to the host and the last
 // one into the context's own count buffer.  Nothing here waits or allocates, so a push burst
 // can be recorded into a graph.
-int enqueue_burst(Ctx* c, int passes, int dense) {
+int enqueue_burst(Ctx* c, int passes, int dense, int slot) {
     hipStream_t st = c->st[0];
     const size_t W = cnt_words(c);
     unsigned long long* const d_cnt = c->d_cnt;
     unsigned long long* const h_small = c->h_small;
+    unsigned long long* const d_reg = c->d_burst + (size_t)slot * EXC_BURST_MAX * W;
+    unsigned long long* const h_reg = c->h_burst + (size_t)slot * EXC_BURST_MAX * W;
     const long long words = (long long)passes * (long long)W;
-    hipLaunchKernelGGL(k_zero_u64, dim3((unsigned)((words + BLOCK - 1) / BLOCK)), dim3(BLOCK), 0, st, c->d_burst, (int)words);
+    hipLaunchKernelGGL(k_zero_u64, dim3((unsigned)((words + BLOCK - 1) / BLOCK)), dim3(BLOCK), 0, st, d_reg, (int)words);
     c->in_burst = true;
     int rc = EXC_OK;
     for (int i = 0; i < passes && rc == EXC_OK; i++) {
         // pass i writes the snapshot of the state it leaves into region i
-        c->d_cnt = c->d_burst + i * W;
-        c->h_small = c->h_burst + i * W;
+        c->d_cnt = d_reg + i * W;
+        c->h_small = h_reg + i * W;
         c->snap_ready = false;
         c->l1_valid = false;
         if (dense) {
@@ -3095,9 +3106,9 @@ int enqueue_burst(Ctx* c, int passes, int dense) {
     c->d_cnt = d_cnt;
     c->h_small = h_small;
     if (rc != EXC_OK) return rc;
-    HIPCHK(hipMemcpyAsync(c->h_burst, c->d_burst, (size_t)words * sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(h_reg, d_reg, (size_t)words * sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
     // the bitmap is already the last snapshot's
-    HIPCHK(hipMemcpyAsync(c->d_cnt, c->d_burst + (passes - 1) * W, W * sizeof(unsigned long long), hipMemcpyDeviceToDevice, st));
+    HIPCHK(hipMemcpyAsync(c->d_cnt, d_reg + (passes - 1) * W, W * sizeof(unsigned long long), hipMemcpyDeviceToDevice, st));
     return EXC_OK;
 }
 
@@ -3123,7 +3134,7 @@ void burst_graphs_clear(Ctx* c) {
     c->burst_graphs.clear();
 }
 
-int launch_push_burst_graph(Ctx* c, int passes) {
+int launch_push_burst_graph(Ctx* c, int passes, int slot) {
     hipStream_t st = c->st[0];
     // the launches reserve the piece buffer for the whole cache: done before recording, so
     // recording never allocates or waits
@@ -3132,16 +3143,17 @@ int launch_push_burst_graph(Ctx* c, int passes) {
     const int phase = c->pb[0].phase;
     BurstGraph* hit = nullptr;
     for (auto& g : c->burst_graphs)
-        if (g.passes == passes && g.phase == phase) hit = &g;
+        if (g.passes == passes && g.phase == phase && g.slot == slot) hit = &g;
     if (!hit) {
         BurstGraph g;
         g.passes = passes;
         g.phase = phase;
+        g.slot = slot;
         const bool timing = c->timing;
         const double sb = c->snap_bytes;
         c->timing = false;   // no event nodes: the burst is timed as a whole
         HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-        rc = enqueue_burst(c, passes, 0);
+        rc = enqueue_burst(c, passes, 0, slot);
         const hipError_t ce = hipStreamEndCapture(st, &g.graph);
         c->timing = timing;
         g.snap_bytes = c->snap_bytes - sb;
@@ -3408,6 +3420,7 @@ int exc_destroy(void* p) {
     for (auto e : c->free_events) hipEventDestroy(e);
     for (auto e : c->bq_ev) if (e) hipEventDestroy(e);
     for (auto e : c->ptune_ev) if (e) hipEventDestroy(e);
+    for (auto e : c->burst_ev) if (e) hipEventDestroy(e);
     burst_graphs_clear(c);
     for (auto& sl : c->slot) {
         if (sl.h_edges) hipHostFree(sl.h_edges);
@@ -3450,6 +3463,7 @@ int exc_upload_offsets(void* p, const uint64_t* h_idx, size_t v0, size_t cnt) {
 int exc_init_measure(void* p, int i, float beta, float lambda) {
     Ctx* c = as_ctx(p);
     if (!c || i < 0 || i >= c->nm) return fail(EXC_EINVAL, "exc_init_measure: bad slot");
+    if (c->burst_subs != c->burst_cols) return burst_busy(__func__);
     HIPCHK(hipSetDevice(c->dev));
     c->snap_ready = false;   // the state changes: a fused snapshot no longer holds
     c->front_ok = false;     // ... nor may the next batched snapshot read the touched rows only
@@ -3475,6 +3489,7 @@ int exc_init_measure(void* p, int i, float beta, float lambda) {
 int exc_set_state(void* p, int i, const float* h_x, const float* h_r) {
     Ctx* c = as_ctx(p);
     if (!c || i < 0 || i >= (c->K ? c->K : c->nm)) return fail(EXC_EINVAL, "exc_set_state: bad slot");
+    if (c->burst_subs != c->burst_cols) return burst_busy(__func__);
     HIPCHK(hipSetDevice(c->dev));
     c->snap_ready = false;   // the state changes: a fused snapshot no longer holds
     c->front_ok = false;     // ... nor may the next batched snapshot read the touched rows only
@@ -3503,6 +3518,7 @@ int exc_read_state(void* p, int i, int which, float* h_out, size_t v0, size_t cn
     if (!c || i < 0 || i >= (c->K ? c->K : c->nm) || v0 + cnt > (size_t)c->n || !h_out)
         return fail(EXC_EINVAL, "exc_read_state: bad arguments");
     if (!cnt) return EXC_OK;
+    if (c->burst_subs != c->burst_cols) return burst_busy(__func__);
     HIPCHK(hipSetDevice(c->dev));
     float* tmp;
     HIPCHK(hipMalloc(&tmp, cnt * sizeof(float)));
@@ -3518,6 +3534,7 @@ int exc_read_state(void* p, int i, int which, float* h_out, size_t v0, size_t cn
 int exc_state_checkpoint(void* p, int restore) {
     Ctx* c = as_ctx(p);
     if (!c) return fail(EXC_EINVAL, "exc_state_checkpoint: null context");
+    if (c->burst_subs != c->burst_cols) return burst_busy(__func__);
     HIPCHK(hipSetDevice(c->dev));
     c->snap_ready = false;   // the state changes: a fused snapshot no longer holds
     c->front_ok = false;     // ... nor may the next batched snapshot read the touched rows only
@@ -3594,6 +3611,7 @@ void parse_counts(Ctx* c, uint64_t* h_cnt, uint64_t* h_hotpend) {
 int exc_snapshot(void* p, int hot_skip, uint64_t* h_cnt, uint64_t* h_hotpend) {
     Ctx* c = as_ctx(p);
     if (!c || !h_cnt) return fail(EXC_EINVAL, "exc_snapshot: bad arguments");
+    if (c->burst_subs != c->burst_cols) return burst_busy(__func__);
     HIPCHK(hipSetDevice(c->dev));
     hipStream_t st = c->st[0];
     if ((c->snap_ready || c->folded) && !hot_skip) {   // computed by the dense pass that ended the last pass
@@ -3615,6 +3633,7 @@ int exc_snapshot(void* p, int hot_skip, uint64_t* h_cnt, uint64_t* h_hotpend) {
 int exc_bitmap_to_host(void* p, uint32_t* h_bitmap) {
     Ctx* c = as_ctx(p);
     if (!c || !h_bitmap) return fail(EXC_EINVAL, "exc_bitmap_to_host: bad arguments");
+    if (c->burst_subs != c->burst_cols) return burst_busy(__func__);
     HIPCHK(hipSetDevice(c->dev));
     HIPCHK(hipMemcpyAsync(h_bitmap, c->d_bitmap, (((size_t)c->n + 31) / 32) * sizeof(uint32_t),
                           hipMemcpyDeviceToHost, c->st[0]));
@@ -3730,6 +3749,7 @@ int exc_push_resident(void* p, const exc_chunk* ch, int use_list, int group, int
     Ctx* c = as_ctx(p);
     if (!c || !ch || ch->v0 < 0 || ch->v1 > c->n || ch->v0 > ch->v1) return fail(EXC_EINVAL, "exc_push_resident: bad chunk");
     if (!c->d_arena || ch->e1 > c->arena_edges) return fail(EXC_ESTATE, "exc_push_resident: chunk not in the edge cache");
+    if (c->burst_subs != c->burst_cols) return burst_busy(__func__);
     HIPCHK(hipSetDevice(c->dev));
     c->snap_ready = false;   // the state changes: a fused snapshot no longer holds
     c->l1_valid = false;
@@ -3758,6 +3778,7 @@ int exc_stream_chunk(void* p, int s, const exc_chunk* ch, int to_resident, int h
     if (c->d_wts) return fail(EXC_ESTATE, "exc_stream_chunk: weighted graphs must be fully cached");
     if (ne > c->max_chunk_edges) return fail(EXC_EINVAL, "exc_stream_chunk: chunk larger than the staging slot");
     if (to_resident && (!c->d_arena || ch->e1 > c->arena_edges)) return fail(EXC_ESTATE, "exc_stream_chunk: chunk not in the edge cache");
+    if (c->burst_subs != c->burst_cols) return burst_busy(__func__);
     HIPCHK(hipSetDevice(c->dev));
     c->snap_ready = false;   // the state changes: a fused snapshot no longer holds
     c->front_ok = false;     // ... nor may the next batched snapshot read the touched rows only
@@ -3800,6 +3821,7 @@ int exc_gather_chunk(void* p, int s, int k, int group) {
     Slot& sl = c->slot[s];
     const uint64_t ne = k ? sl.h_off[k] : 0;
     if (ne > c->max_chunk_edges) return fail(EXC_EINVAL, "exc_gather_chunk: gathered lists exceed the slot");
+    if (c->burst_subs != c->burst_cols) return burst_busy(__func__);
     HIPCHK(hipSetDevice(c->dev));
     c->snap_ready = false;   // the state changes: a fused snapshot no longer holds
     c->front_ok = false;     // ... nor may the next batched snapshot read the touched rows only
@@ -3833,6 +3855,7 @@ int exc_push_hot(void* p, int lo, int hi, int group, int after_slots) {
     Ctx* c = as_ctx(p);
     if (!c || lo < 0 || hi > c->nh || lo > hi) return fail(EXC_EINVAL, "exc_push_hot: bad range");
     if (hi == lo) return EXC_OK;
+    if (c->burst_subs != c->burst_cols) return burst_busy(__func__);
     HIPCHK(hipSetDevice(c->dev));
     c->snap_ready = false;   // the state changes: a fused snapshot no longer holds
     c->front_ok = false;     // ... nor may the next batched snapshot read the touched rows only
@@ -3947,47 +3970,88 @@ int exc_set_edge_weights(void* p, const float* h_w, uint64_t cnt) {
     return EXC_OK;
 }
 
-int exc_run_passes(void* p, int passes, int dense, uint64_t* h_cnt, double* h_l1, float* h_linf) {
+int exc_burst_submit(void* p, int passes, int dense) {
     Ctx* c = as_ctx(p);
-    if (!c || passes < 1 || passes > EXC_BURST_MAX || !h_cnt)
-        return fail(EXC_EINVAL, "exc_run_passes: 1 to %d passes and a count array", EXC_BURST_MAX);
-    if (c->K) return fail(EXC_ESTATE, "exc_run_passes: plain contexts only");
-    if (!c->d_arena || c->arena_edges != c->m) return fail(EXC_ESTATE, "exc_run_passes: the whole graph must be in the edge cache");
-    if (c->d_trace) return fail(EXC_ESTATE, "exc_run_passes: not with the pending-pass trace");
-    if (dense && !(c->d_tadj && (!c->d_wts || c->d_twts))) return fail(EXC_ESTATE, "exc_run_passes: dense passes need exc_enable_pull");
+    if (!c || passes < 1 || passes > EXC_BURST_MAX)
+        return fail(EXC_EINVAL, "exc_burst_submit: 1 to %d passes", EXC_BURST_MAX);
+    if (c->K) return fail(EXC_ESTATE, "exc_burst_submit: plain contexts only");
+    if (!c->d_arena || c->arena_edges != c->m) return fail(EXC_ESTATE, "exc_burst_submit: the whole graph must be in the edge cache");
+    if (c->d_trace) return fail(EXC_ESTATE, "exc_burst_submit: not with the pending-pass trace");
+    if (dense && !(c->d_tadj && (!c->d_wts || c->d_twts))) return fail(EXC_ESTATE, "exc_burst_submit: dense passes need exc_enable_pull");
     if (!dense && c->folded)
-        return fail(EXC_ESTATE, "exc_run_passes: the last dense pass folded the next one; it must be a dense pass");
+        return fail(EXC_ESTATE, "exc_burst_submit: the last dense pass folded the next one; it must be a dense pass");
+    if (c->burst_subs - c->burst_cols >= 2) return fail(EXC_ESTATE, "exc_burst_submit: two bursts are already in flight");
     HIPCHK(hipSetDevice(c->dev));
     const size_t W = cnt_words(c);
     if (!c->d_burst) {
-        HIPCHK(hipMalloc(&c->d_burst, EXC_BURST_MAX * W * sizeof(unsigned long long)));
-        HIPCHK(hipHostMalloc((void**)&c->h_burst, EXC_BURST_MAX * W * sizeof(unsigned long long), hipHostMallocDefault));
+        HIPCHK(hipMalloc(&c->d_burst, 2 * EXC_BURST_MAX * W * sizeof(unsigned long long)));
+        HIPCHK(hipHostMalloc((void**)&c->h_burst, 2 * EXC_BURST_MAX * W * sizeof(unsigned long long), hipHostMallocDefault));
+        for (auto& e : c->burst_ev) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     }
-    int rc = !dense && c->use_burst_graphs ? launch_push_burst_graph(c, passes) : enqueue_burst(c, passes, dense);
+    const int slot = c->burst_subs & 1;
+    int rc = !dense && c->use_burst_graphs ? launch_push_burst_graph(c, passes, slot) : enqueue_burst(c, passes, dense, slot);
     if (rc) return rc;
-    for (auto s2 : c->st) HIPCHK(hipStreamSynchronize(s2));
-    c->counts_enqueued = false;
-    for (auto& sl : c->slot) sl.busy = false;
-    HIPCHK(hipGetLastError());
-    rc = collect_timers(c);
-    if (rc) return rc;
-    for (int i = 0; i < passes; i++) {
-        memcpy(c->h_small, c->h_burst + i * W, W * sizeof(unsigned long long));
+    HIPCHK(hipEventRecord(c->burst_ev[slot], c->st[0]));
+    c->burst_len[slot] = passes;
+    c->burst_subs++;
+    return EXC_OK;
+}
+
+int exc_burst_collect(void* p, uint64_t* h_cnt, double* h_l1, float* h_linf, int* passes) {
+    Ctx* c = as_ctx(p);
+    if (!c || !h_cnt) return fail(EXC_EINVAL, "exc_burst_collect: bad arguments");
+    if (c->burst_cols == c->burst_subs) return fail(EXC_ESTATE, "exc_burst_collect: no burst in flight");
+    HIPCHK(hipSetDevice(c->dev));
+    const int slot = c->burst_cols & 1;
+    HIPCHK(hipEventSynchronize(c->burst_ev[slot]));
+    c->burst_cols++;
+    const bool last = c->burst_cols == c->burst_subs;
+    if (last) {   // nothing else in flight: the context is idle again
+        for (auto s2 : c->st) HIPCHK(hipStreamSynchronize(s2));
+        c->counts_enqueued = false;
+        for (auto& sl : c->slot) sl.busy = false;
+        HIPCHK(hipGetLastError());
+        int rc = collect_timers(c);   // (their events may belong to a burst still running otherwise)
+        if (rc) return rc;
+    }
+    const size_t W = cnt_words(c);
+    const int np = c->burst_len[slot];
+    const unsigned long long* h_reg = c->h_burst + (size_t)slot * EXC_BURST_MAX * W;
+    for (int i = 0; i < np; i++) {
+        memcpy(c->h_small, h_reg + i * W, W * sizeof(unsigned long long));
         parse_counts(c, h_cnt + (size_t)i * 2 * c->nc, nullptr);
         for (int k = 0; k < c->nm; k++) {
             if (h_l1) h_l1[(size_t)i * c->nm + k] = c->l1_last[k];
             if (h_linf) h_linf[(size_t)i * c->nm + k] = c->linf_last[k];
         }
     }
-    // the next exc_snapshot returns the last pass's snapshot without a launch
-    c->snap_ready = true;
-    c->counts_copied = true;
+    if (passes) *passes = np;
+    // with nothing left in flight, the next exc_snapshot returns the last pass's snapshot
+    // without a launch
+    c->snap_ready = last;
+    c->counts_copied = last;
     return EXC_OK;
+}
+
+int exc_run_passes(void* p, int passes, int dense, uint64_t* h_cnt, double* h_l1, float* h_linf) {
+    Ctx* c = as_ctx(p);
+    if (!c || passes < 1 || passes > EXC_BURST_MAX || !h_cnt)
+        return fail(EXC_EINVAL, "exc_run_passes: 1 to %d passes and a count array", EXC_BURST_MAX);
+    if (c->burst_subs != c->burst_cols) return fail(EXC_ESTATE, "exc_run_passes: a submitted burst is not collected");
+    int rc = exc_burst_submit(p, passes, dense);
+    if (rc) {   // the same checks, reported under this name
+        std::string m = exc_last_error();
+        const std::string from = "exc_burst_submit";
+        if (m.compare(0, from.size(), from) == 0) m = "exc_run_passes" + m.substr(from.size());
+        return fail(rc, "%s", m.c_str());
+    }
+    return exc_burst_collect(p, h_cnt, h_l1, h_linf, nullptr);
 }
 
 int exc_pass_sync(void* p, double* waited) {
     Ctx* c = as_ctx(p);
     if (!c) return fail(EXC_EINVAL, "exc_pass_sync: null context");
+    if (c->burst_subs != c->burst_cols) return burst_busy(__func__);
     HIPCHK(hipSetDevice(c->dev));
     auto a = std::chrono::steady_clock::now();
     for (auto s : c->st) HIPCHK(hipStreamSynchronize(s));
@@ -4018,6 +4082,7 @@ int exc_pushed(void* p, uint64_t* edges) {
 int exc_get_stats(void* p, exc_stats* o) {
     Ctx* c = as_ctx(p);
     if (!c || !o) return fail(EXC_EINVAL, "exc_get_stats: bad arguments");
+    if (c->burst_subs != c->burst_cols) return burst_busy(__func__);
     HIPCHK(hipSetDevice(c->dev));
     unsigned long long h[C_NCOUNTERS];
     HIPCHK(hipMemcpy(h, c->d_counters, sizeof h, hipMemcpyDeviceToHost));
@@ -4128,6 +4193,7 @@ int exc_enable_pull(void* p, double threshold) {
     Ctx* c = as_ctx(p);
     if (!c) return fail(EXC_EINVAL, "exc_enable_pull: null context");
     if (!c->d_arena || c->arena_edges != c->m) return fail(EXC_ESTATE, "exc_enable_pull: the whole graph must be in the edge cache");
+    if (c->burst_subs != c->burst_cols) return burst_busy(__func__);
     HIPCHK(hipSetDevice(c->dev));
     HIPCHK(hipDeviceSynchronize());
     c->pull_threshold = threshold;
@@ -4349,6 +4415,7 @@ int exc_residual_max(void* p, float* out) {
 int exc_set_active_measures(void* p, int count) {
     Ctx* c = as_ctx(p);
     if (!c) return fail(EXC_EINVAL, "exc_set_active_measures: null context");
+    if (c->burst_subs != c->burst_cols) return burst_busy(__func__);
     if (c->K) return fail(EXC_ESTATE, "exc_set_active_measures: batched contexts drop columns (exc_batch_set_active)");
     if (count < 1 || count > c->nm) return fail(EXC_EINVAL, "exc_set_active_measures: count must be 1..%d", c->nm);
     if (count > c->nm_act && c->folded)
@@ -4384,6 +4451,7 @@ int exc_dense_required(void* p) {
 int exc_reset_stats(void* p) {
     Ctx* c = as_ctx(p);
     if (!c) return fail(EXC_EINVAL, "exc_reset_stats: null context");
+    if (c->burst_subs != c->burst_cols) return burst_busy(__func__);
     HIPCHK(hipSetDevice(c->dev));
     HIPCHK(hipDeviceSynchronize());
     int rc = collect_timers(c);

@@ -157,6 +157,8 @@ HIP_SIGNATURES = {
     "exc_fold_ahead": (c_int, [c_void_p, c_int]),
     "exc_set_active_measures": (c_int, [c_void_p, c_int]),
     "exc_run_passes": (c_int, [c_void_p, c_int, c_int, P(c_u64), P(c_double), P(ctypes.c_float)]),
+    "exc_burst_submit": (c_int, [c_void_p, c_int, c_int]),
+    "exc_burst_collect": (c_int, [c_void_p, P(c_u64), P(c_double), P(ctypes.c_float), P(c_int)]),
     "exc_set_hub_destinations": (c_int, [c_void_p, c_void_p, c_int]),
     "exc_snapshot_noalloc": (c_int, [c_int, c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_int,
                                      c_void_p, c_void_p, c_void_p, c_void_p]),
@@ -409,6 +411,20 @@ class Context:
         check(F["exc_run_passes"](self.ctx, passes, 1 if dense else 0, cnt.ctypes.data_as(P(c_u64)),
                                   l1.ctypes.data_as(P(c_double)), mx.ctypes.data_as(P(ctypes.c_float))), "run_passes")
         return cnt, l1, mx
+
+    def burst_submit(self, passes, dense):
+        """Enqueue a burst without waiting (at most two in flight)."""
+        check(F["exc_burst_submit"](self.ctx, passes, 1 if dense else 0), "burst_submit")
+
+    def burst_collect(self, passes):
+        """Wait for the oldest burst in flight: as run_passes' outputs, and its pass count."""
+        cnt = np.zeros((passes, 2 * self.nc), np.uint64)
+        l1 = np.zeros((passes, self.nm), np.float64)
+        mx = np.zeros((passes, self.nm), np.float32)
+        got = c_int(0)
+        check(F["exc_burst_collect"](self.ctx, cnt.ctypes.data_as(P(c_u64)), l1.ctypes.data_as(P(c_double)),
+                                     mx.ctypes.data_as(P(ctypes.c_float)), ctypes.byref(got)), "burst_collect")
+        return cnt[:got.value], l1[:got.value], mx[:got.value]
 
     def set_active_measures(self, count):
         """Dense passes over the first `count` measure slots only (the others converged)."""

@@ -267,6 +267,18 @@ int exc_set_active_measures(void* ctx, int count);
  * with the whole graph in the edge cache, no hot set and no trace.  No reference counterpart
  * (the reference synchronises with the host at every pass, src/excent.cu:664-665, 750). */
 int exc_run_passes(void* ctx, int passes, int dense, uint64_t* h_cnt, double* h_l1, float* h_linf);
+/* exc_run_passes in two halves, so that the next burst runs while the host reads this one:
+ * exc_burst_submit enqueues a burst (same arguments and checks) and returns at once; at most
+ * two bursts are in flight, the second starting from the state the first leaves.
+ * exc_burst_collect waits for the oldest one and writes its passes' counts, sums and largest
+ * |r| as exc_run_passes does (*passes: how many).  A burst submitted after the run's last
+ * needed pass changes nothing once a pass started from nothing pending, so the caller may
+ * submit the next burst before it knows whether the current one converged.  While a burst is
+ * in flight the entry points that read or change the state refuse (EXC_ESTATE); after the
+ * last one is collected the next exc_snapshot returns its last pass's snapshot.  No
+ * reference counterpart. */
+int exc_burst_submit(void* ctx, int passes, int dense);
+int exc_burst_collect(void* ctx, uint64_t* h_cnt, double* h_l1, float* h_linf, int* passes);
 
 /* Push a resident chunk run [ch.v0, ch.v1) from the arena on the resident stream
  * (launch_chunk with dst = resident[c], src/excent.cu:634-652, 684-688).  use_list:

@@ -49,21 +49,36 @@ def single_passes(idx, adj, eps, pull, max_passes=500, init=None):
         return snaps, ctx.read_state(0, 0), ctx.read_state(0, 1)
 
 
-def burst_passes(idx, adj, eps, pull, B, dense, max_bursts=100, init=None):
-    """The same run by bursts of B passes, ended at the first snapshot with nothing pending."""
+def burst_passes(idx, adj, eps, pull, B, dense, max_bursts=100, init=None, ahead=False):
+    """The same run by bursts of B passes, ended at the first snapshot with nothing pending.
+    ahead: the next burst is submitted before the current one is collected (two in flight),
+    and the one still in flight when the run ends is drained."""
     snaps = []
     with context(idx, adj, eps, pull, init) as ctx:
         cnt, _ = ctx.snapshot()
         snaps.append((cnt.copy(), ctx.residual_l1()[0], ctx.residual_max()[0]))
+        inflight = 0
         for _ in range(max_bursts):
             if snaps[-1][0][0::2].sum() == 0:
                 break
             d = dense if isinstance(dense, bool) else dense(ctx)
-            cnts, l1, mx = ctx.run_passes(B, d)
+            if ahead:
+                while inflight < 2:
+                    ctx.burst_submit(B, d)
+                    inflight += 1
+                cnts, l1, mx = ctx.burst_collect(B)
+                inflight -= 1
+                assert len(cnts) == B
+            else:
+                cnts, l1, mx = ctx.run_passes(B, d)
             for i in range(B):
                 snaps.append((cnts[i].copy(), float(l1[i][0]), float(mx[i][0])))
                 if cnts[i][0::2].sum() == 0:
                     break
+        while inflight:   # submitted after the run's end: it starts from nothing pending
+            cnts, _, _ = ctx.burst_collect(B)
+            inflight -= 1
+            assert cnts.sum() == 0
         # the next snapshot is the burst's last one, without a launch
         cnt, _ = ctx.snapshot()
         assert np.array_equal(cnt, cnts[B - 1])
@@ -71,23 +86,25 @@ def burst_passes(idx, adj, eps, pull, B, dense, max_bursts=100, init=None):
         return snaps, ctx.read_state(0, 0), ctx.read_state(0, 1)
 
 
+@pytest.mark.parametrize("ahead", [False, True])
 @pytest.mark.parametrize("B", [2, 8, 16])
-def test_burst_dense_cycle_bit_exact(B):
+def test_burst_dense_cycle_bit_exact(B, ahead):
     """Dense passes on a directed cycle (one in-edge per destination): bursts of B passes give
-    every pass's snapshot and the final state bit for bit as pass-by-pass runs."""
-    print("case burst dense cycle", B, flush=True)
+    every pass's snapshot and the final state bit for bit as pass-by-pass runs, also with the
+    next burst submitted before the current one is collected (ahead)."""
+    print("case burst dense cycle", B, ahead, flush=True)
     n = 5000
     idx = np.arange(n + 1, dtype=np.uint64)
     adj = ((np.arange(n) + 1) % n).astype(np.int32)
     a = single_passes(idx, adj, 1e-4, 0.0)
-    b = burst_passes(idx, adj, 1e-4, 0.0, B, True)
+    b = burst_passes(idx, adj, 1e-4, 0.0, B, True, ahead=ahead)
     assert len(a[0]) == len(b[0]) > 10
     for p, (sa, sb) in enumerate(zip(a[0], b[0])):
         assert np.array_equal(sa[0], sb[0]) and sa[1] == sb[1] and sa[2] == sb[2], p
     assert np.array_equal(a[1], b[1]) and np.array_equal(a[2], b[2])
 
 
-@pytest.mark.parametrize("graph", ["1", "0"])
+@pytest.mark.parametrize("graph", ["1", "0", "1-ahead"])
 @pytest.mark.parametrize("B", [2, 8])
 def test_burst_push_paths_bit_exact(B, graph, monkeypatch):
     """Bitmap-driven pushes along 300 disjoint paths of 20 edges with the residual mass at the
@@ -95,8 +112,10 @@ def test_burst_push_paths_bit_exact(B, graph, monkeypatch):
     was not pending, so every destination receives one delta and the arithmetic does not
     depend on atomic order; the bursts equal pass-by-pass runs bit for bit (21 passes).
     graph 1: push bursts replayed from recorded HIP graphs (every replay checked against the
-    pass-by-pass run), 0: launched directly."""
+    pass-by-pass run), 0: launched directly; 1-ahead: graphs, two bursts in flight (each slot
+    of count regions has its own recorded graph)."""
     print("case burst push paths", B, graph, flush=True)
+    graph, _, ahead = graph.partition("-")
     monkeypatch.setenv("EXC_BURST_GRAPH", graph)
     paths, L = 300, 20
     n = paths * (L + 1)
@@ -107,7 +126,7 @@ def test_burst_push_paths_bit_exact(B, graph, monkeypatch):
     r0 = np.where(v % (L + 1) == 0, 1.0, 0.0).astype(np.float32)
     init = (np.zeros(n, np.float32), r0)
     a = single_passes(idx, adj, 1e-3, None, init=init)
-    b = burst_passes(idx, adj, 1e-3, None, B, False, init=init)
+    b = burst_passes(idx, adj, 1e-3, None, B, False, init=init, ahead=ahead == "ahead")
     assert (burst_passes.graph_launches > 0) == (graph == "1")
     assert len(a[0]) == len(b[0]) == L + 2
     for p, (sa, sb) in enumerate(zip(a[0], b[0])):
@@ -185,6 +204,21 @@ def test_burst_refusals(rmat12):
         if ctx.dense_required():
             with pytest.raises(nat.ExcError, match="folded"):
                 ctx.run_passes(2, False)
+    with context(idx, adj, 1e-4, None) as ctx:   # two bursts in flight at most; nothing else meanwhile
+        ctx.snapshot()
+        with pytest.raises(nat.ExcError, match="no burst in flight"):
+            ctx.burst_collect(2)
+        ctx.burst_submit(2, False)
+        ctx.burst_submit(2, False)
+        with pytest.raises(nat.ExcError, match="two bursts"):
+            ctx.burst_submit(2, False)
+        for call in (ctx.snapshot, lambda: ctx.run_passes(2, False), lambda: ctx.read_state(0, 1),
+                     lambda: ctx.push_resident(0, n, 0, m, False)):
+            with pytest.raises(nat.ExcError, match="not collected"):
+                call()
+        ctx.burst_collect(2)
+        ctx.burst_collect(2)
+        ctx.snapshot()
     with nat.Context(n, m, np.array([0], np.int32), [nat.EXC_PR], [1e-4], [0.85], [1]) as ctx:
         ctx.upload_offsets(idx)
         with pytest.raises(nat.ExcError, match="edge cache"):
@@ -198,18 +232,19 @@ PASS = re.compile(r"^pass +(\d+): pending (\d+) v", re.M)
 
 
 def test_cli_burst(tmp_path, rmat14):
-    """bin/excent PageRank with --burst 8 and --burst 1: both end with nothing pending within
-    the FP64 bound; the pass lines are numbered 0 .. passes-1 and the last one's pending count
-    is nonzero (the run ended at the first snapshot with nothing pending)."""
+    """bin/excent PageRank with --burst 8 (the next burst submitted ahead, and not) and
+    --burst 1: each ends with nothing pending within the FP64 bound; the pass lines are
+    numbered 0 .. passes-1 and the last one's pending count is nonzero (the run ended at the
+    first snapshot with nothing pending)."""
     idx, adj = orc.egr_load(rmat14)
     n = len(idx) - 1
     b = np.full(n, np.float64(np.float32(1) - np.float32(0.85)))
     ref, _, _ = orc.fp64_solve_b(idx, adj, b)
-    for burst in (8, 1):
-        print("case cli burst", burst, flush=True)
-        log, pre = str(tmp_path / f"b{burst}.log"), str(tmp_path / f"b{burst}")
+    for burst, ahead in ((8, "on"), (8, "off"), (1, "on")):
+        print("case cli burst", burst, ahead, flush=True)
+        log, pre = str(tmp_path / f"b{burst}{ahead}.log"), str(tmp_path / f"b{burst}{ahead}")
         assert nat.excent([rmat14, "--measure", "pr", "--eps-pr", 1e-4, "--cache", "chunks", "--preload",
-                           "--burst", burst, "--dump", pre, "--log", log]) == 0, open(log).read()
+                           "--burst", burst, "--burst-ahead", ahead, "--dump", pre, "--log", log]) == 0, open(log).read()
         text = open(log).read()
         passes = int(re.search(r"passes (\d+)  measure pr", text).group(1))
         lines = [(int(a), int(p)) for a, p in PASS.findall(text)]
@@ -221,6 +256,7 @@ def test_cli_burst(tmp_path, rmat14):
         assert np.abs(x.astype(np.float64) - ref).sum() <= np.abs(tr).sum() / 0.15 * (1 + 1e-6) + 1e-6
         assert np.abs(tr).max() <= 1e-4 * (1 + 1e-3) + 2e-4 * max(1.0, np.abs(x).max())
     assert nat.excent([rmat14, "--burst", "17", "--log", str(tmp_path / "bad.log")]) == 1
+    assert nat.excent([rmat14, "--burst-ahead", "maybe", "--log", str(tmp_path / "bad2.log")]) == 1
 
 
 def test_burst_graphs_follow_the_hub_table(rmat12):
