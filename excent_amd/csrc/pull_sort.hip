@@ -92,4 +92,36 @@ hipError_t build_sorted_tiles(int n, int T, const uint64_t* tidx, const int32_t*
     return hipSuccess;
 }
 
+hipError_t sort_runs(int32_t* keys, float* vals, uint64_t count, const std::vector<int>& starts, int row_bits,
+                     hipStream_t st) {
+    const int nseg = (int)starts.size() - 1;
+    if (nseg < 1 || count == 0) return hipSuccess;
+    int32_t* kout = nullptr;
+    float* vout = nullptr;
+    int* d_starts = nullptr;
+    void* temp = nullptr;
+    size_t temp_bytes = 0;
+    hipError_t e = hipSuccess;
+    auto step = [&](hipError_t r) { if (e == hipSuccess) e = r; return e == hipSuccess; };
+    step(hipMalloc(&kout, count * sizeof(int32_t)));
+    if (e == hipSuccess && vals) step(hipMalloc(&vout, count * sizeof(float)));
+    if (e == hipSuccess) step(hipMalloc(&d_starts, starts.size() * sizeof(int)));
+    if (e == hipSuccess) step(hipMemcpyAsync(d_starts, starts.data(), starts.size() * sizeof(int), hipMemcpyHostToDevice, st));
+    auto sort = [&](void* t, size_t& b) {
+        return vals ? hipcub::DeviceSegmentedRadixSort::SortPairs(t, b, keys, kout, vals, vout, (int)count, nseg, d_starts,
+                                                                  d_starts + 1, 0, row_bits, st)
+                    : hipcub::DeviceSegmentedRadixSort::SortKeys(t, b, keys, kout, (int)count, nseg, d_starts, d_starts + 1,
+                                                                 0, row_bits, st);
+    };
+    if (e == hipSuccess) step(sort(nullptr, temp_bytes));
+    if (e == hipSuccess) step(hipMalloc(&temp, temp_bytes ? temp_bytes : 1));
+    if (e == hipSuccess) step(sort(temp, temp_bytes));
+    if (e == hipSuccess) step(hipMemcpyAsync(keys, kout, count * sizeof(int32_t), hipMemcpyDeviceToDevice, st));
+    if (e == hipSuccess && vals) step(hipMemcpyAsync(vals, vout, count * sizeof(float), hipMemcpyDeviceToDevice, st));
+    if (e == hipSuccess) step(hipStreamSynchronize(st));
+    void* tmp[] = {kout, vout, d_starts, temp};
+    for (void* x : tmp) if (x) (void)hipFree(x);
+    return e;
+}
+
 }  // namespace excent_dev

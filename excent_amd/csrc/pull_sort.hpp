@@ -7,6 +7,7 @@
 
 #include <cstdint>
 #include <string>
+#include <vector>
 
 namespace excent_dev {
 
@@ -24,5 +25,13 @@ struct SortedTileBuffers {
 hipError_t build_sorted_tiles(int n, int T, const uint64_t* tidx, const int32_t* tadj, const float* tw,
                               uint64_t entries, int row_bits, hipStream_t st, SortedTileBuffers* out);
 void free_sorted_tiles(SortedTileBuffers* b);
+
+// Sorts each run [starts[i], starts[i + 1]) of keys (row ids < 2^row_bits) ascending, in
+// place, with vals (or null) following its keys: the hub in-lists, so that a wave's
+// consecutive entries gather neighbouring rows.  starts: host offsets, starts[0] = 0, the
+// last = count < 2^31.  Temporary buffers are freed before returning; the work runs on `st`
+// and is waited for.
+hipError_t sort_runs(int32_t* keys, float* vals, uint64_t count, const std::vector<int>& starts, int row_bits,
+                     hipStream_t st);
 
 }  // namespace excent_dev
