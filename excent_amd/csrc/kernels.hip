@@ -1259,30 +1259,34 @@ __global__ __launch_bounds__(BLOCK) void k_pull_sorted(PullArgs a, SortedTiles s
     __syncthreads();
     const uint64_t e0 = st.start[blockIdx.x], e1 = st.start[blockIdx.x + 1];
     uint32_t taken = 0;
+    const uint64_t last = e1 ? e1 - 1 : 0;
     for (uint64_t b = e0; b < e1; b += (uint64_t)BLOCK * U) {
         // ids, destinations and coefficients of the lane's entries, then their live-bitmap
-        // words, then the rows: each stage issued as one batch
+        // words, then the rows: each stage issued as one batch (the ids without branches: an
+        // entry past the tile's end reads the tile's last entry and is not live)
         int vid[U];
         uint32_t dl[U];
         float wv[W ? U : 1];
+        bool in[U];
 #pragma unroll
         for (int u = 0; u < U; u++) {
             const uint64_t e = b + (uint64_t)u * BLOCK + tid;
-            const bool ok = e < e1;
-            vid[u] = ok ? st.row[e] : -1;
-            dl[u] = ok ? st.dst[e] : 0u;
-            if constexpr (W) wv[u] = ok ? st.w[e] : 0.f;
+            in[u] = e < e1;
+            const uint64_t ec = in[u] ? e : last;
+            vid[u] = st.row[ec];
+            dl[u] = st.dst[ec];
+            if constexpr (W) wv[u] = st.w[ec];
         }
         bool lv[U];
         if (a.live) {
             uint32_t wd[U];
 #pragma unroll
-            for (int u = 0; u < U; u++) wd[u] = vid[u] >= 0 ? a.live[vid[u] >> 5] : 0u;
+            for (int u = 0; u < U; u++) wd[u] = a.live[vid[u] >> 5];
 #pragma unroll
-            for (int u = 0; u < U; u++) lv[u] = vid[u] >= 0 && ((wd[u] >> (vid[u] & 31)) & 1u);
+            for (int u = 0; u < U; u++) lv[u] = in[u] && ((wd[u] >> (vid[u] & 31)) & 1u);
         } else {
 #pragma unroll
-            for (int u = 0; u < U; u++) lv[u] = vid[u] >= 0;
+            for (int u = 0; u < U; u++) lv[u] = in[u];
         }
         Row<S> rows[U];
 #pragma unroll
