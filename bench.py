@@ -212,7 +212,8 @@ def bench_batched(args, nat, np, work, graph, sample=None):
     n, m = graph_size(nat, graph)
     sample = sorted(sample or {0, K // 3, (2 * K) // 3, K - 1})
     t0 = time.time()
-    sess = nat.Session(common + ["--measure", "ppr", "--log", os.path.join(args.logdir or work, "batched.log")])
+    sess = nat.Session(common + ["--measure", "ppr", "--pull", args.pull,
+                                 "--log", os.path.join(args.logdir or work, "batched.log")])
     log(f"batched setup {time.time() - t0:.1f}s")
     for w in range(args.warmup):
         sess.run(reinit=w > 0)
