@@ -203,6 +203,16 @@ def test_fold_pull_all_measures(tile, monkeypatch):
     assert orc.compare(orc.fp64_reference(idx, adj, orc.ORC_KATZ, alpha=alpha)[0], xs[1])["l1_rel"] < 0.01
 
 
+@pytest.mark.parametrize("live", ["0", "1"])
+@pytest.mark.parametrize("tile", ["0", "2048"])
+@pytest.mark.parametrize("gname", ["rmat12", "star"])
+def test_dense_passes_live_rows_forced(live, gname, tile, rmat12, monkeypatch):
+    """The live-row bitmap (a source's row is gathered only if it pushed since the last pass)
+    forced off, and forced on also in the first passes, where nearly every source is pending:
+    the same bounds and invariant as the default choice by demand."""
+    monkeypatch.setenv("EXC_PULL_LIVE", live)
+    test_fold_pull_dense_passes(0.0, gname, tile, rmat12, monkeypatch)
+
 @pytest.mark.parametrize("tile", ["0", "0:256", "1024", "4096", "4096/1"])
 @pytest.mark.parametrize("measures", ["pr", "all"])
 def test_fused_snapshot_bit_exact(rmat12, measures, tile, monkeypatch):
@@ -251,6 +261,15 @@ def test_fused_snapshot_bit_exact(rmat12, measures, tile, monkeypatch):
             ctx.push_resident(0, n, 0, m, False)
             ctx.pass_sync()
         assert checked >= 3
+
+
+@pytest.mark.parametrize("live", ["0", "1"])
+@pytest.mark.parametrize("tile", ["0", "4096/1"])
+def test_fused_snapshot_live_rows_forced(rmat12, live, tile, monkeypatch):
+    """The fused snapshot stays bit-exact with the live-row bitmap forced off and on, over
+    three-measure rows (k_pull) and the one-measure passes of the source-sorted pull."""
+    monkeypatch.setenv("EXC_PULL_LIVE", live)
+    test_fused_snapshot_bit_exact(rmat12, "all", tile, monkeypatch)
 
 
 def test_fused_fold_state_and_results(rmat12):
